@@ -1,0 +1,193 @@
+"""Benchmark of the plain one-walker drop-in on the single-walker workload:
+3D He-4, N=256 at 0.02198 A^-3, T=1.5 K, tau=0.004 (M=167), Aziz HFDHE2,
+primitive action, FP32 pair terms with FP64 sums.
+
+One step = one pass of the hot path over one seeded configuration:
+  the full potential action (all 5,450,880 i<j slice pairs) and the Delta V of
+  three Bisection segments of 8/16/32 slices at random start slice and particle
+  (old + new partner pair evaluations, 3570 / 7650 / 15810 pairs).
+value = bead-pair evaluations per second over the timed steps.  F(q,tau) for
+all 5377 lattice q with |q| <= 3 A^-1 and all 167 lags is timed beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+F_PAIR = 50          # algorithmic FP32 flops per bead-pair evaluation
+F_QB = 9             # per (q, bead) phasor
+F_QMT = 4            # per (q, m, tau) lag product
+
+
+def pct(a, p):
+    return float(np.percentile(np.asarray(a), p)) if len(a) else None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--isf-reps", type=int, default=5)
+    ap.add_argument("--probe-copy", type=int, default=0,
+                    help="also run a plain copy kernel of this many bytes (HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is a design note "
+                                   "(--gpus must be 1)"}))
+        sys.exit(2)
+
+    import pimc_amd
+    from pimc_amd import worldlines
+    from pimc_amd.workloads import DENSITY, LAMBDA, SEED0, box_side, lattice_q
+
+    N, T, tau, M = 256, 1.5, 0.004, 167      # M as the workload states (the reference would force 166)
+    L = box_side(N, DENSITY)
+    beads = worldlines.generate(SEED0, L, M, N, LAMBDA, tau, 0)
+    ctx = pimc_amd.Context(L, M, N, tau, LAMBDA, L, v_mode=pimc_amd.V_FP32)
+    ctx.upload_path(beads)
+    q = lattice_q(L, qmag_max=3.0)
+
+    rng = np.random.default_rng(SEED0)
+    segs = (8, 16, 32)
+
+    def make_request(seg):
+        start, p = int(rng.integers(M)), int(rng.integers(N))
+        sp = np.array([[(start + k) % M, p] for k in range(1, seg)], dtype=np.int32)
+        new = beads[sp[:, 0], sp[:, 1]] + rng.normal(scale=math.sqrt(LAMBDA * tau), size=(seg - 1, 3))
+        new -= L * np.floor(new / L + 0.5)
+        return sp, new
+
+    pairs_full = sum(n * (n - 1) // 2 for n in [N] * M)
+    pairs_dv = {s: 2 * (s - 1) * (N - 1) for s in segs}
+    pairs_step = pairs_full + sum(pairs_dv.values())
+
+    def step(lat):
+        ctx.potentialAction()
+        for s in segs:
+            sp, new = make_request(s)
+            t0 = time.perf_counter()
+            ctx.delta_action(sp, new)
+            lat[s].append((time.perf_counter() - t0) * 1e6)
+
+    lat = {s: [] for s in segs}
+    for _ in range(args.warmup):
+        step(lat)
+    lat = {s: [] for s in segs}
+    ctx.set_timing(True)
+    pimc_amd.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(lat)
+    pimc_amd.device_synchronize()
+    elapsed = time.perf_counter() - t0
+    pa_ms, pa_n = ctx.kernel_time("pair_action")
+    dv_ms, dv_n = ctx.kernel_time("delta_action")
+    ctx.set_timing(False)
+
+    # F(q, tau), timed beside the step
+    ctx.isf(q)
+    ctx.set_timing(True)
+    pimc_amd.device_synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.isf_reps):
+        ctx.isf(q)
+    isf_s = (time.perf_counter() - t1) / args.isf_reps
+    rho_ms, rho_n = ctx.kernel_time("rho")
+    isf_ms, isf_n = ctx.kernel_time("isf")
+    ctx.set_timing(False)
+
+    value = pairs_step * args.steps / elapsed
+    pa_avg_s = pa_ms / max(pa_n, 1) / 1e3
+    achieved = pairs_full * F_PAIR / pa_avg_s / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "pair_action_c2_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    line = {
+        "metric": "bead-pair evals/s + p50/p99 dV latency, served vs plain GPU at <=1e-6 rel err",
+        "value": value,
+        "unit": "bead-pair evaluations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 transcendentals (exp2/rcp/rsq) + fp64 arithmetic and sums",
+        "data": "synthetic seeded worldlines (MTRand seed 139853: uniform slice 0 + Gaussian steps)",
+        "config": {"workload": "single walker: He-4 N=256, rho=0.02198 A^-3, T=1.5 K (M tau = 1/T), tau=0.004, M=167, "
+                               "Aziz 1979, primitive action, full action + Bisection dV 8/16/32",
+                   "N": N, "M": M, "form": "plain one-walker drop-in (C-ABI)"},
+        "roofline": {"bound": "valu", "kernel": "k_pair_action<mixed>",
+                     "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "avg_launch_us": pa_avg_s * 1e6, "flop_per_pair": F_PAIR, "pairs_per_launch": pairs_full},
+        "delta_v": {str(s): {"pairs": pairs_dv[s], "p50_us": pct(lat[s], 50), "p99_us": pct(lat[s], 99)}
+                    for s in segs},
+        "delta_v_kernel_avg_us": dv_ms / max(dv_n, 1) * 1e3,
+        "isf": {"n_q": int(q.shape[0]), "lags": M, "ms": isf_s * 1e3,
+                "rho_kernel_ms": rho_ms / max(rho_n, 1), "lag_kernel_ms": isf_ms / max(isf_n, 1),
+                "q_bead_phasors_per_s": q.shape[0] * M * N / isf_s,
+                "tflops_algorithmic": (q.shape[0] * M * N * F_QB + q.shape[0] * M * M * F_QMT) / isf_s / 1e12},
+        "served": None,
+    }
+
+    if args.probe_copy:
+        line["copy_probe"] = {"bytes": args.probe_copy,
+                              "avg_ms": pimc_amd.api.copy_probe(args.probe_copy, 5)}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(beads, L, M, N, tau, LAMBDA, pairs_full, pairs_dv)
+    print(json.dumps(line))
+
+
+def cpu_baseline(beads, L, M, N, tau, lam, pairs_full, pairs_dv):
+    """The oracle (FP64 restatement of the reference loops), 1 core, bounded sample."""
+    from oracle import pyoracle as orc
+    st = orc.State(L, M, N, tau, lam, L)
+    st.set_beads(beads)
+    st.total_action()                     # warm-up (table pages in)
+    reps, t = 0, 0.0
+    t0 = time.perf_counter()
+    while t < 10.0 and reps < 20:
+        st.total_action()
+        reps += 1
+        t = time.perf_counter() - t0
+    pairs = pairs_full * reps
+    lat = []
+    rng = np.random.default_rng(7)
+    dv_pairs = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 5.0:
+        s = int(rng.choice([8, 16, 32]))
+        start, p = int(rng.integers(M)), int(rng.integers(N))
+        sp = np.array([[(start + k) % M, p] for k in range(1, s)], dtype=np.int32)
+        t1 = time.perf_counter()
+        st.action_list(sp)
+        st.action_list(sp)
+        lat.append((time.perf_counter() - t1) * 1e6)
+        dv_pairs += pairs_dv[s]
+    dv_t = time.perf_counter() - t0
+    return {"value": (pairs + dv_pairs) / (t + dv_t), "unit": "bead-pair evaluations/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{reps} full potential actions of the N=256 M=167 configuration ({t:.1f} s) + "
+                      f"{len(lat)} Bisection dV (old+new Vnn lists, {dv_t:.1f} s), single thread",
+            "full_action_pairs_per_s": pairs / t, "dv_p50_us": pct(lat, 50), "dv_p99_us": pct(lat, 99)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/*
+ * pimc_gpu.h -- C-ABI of the MI355X (gfx950) all-bead-pairs hot path.
+ *
+ * Drop-in boundary for DelMaestroGroup/pimc's PotentialBase / ActionBase /
+ * EstimatorBase hot path.  Plain pointers and sizes only; every entry point
+ * returns PIMC_OK (0) or a negative PIMC_E* status, never throws, and leaves a
+ * message for pimc_last_error().  Host buffers belong to the caller; device
+ * memory belongs to the context.  One context holds one worldline
+ * configuration (one Markov chain) resident in HBM.  Calls on one context are
+ * not reentrant (the reference's LocalAction is not either,
+ * include/action.h:138-141); distinct contexts may be used from distinct
+ * threads.
+ *
+ * Layout conventions follow the reference's Path (include/path.h:164-168):
+ * beads are [M][N_alloc][3] FP64 (row-major, x fastest), bead locators are
+ * {slice, ptcl} int32 pairs, links are [M][N_alloc][2] int32, XXX = -1.
+ *
+ * All reference citations are file:line in the pimc repository.
+ */
+#ifndef PIMC_GPU_H
+#define PIMC_GPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------- */
+#define PIMC_OK 0
+#define PIMC_E_ARG -1       /* invalid argument (null pointer, bad size, bad index) */
+#define PIMC_E_HIP -2       /* HIP runtime error (message has the HIP error string) */
+#define PIMC_E_NOMEM -3     /* device allocation failed or HBM budget exceeded */
+#define PIMC_E_STATE -4     /* call not valid in the current state */
+#define PIMC_E_UNSUPPORTED -5
+
+/* ---- pair-term evaluation modes -------------------------------------------- */
+/* All modes first compute the reference's bin k = int(|r|/dr), dr = 1e-6 rm,
+ * from an FP64 minimum-image separation (include/potential.h:249-260), then:   */
+#define PIMC_V_MIXED 0  /* Aziz closed form at the bin radius: FP32 transcendental units
+                         * (exp2, rcp) around FP64 arithmetic, FP64 sums (default) */
+#define PIMC_V_FP32 PIMC_V_MIXED
+#define PIMC_V_FP64 1   /* Aziz closed form at the bin radius in FP64 */
+#define PIMC_V_TABLE 2  /* gather lookupV[k] from an FP64 table built exactly as the reference's */
+
+typedef struct pimc_ctx pimc_ctx;
+typedef struct pimc_aziz pimc_aziz;
+
+/* Cube box + Aziz + chain constants.  Replaces Prism(density, N)
+ * (src/container.cpp:84-109), AzizPotential(year, box) (src/potential.cpp:1741-1807),
+ * LookupTable(box, M, N) (src/lookuptable.cpp:26-55) and the constants() values
+ * tau, lambda, rc that LocalAction reads (src/constants.cpp:127-137). */
+typedef struct {
+    double side;        /* cube edge [A] */
+    int num_slices;     /* M (any M >= 1; the reference itself forces even M) */
+    int num_alloc;      /* N_alloc: particle columns of the bead array */
+    double tau;         /* imaginary time step [1/K] */
+    double lambda;      /* hbar^2/2m [A^2 K] */
+    double rc;          /* potential cutoff [A] (Vnn only, src/lookuptable.cpp:504) */
+    int aziz_year;      /* 1979 (HFDHE2) | 1987 | 1995 */
+    int v_mode;         /* PIMC_V_MIXED | PIMC_V_FP64 | PIMC_V_TABLE */
+} pimc_desc;
+
+int  pimc_ctx_create(const pimc_desc* desc, pimc_ctx** out);
+void pimc_ctx_destroy(pimc_ctx* ctx);
+const char* pimc_last_error(const pimc_ctx* ctx);   /* NULL ctx: last global error */
+int  pimc_set_v_mode(pimc_ctx* ctx, int v_mode);
+
+/* Upload a worldline configuration.  Replaces the Path constructor
+ * (src/path.cpp:28-129) / loadState + leftPack + LookupTable::updateGrid
+ * (src/lookuptable.cpp:314-349).  beads [M][N_alloc][3] (required);
+ * bead_on [M][N_alloc] (NULL: all on); num_beads [M] (NULL: N_alloc each);
+ * the device cell lists are rebuilt bit-exactly as updateGrid builds them. */
+int pimc_upload_path(pimc_ctx* ctx, const double* beads, const uint8_t* bead_on,
+                     const int32_t* num_beads);
+/* Worm::head/tail/special1/special2 (include/worm.h:31-34), each {slice,ptcl}
+ * or {-1,-1}; they set the pair weights of Worm::factor (src/worm.cpp:108-118). */
+int pimc_set_worm(pimc_ctx* ctx, const int32_t head[2], const int32_t tail[2],
+                  const int32_t special1[2], const int32_t special2[2]);
+/* Download the device copy of the beads [M][N_alloc][3]. */
+int pimc_download_path(pimc_ctx* ctx, double* beads);
+/* Path::updateBead for n beads in order (src/path.cpp:197-200 with
+ * LookupTable::updateBead, src/lookuptable.cpp:355-393). */
+int pimc_update_beads(pimc_ctx* ctx, const int32_t* slice_ptcl, const double* pos, int n);
+
+/* LocalAction::potentialAction() (src/action.cpp:456-472):
+ * action = sum_m vfactor[m%2] * shift*tau * (V_ext(m) + V_int(m)), all i<j pairs,
+ * no cutoff.  v_slices [M][2] = {V_ext, V_int} of LocalAction::V(int)
+ * (src/action.cpp:902-947), may be NULL. */
+int pimc_total_action(pimc_ctx* ctx, int shift, const double vfactor[2], double* action,
+                      double* v_slices);
+/* LocalAction::V(int) for every slice: v_slices [M][2]; sep_hist [50] (may be NULL)
+ * is the separation histogram the reference leaves behind after evaluating the
+ * last slice, M-1 (src/action.cpp:216-224). */
+int pimc_potential_slices(pimc_ctx* ctx, double* v_slices, int32_t* sep_hist);
+/* EnergyEstimator::accumulate potential and kinetic terms for the primitive
+ * action (src/estimator.cpp:940-1029): out8 = {K, V, V_ext, V_int, E, K/N, V/N, E/N}.
+ * next_links [M][N_alloc][2] may be NULL (periodic closed worldlines). */
+int pimc_energy(pimc_ctx* ctx, const int32_t* next_links, double* out8);
+
+/* LocalAction::potentialAction(std::vector<beadLocator>) (src/action.cpp:572-620):
+ * sum over active listed beads of vfactor[slice%2]*shift*tau*Vnn(bead)
+ * (Vnn: src/action.cpp:1044-1060, pairs inside rc and inside neighbouring cells).
+ * per_bead [n] (may be NULL) receives each bead's Vnn. */
+int pimc_action_list(pimc_ctx* ctx, const int32_t* slice_ptcl, int n, int shift,
+                     const double vfactor[2], double* action, double* per_bead);
+/* ActionBase/LocalAction::potentialAction(start, end) (src/action.cpp:384-399, :522-570):
+ * walks next links from start to end inclusive; next_links as in pimc_energy. */
+int pimc_action_range(pimc_ctx* ctx, const int32_t start[2], const int32_t end[2],
+                      const int32_t* next_links, int shift, const double vfactor[2],
+                      double* action);
+/* One-pass Delta V of a move over n beads on distinct slices (the Bisection /
+ * Staging / CenterOfMass contract, src/move.cpp:1124-1661): old_pos [n][3]
+ * (NULL: the device copy), new_pos [n][3].  Returns
+ *   out3 = {A_old, A_new, dU} with A = sum_b vfactor[s%2]*shift*tau*Vnn(b) and
+ *   dU = sum of the per-pair differences (new - old), accumulated in FP64.
+ * commit != 0 writes new_pos into the device path and updates the device cell
+ * lists (Path::updateBead) in list order. */
+int pimc_delta_action(pimc_ctx* ctx, const int32_t* slice_ptcl, const double* old_pos,
+                      const double* new_pos, int n, int shift, const double vfactor[2],
+                      int commit, double* out3);
+
+/* StaticStructureFactorEstimator::accumulate (src/estimator.cpp:3705-3737):
+ * sq[i] = sf(q_i)/N, sf = sum_m [N_m + 2 sum_{i<j} cos(q . r_ij)] (minimum image).
+ * q [nq][3].  Lattice q (q L / 2pi integer) use the phasor factorisation
+ * sum_m |rho_q(m)|^2; other q use the minimum-image pair sum. */
+int pimc_ssf(pimc_ctx* ctx, const double* q, int nq, double* sq);
+/* IntermediateScatteringFunctionEstimator::accumulate (src/estimator.cpp:3923-3961):
+ * fqt [nq][M], fqt[i][t] = isf(q_i, t)/N for all M lags, raw positions. */
+int pimc_isf(pimc_ctx* ctx, const double* q, int nq, double* fqt);
+
+/* LookupTable dumps (src/lookuptable.cpp:67-516, include/lookuptable.h:139-253).
+ * dims3 = numNNGrid; grid [M][N_alloc][3], bead_label [M][N_alloc],
+ * num_labels [cells][M], hash [cells][M][hash_cap] (any NULL skipped). */
+int pimc_cell_dims(pimc_ctx* ctx, int32_t dims3[3], int32_t* hash_cap);
+int pimc_cell_lists(pimc_ctx* ctx, int32_t* grid, int32_t* bead_label, int32_t* num_labels,
+                    int32_t* hash);
+/* LookupTable::updateInteractionList (src/lookuptable.cpp:464-516) for one bead:
+ * list2 [N_alloc][2], sep3 [N_alloc][3]; *count receives numBeads. */
+int pimc_interaction_list(pimc_ctx* ctx, int slice, int ptcl, int32_t* list2, double* sep3,
+                          int* count);
+
+/* ---- PotentialBase drop-in for the Aziz pair potential ------------------------ */
+/* AzizPotential(year, box) with maxSep and the tail-correction cutoff rc
+ * (src/potential.cpp:1741-1807). */
+int  pimc_aziz_create(int year, double max_sep, double rc, int v_mode, pimc_aziz** out);
+void pimc_aziz_destroy(pimc_aziz* az);
+/* PotentialBase::V(const dVec*, double*, int) batched form (include/potential.h:86,
+ * src/potential.cpp:127-132) for pair separations: out[i] = V(sep[i]). */
+int pimc_aziz_V(pimc_aziz* az, const double* sep3, double* out, int n);
+/* gradV / grad2V (include/potential.h:997-1018): gradV = (dV/dr)(k)/|r| * r. */
+int pimc_aziz_gradV(pimc_aziz* az, const double* sep3, double* out3, int n);
+int pimc_aziz_grad2V(pimc_aziz* az, const double* sep3, double* out, int n);
+double pimc_aziz_tailV(const pimc_aziz* az);   /* PotentialBase::tailV */
+int  pimc_aziz_table_length(const pimc_aziz* az);
+
+/* ---- measurement hooks ------------------------------------------------------- */
+/* When enabled, every launch of a hot kernel is bracketed by HIP events on the
+ * stream it runs on; pimc_kernel_time returns the summed duration [ms] and the
+ * launch count for name in {"pair_action","delta_action","rho","ssf","isf"}. */
+int pimc_set_timing(pimc_ctx* ctx, int enable);
+int pimc_kernel_time(pimc_ctx* ctx, const char* name, double* total_ms, long* count);
+int pimc_device_synchronize(void);
+/* A plain device-to-device streaming copy of `bytes` run `iters` times (no
+ * reference counterpart): a known-size kernel against which the HBM byte
+ * counters of the hot kernels are scaled.  *avg_ms receives the mean duration. */
+int pimc_copy_probe(size_t bytes, int iters, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,9 @@
+"""pimc_amd: MI355X-native (gfx950 HIP) all-bead-pairs hot path of pimc.
+
+The product is libpimc_gpu.so (C-ABI: include/pimc_gpu.h); pimc_amd.api is
+its Python mirror.  Nothing in this package computes the physics on the host.
+"""
+from .api import (V_FP32, V_FP64, V_TABLE, Aziz, Context, PimcError,  # noqa: F401
+                  device_synchronize, lib)
+
+__all__ = ["Context", "Aziz", "PimcError", "V_FP32", "V_FP64", "V_TABLE", "lib", "device_synchronize"]

@@ -1,0 +1,295 @@
+"""Python mirror of the C-ABI in include/pimc_gpu.h (ctypes over libpimc_gpu.so).
+
+The classes mirror the reference's interfaces for the hot path:
+  Context  ~ Path + LookupTable + LocalAction (src/action.cpp, src/path.cpp)
+  Aziz     ~ AzizPotential as a batched PotentialBase (src/potential.cpp:1741-1909)
+Methods keep the reference's names where one exists (potentialAction,
+potential, Vnn-based action lists, updateBead) and raise PimcError on any
+non-zero status, the way the reference aborts on a failed GPU_ASSERT.
+Nothing here computes physics on the host: every result comes from the HIP
+kernels, and importing the module fails loudly when the library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpimc_gpu.so")
+
+V_FP32, V_FP64, V_TABLE = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_bp = C.POINTER(C.c_uint8)
+
+
+class PimcError(RuntimeError):
+    pass
+
+
+class _Desc(C.Structure):
+    _fields_ = [("side", C.c_double), ("num_slices", C.c_int), ("num_alloc", C.c_int),
+                ("tau", C.c_double), ("lam", C.c_double), ("rc", C.c_double),
+                ("aziz_year", C.c_int), ("v_mode", C.c_int)]
+
+
+SIGNATURES = {
+    "pimc_ctx_create": (C.c_int, [C.POINTER(_Desc), C.POINTER(C.c_void_p)]),
+    "pimc_ctx_destroy": (None, [C.c_void_p]),
+    "pimc_last_error": (C.c_char_p, [C.c_void_p]),
+    "pimc_set_v_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "pimc_upload_path": (C.c_int, [C.c_void_p, _dp, _bp, _ip]),
+    "pimc_set_worm": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip]),
+    "pimc_download_path": (C.c_int, [C.c_void_p, _dp]),
+    "pimc_update_beads": (C.c_int, [C.c_void_p, _ip, _dp, C.c_int]),
+    "pimc_total_action": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "pimc_potential_slices": (C.c_int, [C.c_void_p, _dp, _ip]),
+    "pimc_energy": (C.c_int, [C.c_void_p, _ip, _dp]),
+    "pimc_action_list": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "pimc_action_range": (C.c_int, [C.c_void_p, _ip, _ip, _ip, C.c_int, _dp, _dp]),
+    "pimc_delta_action": (C.c_int, [C.c_void_p, _ip, _dp, _dp, C.c_int, C.c_int, _dp, C.c_int, _dp]),
+    "pimc_ssf": (C.c_int, [C.c_void_p, _dp, C.c_int, _dp]),
+    "pimc_isf": (C.c_int, [C.c_void_p, _dp, C.c_int, _dp]),
+    "pimc_cell_dims": (C.c_int, [C.c_void_p, _ip, _ip]),
+    "pimc_cell_lists": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip]),
+    "pimc_interaction_list": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _dp, C.POINTER(C.c_int)]),
+    "pimc_aziz_create": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_void_p)]),
+    "pimc_aziz_destroy": (None, [C.c_void_p]),
+    "pimc_aziz_V": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
+    "pimc_aziz_gradV": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
+    "pimc_aziz_grad2V": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
+    "pimc_aziz_tailV": (C.c_double, [C.c_void_p]),
+    "pimc_aziz_table_length": (C.c_int, [C.c_void_p]),
+    "pimc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "pimc_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_long)]),
+    "pimc_device_synchronize": (C.c_int, []),
+    "pimc_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _dp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libpimc_gpu.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PimcError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a if shape is None else a.reshape(shape)
+
+
+def _i32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a if shape is None else a.reshape(shape)
+
+
+class Context:
+    """One chain's worldlines resident in HBM (include/pimc_gpu.h, pimc_ctx)."""
+
+    def __init__(self, side, num_slices, num_alloc, tau, lam, rc, year=1979, v_mode=V_FP32):
+        d = _Desc(side, num_slices, num_alloc, tau, lam, rc, year, v_mode)
+        h = C.c_void_p()
+        rc_ = lib().pimc_ctx_create(C.byref(d), C.byref(h))
+        if rc_ != 0:
+            raise PimcError(f"pimc_ctx_create: {lib().pimc_last_error(None).decode()} ({rc_})")
+        self.h = h
+        self.M, self.N, self.side = num_slices, num_alloc, side
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pimc_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PimcError(f"{what}: {lib().pimc_last_error(self.h).decode()} ({rc})")
+
+    def set_v_mode(self, mode):
+        self._check(lib().pimc_set_v_mode(self.h, mode), "set_v_mode")
+
+    def upload_path(self, beads, bead_on=None, num_beads=None):
+        b = _f64(beads, (self.M, self.N, 3))
+        on = None if bead_on is None else np.ascontiguousarray(bead_on, dtype=np.uint8)
+        nb = None if num_beads is None else _i32(num_beads)
+        self._check(lib().pimc_upload_path(self.h, _p(b, _dp), _p(on, _bp), _p(nb, _ip)), "upload_path")
+
+    def set_worm(self, head=(-1, -1), tail=(-1, -1), special1=(-1, -1), special2=(-1, -1)):
+        arrs = [_i32(x) for x in (head, tail, special1, special2)]
+        self._check(lib().pimc_set_worm(self.h, *[_p(a, _ip) for a in arrs]), "set_worm")
+
+    def download_path(self):
+        b = np.zeros((self.M, self.N, 3))
+        self._check(lib().pimc_download_path(self.h, _p(b, _dp)), "download_path")
+        return b
+
+    def update_beads(self, slice_ptcl, pos):
+        sp = _i32(slice_ptcl, (-1, 2))
+        p = _f64(pos, (-1, 3))
+        self._check(lib().pimc_update_beads(self.h, _p(sp, _ip), _p(p, _dp), sp.shape[0]), "update_beads")
+
+    def potentialAction(self, shift=1, vfactor=(1.0, 1.0), with_slices=False):
+        vf = _f64(vfactor)
+        a = np.zeros(1)
+        vs = np.zeros((self.M, 2)) if with_slices else None
+        self._check(lib().pimc_total_action(self.h, shift, _p(vf, _dp), _p(a, _dp), _p(vs, _dp)), "total_action")
+        return (a[0], vs) if with_slices else a[0]
+
+    def potential_slices(self, sep_hist=False):
+        vs = np.zeros((self.M, 2))
+        h = np.zeros(50, dtype=np.int32) if sep_hist else None
+        self._check(lib().pimc_potential_slices(self.h, _p(vs, _dp), _p(h, _ip)), "potential_slices")
+        return (vs, h) if sep_hist else vs
+
+    def energy(self, next_links=None):
+        o = np.zeros(8)
+        nx = None if next_links is None else _i32(next_links)
+        self._check(lib().pimc_energy(self.h, _p(nx, _ip), _p(o, _dp)), "energy")
+        return dict(zip(["K", "V", "V_ext", "V_int", "E", "K/N", "V/N", "E/N"], o))
+
+    def action_list(self, slice_ptcl, shift=1, vfactor=(1.0, 1.0), per_bead=False):
+        sp = _i32(slice_ptcl, (-1, 2))
+        vf = _f64(vfactor)
+        a = np.zeros(1)
+        pb = np.zeros(sp.shape[0]) if per_bead else None
+        self._check(lib().pimc_action_list(self.h, _p(sp, _ip), sp.shape[0], shift, _p(vf, _dp), _p(a, _dp),
+                                           _p(pb, _dp)), "action_list")
+        return (a[0], pb) if per_bead else a[0]
+
+    def action_range(self, start, end, next_links=None, shift=1, vfactor=(1.0, 1.0)):
+        s, e = _i32(start), _i32(end)
+        nx = None if next_links is None else _i32(next_links)
+        vf = _f64(vfactor)
+        a = np.zeros(1)
+        self._check(lib().pimc_action_range(self.h, _p(s, _ip), _p(e, _ip), _p(nx, _ip), shift, _p(vf, _dp),
+                                            _p(a, _dp)), "action_range")
+        return a[0]
+
+    def delta_action(self, slice_ptcl, new_pos, old_pos=None, shift=1, vfactor=(1.0, 1.0), commit=False):
+        sp = _i32(slice_ptcl, (-1, 2))
+        npos = _f64(new_pos, (-1, 3))
+        opos = None if old_pos is None else _f64(old_pos, (-1, 3))
+        vf = _f64(vfactor)
+        o = np.zeros(3)
+        self._check(lib().pimc_delta_action(self.h, _p(sp, _ip), _p(opos, _dp), _p(npos, _dp), sp.shape[0], shift,
+                                            _p(vf, _dp), int(bool(commit)), _p(o, _dp)), "delta_action")
+        return o
+
+    def ssf(self, q):
+        q = _f64(q, (-1, 3))
+        o = np.zeros(q.shape[0])
+        self._check(lib().pimc_ssf(self.h, _p(q, _dp), q.shape[0], _p(o, _dp)), "ssf")
+        return o
+
+    def isf(self, q):
+        q = _f64(q, (-1, 3))
+        o = np.zeros((q.shape[0], self.M))
+        self._check(lib().pimc_isf(self.h, _p(q, _dp), q.shape[0], _p(o, _dp)), "isf")
+        return o
+
+    def cell_dims(self):
+        d = np.zeros(3, dtype=np.int32)
+        cap = np.zeros(1, dtype=np.int32)
+        self._check(lib().pimc_cell_dims(self.h, _p(d, _ip), _p(cap, _ip)), "cell_dims")
+        return d, int(cap[0])
+
+    def cell_lists(self):
+        d, cap = self.cell_dims()
+        cells = int(np.prod(d))
+        grid = np.zeros((self.M, self.N, 3), dtype=np.int32)
+        lab = np.zeros((self.M, self.N), dtype=np.int32)
+        nl = np.zeros((cells, self.M), dtype=np.int32)
+        hs = np.zeros((cells, self.M, cap), dtype=np.int32)
+        self._check(lib().pimc_cell_lists(self.h, _p(grid, _ip), _p(lab, _ip), _p(nl, _ip), _p(hs, _ip)),
+                    "cell_lists")
+        return dict(dims=d, cap=cap, grid=grid, bead_label=lab, num_labels=nl, hash=hs)
+
+    def interaction_list(self, slc, ptcl):
+        lst = np.zeros((self.N, 2), dtype=np.int32)
+        sep = np.zeros((self.N, 3))
+        cnt = C.c_int()
+        self._check(lib().pimc_interaction_list(self.h, slc, ptcl, _p(lst, _ip), _p(sep, _dp), C.byref(cnt)),
+                    "interaction_list")
+        return lst[:cnt.value].copy(), sep[:cnt.value].copy()
+
+    def set_timing(self, on=True):
+        self._check(lib().pimc_set_timing(self.h, int(on)), "set_timing")
+
+    def kernel_time(self, name):
+        ms = C.c_double()
+        cnt = C.c_long()
+        self._check(lib().pimc_kernel_time(self.h, name.encode(), C.byref(ms), C.byref(cnt)), "kernel_time")
+        return ms.value, cnt.value
+
+
+class Aziz:
+    """Batched PotentialBase for the Aziz pair potential (include/pimc_gpu.h)."""
+
+    def __init__(self, year, max_sep, rc, v_mode=V_FP32):
+        h = C.c_void_p()
+        rc_ = lib().pimc_aziz_create(year, max_sep, rc, v_mode, C.byref(h))
+        if rc_ != 0:
+            raise PimcError(f"pimc_aziz_create failed ({rc_})")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pimc_aziz_destroy(self.h)
+            self.h = None
+
+    def _run(self, fn, sep, width):
+        s = _f64(sep, (-1, 3))
+        o = np.zeros((s.shape[0], width)) if width > 1 else np.zeros(s.shape[0])
+        rc_ = fn(self.h, _p(s, _dp), _p(o, _dp), s.shape[0])
+        if rc_ != 0:
+            raise PimcError(f"aziz batch failed ({rc_})")
+        return o
+
+    def V(self, sep):
+        return self._run(lib().pimc_aziz_V, sep, 1)
+
+    def gradV(self, sep):
+        return self._run(lib().pimc_aziz_gradV, sep, 3)
+
+    def grad2V(self, sep):
+        return self._run(lib().pimc_aziz_grad2V, sep, 1)
+
+    @property
+    def tailV(self):
+        return lib().pimc_aziz_tailV(self.h)
+
+    @property
+    def table_length(self):
+        return lib().pimc_aziz_table_length(self.h)
+
+
+def copy_probe(nbytes, iters=20):
+    ms = np.zeros(1)
+    rc_ = lib().pimc_copy_probe(nbytes, iters, _p(ms, _dp))
+    if rc_ != 0:
+        raise PimcError(f"copy probe failed ({rc_})")
+    return float(ms[0])
+
+
+def device_synchronize():
+    rc_ = lib().pimc_device_synchronize()
+    if rc_ != 0:
+        raise PimcError(lib().pimc_last_error(None).decode())

@@ -1,0 +1,957 @@
+// pimc_capi.hip -- host side of the C-ABI declared in include/pimc_gpu.h.
+//
+// A pimc_ctx owns one chain's worldlines in HBM (FP64 SoA planes x|y|z of
+// [M][N_alloc]), the bead mask, beads-per-slice, the bit-exact cell lists and
+// the pair work list, plus one HIP stream (default priority) and pinned
+// staging buffers.  Every entry point validates its arguments on the host,
+// launches on the context's stream and synchronises before returning.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pimc_gpu.h"
+#include "pimc_kernels.hpp"
+
+
+#include "pimc_host.hpp"
+
+using namespace pimc;
+
+// ===========================================================================
+// errors
+// ===========================================================================
+static thread_local std::string g_last_error;
+
+static int fail(pimc_ctx* ctx, int code, const std::string& msg) {
+    g_last_error = msg;
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_OR_FAIL(ctx, expr)                                                              \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail((ctx), PIMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// ===========================================================================
+// Aziz constants and tables (src/potential.cpp:1741-1909)
+// ===========================================================================
+namespace pimc {
+
+bool aziz_params(int year, AzizParams* p) {
+    if (year == 1979) {
+        *p = {10.8, 2.9673, 1.241314, 13.353384, 0.0, 1.3732412, 0.4253785, 0.1781, 0.5448504E6};
+    } else if (year == 1987) {
+        *p = {10.948, 2.9673, 1.4826, 10.43329537, -2.27965105, 1.36745214, 0.42123807, 0.17473318, 1.8443101E5};
+    } else if (year == 1995) {
+        *p = {10.956, 2.9683, 1.438, 10.5717543, -2.07758779, 1.35186623, 0.4149514, 0.17151143, 1.86924404E5};
+    } else {
+        return false;
+    }
+    return true;
+}
+
+// valueV / valuedVdr / valued2Vdr2 of the reference, FP64, in its expression order.
+#pragma STDC FP_CONTRACT OFF
+static double aziz_F(const AzizParams& p, double x) {
+    return (x < p.D ? std::exp(-(p.D / x - 1.0) * (p.D / x - 1.0)) : 1.0);
+}
+static double aziz_dF(const AzizParams& p, double x) {
+    double ix = 1.0 / x;
+    double r = 2.0 * p.D * ix * ix * (p.D * ix - 1.0) * std::exp(-(p.D * ix - 1.0) * (p.D * ix - 1.0));
+    return (x < p.D ? r : 0.0);
+}
+static double aziz_d2F(const AzizParams& p, double x) {
+    double ix = 1.0 / x;
+    const double D = p.D;
+    double r = 2.0 * D * ix * ix * ix * (2.0 * D * D * D * ix * ix * ix - 4.0 * D * D * ix * ix - D * ix + 2.0) *
+               std::exp(-(D * ix - 1.0) * (D * ix - 1.0));
+    return (x < D ? r : 0.0);
+}
+double aziz_value(const AzizParams& p, double r, int what) {
+    double x = r / p.rm;
+    if (what == 0) {
+        double Urep = p.A * std::exp(-p.alpha * x + p.beta * x * x);
+        if (x < 1.0e-7) return 0.0;
+        if (x < 0.01) return (p.eps * Urep);
+        double ix2 = 1.0 / (x * x);
+        double ix6 = ix2 * ix2 * ix2;
+        double ix8 = ix6 * ix2;
+        double ix10 = ix8 * ix2;
+        double Uatt = -(p.C6 * ix6 + p.C8 * ix8 + p.C10 * ix10) * aziz_F(p, x);
+        return (p.eps * (Urep + Uatt));
+    }
+    if (what == 1) {
+        double T1 = p.A * (-p.alpha + 2.0 * p.beta * x) * std::exp(-p.alpha * x + p.beta * x * x);
+        if (x < 1.0e-7) return 0.0;
+        if (x < 0.01) return ((p.eps / p.rm) * T1);
+        double ix = 1.0 / x;
+        double ix2 = ix * ix;
+        double ix6 = ix2 * ix2 * ix2;
+        double ix7 = ix6 * ix;
+        double ix8 = ix6 * ix2;
+        double ix9 = ix8 * ix;
+        double ix10 = ix8 * ix2;
+        double ix11 = ix10 * ix;
+        double T2 = (6.0 * p.C6 * ix7 + 8.0 * p.C8 * ix9 + 10.0 * p.C10 * ix11) * aziz_F(p, x);
+        double T3 = -(p.C6 * ix6 + p.C8 * ix8 + p.C10 * ix10) * aziz_dF(p, x);
+        return ((p.eps / p.rm) * (T1 + T2 + T3));
+    }
+    double ab = (p.alpha - 2.0 * p.beta * x) * (p.alpha - 2.0 * p.beta * x);
+    double T1 = p.A * (2 * p.beta + ab) * std::exp(-p.alpha * x + p.beta * x * x);
+    if (x < 1.0e-7) return 0.0;
+    if (x < 0.01) return ((p.eps / p.rm) * T1);
+    double ix = 1.0 / x;
+    double ix2 = ix * ix;
+    double ix6 = ix2 * ix2 * ix2;
+    double ix7 = ix6 * ix;
+    double ix8 = ix6 * ix2;
+    double ix9 = ix8 * ix;
+    double ix10 = ix8 * ix2;
+    double ix11 = ix10 * ix;
+    double ix12 = ix11 * ix;
+    double T2 = -(42.0 * p.C6 * ix8 + 72.0 * p.C8 * ix10 + 110.0 * p.C10 * ix12) * aziz_F(p, x);
+    double T3 = 2.0 * (6.0 * p.C6 * ix7 + 8.0 * p.C8 * ix9 + 10.0 * p.C10 * ix11) * aziz_dF(p, x);
+    double T4 = -(p.C6 * ix6 + p.C8 * ix8 + p.C10 * ix10) * aziz_d2F(p, x);
+    return ((p.eps / (p.rm * p.rm)) * (T1 + T2 + T3 + T4));
+}
+
+AzizDev make_aziz_dev(const AzizParams& p, double max_sep) {
+    AzizDev a{};
+    a.dr = (1.0E-6) * p.rm;
+    a.table_length = int(max_sep / a.dr);
+    a.s = a.dr / p.rm;
+    // branch bins from the accumulated radius r_n (include/potential.h:176-183)
+    a.k_eps = a.k_hc = a.k_d = a.table_length;
+    double r = 0.0;
+    bool f0 = false, f1 = false, f2 = false;
+    for (int n = 0; n < a.table_length; ++n) {
+        const double x = r / p.rm;
+        if (!f0 && !(x < 1.0e-7)) { a.k_eps = n; f0 = true; }
+        if (!f1 && !(x < 0.01)) { a.k_hc = n; f1 = true; }
+        if (!f2 && !(x < p.D)) { a.k_d = n; f2 = true; }
+        if (f0 && f1 && f2) break;
+        r += a.dr;
+    }
+    const double log2e = 1.4426950408889634074;
+    a.rep_c0 = std::log2(p.eps * p.A);
+    a.rep_c1 = -p.alpha * a.s * log2e;
+    a.rep_c2 = p.beta * a.s * a.s * log2e;
+    a.eC6d = p.eps * p.C6;
+    a.eC8d = p.eps * p.C8;
+    a.eC10d = p.eps * p.C10;
+    a.eps = p.eps;
+    a.A = p.A;
+    a.alpha = p.alpha;
+    a.beta = p.beta;
+    a.D = p.D;
+    a.C6 = p.C6;
+    a.C8 = p.C8;
+    a.C10 = p.C10;
+    a.table = nullptr;
+    return a;
+}
+
+// initLookupTable: entry n = value at the n-fold accumulated radius.
+void build_table(const AzizParams& p, int length, int what, std::vector<double>& out) {
+    out.assign(length, 0.0);
+    const double dr = (1.0E-6) * p.rm;
+    double r = 0.0;
+    for (int n = 0; n < length; ++n) {
+        out[n] = aziz_value(p, r, what);
+        r += dr;
+    }
+}
+
+double aziz_tail(const AzizParams& p, double rc) {
+    double rmorc = p.rm / rc;
+    double t2 = p.C6 * std::pow(rmorc, 3.0) / 3.0;
+    double t3 = p.C8 * std::pow(rmorc, 5.0) / 5.0;
+    double t4 = p.C10 * std::pow(rmorc, 7.0) / 7.0;
+    return 2.0 * M_PI * p.eps * (-p.rm * p.rm * p.rm * (t2 + t3 + t4));
+}
+#pragma STDC FP_CONTRACT ON
+
+// Process-wide device tables: one per (year, derivative), grown on demand.
+struct TableCache {
+    std::mutex mu;
+    struct Entry {
+        double* d = nullptr;
+        int length = 0;
+    };
+    std::map<std::pair<int, int>, Entry> tables;
+    const double* get(int year, int what, int length, std::string* err) {
+        std::lock_guard<std::mutex> g(mu);
+        Entry& e = tables[{year, what}];
+        if (e.d && e.length >= length) return e.d;
+        AzizParams p;
+        if (!aziz_params(year, &p)) {
+            *err = "unknown Aziz year";
+            return nullptr;
+        }
+        std::vector<double> h;
+        build_table(p, length, what, h);
+        double* d = nullptr;
+        if (hipMalloc(&d, sizeof(double) * std::max(1, length)) != hipSuccess) {
+            *err = "hipMalloc of the Aziz table failed";
+            return nullptr;
+        }
+        if (hipMemcpy(d, h.data(), sizeof(double) * length, hipMemcpyHostToDevice) != hipSuccess) {
+            *err = "upload of the Aziz table failed";
+            (void)hipFree(d);
+            return nullptr;
+        }
+        // Older, shorter tables stay allocated: contexts may still point at them.
+        e.d = d;
+        e.length = length;
+        return e.d;
+    }
+};
+TableCache& table_cache() {
+    static TableCache* c = new TableCache();
+    return *c;
+}
+
+const double* cached_table(int year, int what, int length, std::string* err) {
+    return table_cache().get(year, what, length, err);
+}
+
+}  // namespace pimc
+
+// ===========================================================================
+// timing
+// ===========================================================================
+static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf", "isf"};
+
+void pimc_ctx::timed_begin(int k) {
+    if (!timing) return;
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    (void)hipEventRecord(a, stream);
+    pending.push_back({k, a, b});
+}
+void pimc_ctx::timed_end() {
+    if (!timing || pending.empty()) return;
+    (void)hipEventRecord(pending.back().b, stream);
+}
+void pimc_ctx::resolve_timing() {
+    for (auto& p : pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            kernel_ms[p.kind] += ms;
+            kernel_count[p.kind] += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    pending.clear();
+}
+
+// ===========================================================================
+// context management
+// ===========================================================================
+template <typename T>
+static hipError_t dalloc(T** p, size_t n) {
+    return hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1));
+}
+
+pimc_ctx::~pimc_ctx() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    resolve_timing();
+    for (void* p : dev_allocs) (void)hipFree(p);
+    for (void* p : host_allocs) (void)hipHostFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+template <typename T>
+static int ctx_dalloc(pimc_ctx* c, T** p, size_t n) {
+    hipError_t e = dalloc(p, n);
+    if (e != hipSuccess) return fail(c, PIMC_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->dev_allocs.push_back((void*)*p);
+    return PIMC_OK;
+}
+template <typename T>
+static int ctx_halloc(pimc_ctx* c, T** p, size_t n) {
+    hipError_t e = hipHostMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1), hipHostMallocDefault);
+    if (e != hipSuccess) return fail(c, PIMC_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    c->host_allocs.push_back((void*)*p);
+    return PIMC_OK;
+}
+
+static void set_box(pimc_ctx* c) {
+    BoxDev& b = c->box;
+    b.L = c->desc.side;
+    b.invL = 1.0 / c->desc.side;
+    b.rc2 = c->desc.rc * c->desc.rc;
+    // LookupTable::setupNNGrid (src/lookuptable.cpp:73-90)
+    int ng = (int)std::floor((c->desc.side / c->desc.rc) + 1.0e-7);
+    if (ng < 1) ng = 1;
+    b.ngrid = ng;
+    b.grid_size = c->desc.side / (1.0 * ng);
+}
+
+static void build_grid_nn(pimc_ctx* c) {
+    // 27 neighbour cells per cell, duplicates -1 keeping the last occurrence
+    // (src/lookuptable.cpp:111-184)
+    const int ng = c->box.ngrid;
+    const int cells = ng * ng * ng;
+    c->grid_nn.assign((size_t)cells * 27, -1);
+    for (int n = 0; n < cells; ++n) {
+        const int g[3] = {n / (ng * ng), (n / ng) % ng, n % ng};
+        int row[27];
+        for (int m = 0; m < 27; ++m) {
+            const int sh[3] = {m / 9 - 1, (m / 3) % 3 - 1, m % 3 - 1};
+            int t[3];
+            for (int i = 0; i < 3; ++i) {
+                t[i] = g[i] + sh[i];
+                if (t[i] == ng) t[i] = 0;
+                else if (t[i] == -1) t[i] = ng - 1;
+            }
+            row[m] = (t[0] * ng + t[1]) * ng + t[2];
+        }
+        for (int m = 26; m >= 0; --m) {
+            if (row[m] < 0) continue;
+            for (int p = m - 1; p >= 0; --p)
+                if (row[p] == row[m]) row[p] = -1;
+        }
+        for (int m = 0; m < 27; ++m) c->grid_nn[(size_t)n * 27 + m] = row[m];
+    }
+}
+
+static int64_t pair_chunk_for(int64_t total_pairs) {
+    // aim for >= ~3000 workgroups on the 256-CU part, at least 4 pairs per lane
+    int64_t chunk = total_pairs / 3000;
+    chunk = std::max<int64_t>(chunk, (int64_t)kPairThreads * 4);
+    chunk = ((chunk + kPairThreads - 1) / kPairThreads) * kPairThreads;
+    return std::min<int64_t>(chunk, (int64_t)kPairThreads * 512);
+}
+
+static int rebuild_work(pimc_ctx* c) {
+    const int M = c->desc.num_slices;
+    std::vector<SliceDesc> sl(M);
+    std::vector<PairItem> items;
+    int64_t total = 0;
+    for (int m = 0; m < M; ++m) {
+        const int64_t n = c->h_nbs[m];
+        total += n * (n / 2);
+    }
+    int64_t chunk = c->pair_chunk > 0 ? c->pair_chunk : pair_chunk_for(total);
+    for (int m = 0; m < M; ++m) {
+        const int n = c->h_nbs[m];
+        sl[m].n = n;
+        sl[m].slice = m;
+        sl[m].walker = 0;
+        sl[m].first_item = (int)items.size();
+        sl[m].pos = (int64_t)m * c->desc.num_alloc;
+        sl[m].plane = c->plane;
+        const int64_t P = (int64_t)n * (n / 2);
+        for (int64_t p = 0; p < P; p += chunk) {
+            PairItem it;
+            it.slice = m;
+            it.p_begin = (int)p;
+            it.p_count = (int)std::min<int64_t>(chunk, P - p);
+            items.push_back(it);
+        }
+    }
+    c->n_items = (int)items.size();
+    int maxn = 0;
+    for (int m = 0; m < M; ++m) maxn = std::max(maxn, c->h_nbs[m]);
+    c->lds_beads = maxn <= kLdsBeads ? std::max(maxn, 1) : 0;
+    if (c->n_items > c->cap_items) {
+        if (c->d_items) {
+            (void)hipFree(c->d_items);
+            (void)hipFree(c->d_partials);
+            c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)c->d_items),
+                                c->dev_allocs.end());
+            c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)c->d_partials),
+                                c->dev_allocs.end());
+        }
+        int r;
+        if ((r = ctx_dalloc(c, &c->d_items, c->n_items))) return r;
+        if ((r = ctx_dalloc(c, &c->d_partials, c->n_items))) return r;
+        c->cap_items = c->n_items;
+    }
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_slices, sl.data(), sizeof(SliceDesc) * M, hipMemcpyHostToDevice, c->stream));
+    if (c->n_items)
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->d_items, items.data(), sizeof(PairItem) * c->n_items,
+                                      hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    return PIMC_OK;
+}
+
+static AzizDev ctx_aziz(pimc_ctx* c, int* rc) {
+    AzizDev a = c->az;
+    *rc = PIMC_OK;
+    if (c->desc.v_mode == PIMC_V_TABLE) {
+        std::string err;
+        a.table = table_cache().get(c->desc.aziz_year, 0, a.table_length, &err);
+        if (!a.table) *rc = fail(c, PIMC_E_NOMEM, err);
+    }
+    return a;
+}
+
+static void fill_walker(pimc_ctx* c, WalkerDesc* w, int shift, const double vf[2]) {
+    w->box = c->box;
+    w->table_length = c->az.table_length;
+    w->num_slices = c->desc.num_slices;
+    w->first_slice = 0;
+    w->shift = shift;
+    w->vfactor0 = vf ? vf[0] : 1.0;
+    w->vfactor1 = vf ? vf[1] : 1.0;
+    w->tau = c->desc.tau;
+    w->worm = c->worm;
+}
+
+extern "C" {
+
+const char* pimc_last_error(const pimc_ctx* ctx) {
+    if (ctx) return ctx->err.c_str();
+    return g_last_error.c_str();
+}
+
+int pimc_ctx_create(const pimc_desc* d, pimc_ctx** out) {
+    if (!d || !out) return fail(nullptr, PIMC_E_ARG, "null argument");
+    *out = nullptr;
+    if (d->num_slices < 1 || d->num_alloc < 1 || !(d->side > 0) || !(d->tau > 0) || !(d->rc > 0))
+        return fail(nullptr, PIMC_E_ARG, "invalid pimc_desc sizes");
+    if (d->v_mode < PIMC_V_FP32 || d->v_mode > PIMC_V_TABLE) return fail(nullptr, PIMC_E_ARG, "invalid v_mode");
+    AzizParams p;
+    if (!aziz_params(d->aziz_year, &p)) return fail(nullptr, PIMC_E_ARG, "aziz_year must be 1979, 1987 or 1995");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, PIMC_E_HIP, "no HIP device available");
+    std::unique_ptr<pimc_ctx> c(new pimc_ctx());
+    c->desc = *d;
+    c->params = p;
+    const int M = d->num_slices, N = d->num_alloc;
+    c->plane = (int64_t)M * N;
+    set_box(c.get());
+    // Prism::maxSep = |side/2| over the three axes (src/container.cpp:100)
+    const double h = d->side / 2.0;
+    double ms = 0.0;
+    ms += h * h;
+    ms += h * h;
+    ms += h * h;
+    c->max_sep = std::sqrt(ms);
+    c->az = make_aziz_dev(p, c->max_sep);
+    c->tailV = aziz_tail(p, d->rc);
+    c->worm = WormDev{{-1, -1}, {-1, -1}, {-1, -1}, {-1, -1}};
+    HIP_OR_FAIL(nullptr, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const int cells = c->box.ngrid * c->box.ngrid * c->box.ngrid;
+    c->cells = cells;
+    int r;
+    if ((r = ctx_dalloc(c.get(), &c->d_pos, 3 * (size_t)M * N))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_on, (size_t)M * N))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_nbs, M))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_cell, (size_t)M * N))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_label, (size_t)M * N))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_numlab, (size_t)cells * M))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_hash, (size_t)cells * M * N))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_grid_nn, (size_t)cells * 27))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_slices, M))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_walker, 1))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_vslice, M))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_out, 16))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_hist, 64))) return r;
+    if ((r = ctx_dalloc(c.get(), &c->d_kin_partials, 1024))) return r;
+    if ((r = ctx_halloc(c.get(), &c->h_stage, std::max<size_t>(3 * (size_t)M * N, 256)))) return r;
+    if ((r = ctx_halloc(c.get(), &c->h_out, 64))) return r;
+    {
+        // small results are written by the kernels straight into host memory
+        hipError_t e = hipHostMalloc((void**)&c->h_map, sizeof(double) * 64,
+                                     hipHostMallocMapped | hipHostMallocCoherent);
+        if (e != hipSuccess) return fail(nullptr, PIMC_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        c->host_allocs.push_back((void*)c->h_map);
+        e = hipHostGetDevicePointer((void**)&c->d_map, c->h_map, 0);
+        if (e != hipSuccess) return fail(nullptr, PIMC_E_HIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+    }
+    HIP_OR_FAIL(c.get(), hipMemsetAsync(c->d_hash, 0xff, sizeof(int) * (size_t)cells * M * N, c->stream));
+    HIP_OR_FAIL(c.get(), hipMemsetAsync(c->d_numlab, 0, sizeof(int) * (size_t)cells * M, c->stream));
+    build_grid_nn(c.get());
+    HIP_OR_FAIL(c.get(), hipMemcpyAsync(c->d_grid_nn, c->grid_nn.data(), sizeof(int) * c->grid_nn.size(),
+                                        hipMemcpyHostToDevice, c->stream));
+    c->h_nbs.assign(M, 0);
+    c->h_on.assign((size_t)M * N, 0);
+    HIP_OR_FAIL(c.get(), hipStreamSynchronize(c->stream));
+    *out = c.release();
+    return PIMC_OK;
+}
+
+void pimc_ctx_destroy(pimc_ctx* ctx) { delete ctx; }
+
+int pimc_set_v_mode(pimc_ctx* c, int v_mode) {
+    if (!c) return fail(nullptr, PIMC_E_ARG, "null ctx");
+    if (v_mode < PIMC_V_FP32 || v_mode > PIMC_V_TABLE) return fail(c, PIMC_E_ARG, "invalid v_mode");
+    c->desc.v_mode = v_mode;
+    return PIMC_OK;
+}
+
+int pimc_upload_path(pimc_ctx* c, const double* beads, const uint8_t* bead_on, const int32_t* num_beads) {
+    if (!c || !beads) return fail(c, PIMC_E_ARG, "null argument");
+    const int M = c->desc.num_slices, N = c->desc.num_alloc;
+    const size_t MN = (size_t)M * N;
+    for (int m = 0; m < M; ++m) {
+        const int nb = num_beads ? num_beads[m] : N;
+        if (nb < 0 || nb > N) return fail(c, PIMC_E_ARG, "num_beads out of range");
+        c->h_nbs[m] = nb;
+    }
+    for (size_t t = 0; t < MN; ++t) c->h_on[t] = bead_on ? (bead_on[t] ? 1 : 0) : 1;
+    // AoS [M][N][3] -> SoA planes
+    for (size_t t = 0; t < MN; ++t) {
+        c->h_stage[t] = beads[3 * t];
+        c->h_stage[t + MN] = beads[3 * t + 1];
+        c->h_stage[t + 2 * MN] = beads[3 * t + 2];
+    }
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_pos, c->h_stage, sizeof(double) * 3 * MN, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_on, c->h_on.data(), MN, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_nbs, c->h_nbs.data(), sizeof(int) * M, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemsetAsync(c->d_hash, 0xff, sizeof(int) * (size_t)c->cells * MN, c->stream));
+    HIP_OR_FAIL(c, launch_cell_build(c->stream, c->d_pos, c->plane, N, M, c->d_on, c->d_cell, c->d_label,
+                                     c->d_numlab, c->d_hash, c->cells, c->box));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    c->have_path = true;
+    return rebuild_work(c);
+}
+
+int pimc_set_worm(pimc_ctx* c, const int32_t head[2], const int32_t tail[2], const int32_t special1[2],
+                  const int32_t special2[2]) {
+    if (!c) return fail(nullptr, PIMC_E_ARG, "null ctx");
+    auto cp = [](int* d, const int32_t* s) {
+        d[0] = s ? s[0] : -1;
+        d[1] = s ? s[1] : -1;
+    };
+    cp(c->worm.head, head);
+    cp(c->worm.tail, tail);
+    cp(c->worm.sp1, special1);
+    cp(c->worm.sp2, special2);
+    return PIMC_OK;
+}
+
+int pimc_download_path(pimc_ctx* c, double* beads) {
+    if (!c || !beads) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    const size_t MN = (size_t)c->desc.num_slices * c->desc.num_alloc;
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->h_stage, c->d_pos, sizeof(double) * 3 * MN, hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    for (size_t t = 0; t < MN; ++t) {
+        beads[3 * t] = c->h_stage[t];
+        beads[3 * t + 1] = c->h_stage[t + MN];
+        beads[3 * t + 2] = c->h_stage[t + 2 * MN];
+    }
+    return PIMC_OK;
+}
+
+static int check_beads(pimc_ctx* c, const int32_t* sp, int n) {
+    if (n < 0 || (n > 0 && !sp)) return fail(c, PIMC_E_ARG, "bad bead list");
+    for (int k = 0; k < n; ++k)
+        if (sp[2 * k] < 0 || sp[2 * k] >= c->desc.num_slices || sp[2 * k + 1] < 0 ||
+            sp[2 * k + 1] >= c->desc.num_alloc)
+            return fail(c, PIMC_E_ARG, "bead locator out of range");
+    return PIMC_OK;
+}
+
+static int ensure_req(pimc_ctx* c, int n) {
+    if (n <= c->cap_req) return PIMC_OK;
+    int cap = std::max(64, n);
+    int r;
+    if ((r = ctx_dalloc(c, &c->d_req_beads, cap))) return r;
+    if ((r = ctx_halloc(c, &c->h_req_beads, cap))) return r;
+    if ((r = ctx_dalloc(c, &c->d_req_sp, 2 * (size_t)cap))) return r;
+    if ((r = ctx_dalloc(c, &c->d_req_pos, 3 * (size_t)cap))) return r;
+    if ((r = ctx_halloc(c, &c->h_req_sp, 2 * (size_t)cap))) return r;
+    if ((r = ctx_halloc(c, &c->h_req_pos, 3 * (size_t)cap))) return r;
+    if ((r = ctx_dalloc(c, &c->d_per_bead, 2 * (size_t)cap))) return r;
+    if ((r = ctx_halloc(c, &c->h_per_bead, 2 * (size_t)cap))) return r;
+    const int chunks = (c->desc.num_alloc + kVnnChunk - 1) / kVnnChunk;
+    if ((r = ctx_dalloc(c, &c->d_vnn_partials, 2 * (size_t)cap * chunks))) return r;
+    c->cap_req = cap;
+    return PIMC_OK;
+}
+
+int pimc_update_beads(pimc_ctx* c, const int32_t* sp, const double* pos, int n) {
+    if (!c) return fail(nullptr, PIMC_E_ARG, "null ctx");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    int r;
+    if ((r = check_beads(c, sp, n))) return r;
+    if (n == 0) return PIMC_OK;
+    if (!pos) return fail(c, PIMC_E_ARG, "null positions");
+    if ((r = ensure_req(c, n))) return r;
+    std::memcpy(c->h_req_sp, sp, sizeof(int) * 2 * n);
+    std::memcpy(c->h_req_pos, pos, sizeof(double) * 3 * n);
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_req_sp, c->h_req_sp, sizeof(int) * 2 * n, hipMemcpyHostToDevice, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_req_pos, c->h_req_pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                                  c->stream));
+    HIP_OR_FAIL(c, launch_update_beads(c->stream, c->d_req_sp, c->d_req_pos, n, c->d_pos, c->plane,
+                                       c->desc.num_alloc, c->desc.num_slices, c->d_cell, c->d_label, c->d_numlab,
+                                       c->d_hash, c->box));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    return PIMC_OK;
+}
+
+int pimc_potential_slices(pimc_ctx* c, double* v_slices, int32_t* sep_hist) {
+    const double vf[2] = {1.0, 1.0};
+    double act = 0.0;
+    int r = pimc_total_action(c, 1, vf, &act, v_slices);
+    if (r) return r;
+    if (sep_hist) {
+        const int m = c->desc.num_slices - 1;
+        const double dsep = 0.5 * std::sqrt(3.0) * c->desc.side / (1.0 * 50);
+        HIP_OR_FAIL(c, hipMemsetAsync(c->d_hist, 0, sizeof(int) * 64, c->stream));
+        HIP_OR_FAIL(c, launch_sep_hist(c->stream, c->d_pos, c->plane, m, c->h_nbs[m], c->desc.num_alloc, c->box,
+                                       dsep, c->d_hist));
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->h_stage, c->d_hist, sizeof(int) * 50, hipMemcpyDeviceToHost, c->stream));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+        std::memcpy(sep_hist, c->h_stage, sizeof(int) * 50);
+    }
+    return PIMC_OK;
+}
+
+int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* action, double* v_slices) {
+    if (!c || !action) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    if (shift < 1) return fail(c, PIMC_E_ARG, "shift must be >= 1");
+    const int M = c->desc.num_slices;
+    int r;
+    AzizDev az = ctx_aziz(c, &r);
+    if (r) return r;
+    WalkerDesc w;
+    std::memset(&w, 0, sizeof(w));
+    fill_walker(c, &w, shift, vfactor);
+    if (!c->walker_valid || std::memcmp(&w, &c->walker_cache, sizeof(w)) != 0) {
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->d_walker, &w, sizeof(w), hipMemcpyHostToDevice, c->stream));
+        c->walker_cache = w;
+        c->walker_valid = true;
+    }
+    if (c->n_items > 0) {
+        c->timed_begin(0);
+        HIP_OR_FAIL(c, launch_pair_action(c->desc.v_mode, c->stream, c->d_items, c->n_items, c->d_slices, M,
+                                          c->d_walker, 1, c->d_pos, az, c->lds_beads, c->d_partials, c->d_vslice,
+                                          c->d_map));
+        c->timed_end();
+        if (v_slices)
+            HIP_OR_FAIL(c, hipMemcpyAsync(c->h_stage, c->d_vslice, sizeof(double) * M, hipMemcpyDeviceToHost,
+                                          c->stream));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+        *action = c->h_map[0];
+        if (v_slices)
+            for (int m = 0; m < M; ++m) {
+                v_slices[2 * m] = 0.0;
+                v_slices[2 * m + 1] = c->h_stage[m];
+            }
+    } else {
+        *action = 0.0;
+        if (v_slices)
+            for (int m = 0; m < 2 * M; ++m) v_slices[m] = 0.0;
+    }
+    c->resolve_timing();
+    return PIMC_OK;
+}
+
+int pimc_energy(pimc_ctx* c, const int32_t* next_links, double* out8) {
+    if (!c || !out8) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    const int M = c->desc.num_slices, N = c->desc.num_alloc;
+    std::vector<double> vs(2 * (size_t)M);
+    int r = pimc_potential_slices(c, vs.data(), nullptr);
+    if (r) return r;
+    int* d_next = nullptr;
+    if (next_links) {
+        for (size_t t = 0; t < (size_t)M * N; ++t) {
+            const int a = next_links[2 * t], b = next_links[2 * t + 1];
+            const bool null_link = (a == -1 && b == -1);
+            if (!null_link && (a < 0 || a >= M || b < 0 || b >= N))
+                return fail(c, PIMC_E_ARG, "next link out of range");
+        }
+        if ((r = ctx_dalloc(c, &d_next, 2 * (size_t)M * N))) return r;
+        HIP_OR_FAIL(c, hipMemcpyAsync(d_next, next_links, sizeof(int) * 2 * (size_t)M * N, hipMemcpyHostToDevice,
+                                      c->stream));
+    }
+    HIP_OR_FAIL(c, launch_kinetic(c->stream, c->d_pos, c->plane, N, M, c->d_nbs, d_next, c->box, c->d_kin_partials,
+                                  1024, c->d_out + 8));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->h_out, c->d_out + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    const double link_sum = c->h_out[0];
+    if (d_next) {
+        (void)hipFree(d_next);
+        c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)d_next),
+                            c->dev_allocs.end());
+    }
+    // EnergyEstimator::accumulate (src/estimator.cpp:940-1029), primitive action
+    long on = 0;
+    for (auto v : c->h_on) on += v;
+    const int Np = (int)(on / M);
+    const double tau = c->desc.tau, lambda = c->desc.lambda;
+    const double volume = c->desc.side * c->desc.side * c->desc.side;
+    const double tailV = (1.0 * Np * Np / volume) * c->tailV;
+    const double kinNorm = (1.0 / (4.0 * lambda * tau)) / (tau * M);
+    const double classical = (0.5 * 3 / tau) * Np;
+    double totK = -link_sum * kinNorm;
+    double t2 = 0.0, vop0 = 0.0, vop1 = 0.0;
+    for (int m = 0; m < M; ++m) {
+        t2 += (vs[2 * m] + vs[2 * m + 1]);
+        vop0 += vs[2 * m];
+        vop1 += vs[2 * m + 1];
+    }
+    t2 /= 1.0 * M;
+    vop0 /= M;
+    vop1 /= M;
+    totK += classical;
+    const double totV = t2 + tailV;
+    vop1 += tailV;
+    out8[0] = totK;
+    out8[1] = totV;
+    out8[2] = vop0;
+    out8[3] = vop1;
+    out8[4] = totK + totV;
+    out8[5] = Np > 0 ? totK / Np : 0.0;
+    out8[6] = Np > 0 ? totV / Np : 0.0;
+    out8[7] = Np > 0 ? (totK + totV) / Np : 0.0;
+    return PIMC_OK;
+}
+
+static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commit_in_kernel, double* out3,
+                   double* per_bead) {
+    int r;
+    AzizDev az = ctx_aziz(c, &r);
+    if (r) return r;
+    VnnArgs A{};
+    const bool inline_req = n <= kVnnInline;
+    A.beads = inline_req ? nullptr : c->d_req_beads;
+    A.n_beads = n;
+    A.chunk = kVnnChunk;
+    A.chunks = (c->desc.num_alloc + kVnnChunk - 1) / kVnnChunk;
+    A.pos = c->d_pos;
+    A.plane = c->plane;
+    A.nalloc = c->desc.num_alloc;
+    A.num_slices = c->desc.num_slices;
+    A.on = c->d_on;
+    A.cell = c->d_cell;
+    A.label = c->d_label;
+    A.numlab = c->d_numlab;
+    A.hash = c->d_hash;
+    A.commit_in_kernel = commit_in_kernel ? 1 : 0;
+    A.box = c->box;
+    A.worm = c->worm;
+    A.shift = shift;
+    A.vfactor0 = vf ? vf[0] : 1.0;
+    A.vfactor1 = vf ? vf[1] : 1.0;
+    A.tau = c->desc.tau;
+    A.partials = c->d_vnn_partials;
+    A.per_bead = c->d_per_bead;
+    A.out = c->d_map + 8;
+    VnnInline req;
+    if (inline_req) {
+        std::memcpy(req.beads, c->h_req_beads, sizeof(VnnBead) * n);
+    } else {
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->d_req_beads, c->h_req_beads, sizeof(VnnBead) * n, hipMemcpyHostToDevice,
+                                      c->stream));
+    }
+    c->timed_begin(1);
+    HIP_OR_FAIL(c, launch_vnn(c->desc.v_mode, c->stream, A, req, az));
+    c->timed_end();
+    if (per_bead)
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->h_per_bead, c->d_per_bead, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
+                                      c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    if (out3)
+        for (int k = 0; k < 3; ++k) out3[k] = c->h_map[8 + k];
+    if (per_bead) std::memcpy(per_bead, c->h_per_bead, sizeof(double) * 2 * n);
+    c->resolve_timing();
+    return PIMC_OK;
+}
+
+int pimc_action_list(pimc_ctx* c, const int32_t* sp, int n, int shift, const double vfactor[2], double* action,
+                     double* per_bead) {
+    if (!c || !action) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    if (shift < 1) return fail(c, PIMC_E_ARG, "shift must be >= 1");
+    int r;
+    if ((r = check_beads(c, sp, n))) return r;
+    if (n == 0) {
+        *action = 0.0;
+        return PIMC_OK;
+    }
+    if ((r = ensure_req(c, n))) return r;
+    for (int k = 0; k < n; ++k) {
+        VnnBead& q = c->h_req_beads[k];
+        q.walker = 0;
+        q.slice = sp[2 * k];
+        q.ptcl = sp[2 * k + 1];
+        q.flags = 0;
+        for (int d = 0; d < 3; ++d) q.old_pos[d] = q.new_pos[d] = 0.0;
+    }
+    double out3[3];
+    std::vector<double> pb(per_bead ? 2 * (size_t)n : 0);
+    if ((r = run_vnn(c, n, shift, vfactor, false, out3, per_bead ? pb.data() : nullptr))) return r;
+    *action = out3[0];
+    if (per_bead)
+        for (int k = 0; k < n; ++k) per_bead[k] = pb[2 * k];
+    return PIMC_OK;
+}
+
+int pimc_action_range(pimc_ctx* c, const int32_t start[2], const int32_t end[2], const int32_t* next_links,
+                      int shift, const double vfactor[2], double* action) {
+    if (!c || !start || !end || !action) return fail(c, PIMC_E_ARG, "null argument");
+    const int M = c->desc.num_slices, N = c->desc.num_alloc;
+    auto nxt = [&](int m, int i, int* nm, int* ni) {
+        if (next_links) {
+            const size_t t = (size_t)m * N + i;
+            *nm = next_links[2 * t];
+            *ni = next_links[2 * t + 1];
+        } else {
+            *nm = (m + 1) % M;
+            *ni = i;
+        }
+    };
+    int sm, si;
+    nxt(end[0], end[1], &sm, &si);
+    std::vector<int32_t> list;
+    int m = start[0], i = start[1];
+    const int limit = M * N + 1;
+    do {
+        if (m < 0 || m >= M || i < 0 || i >= N) return fail(c, PIMC_E_ARG, "range walks off the path");
+        // LocalAction skips inactive beads (src/action.cpp:532-541)
+        if (c->h_on[(size_t)m * N + i]) {
+            list.push_back(m);
+            list.push_back(i);
+        }
+        int a, b;
+        nxt(m, i, &a, &b);
+        m = a;
+        i = b;
+        if ((int)list.size() > 2 * limit) return fail(c, PIMC_E_ARG, "range does not terminate");
+    } while (!(m == sm && i == si));
+    return pimc_action_list(c, list.data(), (int)list.size() / 2, shift, vfactor, action, nullptr);
+}
+
+int pimc_delta_action(pimc_ctx* c, const int32_t* sp, const double* old_pos, const double* new_pos, int n, int shift,
+                      const double vfactor[2], int commit, double* out3) {
+    if (!c || !out3) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    if (shift < 1) return fail(c, PIMC_E_ARG, "shift must be >= 1");
+    int r;
+    if ((r = check_beads(c, sp, n))) return r;
+    if (n == 0) {
+        out3[0] = out3[1] = out3[2] = 0.0;
+        return PIMC_OK;
+    }
+    if (!new_pos) return fail(c, PIMC_E_ARG, "null new positions");
+    // one-pass evaluation needs the moved beads on distinct slices
+    std::vector<char> seen(c->desc.num_slices, 0);
+    for (int k = 0; k < n; ++k) {
+        if (seen[sp[2 * k]]) return fail(c, PIMC_E_ARG, "delta_action beads must sit on distinct slices");
+        seen[sp[2 * k]] = 1;
+    }
+    if ((r = ensure_req(c, n))) return r;
+    for (int k = 0; k < n; ++k) {
+        VnnBead& q = c->h_req_beads[k];
+        q.walker = 0;
+        q.slice = sp[2 * k];
+        q.ptcl = sp[2 * k + 1];
+        q.flags = 2 | (old_pos ? 1 : 0) | (commit ? 4 : 0);
+        for (int d = 0; d < 3; ++d) {
+            q.old_pos[d] = old_pos ? old_pos[3 * k + d] : 0.0;
+            q.new_pos[d] = new_pos[3 * k + d];
+        }
+    }
+    return run_vnn(c, n, shift, vfactor, commit != 0, out3, nullptr);
+}
+
+int pimc_cell_dims(pimc_ctx* c, int32_t dims3[3], int32_t* hash_cap) {
+    if (!c || !dims3) return fail(c, PIMC_E_ARG, "null argument");
+    for (int i = 0; i < 3; ++i) dims3[i] = c->box.ngrid;
+    if (hash_cap) *hash_cap = c->desc.num_alloc;
+    return PIMC_OK;
+}
+
+int pimc_cell_lists(pimc_ctx* c, int32_t* grid, int32_t* bead_label, int32_t* num_labels, int32_t* hash) {
+    if (!c) return fail(nullptr, PIMC_E_ARG, "null ctx");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    const int M = c->desc.num_slices, N = c->desc.num_alloc;
+    const size_t MN = (size_t)M * N;
+    const int ng = c->box.ngrid;
+    if (grid) {
+        std::vector<int> cell(MN);
+        HIP_OR_FAIL(c, hipMemcpy(cell.data(), c->d_cell, sizeof(int) * MN, hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < MN; ++t) {
+            const int cn = cell[t];
+            if (cn < 0) {
+                grid[3 * t] = grid[3 * t + 1] = grid[3 * t + 2] = -1;
+            } else {
+                grid[3 * t] = cn / (ng * ng);
+                grid[3 * t + 1] = (cn / ng) % ng;
+                grid[3 * t + 2] = cn % ng;
+            }
+        }
+    }
+    if (bead_label) HIP_OR_FAIL(c, hipMemcpy(bead_label, c->d_label, sizeof(int) * MN, hipMemcpyDeviceToHost));
+    if (num_labels)
+        HIP_OR_FAIL(c, hipMemcpy(num_labels, c->d_numlab, sizeof(int) * (size_t)c->cells * M, hipMemcpyDeviceToHost));
+    if (hash) HIP_OR_FAIL(c, hipMemcpy(hash, c->d_hash, sizeof(int) * (size_t)c->cells * MN, hipMemcpyDeviceToHost));
+    return PIMC_OK;
+}
+
+int pimc_interaction_list(pimc_ctx* c, int slice, int ptcl, int32_t* list2, double* sep3, int* count) {
+    if (!c || !count) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    const int M = c->desc.num_slices, N = c->desc.num_alloc;
+    if (slice < 0 || slice >= M || ptcl < 0 || ptcl >= N) return fail(c, PIMC_E_ARG, "bead out of range");
+    if (!c->h_on[(size_t)slice * N + ptcl]) return fail(c, PIMC_E_ARG, "bead is not active");
+    int* d_list = nullptr;
+    double* d_sep = nullptr;
+    int* d_cnt = nullptr;
+    HIP_OR_FAIL(c, hipMalloc(&d_list, sizeof(int) * 2 * N));
+    HIP_OR_FAIL(c, hipMalloc(&d_sep, sizeof(double) * 3 * N));
+    HIP_OR_FAIL(c, hipMalloc(&d_cnt, sizeof(int)));
+    hipError_t e = launch_interaction_list(c->stream, slice, ptcl, c->d_pos, c->plane, N, M, c->d_cell, c->d_numlab,
+                                           c->d_hash, c->d_grid_nn, c->box, d_list, d_sep, d_cnt);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    int cnt = 0;
+    if (e == hipSuccess) e = hipMemcpy(&cnt, d_cnt, sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && list2 && cnt > 0) e = hipMemcpy(list2, d_list, sizeof(int) * 2 * cnt, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && sep3 && cnt > 0) e = hipMemcpy(sep3, d_sep, sizeof(double) * 3 * cnt, hipMemcpyDeviceToHost);
+    (void)hipFree(d_list);
+    (void)hipFree(d_sep);
+    (void)hipFree(d_cnt);
+    if (e != hipSuccess) return fail(c, PIMC_E_HIP, std::string("interaction list: ") + hipGetErrorString(e));
+    *count = cnt;
+    return PIMC_OK;
+}
+
+int pimc_set_timing(pimc_ctx* c, int enable) {
+    if (!c) return fail(nullptr, PIMC_E_ARG, "null ctx");
+    c->resolve_timing();
+    c->timing = enable != 0;
+    return PIMC_OK;
+}
+
+int pimc_kernel_time(pimc_ctx* c, const char* name, double* total_ms, long* count) {
+    if (!c || !name) return fail(c, PIMC_E_ARG, "null argument");
+    c->resolve_timing();
+    for (int k = 0; k < 5; ++k)
+        if (std::strcmp(name, kKernelNames[k]) == 0) {
+            if (total_ms) *total_ms = c->kernel_ms[k];
+            if (count) *count = c->kernel_count[k];
+            return PIMC_OK;
+        }
+    return fail(c, PIMC_E_ARG, "unknown kernel name");
+}
+
+int pimc_device_synchronize(void) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(nullptr, PIMC_E_HIP, hipGetErrorString(e));
+    return PIMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,233 @@
+// pimc_device.hpp -- device-side pair-term arithmetic for gfx950.
+//
+// A bead pair's potential follows the reference's step table:
+//   V = lookupV[k],  k = int(|r| / dr),  dr = 1e-6 rm,  V = 0 if k <= 0 or
+//   k >= tableLength   (include/potential.h:249-260, src/potential.cpp:1792-1796)
+// where lookupV[k] = valueV(r_k) at the k-fold accumulated radius r_k
+// (include/potential.h:176-183).  The kernels therefore
+//   1. form the minimum-image separation in FP64 (include/container.h:50-59),
+//   2. compute the bin k exactly as the reference does (FP64 sqrt and division
+//      in the exact modes; a refined FP32 root in the FP32 mode),
+//   3. evaluate valueV at the bin radius x_k = k dr / rm, either in FP32
+//      (PIMC_V_MIXED), FP64 (PIMC_V_FP64), or by gathering the FP64 table
+//      built on the host exactly like the reference's (PIMC_V_TABLE).
+// The three valueV branches (x < EPS, x < 0.01, x < D) are resolved by bin
+// thresholds replayed on the host from the accumulated radius, so they switch at
+// exactly the reference's bins.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pimc {
+
+constexpr int kMaxAziz = 3;
+
+// Aziz constants as the kernels consume them (filled on the host).
+struct AzizDev {
+    int table_length;   // tableLength = int(maxSep/dr)
+    int k_eps;          // first bin with x_k >= EPS (below: V = 0)
+    int k_hc;           // first bin with x_k >= 0.01 (below: hard core eps*A*exp(..))
+    int k_d;            // first bin with x_k >= D (at and above: F = 1)
+    double dr;          // 1e-6 rm
+    double s;           // dr / rm, so x_k = k s
+    // FP32 closed form, epsilon folded in, base-2 exponent of the repulsion:
+    //   eps*A*exp(-alpha x + beta x^2) = 2^(c0 + c1 k + c2 k^2)
+    double rep_c0, rep_c1, rep_c2;
+    double eC6d, eC8d, eC10d;   // epsilon * C6, C8, C10
+    // FP64 closed form
+    double eps, A, alpha, beta, D, C6, C8, C10;
+    const double* table;   // device lookupV (PIMC_V_TABLE)
+};
+
+struct BoxDev {
+    double L, invL;     // cube side and its inverse (sideInv)
+    double rc2;         // cutoff^2 (Vnn / interaction list)
+    int ngrid;          // numNNGrid per dimension (cube)
+    double grid_size;   // sizeNNGrid
+};
+
+// ---------------------------------------------------------------------------
+// Minimum image, exactly as putInBC: r -= L * floor(r * invL + 0.5).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double min_image_exact(double d, double L, double invL) {
+#pragma clang fp contract(off)
+    return d - L * floor(d * invL + 0.5);
+}
+
+__device__ __forceinline__ double min_image_fast(double d, double L, double invL) {
+    return fma(-L, floor(fma(d, invL, 0.5)), d);
+}
+
+// |r|^2 in the reference's order ((0 + x*x) + y*y) + z*z without contraction.
+__device__ __forceinline__ double norm2_exact(double x, double y, double z) {
+#pragma clang fp contract(off)
+    double s = x * x;
+    s = s + y * y;
+    s = s + z * z;
+    return s;
+}
+
+// Exact-mode bin: int(sqrt(r2)/dr) with IEEE sqrt and division, as the host does.
+__device__ __forceinline__ int bin_exact(double r2, double dr, double* r_out) {
+    double r = __builtin_sqrt(r2);
+    *r_out = r;
+    return (int)(r / dr);
+}
+
+// FP32-mode bin: an FP32 root refined by one Newton step in FP64, then a
+// multiply by 1/dr.  It differs from the exact bin only when |r|/dr lies within
+// a few FP64 ulps of an integer.
+__device__ __forceinline__ int bin_fast(double r2, double inv_dr, float* rs_out) {
+    float r2f = (float)r2;
+    float rs = __builtin_amdgcn_rsqf(r2f);          // ~1/|r|
+    float r0 = r2f * rs;                            // ~|r|
+    double r0d = (double)r0;
+    double res = fma(-r0d, r0d, r2);                // r2 - r0^2, exact enough
+    double r = fma((double)(0.5f * rs), res, r0d);  // r0 + res/(2 r0)
+    *rs_out = rs;
+    return (int)(r * inv_dr);
+}
+
+__device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+// valueV at the bin radius with the chip's FP32 transcendental units (exp2, rcp)
+// and FP64 arithmetic around them.  On gfx950 an unpacked FP32 op and an FP64
+// FMA issue at the same rate, so only the transcendentals are worth taking in
+// FP32; each exponent is split hi + lo so the FP32 exp2 sees an exactly
+// representable argument and the lo part is applied in FP64.  Per-pair error
+// is set by v_exp_f32 (about 1 FP32 ulp).  Requires k_eps <= k < table_length.
+__device__ __forceinline__ double split_exp2(double e) {
+    const float eh = (float)e;
+    const double el = e - (double)eh;
+    return (double)fast_exp2(eh) * fma(el, 0.69314718055994530942, 1.0);
+}
+
+__device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
+    const double kd = (double)k;
+    const double rep = split_exp2(fma(fma(a.rep_c2, kd, a.rep_c1), kd, a.rep_c0));
+    const double x = kd * a.s;
+    const double x2 = x * x;
+    double ix2 = (double)__builtin_amdgcn_rcpf((float)x2);
+    ix2 = ix2 * fma(-x2, ix2, 2.0);                  // one Newton step: FP64 accurate
+    const double ix6 = ix2 * ix2 * ix2;
+    const double disp = ix6 * fma(fma(a.eC10d, ix2, a.eC8d), ix2, a.eC6d);
+    // F(x) = exp(-(D/x - 1)^2) for x < D, with D/x - 1 = (D - x) x / x^2
+    const double t = (a.D - x) * x * ix2;
+    const double F = (k < a.k_d) ? split_exp2(-1.4426950408889634074 * t * t) : 1.0;
+    const double full = fma(-disp, F, rep);
+    return (k < a.k_hc) ? rep : full;
+}
+
+// valueV at the bin radius, FP64 arithmetic.  Requires k_eps <= k < table_length.
+__device__ __forceinline__ double aziz_v64(int k, const AzizDev& a) {
+    const double x = (double)k * a.s;
+    const double rep = a.A * exp(-a.alpha * x + a.beta * x * x);
+    if (k < a.k_hc) return a.eps * rep;
+    const double ix2 = 1.0 / (x * x);
+    const double ix6 = ix2 * ix2 * ix2;
+    const double ix8 = ix6 * ix2;
+    const double ix10 = ix8 * ix2;
+    double F = 1.0;
+    if (k < a.k_d) {
+        const double u = a.D / x - 1.0;
+        F = exp(-u * u);
+    }
+    return a.eps * (rep - (a.C6 * ix6 + a.C8 * ix8 + a.C10 * ix10) * F);
+}
+
+// Pair value for separation components (dx,dy,dz) already minimum-imaged.
+// MODE: 0 mixed (FP32 transcendentals), 1 FP64, 2 table.  Returns 0 outside the table.
+template <int MODE>
+__device__ __forceinline__ double pair_value_from_r2(double r2, const AzizDev& a, double inv_dr) {
+    int k;
+    if (MODE == 0) {
+        float rs;
+        k = bin_fast(r2, inv_dr, &rs);
+    } else {
+        double r;
+        k = bin_exact(r2, a.dr, &r);
+    }
+    const bool valid = (k >= a.k_eps) && (k < a.table_length) && (k > 0);
+    double v = 0.0;
+    if (MODE == 0) {
+        const int kk = valid ? k : a.k_hc;  // keep the arithmetic finite on masked lanes
+        v = aziz_vmix(kk, a);
+    } else if (MODE == 1) {
+        const int kk = valid ? k : a.k_hc;
+        v = aziz_v64(kk, a);
+    } else {
+        v = valid ? a.table[k] : 0.0;
+    }
+    return valid ? v : 0.0;
+}
+
+template <int MODE>
+__device__ __forceinline__ double sep_r2(double xi, double yi, double zi, double xj, double yj, double zj,
+                                         const BoxDev& b) {
+    if (MODE == 0) {
+        const double dx = min_image_fast(xj - xi, b.L, b.invL);
+        const double dy = min_image_fast(yj - yi, b.L, b.invL);
+        const double dz = min_image_fast(zj - zi, b.L, b.invL);
+        return fma(dz, dz, fma(dy, dy, dx * dx));
+    } else {
+        const double dx = min_image_exact(xj - xi, b.L, b.invL);
+        const double dy = min_image_exact(yj - yi, b.L, b.invL);
+        const double dz = min_image_exact(zj - zi, b.L, b.invL);
+        return norm2_exact(dx, dy, dz);
+    }
+}
+
+// LookupTable::gridIndex for one coordinate (include/lookuptable.h:139-147).
+__device__ __host__ __forceinline__ int grid_coord(double p, double L, double size) {
+#pragma clang fp contract(off)
+    return (int)(fabs(p + 0.5 * L - 1.0e-7) / (size + 1.0e-7));
+}
+
+// Worm::getState (include/worm.h:110-117): 0 HEADTAIL, 1 SPECIAL, 2 NONE.
+struct WormDev {
+    int head[2], tail[2], sp1[2], sp2[2];
+};
+
+__device__ __forceinline__ int bead_state(const WormDev& w, int slice, int ptcl) {
+    if ((slice == w.head[0] && ptcl == w.head[1]) || (slice == w.tail[0] && ptcl == w.tail[1])) return 0;
+    if ((slice == w.sp1[0] && ptcl == w.sp1[1]) || (slice == w.sp2[0] && ptcl == w.sp2[1])) return 1;
+    return 2;
+}
+
+// Worm::factor(state1, bead2) (src/worm.cpp:108-118)
+__device__ __forceinline__ double pair_factor(int s1, int s2) {
+    if (s1 == 2 && s2 == 2) return 1.0;
+    if (s1 == 0 && s2 == 0) return 0.0;
+    return 0.5;
+}
+
+__device__ __forceinline__ bool worm_touches_slice(const WormDev& w, int slice) {
+    return w.head[0] == slice || w.tail[0] == slice || w.sp1[0] == slice || w.sp2[0] == slice;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic wave64 / workgroup FP64 reductions (fixed shuffle tree).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) r += lds[w];
+    }
+    return r;  // valid in thread 0
+}
+
+}  // namespace pimc

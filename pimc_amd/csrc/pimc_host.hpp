@@ -1,0 +1,109 @@
+// pimc_host.hpp -- host-side context of the C-ABI (one chain resident in HBM).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pimc_gpu.h"
+#include "pimc_kernels.hpp"
+
+namespace pimc {
+
+struct AzizParams {
+    double eps, rm, D, alpha, beta, C6, C8, C10, A;
+};
+
+bool aziz_params(int year, AzizParams* p);
+double aziz_value(const AzizParams& p, double r, int what);  // valueV / dVdr / d2Vdr2
+AzizDev make_aziz_dev(const AzizParams& p, double max_sep);
+void build_table(const AzizParams& p, int length, int what, std::vector<double>& out);
+double aziz_tail(const AzizParams& p, double rc);
+
+struct TableCache;
+TableCache& table_cache();
+const double* cached_table(int year, int what, int length, std::string* err);
+
+}  // namespace pimc
+
+struct pimc_ctx {
+    pimc_desc desc{};
+    pimc::AzizParams params{};
+    pimc::AzizDev az{};
+    pimc::BoxDev box{};
+    pimc::WormDev worm{};
+    double max_sep = 0, tailV = 0;
+    int64_t plane = 0;
+    int cells = 1;
+    hipStream_t stream = nullptr;
+    bool have_path = false;
+    std::string err;
+
+    // device state
+    double* d_pos = nullptr;      // [3][M][N]
+    uint8_t* d_on = nullptr;      // [M][N]
+    int* d_nbs = nullptr;         // [M]
+    int* d_cell = nullptr;        // [M][N] packed cell number (-1: not in the table)
+    int* d_label = nullptr;       // [M][N]
+    int* d_numlab = nullptr;      // [cells][M]
+    int* d_hash = nullptr;        // [cells][M][N]
+    int* d_grid_nn = nullptr;     // [cells][27]
+    pimc::SliceDesc* d_slices = nullptr;
+    pimc::WalkerDesc* d_walker = nullptr;
+    pimc::PairItem* d_items = nullptr;
+    double* d_partials = nullptr;
+    int n_items = 0, cap_items = 0;
+    int64_t pair_chunk = 0;       // 0: automatic
+    double* d_vslice = nullptr;
+    double* d_out = nullptr;
+    double* h_map = nullptr;      // 64 doubles of mapped, coherent pinned host memory
+    double* d_map = nullptr;      // its device address
+    int lds_beads = 0;            // slice positions staged in LDS (0: read from HBM)
+    pimc::WalkerDesc walker_cache{};
+    bool walker_valid = false;
+    int* d_hist = nullptr;
+    double* d_kin_partials = nullptr;
+    // requests
+    int cap_req = 0;
+    pimc::VnnBead* d_req_beads = nullptr;
+    pimc::VnnBead* h_req_beads = nullptr;
+    int* d_req_sp = nullptr;
+    double* d_req_pos = nullptr;
+    int* h_req_sp = nullptr;
+    double* h_req_pos = nullptr;
+    double* d_per_bead = nullptr;
+    double* h_per_bead = nullptr;
+    double* d_vnn_partials = nullptr;
+    // structure factors
+    int cap_q = 0;
+    double* d_q = nullptr;
+    int* d_qn = nullptr;
+    double* d_rho = nullptr;
+    size_t cap_rho = 0;
+    double* d_fqt = nullptr;
+    size_t cap_fqt = 0;
+    // staging
+    double* h_stage = nullptr;    // >= 3*M*N doubles, pinned
+    double* h_out = nullptr;      // 64 doubles, pinned
+    // host mirrors
+    std::vector<int> h_nbs;
+    std::vector<uint8_t> h_on;
+    std::vector<int> grid_nn;
+
+    std::vector<void*> dev_allocs, host_allocs;
+
+    // timing
+    bool timing = false;
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    double kernel_ms[5] = {0, 0, 0, 0, 0};
+    long kernel_count[5] = {0, 0, 0, 0, 0};
+    void timed_begin(int kind);
+    void timed_end();
+    void resolve_timing();
+
+    ~pimc_ctx();
+};

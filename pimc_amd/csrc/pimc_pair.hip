@@ -1,0 +1,558 @@
+// pimc_pair.hip -- all-bead-pairs kernels for gfx950.
+//
+//  k_pair_action : LocalAction::V(int) for every slice of one or many walkers
+//                  (src/action.cpp:902-947) and potentialAction() (:456-472).
+//  k_vnn         : LocalAction::Vnn(bead) (src/action.cpp:1044-1060) for a list of
+//                  beads at old and/or new positions: the Delta V of a move.
+//  k_sep_hist    : ActionBase::updateSepHist over one slice (src/action.cpp:216-224).
+//  k_kinetic     : the link sum of EnergyEstimator (src/estimator.cpp:966-976).
+//  cell kernels  : LookupTable::updateGrid / updateBead / updateInteractionList
+//                  (src/lookuptable.cpp:314-516), bit-exact integer results.
+//  k_aziz_batch  : PotentialBase::V / gradV / grad2V over a batch of separations.
+//
+// No MFMA: the pair term is a transcendental sum.  Bead positions are FP64 SoA
+// planes; a slice is staged in LDS once per workgroup and every lane walks a
+// strided range of the slice's flattened pair index space.  Sums are FP64 with
+// fixed-order wave/workgroup trees and a fixed-order final pass, so results do
+// not depend on how many other slices or walkers share the launch.
+#include <hip/hip_runtime.h>
+
+#include "pimc_kernels.hpp"
+
+namespace pimc {
+
+// ---------------------------------------------------------------------------
+// All i<j pairs of each slice.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(kPairThreads) void k_pair_action(const PairItem* __restrict__ items,
+                                                              const SliceDesc* __restrict__ slices,
+                                                              const WalkerDesc* __restrict__ walkers,
+                                                              const double* __restrict__ pos, AzizDev az,
+                                                              int lds_beads, double* __restrict__ partials) {
+    extern __shared__ double lds[];   // x | y | z of one slice, lds_beads each
+    __shared__ double red[kPairThreads / 64];
+    double* lx = lds;
+    double* ly = lds + lds_beads;
+    double* lz = lds + 2 * lds_beads;
+
+    const PairItem it = items[blockIdx.x];
+    const SliceDesc sd = slices[it.slice];
+    const WalkerDesc& wk = walkers[sd.walker];
+    const BoxDev box = wk.box;
+    AzizDev a = az;
+    a.table_length = wk.table_length;
+    const double inv_dr = 1.0 / a.dr;
+    const int n = sd.n;
+    const double* px = pos + sd.pos;
+    const double* py = px + sd.plane;
+    const double* pz = py + sd.plane;
+    const bool in_lds = n <= lds_beads;
+    if (in_lds) {
+        for (int t = threadIdx.x; t < n; t += kPairThreads) {
+            lx[t] = px[t];
+            ly[t] = py[t];
+            lz[t] = pz[t];
+        }
+    }
+    __syncthreads();
+
+    const int H = n >> 1;
+    const bool even = (n & 1) == 0;
+    const bool factors = worm_touches_slice(wk.worm, sd.slice);
+    int p = it.p_begin + threadIdx.x;
+    const int p_end = it.p_begin + it.p_count;
+    int d = 1 + p / n;
+    int i = p - (d - 1) * n;
+    const int step_d = kPairThreads / n;
+    const int step_i = kPairThreads - step_d * n;
+    double acc = 0.0;
+    for (; p < p_end; p += kPairThreads) {
+        int j = i + d;
+        if (j >= n) j -= n;
+        const bool ok = !(even && d == H && i >= H);
+        double xi, yi, zi, xj, yj, zj;
+        if (in_lds) {
+            xi = lx[i]; yi = ly[i]; zi = lz[i];
+            xj = lx[j]; yj = ly[j]; zj = lz[j];
+        } else {
+            xi = px[i]; yi = py[i]; zi = pz[i];
+            xj = px[j]; yj = py[j]; zj = pz[j];
+        }
+        const double r2 = sep_r2<MODE>(xi, yi, zi, xj, yj, zj, box);
+        double v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+        if (factors) {
+            const int lo = i < j ? i : j, hi = i < j ? j : i;
+            v *= pair_factor(bead_state(wk.worm, sd.slice, lo), bead_state(wk.worm, sd.slice, hi));
+        }
+        acc += ok ? v : 0.0;
+        i += step_i;
+        d += step_d;
+        if (i >= n) {
+            i -= n;
+            d += 1;
+        }
+    }
+    const double s = block_sum<kPairThreads>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Per-slice sums of the item partials (items are grouped by slice, in order) and
+// the per-walker potential action, slices in order (src/action.cpp:462-469).
+__global__ __launch_bounds__(256) void k_pair_finalize(int n_items, const SliceDesc* __restrict__ slices,
+                                                       int n_slices, const WalkerDesc* __restrict__ walkers,
+                                                       int n_walkers, const double* __restrict__ partials,
+                                                       double* __restrict__ v_slice, double* __restrict__ action) {
+    for (int sl = threadIdx.x; sl < n_slices; sl += 256) {
+        const int b = slices[sl].first_item;
+        const int e = (sl + 1 < n_slices) ? slices[sl + 1].first_item : n_items;
+        double v = 0.0;
+        for (int k = b; k < e; ++k) v += partials[k];
+        v_slice[sl] = v;
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < n_walkers; w += 256) {
+        const WalkerDesc& W = walkers[w];
+        const double tau_s = W.shift * W.tau;
+        double tot = 0.0;
+        for (int m = 0; m < W.num_slices; ++m) {
+            const double vf = (m & 1) ? W.vfactor1 : W.vfactor0;
+            tot += vf * tau_s * (0.0 + v_slice[W.first_slice + m]);
+        }
+        action[w] = tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vnn for a list of beads, old and new positions in one pass.
+// Block (c, b): bead b, partners [c*chunk, (c+1)*chunk).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool cells_adjacent(int c1, int c2, int ng) {
+    const int x1 = c1 / (ng * ng), y1 = (c1 / ng) % ng, z1 = c1 % ng;
+    const int x2 = c2 / (ng * ng), y2 = (c2 / ng) % ng, z2 = c2 % ng;
+    int dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+    dx = dx < 0 ? -dx : dx;
+    dy = dy < 0 ? -dy : dy;
+    dz = dz < 0 ? -dz : dz;
+    return (dx <= 1 || dx == ng - 1) && (dy <= 1 || dy == ng - 1) && (dz <= 1 || dz == ng - 1);
+}
+
+__device__ __forceinline__ int cell_of(double x, double y, double z, const BoxDev& b) {
+    const int gx = grid_coord(x, b.L, b.grid_size);
+    const int gy = grid_coord(y, b.L, b.grid_size);
+    const int gz = grid_coord(z, b.L, b.grid_size);
+    return (gx * b.ngrid + gy) * b.ngrid + gz;
+}
+
+// Path::updateBead -> LookupTable::updateBead (src/lookuptable.cpp:355-393, :430-458)
+__device__ void cell_update_bead(int m, int i, int c_new, int nalloc, int num_slices, int* cell, int* label,
+                                 int* numlab, int* hash) {
+    const int bi = m * nalloc + i;
+    const int c_old = cell[bi];
+    if (c_new == c_old || c_old < 0) return;  // unchanged cell, or bead not in the table
+    const int64_t hb_old = ((int64_t)c_old * num_slices + m) * nalloc;
+    const int lab = label[bi];
+    const int last = numlab[c_old * num_slices + m] - 1;
+    if (lab < last) {
+        const int moved = hash[hb_old + last];
+        hash[hb_old + last] = -1;
+        hash[hb_old + lab] = moved;
+        label[m * nalloc + moved] = lab;
+    } else {
+        hash[hb_old + last] = -1;
+    }
+    numlab[c_old * num_slices + m] = last;
+    const int64_t hb_new = ((int64_t)c_new * num_slices + m) * nalloc;
+    const int nl = numlab[c_new * num_slices + m];
+    hash[hb_new + nl] = i;
+    label[bi] = nl;
+    numlab[c_new * num_slices + m] = nl + 1;
+    cell[bi] = c_new;
+}
+
+
+
+template <int MODE>
+__global__ __launch_bounds__(kVnnThreads) void k_vnn(VnnArgs A, VnnInline req, AzizDev az) {
+    __shared__ double red[kVnnThreads / 64];
+    const int b = blockIdx.y;
+    const int c = blockIdx.x;
+    const VnnBead q = A.beads ? A.beads[b] : req.beads[b];
+    const int m = q.slice;
+    const int me = q.ptcl;
+    const BoxDev box = A.box;
+    const double inv_dr = 1.0 / az.dr;
+    const double* px = A.pos + (int64_t)m * A.nalloc;
+    const double* py = px + A.plane;
+    const double* pz = py + A.plane;
+    const bool has_old = q.flags & 1;
+    const bool has_new = q.flags & 2;
+    double ox = q.old_pos[0], oy = q.old_pos[1], oz = q.old_pos[2];
+    if (!has_old) {
+        ox = px[me];
+        oy = py[me];
+        oz = pz[me];
+    }
+    const double nx = q.new_pos[0], ny = q.new_pos[1], nz = q.new_pos[2];
+    const bool use_cells = box.ngrid >= 4;
+    int c_old = 0, c_new = 0;
+    if (use_cells) {
+        c_old = has_old ? cell_of(ox, oy, oz, box) : A.cell[m * A.nalloc + me];
+        c_new = cell_of(nx, ny, nz, box);
+    }
+    const int s1 = bead_state(A.worm, m, me);
+    const bool factors = worm_touches_slice(A.worm, m);
+    const bool me_on = A.on[m * A.nalloc + me] != 0;
+    double acc_o = 0.0, acc_n = 0.0;
+    const int j_end = min((c + 1) * A.chunk, A.nalloc);
+    for (int j = c * A.chunk + threadIdx.x; j < j_end; j += kVnnThreads) {
+        const bool on = (A.on[m * A.nalloc + j] != 0) && (j != me) && me_on;
+        const double xj = px[j], yj = py[j], zj = pz[j];
+        double f = 1.0;
+        if (factors) f = pair_factor(s1, bead_state(A.worm, m, j));
+        int cj = 0;
+        if (use_cells) cj = A.cell[m * A.nalloc + j];
+        {
+            const double r2 = sep_r2<MODE>(ox, oy, oz, xj, yj, zj, box);
+            const bool ok = on && (r2 < box.rc2) && (!use_cells || cells_adjacent(c_old, cj, box.ngrid));
+            const double v = pair_value_from_r2<MODE>(r2, az, inv_dr);
+            acc_o += ok ? f * v : 0.0;
+        }
+        if (has_new) {
+            const double r2 = sep_r2<MODE>(nx, ny, nz, xj, yj, zj, box);
+            const bool ok = on && (r2 < box.rc2) && (!use_cells || cells_adjacent(c_new, cj, box.ngrid));
+            const double v = pair_value_from_r2<MODE>(r2, az, inv_dr);
+            acc_n += ok ? f * v : 0.0;
+        }
+    }
+    const double so = block_sum<kVnnThreads>(acc_o, red);
+    const double sn = block_sum<kVnnThreads>(acc_n, red);
+    if (threadIdx.x == 0) {
+        A.partials[((int64_t)b * A.chunks + c) * 2 + 0] = so;
+        A.partials[((int64_t)b * A.chunks + c) * 2 + 1] = sn;
+    }
+}
+
+// Per-bead Vnn (chunks in order), the two actions and dU (beads in order), then
+// the committed bead updates, one lane per bead (the beads sit on distinct slices).
+__global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) {
+    for (int k = threadIdx.x; k < A.n_beads; k += 256) {
+        double vo = 0.0, vn = 0.0;
+        for (int cc = 0; cc < A.chunks; ++cc) {
+            vo += A.partials[((int64_t)k * A.chunks + cc) * 2 + 0];
+            vn += A.partials[((int64_t)k * A.chunks + cc) * 2 + 1];
+        }
+        A.per_bead[2 * k + 0] = vo;
+        A.per_bead[2 * k + 1] = vn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double tau_s = A.shift * A.tau;
+        double ao = 0.0, an = 0.0, du = 0.0;
+        for (int k = 0; k < A.n_beads; ++k) {
+            const int slice = A.beads ? A.beads[k].slice : req.beads[k].slice;
+            const double vf = (slice & 1) ? A.vfactor1 : A.vfactor0;
+            const double co = vf * tau_s;
+            const double vo = A.per_bead[2 * k + 0], vn = A.per_bead[2 * k + 1];
+            ao += co * vo;
+            an += co * vn;
+            du += co * (vn - vo);
+        }
+        A.out[0] = ao;
+        A.out[1] = an;
+        A.out[2] = du;
+    }
+    if (A.commit_in_kernel) {
+        for (int k = threadIdx.x; k < A.n_beads; k += 256) {
+            const VnnBead qb = A.beads ? A.beads[k] : req.beads[k];
+            if (!(qb.flags & 4)) continue;
+            const int mm = qb.slice, ii = qb.ptcl;
+            const int cn = cell_of(qb.new_pos[0], qb.new_pos[1], qb.new_pos[2], A.box);
+            cell_update_bead(mm, ii, cn, A.nalloc, A.num_slices, A.cell, A.label, A.numlab, A.hash);
+            double* qx = A.pos + (int64_t)mm * A.nalloc + ii;
+            qx[0] = qb.new_pos[0];
+            qx[A.plane] = qb.new_pos[1];
+            qx[2 * A.plane] = qb.new_pos[2];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Sequential bead updates (any slices, list order) -- Path::updateBead.
+// ---------------------------------------------------------------------------
+__global__ void k_update_beads(const int* __restrict__ sp, const double* __restrict__ p3, int n, double* pos,
+                               int64_t plane, int nalloc, int num_slices, int* cell, int* label, int* numlab,
+                               int* hash, BoxDev box) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int k = 0; k < n; ++k) {
+        const int m = sp[2 * k], i = sp[2 * k + 1];
+        const int cn = cell_of(p3[3 * k], p3[3 * k + 1], p3[3 * k + 2], box);
+        cell_update_bead(m, i, cn, nalloc, num_slices, cell, label, numlab, hash);
+        double* q = pos + (int64_t)m * nalloc + i;
+        q[0] = p3[3 * k];
+        q[plane] = p3[3 * k + 1];
+        q[2 * plane] = p3[3 * k + 2];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LookupTable::updateGrid (src/lookuptable.cpp:314-349): cells of every bead,
+// then per slice a sequential label pass in particle order.
+// ---------------------------------------------------------------------------
+__global__ void k_cell_assign(const double* __restrict__ pos, int64_t plane, int total, int* __restrict__ cell,
+                              const uint8_t* __restrict__ on, BoxDev box) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    cell[t] = on[t] ? cell_of(pos[t], pos[t + plane], pos[t + 2 * plane], box) : -1;
+}
+
+__global__ void k_cell_labels(const int* __restrict__ cell, int* __restrict__ label, int* __restrict__ numlab,
+                              int* __restrict__ hash, int nalloc, int num_slices, int cells) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= num_slices) return;
+    for (int c = 0; c < cells; ++c) numlab[c * num_slices + m] = 0;
+    for (int i = 0; i < nalloc; ++i) {
+        const int c = cell[m * nalloc + i];
+        if (c < 0) {
+            label[m * nalloc + i] = -1;
+            continue;
+        }
+        const int nl = numlab[c * num_slices + m];
+        hash[((int64_t)c * num_slices + m) * nalloc + nl] = i;
+        label[m * nalloc + i] = nl;
+        numlab[c * num_slices + m] = nl + 1;
+    }
+}
+
+// LookupTable::updateInteractionList (src/lookuptable.cpp:464-516), one bead,
+// walking the 27 de-duplicated neighbour cells in order.
+__global__ void k_interaction_list(int m, int me, const double* __restrict__ pos, int64_t plane, int nalloc,
+                                   int num_slices, const int* __restrict__ cell, const int* __restrict__ numlab,
+                                   const int* __restrict__ hash, const int* __restrict__ grid_nn, BoxDev box,
+                                   int* __restrict__ out_list, double* __restrict__ out_sep,
+                                   int* __restrict__ out_count) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double* px = pos + (int64_t)m * nalloc;
+    const double* py = px + plane;
+    const double* pz = py + plane;
+    const int c1 = cell[m * nalloc + me];
+    int count = 0;
+    for (int nn = 0; nn < 27; ++nn) {
+        const int cn = grid_nn[c1 * 27 + nn];
+        if (cn < 0) continue;
+        const int maxnl = numlab[cn * num_slices + m];
+        for (int lab = 0; lab < maxnl; ++lab) {
+            const int j = hash[((int64_t)cn * num_slices + m) * nalloc + lab];
+            if (j == me) continue;
+            const double dx = min_image_exact(px[j] - px[me], box.L, box.invL);
+            const double dy = min_image_exact(py[j] - py[me], box.L, box.invL);
+            const double dz = min_image_exact(pz[j] - pz[me], box.L, box.invL);
+            if (norm2_exact(dx, dy, dz) < box.rc2) {
+                out_list[2 * count] = m;
+                out_list[2 * count + 1] = j;
+                out_sep[3 * count] = dx;
+                out_sep[3 * count + 1] = dy;
+                out_sep[3 * count + 2] = dz;
+                ++count;
+            }
+        }
+    }
+    *out_count = count;
+}
+
+// ---------------------------------------------------------------------------
+// Separation histogram of one slice (src/action.cpp:216-224), integer bins.
+// ---------------------------------------------------------------------------
+__global__ void k_sep_hist(const double* __restrict__ pos, int64_t plane, int m, int n, int nalloc, BoxDev box,
+                           double dsep, int* __restrict__ hist) {
+    const double* px = pos + (int64_t)m * nalloc;
+    const double* py = px + plane;
+    const double* pz = py + plane;
+    const int64_t total = (int64_t)n * n;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(t / n), j = (int)(t % n);
+        if (j <= i) continue;
+        const double dx = min_image_exact(px[j] - px[i], box.L, box.invL);
+        const double dy = min_image_exact(py[j] - py[i], box.L, box.invL);
+        const double dz = min_image_exact(pz[j] - pz[i], box.L, box.invL);
+        const int nr = (int)(__builtin_sqrt(norm2_exact(dx, dy, dz)) / dsep);
+        if (nr >= 0 && nr < 50) atomicAdd(&hist[nr], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kinetic link sum: sum over active beads of |putInBC(r(next) - r)|^2.
+// next [M][nalloc][2]; a bead with next = XXX contributes 0.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_kinetic(const double* __restrict__ pos, int64_t plane, int nalloc,
+                                                 int num_slices, const int* __restrict__ nbs,
+                                                 const int* __restrict__ next, BoxDev box,
+                                                 double* __restrict__ partials) {
+    __shared__ double red[4];
+    const int total = num_slices * nalloc;
+    double acc = 0.0;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < total; t += gridDim.x * 256) {
+        const int m = t / nalloc, i = t % nalloc;
+        if (i >= nbs[m]) continue;
+        int nm, ni;
+        if (next) {
+            nm = next[2 * t];
+            ni = next[2 * t + 1];
+        } else {
+            nm = (m + 1 == num_slices) ? 0 : m + 1;
+            ni = i;
+        }
+        if (nm < 0 && ni < 0) continue;
+        const int64_t a = (int64_t)m * nalloc + i, b = (int64_t)nm * nalloc + ni;
+        const double dx = min_image_exact(pos[b] - pos[a], box.L, box.invL);
+        const double dy = min_image_exact(pos[b + plane] - pos[a + plane], box.L, box.invL);
+        const double dz = min_image_exact(pos[b + 2 * plane] - pos[a + 2 * plane], box.L, box.invL);
+        acc += norm2_exact(dx, dy, dz);
+    }
+    const double s = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// out[0] = sum of n partials in order
+__global__ void k_sum_partials(const double* __restrict__ partials, int n, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double tot = 0.0;
+    for (int k = 0; k < n; ++k) tot += partials[k];
+    out[0] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Batched PotentialBase::V / gradV / grad2V on separations (table semantics).
+// what: 0 V, 1 gradV, 2 grad2V.  Tables d1/d2 hold dV/dr and d2V/dr2.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void k_aziz_batch(const double* __restrict__ sep, int n, AzizDev az, const double* __restrict__ d1,
+                             const double* __restrict__ d2, int what, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double x = sep[3 * t], y = sep[3 * t + 1], z = sep[3 * t + 2];
+    const double r2 = norm2_exact(x, y, z);
+    const double r = __builtin_sqrt(r2);
+    const double q = r / az.dr;
+    const bool valid = q < (double)az.table_length && (int)q > 0 && (int)q >= az.k_eps;
+    const int k = valid ? (int)q : 0;
+    if (what == 0) {
+        double v = 0.0;
+        if (valid) v = (MODE == 2) ? az.table[k] : (MODE == 1 ? aziz_v64(k, az) : aziz_vmix(k, az));
+        out[t] = v;
+    } else if (what == 1) {
+        const double g = valid ? d1[k] / r : 0.0;
+        out[3 * t] = valid ? g * x : 0.0;
+        out[3 * t + 1] = valid ? g * y : 0.0;
+        out[3 * t + 2] = valid ? g * z : 0.0;
+    } else {
+        out[t] = valid ? d2[k] : 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (C++ linkage, used by pimc_capi.hip)
+// ---------------------------------------------------------------------------
+hipError_t launch_pair_action(int mode, hipStream_t st, const PairItem* items, int n_items, const SliceDesc* slices,
+                              int n_slices, const WalkerDesc* walkers, int n_walkers, const double* pos,
+                              const AzizDev& az, int lds_beads, double* partials, double* v_slice, double* action) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    if (n_items <= 0) return hipSuccess;
+    dim3 g(n_items), b(kPairThreads);
+    const size_t lds = sizeof(double) * 3 * (size_t)lds_beads;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_pair_action<0>, g, b, lds, st, items, slices, walkers, pos, az, lds_beads, partials);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_pair_action<1>, g, b, lds, st, items, slices, walkers, pos, az, lds_beads, partials);
+    else
+        hipLaunchKernelGGL(k_pair_action<2>, g, b, lds, st, items, slices, walkers, pos, az, lds_beads, partials);
+    hipLaunchKernelGGL(k_pair_finalize, dim3(1), dim3(256), 0, st, n_items, slices, n_slices, walkers, n_walkers,
+                       partials, v_slice, action);
+    return hipGetLastError();
+}
+
+hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    if (A.n_beads <= 0) return hipSuccess;
+    dim3 g(A.chunks, A.n_beads), b(kVnnThreads);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_vnn<0>, g, b, 0, st, A, req, az);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_vnn<1>, g, b, 0, st, A, req, az);
+    else
+        hipLaunchKernelGGL(k_vnn<2>, g, b, 0, st, A, req, az);
+    hipLaunchKernelGGL(k_vnn_finalize, dim3(1), dim3(256), 0, st, A, req);
+    return hipGetLastError();
+}
+
+hipError_t launch_update_beads(hipStream_t st, const int* sp, const double* p3, int n, double* pos, int64_t plane,
+                               int nalloc, int num_slices, int* cell, int* label, int* numlab, int* hash,
+                               const BoxDev& box) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    hipLaunchKernelGGL(k_update_beads, dim3(1), dim3(64), 0, st, sp, p3, n, pos, plane, nalloc, num_slices, cell,
+                       label, numlab, hash, box);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_build(hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
+                             const uint8_t* on, int* cell, int* label, int* numlab, int* hash, int cells,
+                             const BoxDev& box) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    const int total = nalloc * num_slices;
+    hipLaunchKernelGGL(k_cell_assign, dim3((total + 255) / 256), dim3(256), 0, st, pos, plane, total, cell, on, box);
+    hipLaunchKernelGGL(k_cell_labels, dim3((num_slices + 63) / 64), dim3(64), 0, st, cell, label, numlab, hash,
+                       nalloc, num_slices, cells);
+    return hipGetLastError();
+}
+
+hipError_t launch_interaction_list(hipStream_t st, int m, int me, const double* pos, int64_t plane, int nalloc,
+                                   int num_slices, const int* cell, const int* numlab, const int* hash,
+                                   const int* grid_nn, const BoxDev& box, int* out_list, double* out_sep,
+                                   int* out_count) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    hipLaunchKernelGGL(k_interaction_list, dim3(1), dim3(64), 0, st, m, me, pos, plane, nalloc, num_slices, cell,
+                       numlab, hash, grid_nn, box, out_list, out_sep, out_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_sep_hist(hipStream_t st, const double* pos, int64_t plane, int m, int n, int nalloc,
+                           const BoxDev& box, double dsep, int* hist) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    int64_t total = (int64_t)n * n;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_sep_hist, dim3(blocks), dim3(256), 0, st, pos, plane, m, n, nalloc, box, dsep, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_kinetic(hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
+                          const int* nbs, const int* next, const BoxDev& box, double* partials, int max_blocks,
+                          double* out) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    int total = nalloc * num_slices;
+    int blocks = (total + 255) / 256;
+    if (blocks > max_blocks) blocks = max_blocks;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_kinetic, dim3(blocks), dim3(256), 0, st, pos, plane, nalloc, num_slices, nbs, next, box,
+                       partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, st, partials, blocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_aziz_batch(int mode, hipStream_t st, const double* sep, int n, const AzizDev& az, const double* d1,
+                             const double* d2, int what, double* out) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    if (n <= 0) return hipSuccess;
+    dim3 g((n + 255) / 256), b(256);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_aziz_batch<0>, g, b, 0, st, sep, n, az, d1, d2, what, out);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_aziz_batch<1>, g, b, 0, st, sep, n, az, d1, d2, what, out);
+    else
+        hipLaunchKernelGGL(k_aziz_batch<2>, g, b, 0, st, sep, n, az, d1, d2, what, out);
+    return hipGetLastError();
+}
+
+}  // namespace pimc

@@ -1,0 +1,316 @@
+// pimc_sq.hip -- density modes, S(q) and F(q,tau) for gfx950.
+//
+// The reference's estimators are direct pair sums:
+//   S:  sf(q)      = sum_m [N_m + 2 sum_{i<j} cos(q . r_ij)]          (src/estimator.cpp:3705-3737)
+//   F:  isf(q,tau) = sum_m sum_{i in m} sum_{j in m+tau} cos(q.r_i - q.r_j)  (:3923-3961)
+// Both equal sums over the density mode rho_q(m) = sum_i exp(i q . r_i(m)):
+//   sf(q) = sum_m |rho_q(m)|^2,   isf(q,tau) = sum_m Re rho_q(m) conj(rho_q(m+tau)),
+// exactly for F and, for S, whenever q is a reciprocal-lattice vector of the
+// periodic box (then the minimum image drops out of the phase).  The kernels
+// build rho_q(m) once (O(n_q M N)) and then correlate in imaginary time
+// (O(n_q M^2)) instead of O(n_q M^2 N^2) trigonometric pair terms.
+//
+// Lattice q = (2 pi / L) n: exp(i q.r) = prod_d exp(2 pi i n_d s_d), s_d = x_d / L.
+// Each workgroup stages per-bead axis phasors exp(2 pi i n s_d) for |n| <= nmax in
+// LDS (phase reduced in FP64, FP32 sincospi), then each lane accumulates a run of
+// q-vectors as two complex FP32 products per (q, bead) with FP64 per-chunk sums.
+#include <hip/hip_runtime.h>
+
+#include "pimc_kernels.hpp"
+
+namespace pimc {
+
+constexpr int kRhoThreads = 256;
+constexpr int kRhoBeadChunk = 32;  // beads per LDS phasor stage
+constexpr int kMaxNmax = 40;       // largest |n_d| of a lattice q
+constexpr int kReseed = 4;         // exact sincospi every kReseed powers, products in between
+
+// rho_q(m) for lattice q.  Lane t of a workgroup takes q = base + t + 256 k
+// (k < QPT), so a wave walks consecutive q: mostly one (n_x, n_y) and consecutive
+// n_z, which makes the x/y phasor reads LDS broadcasts and the z reads
+// consecutive words.  Tables are bead-major: ph[axis][bead][n].
+template <int QPT>
+__global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
+    extern __shared__ float2 ph[];
+    const int m = blockIdx.y;
+    const int n_beads = A.nbs[m];
+    const int w0 = A.nmax[0] + 1, w1 = A.nmax[1] + 1, w2 = A.nmax[2] + 1;
+    float2* phx = ph;
+    float2* phy = phx + w0 * kRhoBeadChunk;
+    float2* phz = phy + w1 * kRhoBeadChunk;
+    const double* px = A.pos + (int64_t)m * A.nalloc;
+    const double* py = px + A.plane;
+    const double* pz = py + A.plane;
+
+    const int qbase = blockIdx.x * kRhoThreads * QPT + threadIdx.x;
+    int ax[QPT], ay[QPT], az[QPT];
+    float sx[QPT], sy[QPT], sz[QPT];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) {
+        const int qi = qbase + k * kRhoThreads;
+        int nx = 0, ny = 0, nz = 0;
+        if (qi < A.nq) {
+            nx = A.qn[3 * qi];
+            ny = A.qn[3 * qi + 1];
+            nz = A.qn[3 * qi + 2];
+        }
+        ax[k] = nx < 0 ? -nx : nx;
+        ay[k] = ny < 0 ? -ny : ny;
+        az[k] = nz < 0 ? -nz : nz;
+        sx[k] = nx < 0 ? -1.f : 1.f;
+        sy[k] = ny < 0 ? -1.f : 1.f;
+        sz[k] = nz < 0 ? -1.f : 1.f;
+    }
+    double are[QPT], aim[QPT];
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) are[k] = aim[k] = 0.0;
+
+    for (int b0 = 0; b0 < n_beads; b0 += kRhoBeadChunk) {
+        const int nb = min(kRhoBeadChunk, n_beads - b0);
+        __syncthreads();
+        // one lane per (axis, bead) row: exp(2 pi i n s) for n = 0..nmax
+        for (int r = threadIdx.x; r < 3 * kRhoBeadChunk; r += kRhoThreads) {
+            const int axis = r / kRhoBeadChunk, bb = r % kRhoBeadChunk;
+            const int w = axis == 0 ? w0 : (axis == 1 ? w1 : w2);
+            float2* row = (axis == 0 ? phx : (axis == 1 ? phy : phz)) + bb * w;
+            if (bb >= nb) {
+                for (int n = 0; n < w; ++n) row[n] = make_float2(0.f, 0.f);
+                continue;
+            }
+            const double x = (axis == 0 ? px : (axis == 1 ? py : pz))[b0 + bb];
+            const double sfrac = x * A.invL;           // position in box units
+            float s1, c1;
+            {
+                const double red = sfrac - rint(sfrac);
+                sincospif((float)(2.0 * red), &s1, &c1);
+            }
+            float cr = 1.f, ci = 0.f;
+            for (int n = 0; n < w; ++n) {
+                if (n % kReseed == 0) {
+                    const double t = (double)n * sfrac;
+                    const double red = t - rint(t);
+                    sincospif((float)(2.0 * red), &ci, &cr);
+                } else {
+                    const float nr = fmaf(cr, c1, -ci * s1);
+                    const float ni = fmaf(cr, s1, ci * c1);
+                    cr = nr;
+                    ci = ni;
+                }
+                row[n] = make_float2(cr, ci);
+            }
+        }
+        __syncthreads();
+        float sre[QPT], sim[QPT];
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) sre[k] = sim[k] = 0.f;
+        for (int bb = 0; bb < nb; ++bb) {
+#pragma unroll
+            for (int k = 0; k < QPT; ++k) {
+                const float2 ex = phx[bb * w0 + ax[k]];
+                const float2 ey = phy[bb * w1 + ay[k]];
+                const float2 ez = phz[bb * w2 + az[k]];
+                const float exi = sx[k] * ex.y, eyi = sy[k] * ey.y, ezi = sz[k] * ez.y;
+                const float xr = fmaf(ex.x, ey.x, -exi * eyi);
+                const float xi = fmaf(ex.x, eyi, exi * ey.x);
+                sre[k] += fmaf(xr, ez.x, -xi * ezi);
+                sim[k] += fmaf(xr, ezi, xi * ez.x);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) {
+            are[k] += (double)sre[k];
+            aim[k] += (double)sim[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < QPT; ++k) {
+        const int qi = qbase + k * kRhoThreads;
+        if (qi < A.nq) {
+            A.rho[((int64_t)m * A.nq + qi) * 2 + 0] = are[k];
+            A.rho[((int64_t)m * A.nq + qi) * 2 + 1] = aim[k];
+        }
+    }
+}
+
+// General q: rho_q(m) = sum_i exp(i q . r_i) with the phase reduced in FP64.
+__global__ __launch_bounds__(kRhoThreads) void k_rho_general(RhoArgs A) {
+    const int m = blockIdx.y;
+    const int qi = blockIdx.x * kRhoThreads + threadIdx.x;
+    if (qi >= A.nq) return;
+    const double qx = A.q[3 * qi], qy = A.q[3 * qi + 1], qz = A.q[3 * qi + 2];
+    const double* px = A.pos + (int64_t)m * A.nalloc;
+    const double* py = px + A.plane;
+    const double* pz = py + A.plane;
+    const int n = A.nbs[m];
+    double re = 0.0, im = 0.0;
+    constexpr double kInv2Pi = 0.15915494309189533577;
+    for (int b0 = 0; b0 < n; b0 += 64) {
+        float sre = 0.f, sim = 0.f;
+        const int e = min(n, b0 + 64);
+        for (int b = b0; b < e; ++b) {
+            const double phase = qx * px[b] + qy * py[b] + qz * pz[b];
+            const double rev = phase * kInv2Pi;
+            const double red = rev - rint(rev);
+            float s, c;
+            sincospif((float)(2.0 * red), &s, &c);
+            sre += c;
+            sim += s;
+        }
+        re += (double)sre;
+        im += (double)sim;
+    }
+    A.rho[((int64_t)m * A.nq + qi) * 2 + 0] = re;
+    A.rho[((int64_t)m * A.nq + qi) * 2 + 1] = im;
+}
+
+// F(q,tau) = sum_m Re rho(m) conj(rho(m+tau)).  A workgroup stages kIsfQ
+// q-vectors (q-major, rows of M) with coalesced reads of rho[m][q..q+kIsfQ);
+// each wave then takes kIsfQ/4 of them, lanes over lags: rho(m) is an LDS
+// broadcast and rho(m+tau) consecutive words.
+constexpr int kIsfQ = 8;
+constexpr int kIsfLagsPerLane = 8;   // M <= 512
+__global__ __launch_bounds__(256) void k_isf_from_rho(const double* __restrict__ rho, int nq, int M,
+                                                     double inv_n, double* __restrict__ fqt) {
+    extern __shared__ double sr[];  // [kIsfQ][M][2]
+    const int qb = blockIdx.x * kIsfQ;
+    const int nql = min(kIsfQ, nq - qb);
+    for (int t = threadIdx.x; t < M * kIsfQ; t += 256) {
+        const int m = t / kIsfQ, k = t % kIsfQ;
+        double re = 0.0, im = 0.0;
+        if (k < nql) {
+            re = rho[((int64_t)m * nq + qb + k) * 2];
+            im = rho[((int64_t)m * nq + qb + k) * 2 + 1];
+        }
+        sr[((int64_t)k * M + m) * 2] = re;
+        sr[((int64_t)k * M + m) * 2 + 1] = im;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int k = wave; k < nql; k += 4) {
+        const double* row = sr + (int64_t)k * M * 2;
+        double acc[kIsfLagsPerLane];
+        int idx[kIsfLagsPerLane];
+#pragma unroll
+        for (int u = 0; u < kIsfLagsPerLane; ++u) {
+            acc[u] = 0.0;
+            const int tau = lane + 64 * u;
+            idx[u] = tau < M ? tau : 0;
+        }
+        for (int m = 0; m < M; ++m) {
+            const double ar = row[2 * m], ai = row[2 * m + 1];
+#pragma unroll
+            for (int u = 0; u < kIsfLagsPerLane; ++u) {
+                if (64 * u < M) {
+                    const double br = row[2 * idx[u]], bi = row[2 * idx[u] + 1];
+                    acc[u] = fma(ar, br, acc[u]);
+                    acc[u] = fma(ai, bi, acc[u]);
+                    idx[u] = (idx[u] + 1 == M) ? 0 : idx[u] + 1;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kIsfLagsPerLane; ++u) {
+            const int tau = lane + 64 * u;
+            if (tau < M) fqt[(int64_t)(qb + k) * M + tau] = acc[u] * inv_n;
+        }
+    }
+}
+
+// S(q) contribution = sum_m |rho_q(m)|^2 / N, one lane per q (coalesced over q).
+__global__ void k_ssf_from_rho(const double* __restrict__ rho, int nq, int M, double inv_n,
+                               double* __restrict__ sq) {
+    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    double acc = 0.0;
+    for (int m = 0; m < M; ++m) {
+        const double re = rho[((int64_t)m * nq + qi) * 2], im = rho[((int64_t)m * nq + qi) * 2 + 1];
+        acc = fma(re, re, acc);
+        acc = fma(im, im, acc);
+    }
+    sq[qi] = acc * inv_n;
+}
+
+// Direct minimum-image S(q) pair sum for general q: one block per (q, slice).
+__global__ __launch_bounds__(256) void k_ssf_pairs(const double* __restrict__ pos, int64_t plane, int nalloc,
+                                                  const int* __restrict__ nbs, const double* __restrict__ q,
+                                                  BoxDev box, double* __restrict__ partial) {
+    __shared__ double red[4];
+    const int qi = blockIdx.x, m = blockIdx.y;
+    const double qx = q[3 * qi], qy = q[3 * qi + 1], qz = q[3 * qi + 2];
+    const double* px = pos + (int64_t)m * nalloc;
+    const double* py = px + plane;
+    const double* pz = py + plane;
+    const int n = nbs[m];
+    constexpr double kInv2Pi = 0.15915494309189533577;
+    double acc = 0.0;
+    const int64_t total = (int64_t)n * n;
+    for (int64_t t = threadIdx.x; t < total; t += 256) {
+        const int i = (int)(t / n), j = (int)(t % n);
+        if (j <= i) continue;
+        const double dx = min_image_exact(px[i] - px[j], box.L, box.invL);
+        const double dy = min_image_exact(py[i] - py[j], box.L, box.invL);
+        const double dz = min_image_exact(pz[i] - pz[j], box.L, box.invL);
+        const double rev = (dx * qx + dy * qy + dz * qz) * kInv2Pi;
+        const double red2 = rev - rint(rev);
+        acc += 2.0 * (double)cospif((float)(2.0 * red2));
+    }
+    const double s = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) partial[(int64_t)qi * gridDim.y + m] = s + (double)n;
+}
+
+// sq[q] = sum_m partial[q][m] / N in slice order
+__global__ void k_ssf_finish(const double* __restrict__ partial, int nq, int M, double inv_n,
+                             double* __restrict__ sq) {
+    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (qi >= nq) return;
+    double s = 0.0;
+    for (int m = 0; m < M; ++m) s += partial[(int64_t)qi * M + m];
+    sq[qi] = s * inv_n;
+}
+
+hipError_t launch_rho(bool lattice, hipStream_t st, const RhoArgs& A) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    if (A.nq <= 0) return hipSuccess;
+    if (lattice) {
+        const size_t lds = sizeof(float2) * (A.nmax[0] + A.nmax[1] + A.nmax[2] + 3) * kRhoBeadChunk;
+        // the most q per lane that still leaves >= 2048 workgroups
+        auto blocks = [&](int qpt) { return (int64_t)((A.nq + kRhoThreads * qpt - 1) / (kRhoThreads * qpt)) * A.num_slices; };
+        int qpt = 8;
+        while (qpt > 1 && blocks(qpt) < 2048) qpt >>= 1;
+        dim3 g((A.nq + kRhoThreads * qpt - 1) / (kRhoThreads * qpt), A.num_slices), b(kRhoThreads);
+        if (qpt == 8) hipLaunchKernelGGL(k_rho_lattice<8>, g, b, lds, st, A);
+        else if (qpt == 4) hipLaunchKernelGGL(k_rho_lattice<4>, g, b, lds, st, A);
+        else if (qpt == 2) hipLaunchKernelGGL(k_rho_lattice<2>, g, b, lds, st, A);
+        else hipLaunchKernelGGL(k_rho_lattice<1>, g, b, lds, st, A);
+    } else {
+        dim3 g((A.nq + kRhoThreads - 1) / kRhoThreads, A.num_slices), b(kRhoThreads);
+        hipLaunchKernelGGL(k_rho_general, g, b, 0, st, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_isf(hipStream_t st, const double* rho, int nq, int M, double inv_n, double* fqt, int only_tau0) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    if (nq <= 0) return hipSuccess;
+    if (only_tau0) {
+        hipLaunchKernelGGL(k_ssf_from_rho, dim3((nq + 255) / 256), dim3(256), 0, st, rho, nq, M, inv_n, fqt);
+    } else {
+        dim3 g((nq + kIsfQ - 1) / kIsfQ), b(256);
+        const size_t lds = sizeof(double) * 2 * kIsfQ * (size_t)M;
+        hipLaunchKernelGGL(k_isf_from_rho, g, b, lds, st, rho, nq, M, inv_n, fqt);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ssf_pairs(hipStream_t st, const double* pos, int64_t plane, int nalloc, int M, const int* nbs,
+                            const double* q, int nq, const BoxDev& box, double* partial, double inv_n, double* sq) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    if (nq <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ssf_pairs, dim3(nq, M), dim3(256), 0, st, pos, plane, nalloc, nbs, q, box, partial);
+    hipLaunchKernelGGL(k_ssf_finish, dim3((nq + 255) / 256), dim3(256), 0, st, partial, nq, M, inv_n, sq);
+    return hipGetLastError();
+}
+
+int rho_max_nmax() { return kMaxNmax; }
+
+}  // namespace pimc

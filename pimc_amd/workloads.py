@@ -1,0 +1,41 @@
+"""Parameters of the measured workloads (He-4 at 0.02198 A^-3, Aziz 1979).
+
+Sizes follow the reference's Prism(density, N) box (src/container.cpp:84-109);
+tau = 1/(M T) as in src/setup.cpp:999-1008 but without forcing M even.
+"""
+import math
+
+DENSITY = 0.02198
+LAMBDA = 6.0596
+SEED0 = 139853      # the reference's base seed (src/pdrive.cpp:41,63)
+
+
+def box_side(n, density=DENSITY):
+    return (1.0 * n / density) ** (1.0 / 3.0)
+
+
+WORKLOADS = {
+    # plain single walker, small: N=16, T=2 K, tau=0.004, M=125
+    "C1": dict(N=16, M=125, T=2.0, tau=0.004, rc_frac=1.0, q=("max_int", "4 4 4")),
+    # single walker, larger: N=256, T=1.5 K, tau=0.004, M=167
+    "C2": dict(N=256, M=167, T=1.5, tau=0.004, rc_frac=1.0, q=("max_float", "3 0 0")),
+    # large single system: N=2048, T=1 K, tau=0.004, M=250, rc = L/2
+    "C5": dict(N=2048, M=250, T=1.0, tau=0.004, rc_frac=0.5, q=("max_float", "4 0 0")),
+}
+
+
+def lattice_q(side, qmag_max=None, nmax=None):
+    """Lattice wavevectors in the reference's max_float/max_int order (src/estimator.cpp:487-541)."""
+    import numpy as np
+    if nmax is None:
+        nm = 1 + int(qmag_max * side / 2.0 / math.pi)
+        lim = qmag_max
+    else:
+        nm = nmax
+        q1 = nm * 2.0 * math.pi / side
+        lim = math.sqrt(((0.0 + q1 * q1) + q1 * q1) + q1 * q1)
+    r = np.arange(-nm, nm + 1)
+    n = np.stack(np.meshgrid(r, r, r, indexing="ij"), axis=-1).reshape(-1, 3)
+    q = n * 2.0 * math.pi / side
+    mag = np.sqrt(((0.0 + q[:, 0] * q[:, 0]) + q[:, 1] * q[:, 1]) + q[:, 2] * q[:, 2])
+    return q[mag <= lim]

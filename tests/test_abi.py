@@ -1,0 +1,45 @@
+"""CPU-side checks of the drop-in boundary: libpimc_gpu.so loads and exports
+every entry point declared in include/pimc_gpu.h, and the Python mirror binds
+each of them.  No compute call is made (there is no GPU in this container)."""
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "pimc_gpu.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(pimc_[A-Za-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "pimc_ctx_create" in syms and "pimc_delta_action" in syms and "pimc_isf" in syms
+    assert len(syms) >= 25
+
+
+def test_library_exports_every_symbol():
+    import ctypes
+    path = os.path.join(ROOT, "pimc_amd", "libpimc_gpu.so")
+    if not os.path.exists(path):
+        pytest.fail("libpimc_gpu.so not built: run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_python_mirror_binds_every_symbol():
+    from pimc_amd import api
+    assert set(declared_symbols()) == set(api.SIGNATURES)
+    api.lib()
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    from pimc_amd import api
+    monkeypatch.setattr(api, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(api, "_lib", None)
+    with pytest.raises(api.PimcError):
+        api.lib()

@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--isf-reps", type=int, default=5)
+    ap.add_argument("--legs", default="plain,batched,served",
+                    help="comma list of plain, batched, served (plain always runs)")
+    ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (HBM counter scaling)")
     args = ap.parse_args()
@@ -147,12 +150,89 @@ def main():
         "served": None,
     }
 
+    legs = set(args.legs.split(","))
+    dv_pairs_total = sum(pairs_dv[s] * len(lat[s]) for s in segs)
+    dv_us_total = sum(sum(lat[s]) for s in segs)
+    plain_dv_rate = dv_pairs_total / (dv_us_total * 1e-6)
+    line["plain_delta_v_pair_evals_per_s"] = plain_dv_rate
+    if "batched" in legs:
+        line["batched"] = batched_leg(pimc_amd, value)
+    if "served" in legs:
+        line["served"] = served_leg(pimc_amd, plain_dv_rate, args.served_seconds)
     if args.probe_copy:
         line["copy_probe"] = {"bytes": args.probe_copy,
                               "avg_ms": pimc_amd.api.copy_probe(args.probe_copy, 5)}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(beads, L, M, N, tau, LAMBDA, pairs_full, pairs_dv)
     print(json.dumps(line))
+
+
+def batched_leg(pimc_amd, plain_value):
+    """512 walkers, N in {16..256}, M in {64..512}, T in [1,3] K, Gaussian-bridge
+    worldlines rounded to FP32; autotuned tiles; hipGraph replays."""
+    from pimc_amd.workloads import LAMBDA, SEED0, box_side
+    rng = np.random.default_rng(SEED0)
+    ws, beads = [], []
+    for k in range(512):
+        N = int(rng.choice([16, 32, 64, 128, 256]))
+        M = int(rng.choice([64, 128, 256, 512]))
+        T = float(rng.uniform(1.0, 3.0))
+        L = box_side(N)
+        tau = 1.0 / (M * T)
+        b = pimc_amd.api.gen_worldlines(SEED0 + k, L, M, N, LAMBDA, tau, 1).astype(np.float32).astype(np.float64)
+        ws.append(dict(side=L, M=M, N=N, tau=tau, lam=LAMBDA, rc=L))
+        beads.append(b)
+    bt = pimc_amd.Batch(ws)
+    for k, b in enumerate(beads):
+        bt.upload(k, b)
+    bt.use_graph(True)
+    tune = bt.autotune(reps=2)
+    bt.potentialAction()
+    reps = 10
+    t0 = time.perf_counter()
+    ms = []
+    for _ in range(reps):
+        bt.potentialAction()
+        ms.append(bt.info()["last_ms"])
+    wall = (time.perf_counter() - t0) / reps
+    info = bt.info()
+    dev_ms = float(np.mean(ms))
+    rate = info["total_pairs"] / (dev_ms * 1e-3)
+    return {"workload": "512 walkers, N in {16,32,64,128,256}, M in {64,128,256,512}, T in [1,3] K, "
+                        "tau = 1/(M T), persistent launch, hipGraph replay",
+            "pairs_per_launch": info["total_pairs"], "device_ms": dev_ms, "wall_ms": wall * 1e3,
+            "pair_evals_per_s": rate, "tflops_algorithmic": rate * F_PAIR / 1e12,
+            "frac_fp32_peak": rate * F_PAIR / 1e12 / PEAK_FP32_TFLOPS,
+            "tiles": tune, "vs_plain": rate / plain_value}
+
+
+def served_leg(pimc_amd, plain_dv_rate, seconds):
+    """1024 resident chains, N in [32,512], M in [64,512]; Poisson arrivals of
+    Staging-style segment moves (4..64 slices) and 10% CenterOfMass moves
+    (sigma 0.1 A); device-side Metropolis and commit."""
+    from pimc_amd.workloads import LAMBDA, SEED0, box_side
+    rng = np.random.default_rng(SEED0 + 1)
+    srv = pimc_amd.Server(12 << 30, streams=4, max_side=box_side(512) + 1.0)
+    for k in range(1024):
+        N = int(rng.integers(32, 513))
+        M = int(rng.integers(64, 513))
+        T = float(rng.uniform(1.0, 3.0))
+        L = box_side(N)
+        tau = 1.0 / (M * T)
+        b = pimc_amd.api.gen_worldlines(SEED0 + 10000 + k, L, M, N, LAMBDA, tau, 1)
+        srv.add_chain(L, M, N, tau, LAMBDA, L, b)
+    out = {"workload": "1024 chains, N in [32,512], M in [64,512], segment moves of 4..64 slices + 10% CoM, "
+                       "Poisson arrivals", "rates": {}}
+    for rate in (2e3, 2e4, 2e5):
+        st = srv.run_synthetic(1024, rate=rate, duration_s=seconds)
+        out["rates"][f"{int(rate)}"] = {k: float(v) for k, v in st.items()}
+    st = srv.run_synthetic(1024, rate=0.0, duration_s=seconds)
+    out["closed_loop"] = {k: float(v) for k, v in st.items()}
+    best = max([out["closed_loop"]] + list(out["rates"].values()), key=lambda d: d["pair_evals_per_s"])
+    out["pair_evals_per_s"] = best["pair_evals_per_s"]
+    out["vs_plain_delta_v"] = best["pair_evals_per_s"] / plain_dv_rate
+    srv.close()
+    return out
 
 
 def cpu_baseline(beads, L, M, N, tau, lam, pairs_full, pairs_dv):

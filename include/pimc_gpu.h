@@ -154,6 +154,83 @@ int pimc_aziz_grad2V(pimc_aziz* az, const double* sep3, double* out, int n);
 double pimc_aziz_tailV(const pimc_aziz* az);   /* PotentialBase::tailV */
 int  pimc_aziz_table_length(const pimc_aziz* az);
 
+
+/* ---- batched form: many independent walkers in one launch -------------------- */
+/* LocalAction::potentialAction() (src/action.cpp:456-472) of n walkers with uneven
+ * N and M (all primitive, one Aziz year).  Walkers are cut into work items of
+ * pairs_per_lane * 256 pairs each; a persistent grid of workgroups walks them;
+ * the launch sequence can be captured once into a hipGraph and replayed.  No
+ * reference counterpart: the reference evaluates one walker per process. */
+typedef struct pimc_batch pimc_batch;
+int  pimc_batch_create(const pimc_desc* walkers, int n_walkers, pimc_batch** out);
+void pimc_batch_destroy(pimc_batch* b);
+const char* pimc_batch_last_error(const pimc_batch* b);
+int  pimc_batch_upload(pimc_batch* b, int walker, const double* beads);
+int  pimc_batch_set_tiles(pimc_batch* b, int pairs_per_lane, int grid);
+int  pimc_batch_use_graph(pimc_batch* b, int enable);
+/* actions [n_walkers]; v_slices [total slices] (walker-major, may be NULL). */
+int  pimc_batch_total_action(pimc_batch* b, int shift, const double vfactor[2], double* actions,
+                             double* v_slices);
+int  pimc_batch_info(const pimc_batch* b, int64_t* total_pairs, int* n_items, int* n_groups,
+                     double* last_ms);
+/* Autotuner: times pairs_per_lane in {16..256} x grid in {2,4,8,16} per CU and keeps
+ * the fastest; out4 = {pairs_per_lane, grid, ms, candidates}. */
+int  pimc_batch_autotune(pimc_batch* b, int reps, double* out4);
+
+/* ---- serving form: Delta V requests of many chains, continuous batching ------- */
+/* Each request is the Delta V of one move over beads on distinct slices of one
+ * chain (the Staging / CenterOfMass contract, src/move.cpp:1124-1408), decided on
+ * the device by rand() < exp(-(A_new - A_old)) with the caller's uniform deviate
+ * and committed when accepted.  No reference counterpart: the reference runs one
+ * chain per process. */
+typedef struct pimc_server pimc_server;
+typedef struct {
+    size_t hbm_budget;  /* bytes of chain state kept resident (arena size) */
+    int streams;        /* batches in flight, 1..4 */
+    int aziz_year;
+    int v_mode;
+    double max_side;    /* largest box edge any chain will use [A] */
+} pimc_server_desc;
+int  pimc_server_create(const pimc_server_desc* desc, pimc_server** out);
+void pimc_server_destroy(pimc_server* s);
+const char* pimc_server_last_error(const pimc_server* s);
+/* Adds a chain (beads [M][N][3]); it is resident at once when the budget allows,
+ * else it waits (state 0) until pimc_server_remove_chain frees room. */
+int  pimc_server_add_chain(pimc_server* s, const pimc_desc* d, const double* beads, int* chain_id);
+int  pimc_server_chain_state(pimc_server* s, int chain, int* state);  /* 0 waiting, 1 resident, 2 removed */
+int  pimc_server_remove_chain(pimc_server* s, int chain);
+int  pimc_server_download(pimc_server* s, int chain, double* beads);
+/* commit_mode: 0 evaluate only, 1 Metropolis with u, 2 always commit. */
+int  pimc_server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos,
+                        int n, int shift, const double vfactor[2], int commit_mode, double u,
+                        uint64_t* req_id);
+int  pimc_server_poll(pimc_server* s, uint64_t req_id, int* done);
+/* out5 = {A_old, A_new, dU, accepted, latency_us}; releases the request. */
+int  pimc_server_wait(pimc_server* s, uint64_t req_id, double* out5);
+int  pimc_server_stats(pimc_server* s, int64_t* batches, int64_t* requests, int64_t* pair_evals,
+                       size_t* free_bytes);
+typedef struct {
+    int n_chains;       /* chains 0..n_chains-1 receive the load */
+    double rate;        /* Poisson arrival rate [requests/s] over all chains; <= 0: closed loop */
+    double duration_s;
+    int seg_min, seg_max;  /* segment length in slices (moved beads = length - 1) */
+    double p_com;       /* fraction of CenterOfMass requests (whole worldline) */
+    double com_sigma;   /* Gaussian CoM shift per axis [A] */
+    double lambda;      /* segment proposal width sqrt(lambda tau) */
+    uint32_t seed;
+} pimc_load_desc;
+/* stats10 = {completed, req/s, pair evals/s, p50_us, p99_us, mean_us, accept frac,
+ *            batches, requests per batch, seconds}; latency counts from arrival. */
+int  pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* load, double* stats10);
+
+/* ---- seeded synthetic worldlines -------------------------------------------- */
+/* MTRand(seed) stream (include/MersenneTwister.h); slice 0 Prism::randPosition
+ * (src/container.cpp:155-160); kind 0: randNorm(prev, sqrt(2 lambda tau)) + putInBC
+ * per slice; kind 1: closed Gaussian bridge by the staging construction
+ * (src/move.cpp:270-291).  beads [M][N][3].  Host only. */
+int pimc_gen_worldlines(uint32_t seed, double side, int M, int N, double lambda, double tau, int kind,
+                        double* beads);
+
 /* ---- measurement hooks ------------------------------------------------------- */
 /* When enabled, every launch of a hot kernel is bracketed by HIP events on the
  * stream it runs on; pimc_kernel_time returns the summed duration [ms] and the

@@ -34,6 +34,21 @@ class _Desc(C.Structure):
                 ("aziz_year", C.c_int), ("v_mode", C.c_int)]
 
 
+class _ServerDesc(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("streams", C.c_int), ("aziz_year", C.c_int), ("v_mode", C.c_int),
+                ("max_side", C.c_double)]
+
+
+class _LoadDesc(C.Structure):
+    _fields_ = [("n_chains", C.c_int), ("rate", C.c_double), ("duration_s", C.c_double), ("seg_min", C.c_int),
+                ("seg_max", C.c_int), ("p_com", C.c_double), ("com_sigma", C.c_double), ("lam", C.c_double),
+                ("seed", C.c_uint32)]
+
+
+_vp = C.c_void_p
+_u64p = C.POINTER(C.c_uint64)
+_i64p = C.POINTER(C.c_int64)
+
 SIGNATURES = {
     "pimc_ctx_create": (C.c_int, [C.POINTER(_Desc), C.POINTER(C.c_void_p)]),
     "pimc_ctx_destroy": (None, [C.c_void_p]),
@@ -65,6 +80,29 @@ SIGNATURES = {
     "pimc_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_long)]),
     "pimc_device_synchronize": (C.c_int, []),
     "pimc_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _dp]),
+    "pimc_gen_worldlines": (C.c_int, [C.c_uint32, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                                      _dp]),
+    "pimc_batch_create": (C.c_int, [C.POINTER(_Desc), C.c_int, C.POINTER(C.c_void_p)]),
+    "pimc_batch_destroy": (None, [_vp]),
+    "pimc_batch_last_error": (C.c_char_p, [_vp]),
+    "pimc_batch_upload": (C.c_int, [_vp, C.c_int, _dp]),
+    "pimc_batch_set_tiles": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "pimc_batch_use_graph": (C.c_int, [_vp, C.c_int]),
+    "pimc_batch_total_action": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "pimc_batch_info": (C.c_int, [_vp, _i64p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]),
+    "pimc_batch_autotune": (C.c_int, [_vp, C.c_int, _dp]),
+    "pimc_server_create": (C.c_int, [C.POINTER(_ServerDesc), C.POINTER(C.c_void_p)]),
+    "pimc_server_destroy": (None, [_vp]),
+    "pimc_server_last_error": (C.c_char_p, [_vp]),
+    "pimc_server_add_chain": (C.c_int, [_vp, C.POINTER(_Desc), _dp, C.POINTER(C.c_int)]),
+    "pimc_server_chain_state": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
+    "pimc_server_remove_chain": (C.c_int, [_vp, C.c_int]),
+    "pimc_server_download": (C.c_int, [_vp, C.c_int, _dp]),
+    "pimc_server_submit": (C.c_int, [_vp, C.c_int, _ip, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_double, _u64p]),
+    "pimc_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(C.c_int)]),
+    "pimc_server_wait": (C.c_int, [_vp, C.c_uint64, _dp]),
+    "pimc_server_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p, C.POINTER(C.c_size_t)]),
+    "pimc_server_run_synthetic": (C.c_int, [_vp, C.POINTER(_LoadDesc), _dp]),
 }
 
 _lib = None
@@ -279,6 +317,153 @@ class Aziz:
     @property
     def table_length(self):
         return lib().pimc_aziz_table_length(self.h)
+
+
+class Batch:
+    """Batched form: potentialAction() of many walkers in one persistent launch."""
+
+    def __init__(self, walkers, year=1979, v_mode=V_FP32):
+        """walkers: list of dicts with side, M, N, tau, lam, rc."""
+        arr = (_Desc * len(walkers))()
+        for k, w in enumerate(walkers):
+            arr[k] = _Desc(w["side"], w["M"], w["N"], w["tau"], w["lam"], w["rc"], year, v_mode)
+        h = C.c_void_p()
+        rc_ = lib().pimc_batch_create(arr, len(walkers), C.byref(h))
+        if rc_ != 0:
+            raise PimcError(f"pimc_batch_create failed ({rc_})")
+        self.h = h
+        self.walkers = walkers
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().pimc_batch_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PimcError(f"{what}: {lib().pimc_batch_last_error(self.h).decode()} ({rc})")
+
+    def upload(self, w, beads):
+        b = _f64(beads, (self.walkers[w]["M"], self.walkers[w]["N"], 3))
+        self._check(lib().pimc_batch_upload(self.h, w, _p(b, _dp)), "batch_upload")
+
+    def set_tiles(self, pairs_per_lane, grid=0):
+        self._check(lib().pimc_batch_set_tiles(self.h, pairs_per_lane, grid), "batch_set_tiles")
+
+    def use_graph(self, on=True):
+        self._check(lib().pimc_batch_use_graph(self.h, int(on)), "batch_use_graph")
+
+    def potentialAction(self, shift=1, vfactor=(1.0, 1.0), with_slices=False):
+        vf = _f64(vfactor)
+        a = np.zeros(len(self.walkers))
+        ns = sum(w["M"] for w in self.walkers)
+        vs = np.zeros(ns) if with_slices else None
+        self._check(lib().pimc_batch_total_action(self.h, shift, _p(vf, _dp), _p(a, _dp), _p(vs, _dp)),
+                    "batch_total_action")
+        return (a, vs) if with_slices else a
+
+    def info(self):
+        tp = C.c_int64()
+        ni, ng = C.c_int(), C.c_int()
+        ms = np.zeros(1)
+        self._check(lib().pimc_batch_info(self.h, C.byref(tp), C.byref(ni), C.byref(ng), _p(ms, _dp)), "batch_info")
+        return dict(total_pairs=tp.value, n_items=ni.value, n_groups=ng.value, last_ms=float(ms[0]))
+
+    def autotune(self, reps=3):
+        o = np.zeros(4)
+        self._check(lib().pimc_batch_autotune(self.h, reps, _p(o, _dp)), "batch_autotune")
+        return dict(pairs_per_lane=int(o[0]), grid=int(o[1]), ms=float(o[2]), candidates=int(o[3]))
+
+
+class Server:
+    """Serving form: Delta V requests of many resident chains, continuous batching."""
+
+    def __init__(self, hbm_budget, streams=2, year=1979, v_mode=V_FP32, max_side=60.0):
+        d = _ServerDesc(int(hbm_budget), streams, year, v_mode, max_side)
+        h = C.c_void_p()
+        rc_ = lib().pimc_server_create(C.byref(d), C.byref(h))
+        if rc_ != 0:
+            raise PimcError(f"pimc_server_create failed ({rc_})")
+        self.h = h
+        self.year, self.v_mode = year, v_mode
+        self.shapes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pimc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PimcError(f"{what}: {lib().pimc_server_last_error(self.h).decode()} ({rc})")
+
+    def add_chain(self, side, M, N, tau, lam, rc, beads):
+        d = _Desc(side, M, N, tau, lam, rc, self.year, self.v_mode)
+        b = _f64(beads, (M, N, 3))
+        cid = C.c_int()
+        self._check(lib().pimc_server_add_chain(self.h, C.byref(d), _p(b, _dp), C.byref(cid)), "add_chain")
+        self.shapes[cid.value] = (M, N)
+        return cid.value
+
+    def chain_state(self, chain):
+        st = C.c_int()
+        self._check(lib().pimc_server_chain_state(self.h, chain, C.byref(st)), "chain_state")
+        return st.value
+
+    def remove_chain(self, chain):
+        self._check(lib().pimc_server_remove_chain(self.h, chain), "remove_chain")
+
+    def download(self, chain):
+        M, N = self.shapes[chain]
+        b = np.zeros((M, N, 3))
+        self._check(lib().pimc_server_download(self.h, chain, _p(b, _dp)), "download")
+        return b
+
+    def submit(self, chain, slice_ptcl, new_pos, shift=1, vfactor=(1.0, 1.0), commit_mode=0, u=0.0):
+        sp = _i32(slice_ptcl, (-1, 2))
+        npos = _f64(new_pos, (-1, 3))
+        vf = _f64(vfactor)
+        rid = C.c_uint64()
+        self._check(lib().pimc_server_submit(self.h, chain, _p(sp, _ip), _p(npos, _dp), sp.shape[0], shift,
+                                             _p(vf, _dp), commit_mode, u, C.byref(rid)), "submit")
+        return rid.value
+
+    def poll(self, rid):
+        d = C.c_int()
+        self._check(lib().pimc_server_poll(self.h, rid, C.byref(d)), "poll")
+        return bool(d.value)
+
+    def wait(self, rid):
+        o = np.zeros(5)
+        self._check(lib().pimc_server_wait(self.h, rid, _p(o, _dp)), "wait")
+        return dict(A_old=o[0], A_new=o[1], dU=o[2], accepted=bool(o[3] > 0.5), latency_us=o[4], raw=o)
+
+    def stats(self):
+        b, r, p = C.c_int64(), C.c_int64(), C.c_int64()
+        f = C.c_size_t()
+        self._check(lib().pimc_server_stats(self.h, C.byref(b), C.byref(r), C.byref(p), C.byref(f)), "stats")
+        return dict(batches=b.value, requests=r.value, pair_evals=p.value, free_bytes=f.value)
+
+    def run_synthetic(self, n_chains, rate, duration_s, seg=(4, 64), p_com=0.1, com_sigma=0.1, lam=6.0596,
+                      seed=139853):
+        d = _LoadDesc(n_chains, rate, duration_s, seg[0], seg[1], p_com, com_sigma, lam, seed)
+        o = np.zeros(10)
+        self._check(lib().pimc_server_run_synthetic(self.h, C.byref(d), _p(o, _dp)), "run_synthetic")
+        keys = ["completed", "req_per_s", "pair_evals_per_s", "p50_us", "p99_us", "mean_us", "accept_frac",
+                "batches", "req_per_batch", "seconds"]
+        return dict(zip(keys, o))
+
+
+def gen_worldlines(seed, side, M, N, lam, tau, kind=0):
+    """Seeded worldlines [M][N][3] from the library's MTRand stream (host code)."""
+    b = np.zeros((M, N, 3))
+    rc_ = lib().pimc_gen_worldlines(seed, side, M, N, lam, tau, kind, _p(b, _dp))
+    if rc_ != 0:
+        raise PimcError(f"gen_worldlines failed ({rc_})")
+    return b
 
 
 def copy_probe(nbytes, iters=20):

@@ -330,11 +330,11 @@ static void build_grid_nn(pimc_ctx* c) {
 }
 
 static int64_t pair_chunk_for(int64_t total_pairs) {
-    // aim for >= ~3000 workgroups on the 256-CU part, at least 4 pairs per lane
-    int64_t chunk = total_pairs / 3000;
-    chunk = std::max<int64_t>(chunk, (int64_t)kPairThreads * 4);
-    chunk = ((chunk + kPairThreads - 1) / kPairThreads) * kPairThreads;
-    return std::min<int64_t>(chunk, (int64_t)kPairThreads * 512);
+    // >= 16 pairs per lane; shrink no further than that, grow (to at most 512 per
+    // lane) while more than ~4096 workgroups would remain
+    int64_t per_lane = total_pairs / ((int64_t)kPairThreads * 4096);
+    per_lane = std::max<int64_t>(16, std::min<int64_t>(512, per_lane));
+    return per_lane * kPairThreads;
 }
 
 static int rebuild_work(pimc_ctx* c) {

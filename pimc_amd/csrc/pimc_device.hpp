@@ -183,6 +183,26 @@ __device__ __host__ __forceinline__ int grid_coord(double p, double L, double si
     return (int)(fabs(p + 0.5 * L - 1.0e-7) / (size + 1.0e-7));
 }
 
+// Neighbouring cells of the 27-cell stencil with periodic wrap (the set
+// LookupTable::updateInteractionList walks, src/lookuptable.cpp:480-486).
+__device__ __forceinline__ bool cells_adjacent(int c1, int c2, int ng) {
+    const int x1 = c1 / (ng * ng), y1 = (c1 / ng) % ng, z1 = c1 % ng;
+    const int x2 = c2 / (ng * ng), y2 = (c2 / ng) % ng, z2 = c2 % ng;
+    int dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+    dx = dx < 0 ? -dx : dx;
+    dy = dy < 0 ? -dy : dy;
+    dz = dz < 0 ? -dz : dz;
+    return (dx <= 1 || dx == ng - 1) && (dy <= 1 || dy == ng - 1) && (dz <= 1 || dz == ng - 1);
+}
+
+__device__ __forceinline__ int cell_of(double x, double y, double z, const BoxDev& b) {
+    // positions inside the box map into [0, ngrid); the clamp only guards memory
+    const int gx = min(max(grid_coord(x, b.L, b.grid_size), 0), b.ngrid - 1);
+    const int gy = min(max(grid_coord(y, b.L, b.grid_size), 0), b.ngrid - 1);
+    const int gz = min(max(grid_coord(z, b.L, b.grid_size), 0), b.ngrid - 1);
+    return (gx * b.ngrid + gy) * b.ngrid + gz;
+}
+
 // Worm::getState (include/worm.h:110-117): 0 HEADTAIL, 1 SPECIAL, 2 NONE.
 struct WormDev {
     int head[2], tail[2], sp1[2], sp2[2];

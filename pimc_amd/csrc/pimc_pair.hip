@@ -24,6 +24,43 @@ namespace pimc {
 // ---------------------------------------------------------------------------
 // All i<j pairs of each slice.
 // ---------------------------------------------------------------------------
+// Strided walk over the flattened pairs [p_begin, p_begin + p_count) of one
+// slice: lane t takes p = p_begin + t + 256 s; pair p is (i, (i + d) mod n) with
+// d = 1 + p / n, i = p % n, and the repeated d = n/2 pairs of even n masked.
+template <int MODE, typename P>
+__device__ __forceinline__ double pair_loop(P px, P py, P pz, int n, const PairItem it, const BoxDev& box,
+                                            const AzizDev& a, double inv_dr, bool factors, const WormDev& worm,
+                                            int slice) {
+    const int H = n >> 1;
+    const bool even = (n & 1) == 0;
+    int p = it.p_begin + threadIdx.x;
+    const int p_end = it.p_begin + it.p_count;
+    int d = 1 + p / n;
+    int i = p - (d - 1) * n;
+    const int step_d = kPairThreads / n;
+    const int step_i = kPairThreads - step_d * n;
+    double acc = 0.0;
+    for (; p < p_end; p += kPairThreads) {
+        int j = i + d;
+        if (j >= n) j -= n;
+        const bool ok = !(even && d == H && i >= H);
+        const double r2 = sep_r2<MODE>(px[i], py[i], pz[i], px[j], py[j], pz[j], box);
+        double v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+        if (factors) {
+            const int lo = i < j ? i : j, hi = i < j ? j : i;
+            v *= pair_factor(bead_state(worm, slice, lo), bead_state(worm, slice, hi));
+        }
+        acc += ok ? v : 0.0;
+        i += step_i;
+        d += step_d;
+        if (i >= n) {
+            i -= n;
+            d += 1;
+        }
+    }
+    return acc;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kPairThreads) void k_pair_action(const PairItem* __restrict__ items,
                                                               const SliceDesc* __restrict__ slices,
@@ -57,69 +94,55 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const PairItem* __
     }
     __syncthreads();
 
-    const int H = n >> 1;
-    const bool even = (n & 1) == 0;
     const bool factors = worm_touches_slice(wk.worm, sd.slice);
-    int p = it.p_begin + threadIdx.x;
-    const int p_end = it.p_begin + it.p_count;
-    int d = 1 + p / n;
-    int i = p - (d - 1) * n;
-    const int step_d = kPairThreads / n;
-    const int step_i = kPairThreads - step_d * n;
-    double acc = 0.0;
-    for (; p < p_end; p += kPairThreads) {
-        int j = i + d;
-        if (j >= n) j -= n;
-        const bool ok = !(even && d == H && i >= H);
-        double xi, yi, zi, xj, yj, zj;
-        if (in_lds) {
-            xi = lx[i]; yi = ly[i]; zi = lz[i];
-            xj = lx[j]; yj = ly[j]; zj = lz[j];
-        } else {
-            xi = px[i]; yi = py[i]; zi = pz[i];
-            xj = px[j]; yj = py[j]; zj = pz[j];
-        }
-        const double r2 = sep_r2<MODE>(xi, yi, zi, xj, yj, zj, box);
-        double v = pair_value_from_r2<MODE>(r2, a, inv_dr);
-        if (factors) {
-            const int lo = i < j ? i : j, hi = i < j ? j : i;
-            v *= pair_factor(bead_state(wk.worm, sd.slice, lo), bead_state(wk.worm, sd.slice, hi));
-        }
-        acc += ok ? v : 0.0;
-        i += step_i;
-        d += step_d;
-        if (i >= n) {
-            i -= n;
-            d += 1;
-        }
-    }
+    // two copies of the loop so the LDS copy compiles to ds_read (no flat addressing)
+    const double acc = in_lds ? pair_loop<MODE>(lx, ly, lz, n, it, box, a, inv_dr, factors, wk.worm, sd.slice)
+                              : pair_loop<MODE>(px, py, pz, n, it, box, a, inv_dr, factors, wk.worm, sd.slice);
     const double s = block_sum<kPairThreads>(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 // Per-slice sums of the item partials (items are grouped by slice, in order) and
-// the per-walker potential action, slices in order (src/action.cpp:462-469).
-__global__ __launch_bounds__(256) void k_pair_finalize(int n_items, const SliceDesc* __restrict__ slices,
-                                                       int n_slices, const WalkerDesc* __restrict__ walkers,
-                                                       int n_walkers, const double* __restrict__ partials,
-                                                       double* __restrict__ v_slice, double* __restrict__ action) {
-    for (int sl = threadIdx.x; sl < n_slices; sl += 256) {
+// the per-walker potential action (src/action.cpp:462-469).  One workgroup of
+// 1024 lanes: the partials are first pulled into LDS with every load in flight,
+// then each wave reduces whole slices (lanes strided over items, fixed shuffle
+// tree) and whole walkers (lanes strided over slices), so the result depends
+// only on the work list, never on timing.
+constexpr int kFinThreads = 1024;
+constexpr int kFinLds = 6144;   // partials staged in LDS (beyond: read from HBM)
+__global__ __launch_bounds__(kFinThreads) void k_pair_finalize(int n_items, const SliceDesc* __restrict__ slices,
+                                                              int n_slices, const WalkerDesc* __restrict__ walkers,
+                                                              int n_walkers, const double* __restrict__ partials,
+                                                              double* __restrict__ v_slice,
+                                                              double* __restrict__ action) {
+    __shared__ double sp[kFinLds];
+    const bool staged = n_items <= kFinLds;
+    if (staged)
+        for (int t = threadIdx.x; t < n_items; t += kFinThreads) sp[t] = partials[t];
+    __syncthreads();
+    const double* src = staged ? sp : partials;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int kWaves = kFinThreads / 64;
+    for (int sl = wave; sl < n_slices; sl += kWaves) {
         const int b = slices[sl].first_item;
         const int e = (sl + 1 < n_slices) ? slices[sl + 1].first_item : n_items;
         double v = 0.0;
-        for (int k = b; k < e; ++k) v += partials[k];
-        v_slice[sl] = v;
+        for (int k = b + lane; k < e; k += 64) v += src[k];
+        v = wave_sum(v);
+        if (lane == 0) v_slice[sl] = v;
     }
+    __threadfence_block();
     __syncthreads();
-    for (int w = threadIdx.x; w < n_walkers; w += 256) {
+    for (int w = wave; w < n_walkers; w += kWaves) {
         const WalkerDesc& W = walkers[w];
         const double tau_s = W.shift * W.tau;
         double tot = 0.0;
-        for (int m = 0; m < W.num_slices; ++m) {
+        for (int m = lane; m < W.num_slices; m += 64) {
             const double vf = (m & 1) ? W.vfactor1 : W.vfactor0;
             tot += vf * tau_s * (0.0 + v_slice[W.first_slice + m]);
         }
-        action[w] = tot;
+        tot = wave_sum(tot);
+        if (lane == 0) action[w] = tot;
     }
 }
 
@@ -127,23 +150,6 @@ __global__ __launch_bounds__(256) void k_pair_finalize(int n_items, const SliceD
 // Vnn for a list of beads, old and new positions in one pass.
 // Block (c, b): bead b, partners [c*chunk, (c+1)*chunk).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool cells_adjacent(int c1, int c2, int ng) {
-    const int x1 = c1 / (ng * ng), y1 = (c1 / ng) % ng, z1 = c1 % ng;
-    const int x2 = c2 / (ng * ng), y2 = (c2 / ng) % ng, z2 = c2 % ng;
-    int dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
-    dx = dx < 0 ? -dx : dx;
-    dy = dy < 0 ? -dy : dy;
-    dz = dz < 0 ? -dz : dz;
-    return (dx <= 1 || dx == ng - 1) && (dy <= 1 || dy == ng - 1) && (dz <= 1 || dz == ng - 1);
-}
-
-__device__ __forceinline__ int cell_of(double x, double y, double z, const BoxDev& b) {
-    const int gx = grid_coord(x, b.L, b.grid_size);
-    const int gy = grid_coord(y, b.L, b.grid_size);
-    const int gz = grid_coord(z, b.L, b.grid_size);
-    return (gx * b.ngrid + gy) * b.ngrid + gz;
-}
-
 // Path::updateBead -> LookupTable::updateBead (src/lookuptable.cpp:355-393, :430-458)
 __device__ void cell_update_bead(int m, int i, int c_new, int nalloc, int num_slices, int* cell, int* label,
                                  int* numlab, int* hash) {
@@ -233,9 +239,13 @@ __global__ __launch_bounds__(kVnnThreads) void k_vnn(VnnArgs A, VnnInline req, A
     }
 }
 
-// Per-bead Vnn (chunks in order), the two actions and dU (beads in order), then
-// the committed bead updates, one lane per bead (the beads sit on distinct slices).
+// Per-bead Vnn (chunks in order), the two actions and dU (a fixed shuffle tree
+// over beads), then the committed bead updates, one lane per bead (the beads sit
+// on distinct slices).
 __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) {
+    __shared__ double red[3][4];
+    double ao = 0.0, an = 0.0, du = 0.0;
+    const double tau_s = A.shift * A.tau;
     for (int k = threadIdx.x; k < A.n_beads; k += 256) {
         double vo = 0.0, vn = 0.0;
         for (int cc = 0; cc < A.chunks; ++cc) {
@@ -244,23 +254,25 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
         }
         A.per_bead[2 * k + 0] = vo;
         A.per_bead[2 * k + 1] = vn;
+        const int slice = A.beads ? A.beads[k].slice : req.beads[k].slice;
+        const double co = ((slice & 1) ? A.vfactor1 : A.vfactor0) * tau_s;
+        ao += co * vo;
+        an += co * vn;
+        du += co * (vn - vo);
+    }
+    ao = wave_sum(ao);
+    an = wave_sum(an);
+    du = wave_sum(du);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[0][wave] = ao;
+        red[1][wave] = an;
+        red[2][wave] = du;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const double tau_s = A.shift * A.tau;
-        double ao = 0.0, an = 0.0, du = 0.0;
-        for (int k = 0; k < A.n_beads; ++k) {
-            const int slice = A.beads ? A.beads[k].slice : req.beads[k].slice;
-            const double vf = (slice & 1) ? A.vfactor1 : A.vfactor0;
-            const double co = vf * tau_s;
-            const double vo = A.per_bead[2 * k + 0], vn = A.per_bead[2 * k + 1];
-            ao += co * vo;
-            an += co * vn;
-            du += co * (vn - vo);
-        }
-        A.out[0] = ao;
-        A.out[1] = an;
-        A.out[2] = du;
+    if (threadIdx.x < 3) {
+        const int r = threadIdx.x;
+        A.out[r] = ((red[r][0] + red[r][1]) + red[r][2]) + red[r][3];
     }
     if (A.commit_in_kernel) {
         for (int k = threadIdx.x; k < A.n_beads; k += 256) {
@@ -467,8 +479,8 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const PairItem* items, i
         hipLaunchKernelGGL(k_pair_action<1>, g, b, lds, st, items, slices, walkers, pos, az, lds_beads, partials);
     else
         hipLaunchKernelGGL(k_pair_action<2>, g, b, lds, st, items, slices, walkers, pos, az, lds_beads, partials);
-    hipLaunchKernelGGL(k_pair_finalize, dim3(1), dim3(256), 0, st, n_items, slices, n_slices, walkers, n_walkers,
-                       partials, v_slice, action);
+    hipLaunchKernelGGL(k_pair_finalize, dim3(1), dim3(kFinThreads), 0, st, n_items, slices, n_slices, walkers,
+                       n_walkers, partials, v_slice, action);
     return hipGetLastError();
 }
 

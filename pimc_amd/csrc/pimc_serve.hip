@@ -1,0 +1,984 @@
+// pimc_serve.hip -- the serving form: Delta V (and S(q)) requests from many
+// concurrent Markov chains with continuous batching.
+//
+// * Chains are admitted into a pre-allocated HBM arena (the budget) by first fit;
+//   a chain that does not fit waits until another chain is removed.
+// * Worldlines stay resident; a request carries only its bead list, the proposed
+//   positions and the uniform deviate of its Metropolis test
+//   (rand() < exp(-(A_new - A_old)), src/move.cpp:1248, :1394).
+// * A scheduler thread packs every ready request (at most one in flight per
+//   chain, so a chain's moves stay ordered) into a batch, uploads one
+//   descriptor, and launches on one of up to four streams (default priority)
+//   while earlier batches still run, so batches overlap on the device.
+// * Each request is reduced by its own fixed-order tree and decided and
+//   committed on the device, so its result does not depend on the batch it
+//   travelled in.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <random>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "pimc_host.hpp"
+
+namespace pimc {
+
+struct ServeChainDev {
+    double* pos;        // SoA planes x|y|z of [M][N]
+    int64_t plane;
+    const uint8_t* on;  // [M][N]
+    int* cell;          // [M][N] packed cell numbers
+    int nalloc, num_slices;
+    int table_length;
+    BoxDev box;
+    double tau;
+};
+
+struct ServeReqDev {
+    int chain;          // index into the chain table
+    int n_beads;
+    int first_bead;     // into the batch bead array
+    int chunks;         // partner chunks per bead
+    int first_group;
+    int commit_mode;    // 0: evaluate only, 1: Metropolis with u, 2: always commit
+    int shift;
+    int out_slot;
+    double u;
+    double vfactor0, vfactor1;
+};
+
+struct ServeGroupDev {
+    int req;
+    int bead;           // index into the batch bead array
+    int chunk;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kVnnThreads) void k_serve_vnn(const ServeGroupDev* __restrict__ groups,
+                                                           const ServeReqDev* __restrict__ reqs,
+                                                           const VnnBead* __restrict__ beads,
+                                                           const ServeChainDev* __restrict__ chains, AzizDev az,
+                                                           int chunk, double* __restrict__ partials) {
+    __shared__ double red[kVnnThreads / 64];
+    const ServeGroupDev g = groups[blockIdx.x];
+    const ServeReqDev& R = reqs[g.req];
+    const ServeChainDev C = chains[R.chain];
+    const VnnBead q = beads[g.bead];
+    AzizDev a = az;
+    a.table_length = C.table_length;
+    const double inv_dr = 1.0 / a.dr;
+    const int m = q.slice, me = q.ptcl;
+    const double* px = C.pos + (int64_t)m * C.nalloc;
+    const double* py = px + C.plane;
+    const double* pz = py + C.plane;
+    const double ox = px[me], oy = py[me], oz = pz[me];
+    const double nx = q.new_pos[0], ny = q.new_pos[1], nz = q.new_pos[2];
+    const BoxDev box = C.box;
+    const bool use_cells = box.ngrid >= 4;
+    int c_old = 0, c_new = 0;
+    if (use_cells) {
+        c_old = C.cell[m * C.nalloc + me];
+        c_new = cell_of(nx, ny, nz, box);
+    }
+    const bool me_on = C.on[m * C.nalloc + me] != 0;
+    double acc_o = 0.0, acc_n = 0.0;
+    const int j_end = min((g.chunk + 1) * chunk, C.nalloc);
+    for (int j = g.chunk * chunk + threadIdx.x; j < j_end; j += kVnnThreads) {
+        const bool on = (C.on[m * C.nalloc + j] != 0) && (j != me) && me_on;
+        const double xj = px[j], yj = py[j], zj = pz[j];
+        const int cj = use_cells ? C.cell[m * C.nalloc + j] : 0;
+        {
+            const double r2 = sep_r2<MODE>(ox, oy, oz, xj, yj, zj, box);
+            const bool ok = on && (r2 < box.rc2) && (!use_cells || cells_adjacent(c_old, cj, box.ngrid));
+            const double v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+            acc_o += ok ? v : 0.0;
+        }
+        {
+            const double r2 = sep_r2<MODE>(nx, ny, nz, xj, yj, zj, box);
+            const bool ok = on && (r2 < box.rc2) && (!use_cells || cells_adjacent(c_new, cj, box.ngrid));
+            const double v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+            acc_n += ok ? v : 0.0;
+        }
+    }
+    const double so = block_sum<kVnnThreads>(acc_o, red);
+    const double sn = block_sum<kVnnThreads>(acc_n, red);
+    if (threadIdx.x == 0) {
+        partials[2 * (int64_t)blockIdx.x + 0] = so;
+        partials[2 * (int64_t)blockIdx.x + 1] = sn;
+    }
+}
+
+// One wave per request: per-bead sums (chunks in order), the actions and dU
+// (fixed shuffle tree), the Metropolis decision, the commit, the result.
+__global__ __launch_bounds__(64) void k_serve_finalize(const ServeReqDev* __restrict__ reqs,
+                                                       const VnnBead* __restrict__ beads,
+                                                       const ServeChainDev* __restrict__ chains,
+                                                       const double* __restrict__ partials,
+                                                       double* __restrict__ out) {
+    const ServeReqDev R = reqs[blockIdx.x];
+    const ServeChainDev C = chains[R.chain];
+    const int lane = threadIdx.x;
+    const double tau_s = R.shift * C.tau;
+    double ao = 0.0, an = 0.0, du = 0.0;
+    for (int k = lane; k < R.n_beads; k += 64) {
+        double vo = 0.0, vn = 0.0;
+        const int g0 = R.first_group + k * R.chunks;
+        for (int c = 0; c < R.chunks; ++c) {
+            vo += partials[2 * (int64_t)(g0 + c)];
+            vn += partials[2 * (int64_t)(g0 + c) + 1];
+        }
+        const double co = ((beads[R.first_bead + k].slice & 1) ? R.vfactor1 : R.vfactor0) * tau_s;
+        ao += co * vo;
+        an += co * vn;
+        du += co * (vn - vo);
+    }
+    ao = wave_sum(ao);
+    an = wave_sum(an);
+    du = wave_sum(du);
+    const bool accept = R.commit_mode == 2 || (R.commit_mode == 1 && R.u < exp(-du));
+    if (accept) {
+        for (int k = lane; k < R.n_beads; k += 64) {
+            const VnnBead qb = beads[R.first_bead + k];
+            const int64_t bi = (int64_t)qb.slice * C.nalloc + qb.ptcl;
+            C.pos[bi] = qb.new_pos[0];
+            C.pos[bi + C.plane] = qb.new_pos[1];
+            C.pos[bi + 2 * C.plane] = qb.new_pos[2];
+            if (C.box.ngrid >= 4) C.cell[bi] = cell_of(qb.new_pos[0], qb.new_pos[1], qb.new_pos[2], C.box);
+        }
+    }
+    if (lane == 0) {
+        double* o = out + 4 * (int64_t)R.out_slot;
+        o[0] = ao;
+        o[1] = an;
+        o[2] = du;
+        o[3] = accept ? 1.0 : 0.0;
+    }
+}
+
+__global__ void k_serve_cells(const double* __restrict__ pos, int64_t plane, int total, int* __restrict__ cell,
+                              BoxDev box) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < total) cell[t] = cell_of(pos[t], pos[t + plane], pos[t + 2 * plane], box);
+}
+
+}  // namespace pimc
+
+using namespace pimc;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+constexpr int kSlots = 4;            // batches in flight
+constexpr int kMaxReqPerBatch = 4096;
+constexpr int kMaxBeadsPerBatch = 1 << 17;
+constexpr int kMaxGroupsPerBatch = 1 << 18;
+
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_;   // offset -> length
+    bool alloc(size_t n, size_t* off) {
+        n = (n + 255) & ~size_t(255);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                *off = it->first;
+                size_t rest = it->second - n, at = it->first + n;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                return true;
+            }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = (n + 255) & ~size_t(255);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t free_bytes() const {
+        size_t s = 0;
+        for (auto& kv : free_) s += kv.second;
+        return s;
+    }
+};
+
+struct Chain {
+    pimc_desc d{};
+    int state = 0;                    // 0 waiting for memory, 1 resident, 2 removed
+    size_t off = 0, bytes = 0;
+    ServeChainDev dev{};
+    int dev_index = -1;
+    bool busy = false;
+    std::deque<uint64_t> queue;
+    std::vector<double> pending_beads;  // positions while waiting for admission
+};
+
+struct Request {
+    uint64_t id = 0;
+    int chain = -1;
+    int shift = 1;
+    double vf[2] = {1.0, 1.0};
+    int commit_mode = 0;
+    double u = 0.0;
+    std::vector<VnnBead> beads;
+    Clock::time_point t_submit, t_done;
+    bool done = false;
+    double out[4] = {0, 0, 0, 0};
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    bool busy = false;
+    char* h_desc = nullptr;           // pinned: reqs | beads | groups
+    char* d_desc = nullptr;
+    double* d_partials = nullptr;
+    double* h_out = nullptr;          // mapped pinned results [kMaxReqPerBatch][4]
+    double* d_out = nullptr;
+    std::vector<uint64_t> reqs;
+    size_t desc_bytes = 0;
+};
+
+}  // namespace
+
+struct pimc_server {
+    pimc_server_desc desc{};
+    AzizParams params{};
+    AzizDev az{};
+    Arena arena;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::vector<Chain> chains;
+    std::deque<int> waiting_chains;
+    std::vector<ServeChainDev> chain_table;
+    ServeChainDev* d_chains = nullptr;
+    int chain_cap = 0;
+    bool chains_dirty = false;
+    std::unordered_map<uint64_t, Request> reqs;
+    std::deque<uint64_t> ready;       // chains with queued work, by request id order
+    uint64_t next_id = 1;
+    Slot slots[kSlots];
+    int n_slots = 2;
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    std::string err;
+    int64_t launched_batches = 0, launched_reqs = 0;
+    int64_t pairs_done = 0;
+    double max_table_sep = 0;
+    int kernel_error = 0;
+    std::string kernel_error_msg;
+    bool track_done = false;
+    std::vector<uint64_t> done_list;   // completed request ids (load generator)
+
+    void run();
+    int launch_ready(std::unique_lock<std::mutex>& lk);
+    void complete(Slot& s);
+    void admit_waiting();
+};
+
+namespace {
+
+size_t chain_bytes(const pimc_desc& d) {
+    const size_t MN = (size_t)d.num_slices * d.num_alloc;
+    return MN * 3 * sizeof(double) + MN * sizeof(int) + MN + 3 * 256;
+}
+
+int sfail(pimc_server* s, int code, const std::string& m) {
+    if (s) s->err = m;
+    return code;
+}
+
+// Place a chain into the arena: positions, bead mask, cell numbers.
+int place_chain(pimc_server* s, int id, const double* beads) {
+    Chain& c = s->chains[id];
+    const pimc_desc& d = c.d;
+    const int M = d.num_slices, N = d.num_alloc;
+    const size_t MN = (size_t)M * N;
+    size_t off;
+    if (!s->arena.alloc(c.bytes, &off)) return 1;   // no room yet
+    c.off = off;
+    char* base = s->arena.base + off;
+    double* pos = (double*)base;
+    int* cell = (int*)(base + ((MN * 3 * sizeof(double) + 255) & ~size_t(255)));
+    uint8_t* on = (uint8_t*)((char*)cell + ((MN * sizeof(int) + 255) & ~size_t(255)));
+    std::vector<double> soa(3 * MN);
+    for (size_t t = 0; t < MN; ++t) {
+        soa[t] = beads[3 * t];
+        soa[t + MN] = beads[3 * t + 1];
+        soa[t + 2 * MN] = beads[3 * t + 2];
+    }
+    hipStream_t st = s->slots[0].stream;
+    if (hipMemcpyAsync(pos, soa.data(), sizeof(double) * 3 * MN, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+    std::vector<uint8_t> ones(MN, 1);
+    if (hipMemcpyAsync(on, ones.data(), MN, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+    ServeChainDev& cd = c.dev;
+    cd.pos = pos;
+    cd.plane = (int64_t)MN;
+    cd.on = on;
+    cd.cell = cell;
+    cd.nalloc = N;
+    cd.num_slices = M;
+    BoxDev& b = cd.box;
+    b.L = d.side;
+    b.invL = 1.0 / d.side;
+    b.rc2 = d.rc * d.rc;
+    int ng = (int)std::floor((d.side / d.rc) + 1.0e-7);
+    b.ngrid = ng < 1 ? 1 : ng;
+    b.grid_size = d.side / (1.0 * b.ngrid);
+    const double h = d.side / 2.0;
+    double ms = 0.0;
+    ms += h * h;
+    ms += h * h;
+    ms += h * h;
+    cd.table_length = int(std::sqrt(ms) / ((1.0E-6) * s->params.rm));
+    cd.tau = d.tau;
+    hipLaunchKernelGGL(k_serve_cells, dim3((unsigned)((MN + 255) / 256)), dim3(256), 0, st, pos, (int64_t)MN,
+                       (int)MN, cell, b);
+    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    // chain table entry
+    int idx = -1;
+    for (int k = 0; k < (int)s->chain_table.size(); ++k)
+        if (s->chain_table[k].pos == nullptr) {
+            idx = k;
+            break;
+        }
+    if (idx < 0) {
+        idx = (int)s->chain_table.size();
+        s->chain_table.push_back(cd);
+    } else {
+        s->chain_table[idx] = cd;
+    }
+    c.dev_index = idx;
+    c.state = 1;
+    s->chains_dirty = true;
+    return 0;
+}
+
+int sync_chain_table(pimc_server* s, hipStream_t st) {
+    if (!s->chains_dirty) return 0;
+    const int n = (int)s->chain_table.size();
+    if (n > s->chain_cap) {
+        if (s->d_chains) {
+            for (auto& sl : s->slots)
+                if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+            (void)hipFree(s->d_chains);
+        }
+        s->chain_cap = std::max(64, 2 * n);
+        if (hipMalloc(&s->d_chains, sizeof(ServeChainDev) * s->chain_cap) != hipSuccess) return -1;
+    }
+    // every slot stream must see the new table before its next batch
+    for (auto& sl : s->slots)
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (hipMemcpy(s->d_chains, s->chain_table.data(), sizeof(ServeChainDev) * n, hipMemcpyHostToDevice) != hipSuccess)
+        return -1;
+    s->chains_dirty = false;
+    (void)st;
+    return 0;
+}
+
+}  // namespace
+
+void pimc_server::admit_waiting() {
+    while (!waiting_chains.empty()) {
+        int id = waiting_chains.front();
+        Chain& c = chains[id];
+        if (c.state != 0) {
+            waiting_chains.pop_front();
+            continue;
+        }
+        int r = place_chain(this, id, c.pending_beads.data());
+        if (r != 0) break;   // still no room: keep arrival order
+        c.pending_beads.clear();
+        c.pending_beads.shrink_to_fit();
+        waiting_chains.pop_front();
+        if (!c.queue.empty()) ready.push_back(c.queue.front());
+    }
+}
+
+int pimc_server::launch_ready(std::unique_lock<std::mutex>& lk) {
+    (void)lk;
+    int si = -1;
+    for (int k = 0; k < n_slots; ++k)
+        if (!slots[k].busy) {
+            si = k;
+            break;
+        }
+    if (si < 0 || ready.empty()) return 0;
+    Slot& S = slots[si];
+    if (sync_chain_table(this, S.stream) != 0) {
+        kernel_error = PIMC_E_HIP;
+        kernel_error_msg = "chain table upload failed";
+        return -1;
+    }
+    std::vector<ServeReqDev> rq;
+    std::vector<VnnBead> bd;
+    std::vector<ServeGroupDev> gr;
+    S.reqs.clear();
+    while (!ready.empty() && (int)rq.size() < kMaxReqPerBatch) {
+        const uint64_t id = ready.front();
+        Request& R = reqs[id];
+        Chain& C = chains[R.chain];
+        const int chunks = (C.d.num_alloc + kVnnChunk - 1) / kVnnChunk;
+        const int nb = (int)R.beads.size();
+        if (!rq.empty() && ((int)bd.size() + nb > kMaxBeadsPerBatch || (int)gr.size() + nb * chunks > kMaxGroupsPerBatch))
+            break;
+        ready.pop_front();
+        ServeReqDev d{};
+        d.chain = C.dev_index;
+        d.n_beads = nb;
+        d.first_bead = (int)bd.size();
+        d.chunks = chunks;
+        d.first_group = (int)gr.size();
+        d.commit_mode = R.commit_mode;
+        d.shift = R.shift;
+        d.out_slot = (int)rq.size();
+        d.u = R.u;
+        d.vfactor0 = R.vf[0];
+        d.vfactor1 = R.vf[1];
+        for (int k = 0; k < nb; ++k) {
+            bd.push_back(R.beads[k]);
+            for (int c = 0; c < chunks; ++c) gr.push_back(ServeGroupDev{(int)rq.size(), d.first_bead + k, c});
+        }
+        rq.push_back(d);
+        S.reqs.push_back(id);
+        C.busy = true;
+        pairs_done += 2LL * nb * (C.d.num_alloc - 1);
+    }
+    if (rq.empty()) return 0;
+    const size_t b_rq = sizeof(ServeReqDev) * rq.size();
+    const size_t b_bd = sizeof(VnnBead) * bd.size();
+    const size_t b_gr = sizeof(ServeGroupDev) * gr.size();
+    const size_t o_bd = (b_rq + 255) & ~size_t(255);
+    const size_t o_gr = (o_bd + b_bd + 255) & ~size_t(255);
+    std::memcpy(S.h_desc, rq.data(), b_rq);
+    std::memcpy(S.h_desc + o_bd, bd.data(), b_bd);
+    std::memcpy(S.h_desc + o_gr, gr.data(), b_gr);
+    AzizDev az_l = az;
+    if (desc.v_mode == PIMC_V_TABLE) {
+        std::string e;
+        az_l.table = cached_table(desc.aziz_year, 0, (int)(max_table_sep / ((1.0E-6) * params.rm)) + 1, &e);
+    }
+    (void)hipGetLastError();
+    hipError_t e = hipMemcpyAsync(S.d_desc, S.h_desc, o_gr + b_gr, hipMemcpyHostToDevice, S.stream);
+    const ServeReqDev* d_rq = (const ServeReqDev*)S.d_desc;
+    const VnnBead* d_bd = (const VnnBead*)(S.d_desc + o_bd);
+    const ServeGroupDev* d_gr = (const ServeGroupDev*)(S.d_desc + o_gr);
+    if (e == hipSuccess) {
+        dim3 g((unsigned)gr.size()), b(kVnnThreads);
+        if (desc.v_mode == 0)
+            hipLaunchKernelGGL(k_serve_vnn<0>, g, b, 0, S.stream, d_gr, d_rq, d_bd, d_chains, az_l, kVnnChunk,
+                               S.d_partials);
+        else if (desc.v_mode == 1)
+            hipLaunchKernelGGL(k_serve_vnn<1>, g, b, 0, S.stream, d_gr, d_rq, d_bd, d_chains, az_l, kVnnChunk,
+                               S.d_partials);
+        else
+            hipLaunchKernelGGL(k_serve_vnn<2>, g, b, 0, S.stream, d_gr, d_rq, d_bd, d_chains, az_l, kVnnChunk,
+                               S.d_partials);
+        hipLaunchKernelGGL(k_serve_finalize, dim3((unsigned)rq.size()), dim3(64), 0, S.stream, d_rq, d_bd, d_chains,
+                           S.d_partials, S.d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(S.ev, S.stream);
+    if (e != hipSuccess) {
+        kernel_error = PIMC_E_HIP;
+        kernel_error_msg = std::string("serve launch: ") + hipGetErrorString(e);
+        return -1;
+    }
+    S.busy = true;
+    ++launched_batches;
+    launched_reqs += (int64_t)rq.size();
+    return 1;
+}
+
+void pimc_server::complete(Slot& S) {
+    const auto now = Clock::now();
+    for (size_t k = 0; k < S.reqs.size(); ++k) {
+        Request& R = reqs[S.reqs[k]];
+        for (int t = 0; t < 4; ++t) R.out[t] = S.h_out[4 * k + t];
+        R.done = true;
+        R.t_done = now;
+        if (track_done) done_list.push_back(R.id);
+        Chain& C = chains[R.chain];
+        C.busy = false;
+        if (!C.queue.empty() && C.queue.front() == R.id) C.queue.pop_front();
+        if (!C.queue.empty() && C.state == 1) ready.push_back(C.queue.front());
+    }
+    S.reqs.clear();
+    S.busy = false;
+}
+
+void pimc_server::run() {
+    std::unique_lock<std::mutex> lk(mu);
+    while (!stop.load()) {
+        bool progressed = false;
+        for (int k = 0; k < n_slots; ++k) {
+            Slot& S = slots[k];
+            if (!S.busy) continue;
+            hipError_t q = hipEventQuery(S.ev);
+            if (q == hipSuccess) {
+                complete(S);
+                progressed = true;
+            } else if (q != hipErrorNotReady) {
+                kernel_error = PIMC_E_HIP;
+                kernel_error_msg = std::string("batch failed: ") + hipGetErrorString(q);
+                complete(S);
+                progressed = true;
+            }
+        }
+        if (progressed) cv_done.notify_all();
+        int lr = 0;
+        while ((lr = launch_ready(lk)) > 0) progressed = true;
+        bool any_busy = false;
+        for (int k = 0; k < n_slots; ++k) any_busy |= slots[k].busy;
+        if (!progressed) {
+            if (any_busy) {
+                // batches in flight: spin briefly without the lock so submitters get in
+                lk.unlock();
+                std::this_thread::yield();
+                lk.lock();
+            } else {
+                cv_work.wait_for(lk, std::chrono::milliseconds(1));
+            }
+        }
+    }
+}
+
+extern "C" {
+
+int pimc_server_create(const pimc_server_desc* d, pimc_server** out) {
+    if (!d || !out) return PIMC_E_ARG;
+    *out = nullptr;
+    std::unique_ptr<pimc_server> s(new pimc_server());
+    s->desc = *d;
+    if (!aziz_params(d->aziz_year, &s->params)) return PIMC_E_ARG;
+    if (d->v_mode < 0 || d->v_mode > 2 || d->hbm_budget < 4096) return PIMC_E_ARG;
+    s->n_slots = std::max(1, std::min(kSlots, d->streams <= 0 ? 2 : d->streams));
+    s->max_table_sep = d->max_side > 0 ? std::sqrt(3.0) * d->max_side / 2.0 : 200.0;
+    s->az = make_aziz_dev(s->params, s->max_table_sep);
+    if (hipMalloc(&s->arena.base, d->hbm_budget) != hipSuccess) return PIMC_E_NOMEM;
+    s->arena.size = d->hbm_budget;
+    s->arena.free_[0] = d->hbm_budget;
+    for (int k = 0; k < kSlots; ++k) {
+        Slot& S = s->slots[k];
+        if (hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking) != hipSuccess) return PIMC_E_HIP;
+        if (hipEventCreateWithFlags(&S.ev, hipEventDisableTiming) != hipSuccess) return PIMC_E_HIP;
+        S.desc_bytes = sizeof(ServeReqDev) * kMaxReqPerBatch + sizeof(VnnBead) * kMaxBeadsPerBatch +
+                       sizeof(ServeGroupDev) * kMaxGroupsPerBatch + 1024;
+        if (hipHostMalloc((void**)&S.h_desc, S.desc_bytes, hipHostMallocDefault) != hipSuccess) return PIMC_E_NOMEM;
+        if (hipMalloc(&S.d_desc, S.desc_bytes) != hipSuccess) return PIMC_E_NOMEM;
+        if (hipMalloc(&S.d_partials, sizeof(double) * 2 * kMaxGroupsPerBatch) != hipSuccess) return PIMC_E_NOMEM;
+        if (hipHostMalloc((void**)&S.h_out, sizeof(double) * 4 * kMaxReqPerBatch,
+                          hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+            return PIMC_E_NOMEM;
+        if (hipHostGetDevicePointer((void**)&S.d_out, S.h_out, 0) != hipSuccess) return PIMC_E_HIP;
+    }
+    pimc_server* raw = s.get();
+    raw->worker = std::thread([raw] { raw->run(); });
+    *out = s.release();
+    return PIMC_OK;
+}
+
+void pimc_server_destroy(pimc_server* s) {
+    if (!s) return;
+    s->stop.store(true);
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (auto& S : s->slots) {
+        if (S.stream) (void)hipStreamSynchronize(S.stream);
+        if (S.ev) (void)hipEventDestroy(S.ev);
+        if (S.h_desc) (void)hipHostFree(S.h_desc);
+        if (S.d_desc) (void)hipFree(S.d_desc);
+        if (S.d_partials) (void)hipFree(S.d_partials);
+        if (S.h_out) (void)hipHostFree(S.h_out);
+        if (S.stream) (void)hipStreamDestroy(S.stream);
+    }
+    if (s->d_chains) (void)hipFree(s->d_chains);
+    if (s->arena.base) (void)hipFree(s->arena.base);
+    delete s;
+}
+
+const char* pimc_server_last_error(const pimc_server* s) { return s ? s->err.c_str() : ""; }
+
+int pimc_server_add_chain(pimc_server* s, const pimc_desc* d, const double* beads, int* chain_id) {
+    if (!s || !d || !beads || !chain_id) return PIMC_E_ARG;
+    if (d->num_slices < 1 || d->num_alloc < 2 || !(d->side > 0) || !(d->tau > 0) || !(d->rc > 0))
+        return sfail(s, PIMC_E_ARG, "invalid chain description");
+    if (d->aziz_year != s->desc.aziz_year) return sfail(s, PIMC_E_ARG, "chain Aziz year differs from the server's");
+    if (std::sqrt(3.0) * d->side / 2.0 > s->max_table_sep) return sfail(s, PIMC_E_ARG, "box larger than max_side");
+    std::lock_guard<std::mutex> g(s->mu);
+    Chain c;
+    c.d = *d;
+    c.bytes = chain_bytes(*d);
+    if (c.bytes > s->arena.size) return sfail(s, PIMC_E_NOMEM, "chain larger than the HBM budget");
+    const int id = (int)s->chains.size();
+    s->chains.push_back(c);
+    const bool queue_empty = s->waiting_chains.empty();
+    int r = queue_empty ? place_chain(s, id, beads) : 1;
+    if (r < 0) return sfail(s, PIMC_E_HIP, "chain upload failed");
+    if (r > 0) {
+        Chain& cc = s->chains[id];
+        cc.pending_beads.assign(beads, beads + 3 * (size_t)d->num_slices * d->num_alloc);
+        s->waiting_chains.push_back(id);
+    }
+    *chain_id = id;
+    return PIMC_OK;
+}
+
+int pimc_server_chain_state(pimc_server* s, int chain, int* state) {
+    if (!s || !state) return PIMC_E_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (chain < 0 || chain >= (int)s->chains.size()) return PIMC_E_ARG;
+    *state = s->chains[chain].state;
+    return PIMC_OK;
+}
+
+int pimc_server_remove_chain(pimc_server* s, int chain) {
+    if (!s) return PIMC_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (chain < 0 || chain >= (int)s->chains.size()) return PIMC_E_ARG;
+    Chain& c = s->chains[chain];
+    s->cv_done.wait(lk, [&] { return !c.busy && c.queue.empty(); });
+    if (c.state == 1) {
+        s->arena.release(c.off, c.bytes);
+        s->chain_table[c.dev_index].pos = nullptr;
+        s->chains_dirty = true;
+    }
+    c.state = 2;
+    s->admit_waiting();
+    s->cv_work.notify_all();
+    return PIMC_OK;
+}
+
+int pimc_server_download(pimc_server* s, int chain, double* beads) {
+    if (!s || !beads) return PIMC_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (chain < 0 || chain >= (int)s->chains.size()) return PIMC_E_ARG;
+    Chain& c = s->chains[chain];
+    s->cv_done.wait(lk, [&] { return !c.busy && c.queue.empty(); });
+    const size_t MN = (size_t)c.d.num_slices * c.d.num_alloc;
+    if (c.state == 0) {
+        std::memcpy(beads, c.pending_beads.data(), sizeof(double) * 3 * MN);
+        return PIMC_OK;
+    }
+    if (c.state != 1) return sfail(s, PIMC_E_STATE, "chain removed");
+    std::vector<double> soa(3 * MN);
+    if (hipMemcpy(soa.data(), c.dev.pos, sizeof(double) * 3 * MN, hipMemcpyDeviceToHost) != hipSuccess)
+        return PIMC_E_HIP;
+    for (size_t t = 0; t < MN; ++t) {
+        beads[3 * t] = soa[t];
+        beads[3 * t + 1] = soa[t + MN];
+        beads[3 * t + 2] = soa[t + 2 * MN];
+    }
+    return PIMC_OK;
+}
+
+int pimc_server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos, int n,
+                       int shift, const double vfactor[2], int commit_mode, double u, uint64_t* req_id) {
+    if (!s || !slice_ptcl || !new_pos || n < 1 || !req_id || shift < 1 || commit_mode < 0 || commit_mode > 2)
+        return PIMC_E_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
+    if (chain < 0 || chain >= (int)s->chains.size() || s->chains[chain].state == 2)
+        return sfail(s, PIMC_E_ARG, "unknown chain");
+    Chain& c = s->chains[chain];
+    if (n > kMaxBeadsPerBatch / 4) return sfail(s, PIMC_E_ARG, "too many beads in one request");
+    std::vector<char> seen(c.d.num_slices, 0);
+    Request R;
+    R.id = s->next_id++;
+    R.chain = chain;
+    R.shift = shift;
+    R.vf[0] = vfactor ? vfactor[0] : 1.0;
+    R.vf[1] = vfactor ? vfactor[1] : 1.0;
+    R.commit_mode = commit_mode;
+    R.u = u;
+    R.beads.resize(n);
+    for (int k = 0; k < n; ++k) {
+        const int m = slice_ptcl[2 * k], i = slice_ptcl[2 * k + 1];
+        if (m < 0 || m >= c.d.num_slices || i < 0 || i >= c.d.num_alloc)
+            return sfail(s, PIMC_E_ARG, "bead locator out of range");
+        if (seen[m]) return sfail(s, PIMC_E_ARG, "request beads must sit on distinct slices");
+        seen[m] = 1;
+        VnnBead& b = R.beads[k];
+        b.walker = chain;
+        b.slice = m;
+        b.ptcl = i;
+        b.flags = 2;
+        for (int t = 0; t < 3; ++t) {
+            b.old_pos[t] = 0.0;
+            b.new_pos[t] = new_pos[3 * k + t];
+        }
+    }
+    R.t_submit = Clock::now();
+    const uint64_t id = R.id;
+    s->reqs.emplace(id, std::move(R));
+    c.queue.push_back(id);
+    if (c.queue.size() == 1 && c.state == 1 && !c.busy) s->ready.push_back(id);
+    *req_id = id;
+    s->cv_work.notify_one();
+    return PIMC_OK;
+}
+
+// out5 = {A_old, A_new, dU, accepted, latency_us}; releases the request.
+int pimc_server_wait(pimc_server* s, uint64_t req_id, double* out5) {
+    if (!s) return PIMC_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return sfail(s, PIMC_E_ARG, "unknown request");
+    s->cv_done.wait(lk, [&] { return it->second.done || s->kernel_error; });
+    if (!it->second.done) return sfail(s, s->kernel_error, s->kernel_error_msg);
+    if (out5) {
+        for (int t = 0; t < 4; ++t) out5[t] = it->second.out[t];
+        out5[4] = std::chrono::duration<double, std::micro>(it->second.t_done - it->second.t_submit).count();
+    }
+    s->reqs.erase(it);
+    return PIMC_OK;
+}
+
+int pimc_server_poll(pimc_server* s, uint64_t req_id, int* done) {
+    if (!s || !done) return PIMC_E_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return sfail(s, PIMC_E_ARG, "unknown request");
+    *done = it->second.done ? 1 : 0;
+    return PIMC_OK;
+}
+
+int pimc_server_stats(pimc_server* s, int64_t* batches, int64_t* requests, int64_t* pairs, size_t* free_bytes) {
+    if (!s) return PIMC_E_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (batches) *batches = s->launched_batches;
+    if (requests) *requests = s->launched_reqs;
+    if (pairs) *pairs = s->pairs_done;
+    if (free_bytes) *free_bytes = s->arena.free_bytes();
+    return PIMC_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Synthetic load: Poisson arrivals of Delta V requests over the resident chains.
+// ===========================================================================
+namespace {
+
+struct GenChain;
+std::vector<int> pending_chains_scan(const std::vector<GenChain>& gc);
+
+struct GenChain {
+    std::vector<double> mirror;        // [M][N][3], kept equal to the device copy
+    std::deque<Clock::time_point> arrivals;
+    uint64_t inflight = 0;
+    Clock::time_point t_arrival;
+    std::vector<int32_t> sp;
+    std::vector<double> prop;
+};
+
+std::vector<int> pending_chains_scan(const std::vector<GenChain>& gc) {
+    std::vector<int> out;
+    for (int c = 0; c < (int)gc.size(); ++c)
+        if (!gc[c].inflight && !gc[c].arrivals.empty()) out.push_back(c);
+    return out;
+}
+
+}  // namespace
+
+extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L, double* stats) {
+    if (!s || !L || !stats || L->n_chains < 1 || L->duration_s <= 0 || L->seg_min < 2 || L->seg_max < L->seg_min)
+        return PIMC_E_ARG;
+    const int nc = L->n_chains;
+    std::vector<GenChain> gc(nc);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (nc > (int)s->chains.size()) return sfail(s, PIMC_E_ARG, "more load chains than server chains");
+        s->track_done = true;
+        s->done_list.clear();
+    }
+    struct TrackOff {
+        pimc_server* s;
+        ~TrackOff() {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->track_done = false;
+            s->done_list.clear();
+        }
+    } track_off{s};
+    for (int c = 0; c < nc; ++c) {
+        const pimc_desc& d = s->chains[c].d;
+        gc[c].mirror.resize(3 * (size_t)d.num_slices * d.num_alloc);
+        int r = pimc_server_download(s, c, gc[c].mirror.data());
+        if (r) return r;
+    }
+    std::mt19937_64 rng(L->seed);
+    std::uniform_real_distribution<double> U(0.0, 1.0);
+    std::normal_distribution<double> G(0.0, 1.0);
+    std::exponential_distribution<double> Exp(L->rate > 0 ? L->rate : 1.0);
+    const bool closed = L->rate <= 0;
+    std::vector<double> lat;
+    lat.reserve(1 << 20);
+    int64_t completed = 0, accepted = 0, pairs = 0;
+    int64_t b0 = 0, r0 = 0;
+    pimc_server_stats(s, &b0, &r0, nullptr, nullptr);
+    const auto t0 = Clock::now();
+    const auto t_end = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(L->duration_s));
+    auto next_arrival = t0;
+    int inflight = 0;
+    auto propose = [&](int c) {
+        GenChain& g = gc[c];
+        const pimc_desc& d = s->chains[c].d;
+        const int M = d.num_slices, N = d.num_alloc;
+        g.sp.clear();
+        g.prop.clear();
+        const int p = (int)(U(rng) * N) % N;
+        if (U(rng) < L->p_com) {
+            // CenterOfMass: the whole worldline of p shifted by one Gaussian vector
+            double sh[3];
+            for (double& x : sh) x = L->com_sigma * G(rng);
+            for (int m = 0; m < M; ++m) {
+                g.sp.push_back(m);
+                g.sp.push_back(p);
+                for (int t = 0; t < 3; ++t) {
+                    double x = g.mirror[3 * ((size_t)m * N + p) + t] + sh[t];
+                    x -= d.side * std::floor(x * (1.0 / d.side) + 0.5);
+                    g.prop.push_back(x);
+                }
+            }
+        } else {
+            int seg = L->seg_min + (int)(U(rng) * (L->seg_max - L->seg_min + 1));
+            seg = std::min(seg, M);
+            const int m0 = (int)(U(rng) * M) % M;
+            const double sd = std::sqrt(L->lambda * d.tau);
+            for (int k = 1; k < seg; ++k) {
+                const int m = (m0 + k) % M;
+                g.sp.push_back(m);
+                g.sp.push_back(p);
+                for (int t = 0; t < 3; ++t) {
+                    double x = g.mirror[3 * ((size_t)m * N + p) + t] + sd * G(rng);
+                    x -= d.side * std::floor(x * (1.0 / d.side) + 0.5);
+                    g.prop.push_back(x);
+                }
+            }
+        }
+    };
+    std::unordered_map<uint64_t, int> by_id;
+    auto submit = [&](int c, Clock::time_point t_arr) -> int {
+        GenChain& g = gc[c];
+        propose(c);
+        uint64_t id = 0;
+        const double vf[2] = {1.0, 1.0};
+        int r = pimc_server_submit(s, c, g.sp.data(), g.prop.data(), (int)g.sp.size() / 2, 1, vf, 1, U(rng), &id);
+        if (r) return r;
+        g.inflight = id;
+        g.t_arrival = t_arr;
+        by_id[id] = c;
+        ++inflight;
+        return 0;
+    };
+    if (closed)
+        for (int c = 0; c < nc; ++c) {
+            int r = submit(c, Clock::now());
+            if (r) return r;
+        }
+    while (true) {
+        const auto now = Clock::now();
+        const bool open = now < t_end;
+        if (!closed && open) {
+            while (next_arrival <= now) {
+                const int c = (int)(U(rng) * nc) % nc;
+                gc[c].arrivals.push_back(next_arrival);
+                next_arrival += std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(Exp(rng)));
+            }
+        }
+        // completions, in the order the server finished them
+        std::vector<uint64_t> finished;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            finished.swap(s->done_list);
+            if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
+        }
+        for (uint64_t id : finished) {
+            auto it = by_id.find(id);
+            if (it == by_id.end()) continue;
+            const int c = it->second;
+            by_id.erase(it);
+            GenChain& g = gc[c];
+            double o[5];
+            int r = pimc_server_wait(s, id, o);
+            if (r) return r;
+            const auto t_done = Clock::now();
+            lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
+            ++completed;
+            const pimc_desc& d = s->chains[c].d;
+            pairs += 2LL * ((int64_t)g.sp.size() / 2) * (d.num_alloc - 1);
+            if (o[3] > 0.5) {
+                ++accepted;
+                for (size_t k = 0; k < g.sp.size() / 2; ++k) {
+                    const size_t bi = (size_t)g.sp[2 * k] * d.num_alloc + g.sp[2 * k + 1];
+                    for (int t = 0; t < 3; ++t) g.mirror[3 * bi + t] = g.prop[3 * k + t];
+                }
+            }
+            g.inflight = 0;
+            --inflight;
+            if (closed && open) {
+                r = submit(c, Clock::now());
+                if (r) return r;
+            }
+        }
+        if (!closed)
+            for (int c : pending_chains_scan(gc)) {
+                GenChain& g = gc[c];
+                if (!g.inflight && !g.arrivals.empty()) {
+                    const auto ta = g.arrivals.front();
+                    g.arrivals.pop_front();
+                    int r = submit(c, ta);
+                    if (r) return r;
+                }
+            }
+        if (finished.empty()) std::this_thread::yield();
+        if (!open && inflight == 0) break;
+        if (!open && now - t_end > std::chrono::seconds(30))
+            return sfail(s, PIMC_E_STATE, "synthetic load: requests still pending 30 s after the load window");
+        if (!open) {
+            // stop accepting arrivals; drop the ones not yet submitted
+            for (auto& g : gc) g.arrivals.clear();
+        }
+    }
+    const double secs = std::chrono::duration<double>(Clock::now() - t0).count();
+    int64_t b1 = 0, r1 = 0;
+    pimc_server_stats(s, &b1, &r1, nullptr, nullptr);
+    std::sort(lat.begin(), lat.end());
+    auto pct = [&](double q) { return lat.empty() ? 0.0 : lat[std::min(lat.size() - 1, (size_t)(q * (lat.size() - 1)))]; };
+    double mean = 0;
+    for (double x : lat) mean += x;
+    mean = lat.empty() ? 0 : mean / lat.size();
+    stats[0] = (double)completed;
+    stats[1] = completed / secs;
+    stats[2] = pairs / secs;
+    stats[3] = pct(0.50);
+    stats[4] = pct(0.99);
+    stats[5] = mean;
+    stats[6] = completed ? (double)accepted / completed : 0.0;
+    stats[7] = (double)(b1 - b0);
+    stats[8] = (b1 > b0) ? (double)(r1 - r0) / (b1 - b0) : 0.0;
+    stats[9] = secs;
+    return PIMC_OK;
+}

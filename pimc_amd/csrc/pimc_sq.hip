@@ -163,56 +163,51 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_general(RhoArgs A) {
     A.rho[((int64_t)m * A.nq + qi) * 2 + 1] = im;
 }
 
-// F(q,tau) = sum_m Re rho(m) conj(rho(m+tau)).  A workgroup stages kIsfQ
-// q-vectors (q-major, rows of M) with coalesced reads of rho[m][q..q+kIsfQ);
-// each wave then takes kIsfQ/4 of them, lanes over lags: rho(m) is an LDS
-// broadcast and rho(m+tau) consecutive words.
-constexpr int kIsfQ = 8;
+// F(q,tau) = sum_m Re rho(m) conj(rho(m+tau)).  A workgroup of 4 waves takes
+// kIsfQ = 4 q-vectors, one per wave; rho_q(m) is staged twice over (m and
+// m + M) so lag tau reads row[m + tau] with no wrap test: rho(m) is an LDS
+// broadcast, rho(m + tau) consecutive words, each lag at a fixed offset.  Rows
+// are 2M + 64 long: lanes past the last lag read the zero padding.
+constexpr int kIsfQ = 4;
 constexpr int kIsfLagsPerLane = 8;   // M <= 512
 __global__ __launch_bounds__(256) void k_isf_from_rho(const double* __restrict__ rho, int nq, int M,
                                                      double inv_n, double* __restrict__ fqt) {
-    extern __shared__ double sr[];  // [kIsfQ][M][2]
+    extern __shared__ double2 sr[];  // [kIsfQ][2M + 64]
+    const int R = 2 * M + 64;
     const int qb = blockIdx.x * kIsfQ;
     const int nql = min(kIsfQ, nq - qb);
     for (int t = threadIdx.x; t < M * kIsfQ; t += 256) {
         const int m = t / kIsfQ, k = t % kIsfQ;
-        double re = 0.0, im = 0.0;
-        if (k < nql) {
-            re = rho[((int64_t)m * nq + qb + k) * 2];
-            im = rho[((int64_t)m * nq + qb + k) * 2 + 1];
-        }
-        sr[((int64_t)k * M + m) * 2] = re;
-        sr[((int64_t)k * M + m) * 2 + 1] = im;
+        double2 v = make_double2(0.0, 0.0);
+        if (k < nql) v = make_double2(rho[((int64_t)m * nq + qb + k) * 2], rho[((int64_t)m * nq + qb + k) * 2 + 1]);
+        sr[(int64_t)k * R + m] = v;
+        sr[(int64_t)k * R + m + M] = v;
     }
+    for (int t = threadIdx.x; t < 64 * kIsfQ; t += 256) sr[(int64_t)(t / 64) * R + 2 * M + (t % 64)] = make_double2(0.0, 0.0);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int k = wave; k < nql; k += 4) {
-        const double* row = sr + (int64_t)k * M * 2;
-        double acc[kIsfLagsPerLane];
-        int idx[kIsfLagsPerLane];
+    const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (k >= nql) return;
+    const double2* row = sr + (int64_t)k * R;
+    double acc[kIsfLagsPerLane];
+#pragma unroll
+    for (int u = 0; u < kIsfLagsPerLane; ++u) acc[u] = 0.0;
+    const int nu = (M + 63) / 64;
+    const double2* b = row + lane;
+    for (int m = 0; m < M; ++m, ++b) {
+        const double2 a = row[m];
 #pragma unroll
         for (int u = 0; u < kIsfLagsPerLane; ++u) {
-            acc[u] = 0.0;
-            const int tau = lane + 64 * u;
-            idx[u] = tau < M ? tau : 0;
-        }
-        for (int m = 0; m < M; ++m) {
-            const double ar = row[2 * m], ai = row[2 * m + 1];
-#pragma unroll
-            for (int u = 0; u < kIsfLagsPerLane; ++u) {
-                if (64 * u < M) {
-                    const double br = row[2 * idx[u]], bi = row[2 * idx[u] + 1];
-                    acc[u] = fma(ar, br, acc[u]);
-                    acc[u] = fma(ai, bi, acc[u]);
-                    idx[u] = (idx[u] + 1 == M) ? 0 : idx[u] + 1;
-                }
+            if (u < nu) {
+                const double2 c = b[64 * u];
+                acc[u] = fma(a.x, c.x, acc[u]);
+                acc[u] = fma(a.y, c.y, acc[u]);
             }
         }
+    }
 #pragma unroll
-        for (int u = 0; u < kIsfLagsPerLane; ++u) {
-            const int tau = lane + 64 * u;
-            if (tau < M) fqt[(int64_t)(qb + k) * M + tau] = acc[u] * inv_n;
-        }
+    for (int u = 0; u < kIsfLagsPerLane; ++u) {
+        const int tau = lane + 64 * u;
+        if (u < nu && tau < M) fqt[(int64_t)(qb + k) * M + tau] = acc[u] * inv_n;
     }
 }
 
@@ -296,7 +291,15 @@ hipError_t launch_isf(hipStream_t st, const double* rho, int nq, int M, double i
         hipLaunchKernelGGL(k_ssf_from_rho, dim3((nq + 255) / 256), dim3(256), 0, st, rho, nq, M, inv_n, fqt);
     } else {
         dim3 g((nq + kIsfQ - 1) / kIsfQ), b(256);
-        const size_t lds = sizeof(double) * 2 * kIsfQ * (size_t)M;
+        const size_t lds = sizeof(double2) * kIsfQ * (2 * (size_t)M + 64);
+        static bool attr_set = false;
+        if (!attr_set) {
+            // rows of M <= 512 slices need up to 68 KB of the 160 KB LDS
+            hipError_t e = hipFuncSetAttribute((const void*)k_isf_from_rho,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            if (e != hipSuccess) return e;
+            attr_set = true;
+        }
         hipLaunchKernelGGL(k_isf_from_rho, g, b, lds, st, rho, nq, M, inv_n, fqt);
     }
     return hipGetLastError();

@@ -1,0 +1,170 @@
+"""GPU tests of the batched and serving forms.
+
+Batched: per-walker potentialAction() of uneven walkers against the oracle
+(rel <= 1e-6, mixed mode), a walker's result bit-identical alone and in the mix,
+hipGraph replays bit-identical to eager launches with eager work in between.
+Serving: Delta V against the oracle, results bit-identical alone and in the mix,
+commit / Metropolis semantics, the HBM-budget wait queue, a short synthetic load."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DENSITY, LAMBDA, SEED0 = 0.02198, 6.0596, 139853
+
+
+def walker(orc, N, M, T, seed, kind=1):
+    L = orc.box_side(N, DENSITY)
+    tau = 1.0 / (M * T)
+    b = orc.gen_worldlines(seed, L, M, N, LAMBDA, tau, kind).astype(np.float32).astype(np.float64)
+    b -= L * np.floor(b / L + 0.5)
+    return dict(side=L, M=M, N=N, tau=tau, lam=LAMBDA, rc=L), b
+
+
+def batch_set(orc, n, seed=7):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        N = int(rng.choice([16, 32, 64, 128]))
+        M = int(rng.choice([8, 16, 24]))
+        T = float(rng.uniform(1.0, 3.0))
+        out.append(walker(orc, N, M, T, SEED0 + k))
+    return out
+
+
+def oracle_action(orc, w, b):
+    st = orc.State(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"])
+    st.set_beads(b)
+    return st.total_action()
+
+
+def test_batch_against_oracle_and_mix(orc):
+    import pimc_amd
+    ws = batch_set(orc, 12)
+    bt = pimc_amd.Batch([w for w, _ in ws])
+    for k, (_, b) in enumerate(ws):
+        bt.upload(k, b)
+    a = bt.potentialAction()
+    for k, (w, b) in enumerate(ws):
+        ref = oracle_action(orc, w, b)
+        assert abs(a[k] - ref) <= 1e-6 * abs(ref), (k, a[k], ref)
+    # alone vs in the mix, same tiles: bit-identical
+    for k in (0, 5, 11):
+        solo = pimc_amd.Batch([ws[k][0]])
+        solo.upload(0, ws[k][1])
+        assert solo.potentialAction()[0] == a[k]
+
+
+def test_batch_graph_replays(orc):
+    import pimc_amd
+    ws = batch_set(orc, 8, seed=3)
+    bt = pimc_amd.Batch([w for w, _ in ws])
+    for k, (_, b) in enumerate(ws):
+        bt.upload(k, b)
+    eager = bt.potentialAction()
+    bt.use_graph(True)
+    ctx = pimc_amd.Context(ws[0][0]["side"], ws[0][0]["M"], ws[0][0]["N"], ws[0][0]["tau"], LAMBDA, ws[0][0]["rc"])
+    ctx.upload_path(ws[0][1])
+    for rep in range(4):
+        g = bt.potentialAction()
+        assert np.array_equal(g, eager), rep
+        ctx.potentialAction()          # eager work between replays
+        pimc_amd.device_synchronize()
+    bt.set_tiles(16, 64)
+    g2 = bt.potentialAction()
+    for k, (w, b) in enumerate(ws):
+        assert abs(g2[k] - eager[k]) <= 1e-12 * abs(eager[k])
+
+
+def test_batch_autotune_small(orc):
+    import pimc_amd
+    ws = batch_set(orc, 4, seed=5)
+    bt = pimc_amd.Batch([w for w, _ in ws])
+    for k, (_, b) in enumerate(ws):
+        bt.upload(k, b)
+    ref = bt.potentialAction()
+    best = bt.autotune(reps=1)
+    assert best["candidates"] == 20 and best["ms"] > 0
+    after = bt.potentialAction()
+    assert np.allclose(after, ref, rtol=1e-12)
+
+
+def _segment(rng, M, N, L, beads, seg):
+    start, p = int(rng.integers(M)), int(rng.integers(N))
+    sp = np.array([[(start + k) % M, p] for k in range(1, seg)], dtype=np.int32)
+    new = beads[sp[:, 0], sp[:, 1]] + rng.normal(scale=0.3, size=(seg - 1, 3))
+    new -= L * np.floor(new / L + 0.5)
+    return sp, new
+
+
+def test_server_delta_v_and_mix(orc):
+    import pimc_amd
+    chains = [walker(orc, N, M, 1.5, SEED0 + k, kind=0) for k, (N, M) in enumerate([(64, 32), (128, 16), (32, 40)])]
+    srv = pimc_amd.Server(1 << 28, streams=2, max_side=40.0)
+    ids = [srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b) for w, b in chains]
+    rng = np.random.default_rng(1)
+    w0, b0 = chains[0]
+    sp, new = _segment(rng, w0["M"], w0["N"], w0["side"], b0, 12)
+    alone = srv.wait(srv.submit(ids[0], sp, new))
+    st = orc.State(w0["side"], w0["M"], w0["N"], w0["tau"], LAMBDA, w0["rc"])
+    st.set_beads(b0)
+    old = st.action_list(sp)
+    for (m, i), q in zip(sp, new):
+        st.update_bead(int(m), int(i), q)
+    newa = st.action_list(sp)
+    scale = abs(old) + abs(newa)
+    assert abs(alone["A_old"] - old) <= 1e-6 * scale
+    assert abs(alone["dU"] - (newa - old)) <= 1e-6 * scale
+    # the same request in a mix of many others
+    others = []
+    for rep in range(30):
+        k = 1 + rep % 2
+        w, b = chains[k]
+        s2, n2 = _segment(rng, w["M"], w["N"], w["side"], b, int(rng.integers(4, 12)))
+        others.append(srv.submit(ids[k], s2, n2))
+    mixed = srv.submit(ids[0], sp, new)
+    res = [srv.wait(r) for r in others]
+    m = srv.wait(mixed)
+    assert np.array_equal(m["raw"][:4], alone["raw"][:4])
+    assert all(np.isfinite(r["dU"]) for r in res)
+
+
+def test_server_commit_metropolis_and_budget(orc):
+    import pimc_amd
+    w, b = walker(orc, 64, 16, 1.5, SEED0, kind=0)
+    need = 16 * 64 * (24 + 4 + 1) + 3 * 256
+    srv = pimc_amd.Server(int(need * 1.5), streams=2, max_side=40.0)
+    c0 = srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b)
+    c1 = srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b)
+    assert srv.chain_state(c0) == 1 and srv.chain_state(c1) == 0
+    rng = np.random.default_rng(2)
+    sp, new = _segment(rng, 16, 64, w["side"], b, 6)
+    r1 = srv.submit(c1, sp, new, commit_mode=2)          # queued until c1 is admitted
+    assert not srv.poll(r1)
+    r0 = srv.wait(srv.submit(c0, sp, new, commit_mode=1, u=0.0))
+    assert r0["accepted"] == (0.0 < np.exp(-r0["dU"]))
+    srv.remove_chain(c0)
+    out = srv.wait(r1)
+    assert out["accepted"] and srv.chain_state(c1) == 1
+    got = srv.download(c1)
+    exp = b.copy()
+    exp[sp[:, 0], sp[:, 1]] = new
+    assert np.array_equal(got, exp)
+    # u = 1 rejects unless dU < 0
+    r = srv.wait(srv.submit(c1, sp, b[sp[:, 0], sp[:, 1]], commit_mode=1, u=1.0))
+    assert r["accepted"] == (1.0 < np.exp(-r["dU"]))
+
+
+def test_server_synthetic_load(orc):
+    import pimc_amd
+    srv = pimc_amd.Server(1 << 30, streams=4, max_side=40.0)
+    rng = np.random.default_rng(3)
+    for k in range(32):
+        N = int(rng.integers(32, 129))
+        M = int(rng.integers(64, 129))
+        w, b = walker(orc, N, M, float(rng.uniform(1, 3)), SEED0 + k, kind=1)
+        srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b)
+    st = srv.run_synthetic(32, rate=20000.0, duration_s=0.5)
+    assert st["completed"] > 100 and st["p99_us"] >= st["p50_us"] > 0
+    st2 = srv.run_synthetic(32, rate=0.0, duration_s=0.3)
+    assert st2["completed"] > 100 and st2["req_per_batch"] >= 1.0

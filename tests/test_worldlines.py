@@ -21,3 +21,13 @@ def test_lattice_q_matches_reference_order(orc):
     for N, kind, text, kw in ((16, "max_int", "4 4 4", dict(nmax=4)), (256, "max_float", "3 0 0", dict(qmag_max=3.0))):
         L = box_side(N)
         assert np.array_equal(lattice_q(L, **kw), orc.qvectors(L, kind, text))
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_native_generator_matches_oracle(orc, kind):
+    from pimc_amd import api
+    for (N, M) in ((16, 125), (256, 20), (7, 3)):
+        L = orc.box_side(N, 0.02198)
+        a = api.gen_worldlines(139853 + N, L, M, N, 6.0596, 0.004, kind)
+        b = orc.gen_worldlines(139853 + N, L, M, N, 6.0596, 0.004, kind)
+        assert np.array_equal(a, b)

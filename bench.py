@@ -115,7 +115,7 @@ def main():
     pa_avg_s = pa_ms / max(pa_n, 1) / 1e3
     achieved = pairs_full * F_PAIR / pa_avg_s / 1e12
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "pair_action_c2_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "pair_action_n256_m167_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")

@@ -49,7 +49,22 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_adapter_check()
     return LIB
+
+
+def _build_adapter_check():
+    """tests/cpp/adapter_check: the C++ adapter (include/pimc_gpu.hpp) as a caller uses it."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "adapter_check.cpp")
+    if not os.path.exists(src):
+        return
+    out = os.path.join(root, "tests", "cpp", "adapter_check")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src, "-o", out,
+           "-L", HERE, "-lpimc_gpu", "-Wl,-rpath,$ORIGIN/../../pimc_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"adapter_check build failed:\n{r.stderr}")
 
 
 if __name__ == "__main__":

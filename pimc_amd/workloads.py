@@ -15,12 +15,9 @@ def box_side(n, density=DENSITY):
 
 
 WORKLOADS = {
-    # plain single walker, small: N=16, T=2 K, tau=0.004, M=125
-    "C1": dict(N=16, M=125, T=2.0, tau=0.004, rc_frac=1.0, q=("max_int", "4 4 4")),
-    # single walker, larger: N=256, T=1.5 K, tau=0.004, M=167
-    "C2": dict(N=256, M=167, T=1.5, tau=0.004, rc_frac=1.0, q=("max_float", "3 0 0")),
-    # large single system: N=2048, T=1 K, tau=0.004, M=250, rc = L/2
-    "C5": dict(N=2048, M=250, T=1.0, tau=0.004, rc_frac=0.5, q=("max_float", "4 0 0")),
+    "small_n16_m125": dict(N=16, M=125, T=2.0, tau=0.004, rc_frac=1.0, q=("max_int", "4 4 4")),
+    "walker_n256_m167": dict(N=256, M=167, T=1.5, tau=0.004, rc_frac=1.0, q=("max_float", "3 0 0")),
+    "large_n2048_m250": dict(N=2048, M=250, T=1.0, tau=0.004, rc_frac=0.5, q=("max_float", "4 0 0")),
 }
 
 

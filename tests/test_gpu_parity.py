@@ -33,20 +33,20 @@ def setup_pair(orc, gpu, N, M, tau, rc_frac=1.0, seed=SEED0, kind=0, mode=0):
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])
-def test_total_action_c1_seeds(orc, gpu, mode):
-    """Workload C1: N=16, M=125, tau=0.004; 100 seeded configurations."""
+def test_total_action_n16_m125_seeds(orc, gpu, mode):
+    """Small walker N=16, M=125, tau=0.004; 100 seeded configurations."""
     worst = 0.0
     for k in range(100):
         ref, ctx, _, _ = setup_pair(orc, gpu, 16, 125, 0.004, seed=SEED0 + k, mode=mode)
         a, b = ctx.potentialAction(), ref.total_action()
         worst = max(worst, abs(a - b) / abs(b))
-    print(f"C1 mode {mode}: worst rel err {worst:.3e}")
+    print(f"N16 M125 mode {mode}: worst rel err {worst:.3e}")
     assert worst <= TOL[mode]
 
 
 @pytest.mark.parametrize("mode", [0, 2])
-def test_total_action_and_slices_c2(orc, gpu, mode):
-    """Workload C2: N=256, M=167, tau=0.004."""
+def test_total_action_and_slices_n256_m167(orc, gpu, mode):
+    """Single walker N=256, M=167, tau=0.004."""
     ref, ctx, _, _ = setup_pair(orc, gpu, 256, 167, 0.004, mode=mode)
     a, vs = ctx.potentialAction(with_slices=True)
     b = ref.total_action()
@@ -82,8 +82,8 @@ def _bisection_case(orc, gpu, N, M, seg, seed, mode=0, rc_frac=1.0):
 
 @pytest.mark.parametrize("seg", [8, 16, 32])
 @pytest.mark.parametrize("mode", [0, 2])
-def test_delta_v_bisection_c2(orc, gpu, seg, mode):
-    """Bisection segments of 8/16/32 slices (7/15/31 moved beads) on C2."""
+def test_delta_v_bisection_n256_m167(orc, gpu, seg, mode):
+    """Bisection segments of 8/16/32 slices (7/15/31 moved beads) on N=256, M=167."""
     for seed in range(5):
         ref, ctx, sp, old, new, L = _bisection_case(orc, gpu, 256, 167, seg, SEED0 + seed, mode)
         a_old = ref.action_list(sp)
@@ -189,8 +189,8 @@ def test_energy_and_sep_hist(orc, gpu):
     assert np.array_equal(h, h_ref)
 
 
-def test_ssf_c1(orc, gpu):
-    """S(q) on C1 (N=16, M=125) for the 729-vector cube |n_i| <= 4."""
+def test_ssf_n16_m125(orc, gpu):
+    """S(q) on N=16, M=125 for the 729-vector cube |n_i| <= 4."""
     for seed in (SEED0, SEED0 + 1):
         ref, ctx, beads, L = setup_pair(orc, gpu, 16, 125, 0.004, seed=seed)
         q = orc.qvectors(L, "max_int", "4 4 4")
@@ -235,19 +235,19 @@ def test_aziz_batch(orc, gpu, mode):
     assert az_g.tailV == az_r.tailV
 
 
-def test_total_action_c5(orc, gpu):
-    """Workload C5: N=2048, M=250, rc = L/2, Gaussian-bridge worldlines (full size)."""
+def test_total_action_n2048_m250(orc, gpu):
+    """Large system N=2048, M=250, rc = L/2, Gaussian-bridge worldlines (full size)."""
     ref, ctx, beads, L = setup_pair(orc, gpu, 2048, 250, 0.004, rc_frac=0.5, kind=1)
     b = ref.total_action()
     for mode in (0, 2):
         ctx.set_v_mode(mode)
         a = ctx.potentialAction()
-        print(f"C5 mode {mode}: rel err {abs(a - b) / abs(b):.3e}")
+        print(f"N2048 M250 mode {mode}: rel err {abs(a - b) / abs(b):.3e}")
         assert abs(a - b) <= TOL[mode] * abs(b)
 
 
-def test_isf_c2_properties(orc, gpu):
-    """F(q,tau) on the C2 q-set (5377 lattice q, |q| <= 3 A^-1, all 167 lags):
+def test_isf_n256_m167_properties(orc, gpu):
+    """F(q,tau) on the N=256, M=167 q-set (5377 lattice q, |q| <= 3 A^-1, all 167 lags):
     F(q,0) = S(q), F(q,tau) = F(q,M-tau), and the oracle on a few q."""
     from pimc_amd.workloads import lattice_q
     ref, ctx, beads, L = setup_pair(orc, gpu, 256, 167, 0.004)

@@ -43,3 +43,11 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(api, "_lib", None)
     with pytest.raises(api.PimcError):
         api.lib()
+
+
+def test_package_exports_the_mirror_classes():
+    import pimc_amd
+    for name in ("Context", "Aziz", "Batch", "Server", "PimcError", "gen_worldlines",
+                 "device_synchronize", "copy_probe", "V_FP32", "V_FP64", "V_TABLE"):
+        assert hasattr(pimc_amd, name), name
+    assert set(pimc_amd.__all__) <= set(dir(pimc_amd))

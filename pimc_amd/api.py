@@ -58,12 +58,14 @@ SIGNATURES = {
     "pimc_set_worm": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip]),
     "pimc_download_path": (C.c_int, [C.c_void_p, _dp]),
     "pimc_update_beads": (C.c_int, [C.c_void_p, _ip, _dp, C.c_int]),
-    "pimc_total_action": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    # the two per-move calls take raw addresses (c_void_p): no ctypes pointer objects per call
+    "pimc_total_action": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pimc_potential_slices": (C.c_int, [C.c_void_p, _dp, _ip]),
     "pimc_energy": (C.c_int, [C.c_void_p, _ip, _dp]),
     "pimc_action_list": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp]),
     "pimc_action_range": (C.c_int, [C.c_void_p, _ip, _ip, _ip, C.c_int, _dp, _dp]),
-    "pimc_delta_action": (C.c_int, [C.c_void_p, _ip, _dp, _dp, C.c_int, C.c_int, _dp, C.c_int, _dp]),
+    "pimc_delta_action": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                    C.c_int, C.c_void_p]),
     "pimc_ssf": (C.c_int, [C.c_void_p, _dp, C.c_int, _dp]),
     "pimc_isf": (C.c_int, [C.c_void_p, _dp, C.c_int, _dp]),
     "pimc_cell_dims": (C.c_int, [C.c_void_p, _ip, _ip]),
@@ -148,6 +150,8 @@ class Context:
             raise PimcError(f"pimc_ctx_create: {lib().pimc_last_error(None).decode()} ({rc_})")
         self.h = h
         self.M, self.N, self.side = num_slices, num_alloc, side
+        self._vf1 = np.ones(2)          # default vfactor
+        self._out = np.zeros(4)         # reused result buffer of the per-move calls
 
     def close(self):
         if getattr(self, "h", None):
@@ -185,11 +189,12 @@ class Context:
         self._check(lib().pimc_update_beads(self.h, _p(sp, _ip), _p(p, _dp), sp.shape[0]), "update_beads")
 
     def potentialAction(self, shift=1, vfactor=(1.0, 1.0), with_slices=False):
-        vf = _f64(vfactor)
-        a = np.zeros(1)
+        vf = self._vf1 if vfactor == (1.0, 1.0) else _f64(vfactor)
         vs = np.zeros((self.M, 2)) if with_slices else None
-        self._check(lib().pimc_total_action(self.h, shift, _p(vf, _dp), _p(a, _dp), _p(vs, _dp)), "total_action")
-        return (a[0], vs) if with_slices else a[0]
+        self._check(lib().pimc_total_action(self.h, shift, vf.ctypes.data, self._out.ctypes.data,
+                                            None if vs is None else vs.ctypes.data), "total_action")
+        a = float(self._out[0])
+        return (a, vs) if with_slices else a
 
     def potential_slices(self, sep_hist=False):
         vs = np.zeros((self.M, 2))
@@ -225,11 +230,14 @@ class Context:
         sp = _i32(slice_ptcl, (-1, 2))
         npos = _f64(new_pos, (-1, 3))
         opos = None if old_pos is None else _f64(old_pos, (-1, 3))
-        vf = _f64(vfactor)
-        o = np.zeros(3)
-        self._check(lib().pimc_delta_action(self.h, _p(sp, _ip), _p(opos, _dp), _p(npos, _dp), sp.shape[0], shift,
-                                            _p(vf, _dp), int(bool(commit)), _p(o, _dp)), "delta_action")
-        return o
+        if npos.shape[0] != sp.shape[0] or (opos is not None and opos.shape[0] != sp.shape[0]):
+            raise PimcError("delta_action: bead list and positions differ in length")
+        vf = self._vf1 if vfactor == (1.0, 1.0) else _f64(vfactor)
+        o = self._out
+        self._check(lib().pimc_delta_action(self.h, sp.ctypes.data, None if opos is None else opos.ctypes.data,
+                                            npos.ctypes.data, sp.shape[0], shift, vf.ctypes.data, int(bool(commit)),
+                                            o.ctypes.data), "delta_action")
+        return o[:3].copy()
 
     def ssf(self, q):
         q = _f64(q, (-1, 3))

@@ -2,12 +2,13 @@
 // (LocalAction::potentialAction(), src/action.cpp:456-472) of many independent
 // walkers with uneven N and M in one persistent launch.
 //
-// Every walker's slices are cut into work items of a fixed number of pairs per
-// lane (the walker's tile), so a walker's sums never depend on which other
-// walkers share the batch.  A fixed grid of G workgroups walks contiguous item
-// ranges balanced by pair count; a workgroup restages a slice in LDS only when
-// the slice changes.  Slice and walker sums follow in two fixed-order passes.
-// The three launches can be captured once into a hipGraph and replayed.
+// Every walker's slices are cut into row tiles of a fixed number of pairs per
+// lane (the persistent pair kernel of pimc_pair.hip), so a walker's sums never
+// depend on which other walkers share the batch.  A fixed grid of G workgroups
+// walks contiguous tile ranges balanced by pair count; a workgroup restages a
+// slice in LDS only when the slice changes.  Slice and walker sums follow in two
+// fixed-order passes.  The three launches can be captured once into a hipGraph
+// and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,79 +22,6 @@
 #include "pimc_host.hpp"
 
 namespace pimc {
-
-template <int MODE, typename P>
-__device__ __forceinline__ double batch_pair_loop(P px, P py, P pz, int n, int p_begin, int p_count,
-                                                  const BoxDev& box, const AzizDev& a, double inv_dr) {
-    const int H = n >> 1;
-    const bool even = (n & 1) == 0;
-    int p = p_begin + threadIdx.x;
-    const int p_end = p_begin + p_count;
-    int d = 1 + p / n;
-    int i = p - (d - 1) * n;
-    const int step_d = kPairThreads / n;
-    const int step_i = kPairThreads - step_d * n;
-    double acc = 0.0;
-    for (; p < p_end; p += kPairThreads) {
-        int j = i + d;
-        if (j >= n) j -= n;
-        const bool ok = !(even && d == H && i >= H);
-        const double r2 = sep_r2<MODE>(px[i], py[i], pz[i], px[j], py[j], pz[j], box);
-        const double v = pair_value_from_r2<MODE>(r2, a, inv_dr);
-        acc += ok ? v : 0.0;
-        i += step_i;
-        d += step_d;
-        if (i >= n) {
-            i -= n;
-            d += 1;
-        }
-    }
-    return acc;
-}
-
-// Persistent: workgroup g takes items [ranges[g], ranges[g+1]).
-template <int MODE>
-__global__ __launch_bounds__(kPairThreads) void k_batch_action(const PairItem* __restrict__ items,
-                                                               const int* __restrict__ ranges,
-                                                               const SliceDesc* __restrict__ slices,
-                                                               const WalkerDesc* __restrict__ walkers,
-                                                               const double* __restrict__ pos, AzizDev az,
-                                                               int lds_beads, double* __restrict__ partials) {
-    extern __shared__ double lds[];
-    __shared__ double red[kPairThreads / 64];
-    double* lx = lds;
-    double* ly = lds + lds_beads;
-    double* lz = lds + 2 * lds_beads;
-    const double inv_dr = 1.0 / az.dr;
-    int staged = -1;
-    const int e = ranges[blockIdx.x + 1];
-    for (int k = ranges[blockIdx.x]; k < e; ++k) {
-        const PairItem it = items[k];
-        const SliceDesc sd = slices[it.slice];
-        const WalkerDesc& wk = walkers[sd.walker];
-        AzizDev a = az;
-        a.table_length = wk.table_length;
-        const double* px = pos + sd.pos;
-        const double* py = px + sd.plane;
-        const double* pz = py + sd.plane;
-        const int n = sd.n;
-        const bool in_lds = n <= lds_beads;
-        if (in_lds && staged != it.slice) {
-            __syncthreads();
-            for (int t = threadIdx.x; t < n; t += kPairThreads) {
-                lx[t] = px[t];
-                ly[t] = py[t];
-                lz[t] = pz[t];
-            }
-            staged = it.slice;
-            __syncthreads();
-        }
-        const double acc = in_lds ? batch_pair_loop<MODE>(lx, ly, lz, n, it.p_begin, it.p_count, wk.box, a, inv_dr)
-                                  : batch_pair_loop<MODE>(px, py, pz, n, it.p_begin, it.p_count, wk.box, a, inv_dr);
-        const double s = block_sum<kPairThreads>(acc, red);
-        if (threadIdx.x == 0) partials[k] = s;
-    }
-}
 
 // One wave per slice: items of the slice (lanes strided), fixed shuffle tree.
 __global__ __launch_bounds__(256) void k_slice_sums(int n_items, const SliceDesc* __restrict__ slices, int n_slices,
@@ -151,7 +79,7 @@ struct pimc_batch {
     double* d_pos = nullptr;
     SliceDesc* d_slices = nullptr;
     WalkerDesc* d_walkers = nullptr;
-    PairItem* d_items = nullptr;
+    RowTile* d_items = nullptr;
     int* d_ranges = nullptr;
     double* d_partials = nullptr;
     double* d_vslice = nullptr;
@@ -205,54 +133,36 @@ void drop_graph(pimc_batch* b) {
     b->graph = nullptr;
 }
 
-// Work list: per slice, items of pairs_per_lane * 256 pairs; grid ranges balanced
-// by pair count.
+// Work list: per slice, row tiles of about pairs_per_lane pairs per lane; grid
+// ranges balanced by pair count.
 int build_work(pimc_batch* b) {
     const int W = (int)b->desc.size();
     std::vector<SliceDesc> sl(b->total_slices);
-    std::vector<PairItem> items;
-    const int64_t chunk = (int64_t)b->pairs_per_lane * kPairThreads;
+    std::vector<RowTile> items;
     int maxn = 0;
     b->total_pairs = 0;
     for (int w = 0; w < W; ++w) {
         const int M = b->desc[w].num_slices, N = b->desc[w].num_alloc;
         maxn = std::max(maxn, N);
         for (int m = 0; m < M; ++m) {
-            SliceDesc& s = sl[b->first_slice[w] + m];
+            const int si = b->first_slice[w] + m;
+            SliceDesc& s = sl[si];
             s.n = N;
             s.slice = m;
             s.walker = w;
             s.first_item = (int)items.size();
             s.pos = b->pos_off[w] + (int64_t)m * N;
             s.plane = (int64_t)M * N;
-            const int64_t P = (int64_t)N * (N / 2);
-            b->total_pairs += (int64_t)N * (N - 1) / 2;
-            for (int64_t p = 0; p < P; p += chunk) {
-                PairItem it;
-                it.slice = b->first_slice[w] + m;
-                it.p_begin = (int)p;
-                it.p_count = (int)std::min<int64_t>(chunk, P - p);
-                items.push_back(it);
-            }
+            b->total_pairs += append_row_tiles(si, N, b->pairs_per_lane, items);
         }
     }
     b->n_items = (int)items.size();
     b->lds_beads = maxn <= kLdsBeads ? std::max(maxn, 1) : 0;
-    int G = b->grid > 0 ? b->grid : 4 * b->num_cus;
+    int G = b->grid > 0 ? b->grid : pair_grid_for(b->lds_beads, b->num_cus);
     G = std::max(1, std::min(G, std::max(b->n_items, 1)));
     b->n_groups = G;
-    // contiguous ranges with ~equal pair counts
-    std::vector<int> ranges(G + 1, 0);
-    int64_t tot = 0;
-    for (auto& it : items) tot += it.p_count;
-    int64_t acc = 0;
-    int g = 1;
-    for (int k = 0; k < b->n_items && g < G; ++k) {
-        acc += items[k].p_count;
-        while (g < G && acc >= (tot * g) / G) ranges[g++] = k + 1;
-    }
-    while (g < G) ranges[g++] = b->n_items;
-    ranges[G] = b->n_items;
+    std::vector<int> ranges;
+    balance_row_tiles(items, G, ranges);
     for (void* p : {(void*)b->d_items, (void*)b->d_ranges, (void*)b->d_partials}) {
         if (!p) continue;
         (void)hipFree(p);
@@ -263,7 +173,7 @@ int build_work(pimc_batch* b) {
     if ((r = balloc(b, &b->d_ranges, G + 1))) return r;
     if ((r = balloc(b, &b->d_partials, b->n_items))) return r;
     BHIP(b, hipMemcpyAsync(b->d_slices, sl.data(), sizeof(SliceDesc) * sl.size(), hipMemcpyHostToDevice, b->stream));
-    BHIP(b, hipMemcpyAsync(b->d_items, items.data(), sizeof(PairItem) * items.size(), hipMemcpyHostToDevice,
+    BHIP(b, hipMemcpyAsync(b->d_items, items.data(), sizeof(RowTile) * items.size(), hipMemcpyHostToDevice,
                            b->stream));
     BHIP(b, hipMemcpyAsync(b->d_ranges, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice,
                            b->stream));
@@ -273,18 +183,10 @@ int build_work(pimc_batch* b) {
 }
 
 hipError_t enqueue(pimc_batch* b, const AzizDev& az) {
-    (void)hipGetLastError();
-    const size_t lds = sizeof(double) * 3 * (size_t)b->lds_beads;
-    dim3 g(b->n_groups), t(kPairThreads);
-    if (b->v_mode == 0)
-        hipLaunchKernelGGL(k_batch_action<0>, g, t, lds, b->stream, b->d_items, b->d_ranges, b->d_slices,
-                           b->d_walkers, b->d_pos, az, b->lds_beads, b->d_partials);
-    else if (b->v_mode == 1)
-        hipLaunchKernelGGL(k_batch_action<1>, g, t, lds, b->stream, b->d_items, b->d_ranges, b->d_slices,
-                           b->d_walkers, b->d_pos, az, b->lds_beads, b->d_partials);
-    else
-        hipLaunchKernelGGL(k_batch_action<2>, g, t, lds, b->stream, b->d_items, b->d_ranges, b->d_slices,
-                           b->d_walkers, b->d_pos, az, b->lds_beads, b->d_partials);
+    hipError_t e = launch_pair_action(b->v_mode, b->stream, b->d_items, b->n_items, b->d_ranges, b->n_groups,
+                                      b->d_slices, b->total_slices, b->d_walkers, (int)b->desc.size(), b->d_pos, az,
+                                      b->lds_beads, b->d_partials, nullptr, nullptr, nullptr, false);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_slice_sums, dim3((b->total_slices + 3) / 4), dim3(256), 0, b->stream, b->n_items,
                        b->d_slices, b->total_slices, b->d_partials, b->d_vslice);
     const int W = (int)b->desc.size();
@@ -482,8 +384,8 @@ int pimc_batch_info(const pimc_batch* b, int64_t* total_pairs, int* n_items, int
 // best grid, best ms, candidates tried}.
 int pimc_batch_autotune(pimc_batch* b, int reps, double* out4) {
     if (!b || reps < 1) return bfail(b, PIMC_E_ARG, "bad argument");
-    const int ppls[] = {16, 32, 64, 128, 256};
-    const int gmul[] = {2, 4, 8, 16};
+    const int ppls[] = {4, 8, 16, 32, 64, 128};
+    const int gmul[] = {1, 2, 4, 8};
     const bool graph = b->use_graph;
     b->use_graph = true;
     std::vector<double> act(b->desc.size());

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -233,29 +234,67 @@ const double* cached_table(int year, int what, int length, std::string* err) {
 // ===========================================================================
 static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf", "isf"};
 
+constexpr int kTimingSlots = 2048;
+
+int pimc_ctx::timed_slot(int k) {
+    if (!timing) return -1;
+    if (ev_pool.empty()) {
+        ev_pool.assign(2 * kTimingSlots, nullptr);
+        for (auto& e : ev_pool)
+            if (hipEventCreate(&e) != hipSuccess) {
+                timing = false;
+                return -1;
+            }
+    }
+    if (ev_next == kTimingSlots) resolve_timing();
+    const int slot = ev_next++;
+    pending.push_back({k, slot});
+    return slot;
+}
 void pimc_ctx::timed_begin(int k) {
-    if (!timing) return;
-    hipEvent_t a, b;
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
-    (void)hipEventRecord(a, stream);
-    pending.push_back({k, a, b});
+    pending_end = timed_slot(k);
+    if (pending_end >= 0) (void)hipEventRecord(ev_pool[2 * pending_end], stream);
 }
 void pimc_ctx::timed_end() {
-    if (!timing || pending.empty()) return;
-    (void)hipEventRecord(pending.back().b, stream);
+    if (pending_end >= 0) (void)hipEventRecord(ev_pool[2 * pending_end + 1], stream);
+    pending_end = -1;
+}
+pimc::KernelTimer pimc_ctx::kernel_timer(int k) {
+    pimc::KernelTimer t;
+    const int slot = timed_slot(k);
+    if (slot < 0) return t;
+    t.start = ev_pool[2 * slot];
+    t.stop = ev_pool[2 * slot + 1];
+    return t;
 }
 void pimc_ctx::resolve_timing() {
     for (auto& p : pending) {
         float ms = 0.f;
-        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        hipEvent_t a = ev_pool[2 * p.ev], b = ev_pool[2 * p.ev + 1];
+        if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) {
             kernel_ms[p.kind] += ms;
             kernel_count[p.kind] += 1;
         }
-        (void)hipEventDestroy(p.a);
-        (void)hipEventDestroy(p.b);
     }
     pending.clear();
+    ev_next = 0;
+}
+
+// Waits until the call's last kernel has published seq s in mapped memory: a
+// spin on a host cache line instead of a stream synchronisation (whose wake-up
+// costs several microseconds).  After 0.2 s it falls back to the stream
+// synchronisation, which also returns an error the stream may hold.
+int pimc_ctx::wait_done(unsigned long long s) {
+    const unsigned long long* f = reinterpret_cast<const unsigned long long*>(h_map + kDoneSlot);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned it = 1;; ++it) {
+        if (__atomic_load_n(f, __ATOMIC_ACQUIRE) == s) return 0;
+        if ((it & 4095u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) break;
+        __builtin_ia32_pause();
+    }
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return (int)e;
+    return __atomic_load_n(f, __ATOMIC_ACQUIRE) == s ? 0 : -1;
 }
 
 // ===========================================================================
@@ -269,6 +308,8 @@ static hipError_t dalloc(T** p, size_t n) {
 pimc_ctx::~pimc_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
     resolve_timing();
+    for (auto& e : ev_pool)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : dev_allocs) (void)hipFree(p);
     for (void* p : host_allocs) (void)hipHostFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -329,63 +370,70 @@ static void build_grid_nn(pimc_ctx* c) {
     }
 }
 
-static int64_t pair_chunk_for(int64_t total_pairs) {
-    // >= 16 pairs per lane; shrink no further than that, grow (to at most 512 per
-    // lane) while more than ~4096 workgroups would remain
-    int64_t per_lane = total_pairs / ((int64_t)kPairThreads * 4096);
-    per_lane = std::max<int64_t>(16, std::min<int64_t>(512, per_lane));
-    return per_lane * kPairThreads;
+// Pairs per lane of a tile: about two tiles per persistent workgroup (fine
+// enough for the pair-count balance of the ranges, few enough partials), within
+// [2, 1024].
+static int pairs_per_lane_for(int64_t total_pairs, int G) {
+    const int64_t per_lane = total_pairs / ((int64_t)G * kPairThreads * 2);
+    return (int)std::max<int64_t>(2, std::min<int64_t>(1024, per_lane));
+}
+
+static void free_dev(pimc_ctx* c, void* p) {
+    if (!p) return;
+    (void)hipFree(p);
+    c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), p), c->dev_allocs.end());
 }
 
 static int rebuild_work(pimc_ctx* c) {
     const int M = c->desc.num_slices;
     std::vector<SliceDesc> sl(M);
-    std::vector<PairItem> items;
     int64_t total = 0;
+    int maxn = 0;
     for (int m = 0; m < M; ++m) {
         const int64_t n = c->h_nbs[m];
-        total += n * (n / 2);
+        total += n * (n - 1) / 2;
+        maxn = std::max(maxn, c->h_nbs[m]);
     }
-    int64_t chunk = c->pair_chunk > 0 ? c->pair_chunk : pair_chunk_for(total);
+    c->lds_beads = maxn <= kLdsBeads ? std::max(maxn, 1) : 0;
+    const int G0 = pair_grid_for(c->lds_beads, c->num_cus);
+    const int ppl = pairs_per_lane_for(total, G0);
+    std::vector<RowTile> tiles;
     for (int m = 0; m < M; ++m) {
         const int n = c->h_nbs[m];
         sl[m].n = n;
         sl[m].slice = m;
         sl[m].walker = 0;
-        sl[m].first_item = (int)items.size();
+        sl[m].first_item = (int)tiles.size();
         sl[m].pos = (int64_t)m * c->desc.num_alloc;
         sl[m].plane = c->plane;
-        const int64_t P = (int64_t)n * (n / 2);
-        for (int64_t p = 0; p < P; p += chunk) {
-            PairItem it;
-            it.slice = m;
-            it.p_begin = (int)p;
-            it.p_count = (int)std::min<int64_t>(chunk, P - p);
-            items.push_back(it);
-        }
+        append_row_tiles(m, n, ppl, tiles);
     }
-    c->n_items = (int)items.size();
-    int maxn = 0;
-    for (int m = 0; m < M; ++m) maxn = std::max(maxn, c->h_nbs[m]);
-    c->lds_beads = maxn <= kLdsBeads ? std::max(maxn, 1) : 0;
-    if (c->n_items > c->cap_items) {
-        if (c->d_items) {
-            (void)hipFree(c->d_items);
-            (void)hipFree(c->d_partials);
-            c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)c->d_items),
-                                c->dev_allocs.end());
-            c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)c->d_partials),
-                                c->dev_allocs.end());
-        }
-        int r;
-        if ((r = ctx_dalloc(c, &c->d_items, c->n_items))) return r;
-        if ((r = ctx_dalloc(c, &c->d_partials, c->n_items))) return r;
-        c->cap_items = c->n_items;
+    c->n_tiles = (int)tiles.size();
+    c->n_groups = std::max(1, std::min(G0, c->n_tiles));
+    std::vector<int> ranges;
+    balance_row_tiles(tiles, c->n_groups, ranges);
+    int r;
+    if (c->n_tiles > c->cap_tiles) {
+        free_dev(c, c->d_tiles);
+        free_dev(c, c->d_partials);
+        c->d_tiles = nullptr;
+        c->d_partials = nullptr;
+        if ((r = ctx_dalloc(c, &c->d_tiles, c->n_tiles))) return r;
+        if ((r = ctx_dalloc(c, &c->d_partials, c->n_tiles))) return r;
+        c->cap_tiles = c->n_tiles;
+    }
+    if (c->n_groups + 1 > c->cap_groups) {
+        free_dev(c, c->d_ranges);
+        c->d_ranges = nullptr;
+        if ((r = ctx_dalloc(c, &c->d_ranges, c->n_groups + 1))) return r;
+        c->cap_groups = c->n_groups + 1;
     }
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_slices, sl.data(), sizeof(SliceDesc) * M, hipMemcpyHostToDevice, c->stream));
-    if (c->n_items)
-        HIP_OR_FAIL(c, hipMemcpyAsync(c->d_items, items.data(), sizeof(PairItem) * c->n_items,
-                                      hipMemcpyHostToDevice, c->stream));
+    if (c->n_tiles)
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->d_tiles, tiles.data(), sizeof(RowTile) * c->n_tiles, hipMemcpyHostToDevice,
+                                      c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->d_ranges, ranges.data(), sizeof(int) * ranges.size(), hipMemcpyHostToDevice,
+                                  c->stream));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     return PIMC_OK;
 }
@@ -434,6 +482,12 @@ int pimc_ctx_create(const pimc_desc* d, pimc_ctx** out) {
     std::unique_ptr<pimc_ctx> c(new pimc_ctx());
     c->desc = *d;
     c->params = p;
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            c->num_cus = cus;
+    }
     const int M = d->num_slices, N = d->num_alloc;
     c->plane = (int64_t)M * N;
     set_box(c.get());
@@ -473,6 +527,7 @@ int pimc_ctx_create(const pimc_desc* d, pimc_ctx** out) {
                                      hipHostMallocMapped | hipHostMallocCoherent);
         if (e != hipSuccess) return fail(nullptr, PIMC_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
         c->host_allocs.push_back((void*)c->h_map);
+        std::memset(c->h_map, 0, sizeof(double) * 64);
         e = hipHostGetDevicePointer((void**)&c->d_map, c->h_map, 0);
         if (e != hipSuccess) return fail(nullptr, PIMC_E_HIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
     }
@@ -633,16 +688,19 @@ int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* a
         c->walker_cache = w;
         c->walker_valid = true;
     }
-    if (c->n_items > 0) {
-        c->timed_begin(0);
-        HIP_OR_FAIL(c, launch_pair_action(c->desc.v_mode, c->stream, c->d_items, c->n_items, c->d_slices, M,
-                                          c->d_walker, 1, c->d_pos, az, c->lds_beads, c->d_partials, c->d_vslice,
-                                          c->d_map));
-        c->timed_end();
-        if (v_slices)
+    if (c->n_tiles > 0) {
+        const KernelTimer kt = c->kernel_timer(0);
+        const unsigned long long seq = ++c->seq;
+        HIP_OR_FAIL(c, launch_pair_action(c->desc.v_mode, c->stream, c->d_tiles, c->n_tiles, c->d_ranges, c->n_groups,
+                                          c->d_slices, M, c->d_walker, 1, c->d_pos, az, c->lds_beads, c->d_partials,
+                                          c->d_vslice, c->d_map, &kt, true, c->d_done(), seq));
+        if (v_slices) {
             HIP_OR_FAIL(c, hipMemcpyAsync(c->h_stage, c->d_vslice, sizeof(double) * M, hipMemcpyDeviceToHost,
                                           c->stream));
-        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+            HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+        } else if (c->wait_done(seq) != 0) {
+            return fail(c, PIMC_E_HIP, "pair action: completion not published");
+        }
         *action = c->h_map[0];
         if (v_slices)
             for (int m = 0; m < M; ++m) {
@@ -654,7 +712,6 @@ int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* a
         if (v_slices)
             for (int m = 0; m < 2 * M; ++m) v_slices[m] = 0.0;
     }
-    c->resolve_timing();
     return PIMC_OK;
 }
 
@@ -750,6 +807,8 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
     A.partials = c->d_vnn_partials;
     A.per_bead = c->d_per_bead;
     A.out = c->d_map + 8;
+    A.done = c->d_done();
+    A.seq = ++c->seq;
     VnnInline req;
     if (inline_req) {
         std::memcpy(req.beads, c->h_req_beads, sizeof(VnnBead) * n);
@@ -757,17 +816,18 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
         HIP_OR_FAIL(c, hipMemcpyAsync(c->d_req_beads, c->h_req_beads, sizeof(VnnBead) * n, hipMemcpyHostToDevice,
                                       c->stream));
     }
-    c->timed_begin(1);
-    HIP_OR_FAIL(c, launch_vnn(c->desc.v_mode, c->stream, A, req, az));
-    c->timed_end();
-    if (per_bead)
+    const KernelTimer kt = c->kernel_timer(1);
+    HIP_OR_FAIL(c, launch_vnn(c->desc.v_mode, c->stream, A, req, az, &kt));
+    if (per_bead) {
         HIP_OR_FAIL(c, hipMemcpyAsync(c->h_per_bead, c->d_per_bead, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
                                       c->stream));
-    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+        HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    } else if (c->wait_done(A.seq) != 0) {
+        return fail(c, PIMC_E_HIP, "delta action: completion not published");
+    }
     if (out3)
         for (int k = 0; k < 3; ++k) out3[k] = c->h_map[8 + k];
     if (per_bead) std::memcpy(per_bead, c->h_per_bead, sizeof(double) * 2 * n);
-    c->resolve_timing();
     return PIMC_OK;
 }
 

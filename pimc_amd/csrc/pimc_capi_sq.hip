@@ -159,7 +159,6 @@ int pimc_ssf(pimc_ctx* c, const double* q, int nq, double* sq) {
         SHIP(c, hipStreamSynchronize(c->stream));
         for (int k = 0; k < ng; ++k) sq[idx_g[k]] = host[k];
     }
-    c->resolve_timing();
     return PIMC_OK;
 }
 
@@ -196,7 +195,6 @@ int pimc_isf(pimc_ctx* c, const double* q, int nq, double* fqt) {
         SHIP(c, hipStreamSynchronize(c->stream));
         for (int k = 0; k < n; ++k) std::memcpy(fqt + (size_t)idx[k] * M, host.data() + (size_t)k * M, sizeof(double) * M);
     }
-    c->resolve_timing();
     return PIMC_OK;
 }
 

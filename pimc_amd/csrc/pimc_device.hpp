@@ -114,8 +114,10 @@ __device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
     // F(x) = exp(-(D/x - 1)^2) for x < D, with D/x - 1 = (D - x) x / x^2
     const double t = (a.D - x) * x * ix2;
     const double F = (k < a.k_d) ? split_exp2(-1.4426950408889634074 * t * t) : 1.0;
-    const double full = fma(-disp, F, rep);
-    return (k < a.k_hc) ? rep : full;
+    // Below x = 0.01 (k < k_hc) valueV is the bare repulsion; there t >= D/0.01 - 1
+    // > 120, so exp2(-t^2 log2 e) underflows to exactly 0 in FP32, F = 0 and
+    // fma(-disp, 0, rep) == rep: the hard-core branch needs no select.
+    return fma(-disp, F, rep);
 }
 
 // valueV at the bin radius, FP64 arithmetic.  Requires k_eps <= k < table_length.
@@ -147,7 +149,8 @@ __device__ __forceinline__ double pair_value_from_r2(double r2, const AzizDev& a
         double r;
         k = bin_exact(r2, a.dr, &r);
     }
-    const bool valid = (k >= a.k_eps) && (k < a.table_length) && (k > 0);
+    // k_eps <= k < tableLength (k_eps >= 1 always: bin 0 has x = 0 < EPS), one compare
+    const bool valid = (unsigned)(k - a.k_eps) < (unsigned)(a.table_length - a.k_eps);
     double v = 0.0;
     if (MODE == 0) {
         const int kk = valid ? k : a.k_hc;  // keep the arithmetic finite on masked lanes

@@ -50,14 +50,22 @@ struct pimc_ctx {
     int* d_grid_nn = nullptr;     // [cells][27]
     pimc::SliceDesc* d_slices = nullptr;
     pimc::WalkerDesc* d_walker = nullptr;
-    pimc::PairItem* d_items = nullptr;
-    double* d_partials = nullptr;
-    int n_items = 0, cap_items = 0;
-    int64_t pair_chunk = 0;       // 0: automatic
+    pimc::RowTile* d_tiles = nullptr;
+    int* d_ranges = nullptr;      // [n_groups + 1] tile ranges of the persistent pair kernel
+    double* d_partials = nullptr; // one per tile
+    int n_tiles = 0, cap_tiles = 0;
+    int n_groups = 0, cap_groups = 0;
+    int num_cus = 256;
     double* d_vslice = nullptr;
     double* d_out = nullptr;
     double* h_map = nullptr;      // 64 doubles of mapped, coherent pinned host memory
     double* d_map = nullptr;      // its device address
+    // completion word (slot kDoneSlot of the mapped block): the last kernel of a
+    // call publishes the call's sequence number there
+    static constexpr int kDoneSlot = 63;
+    unsigned long long seq = 0;
+    unsigned long long* d_done() { return reinterpret_cast<unsigned long long*>(d_map + kDoneSlot); }
+    int wait_done(unsigned long long s);   // spin, then fall back to a stream sync
     int lds_beads = 0;            // slice positions staged in LDS (0: read from HBM)
     pimc::WalkerDesc walker_cache{};
     bool walker_valid = false;
@@ -92,18 +100,25 @@ struct pimc_ctx {
 
     std::vector<void*> dev_allocs, host_allocs;
 
-    // timing
+    // timing: a pool of reusable event pairs; durations are summed lazily, when
+    // the pool is exhausted or the totals are read, so timed calls add only the
+    // event records
     bool timing = false;
     struct Pending {
         int kind;
-        hipEvent_t a, b;
+        int ev;    // index into ev_pool
     };
+    std::vector<hipEvent_t> ev_pool;   // 2 per slot
     std::vector<Pending> pending;
+    int ev_next = 0;
     double kernel_ms[5] = {0, 0, 0, 0, 0};
     long kernel_count[5] = {0, 0, 0, 0, 0};
+    int timed_slot(int kind);          // -1 when timing is off
     void timed_begin(int kind);
     void timed_end();
-    void resolve_timing();
+    pimc::KernelTimer kernel_timer(int kind);   // events for one kernel's own duration
+    void resolve_timing();             // sums the pending durations (waits for them)
+    int pending_end = -1;              // slot opened by timed_begin
 
     ~pimc_ctx();
 };

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "pimc_device.hpp"
 
 namespace pimc {
@@ -33,14 +35,25 @@ struct WalkerDesc {
     WormDev worm;
 };
 
-// A contiguous range of flattened pair indices of one slice.  Pair p of a slice
-// with n beads is (i, j = (i + d) mod n) with d = 1 + p / n, i = p % n,
-// 0 <= p < n * (n/2); for even n the pairs d = n/2, i >= n/2 repeat and are masked.
-struct PairItem {
+// Every i<j pair of a slice with n beads is visited once as (i, j = (i + d) mod n)
+// with offsets 1 <= d <= (n-1)/2 for all rows i, plus, for even n, d = n/2 for
+// rows i < n/2.  A tile is rows [i0, i_end) against offsets [d0, d0 + dn).  A
+// workgroup's 256 lanes hold one row each (n >= 256) or form 256/n groups of n
+// rows (n < 256), group g taking offsets d0 + g, d0 + g + 256/n, ...; a lane's
+// row bead stays in registers while its partner index steps through the slice.
+struct RowTile {
     int slice;      // SliceDesc index
-    int p_begin;
-    int p_count;
+    int i0, i_end;  // rows
+    int d0, dn;     // partner offsets
 };
+
+// Appends the tiles of slice `slice_index` (n active beads) with about
+// pairs_per_lane pairs per lane each; returns the slice's pair count n(n-1)/2.
+int64_t append_row_tiles(int slice_index, int n, int pairs_per_lane, std::vector<RowTile>& tiles);
+// Splits tiles [0, T) into G contiguous ranges of about equal pair count:
+// workgroup g of the persistent launch takes tiles [ranges[g], ranges[g+1]).
+void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int>& ranges);
+inline int64_t row_tile_pairs(const RowTile& t) { return (int64_t)(t.i_end - t.i0) * t.dn; }
 
 // A Vnn evaluation request for one bead: partners are the active beads of its slice.
 struct VnnBead {
@@ -75,6 +88,8 @@ struct VnnArgs {
     double* partials;  // [n_beads][chunks][2]
     double* per_bead;  // [n_beads][2]
     double* out;       // {A_old, A_new, dU} (host-mapped pinned memory)
+    unsigned long long* done;   // host-mapped: set to seq once out is written (or null)
+    unsigned long long seq;
 };
 
 // Requests of up to kVnnInline beads travel in the kernel arguments.
@@ -96,11 +111,41 @@ struct RhoArgs {
     double* rho;           // [M][nq][2] FP64
 };
 
+// Publishes seq to a host-mapped completion word after the calling thread's
+// earlier writes (the host spins on it instead of a stream synchronisation).
+__device__ __forceinline__ void publish_done(unsigned long long* done, unsigned long long seq) {
+    __threadfence_system();
+    __hip_atomic_store(done, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Start/stop events recorded on the stream right before and after one kernel's
+// launch, so their elapsed time is that kernel's duration; null: not timed.
+struct KernelTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+// Workgroup b of a launch is dispatched to XCD b % 8.  Work item of workgroup b
+// such that each XCD walks one contiguous run of the G items (items are grouped
+// by slice, so a slice's beads are fetched into one XCD's L2 instead of eight).
+__device__ __forceinline__ int xcd_contiguous_item(int b, int G) {
+    const int x = b & 7, local = b >> 3;
+    const int per = G >> 3, rem = G & 7;
+    return x < rem ? x * (per + 1) + local : rem * (per + 1) + (x - rem) * per + local;
+}
+
 // ---- launchers (defined in pimc_pair.hip / pimc_sq.hip) ----
-hipError_t launch_pair_action(int mode, hipStream_t st, const PairItem* items, int n_items, const SliceDesc* slices,
-                              int n_slices, const WalkerDesc* walkers, int n_walkers, const double* pos,
-                              const AzizDev& az, int lds_beads, double* partials, double* v_slice, double* action);
-hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az);
+// The persistent row-tile pair kernel over G workgroups, then (finalize != 0)
+// the single-workgroup slice and walker sums.
+// done (host-mapped, or null) is set to seq after action has been written.
+hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, int n_tiles, const int* ranges, int G,
+                              const SliceDesc* slices, int n_slices, const WalkerDesc* walkers, int n_walkers,
+                              const double* pos, const AzizDev& az, int lds_beads, double* partials, double* v_slice,
+                              double* action, const KernelTimer* timer = nullptr, bool finalize = true,
+                              unsigned long long* done = nullptr, unsigned long long seq = 0);
+// Workgroups of the persistent pair kernel for a slice staging of lds_beads beads.
+int pair_grid_for(int lds_beads, int num_cus);
+hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
+                      const KernelTimer* timer = nullptr);
 hipError_t launch_update_beads(hipStream_t st, const int* sp, const double* p3, int n, double* pos, int64_t plane,
                                int nalloc, int num_slices, int* cell, int* label, int* numlab, int* hash,
                                const BoxDev& box);

@@ -15,6 +15,7 @@
 // strided range of the slice's flattened pair index space.  Sums are FP64 with
 // fixed-order wave/workgroup trees and a fixed-order final pass, so results do
 // not depend on how many other slices or walkers share the launch.
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include "pimc_kernels.hpp"
@@ -22,47 +23,40 @@
 namespace pimc {
 
 // ---------------------------------------------------------------------------
-// All i<j pairs of each slice.
+// All i<j pairs of each slice, row tiles (RowTile in pimc_kernels.hpp).
 // ---------------------------------------------------------------------------
-// Strided walk over the flattened pairs [p_begin, p_begin + p_count) of one
-// slice: lane t takes p = p_begin + t + 256 s; pair p is (i, (i + d) mod n) with
-// d = 1 + p / n, i = p % n, and the repeated d = n/2 pairs of even n masked.
+// One lane: row bead i (held in registers) against partners j = (i + d) mod n for
+// d = d, d + G, ... < d_end (0 <= i < n, 1 <= d <= n/2).
 template <int MODE, typename P>
-__device__ __forceinline__ double pair_loop(P px, P py, P pz, int n, const PairItem it, const BoxDev& box,
-                                            const AzizDev& a, double inv_dr, bool factors, const WormDev& worm,
-                                            int slice) {
-    const int H = n >> 1;
-    const bool even = (n & 1) == 0;
-    int p = it.p_begin + threadIdx.x;
-    const int p_end = it.p_begin + it.p_count;
-    int d = 1 + p / n;
-    int i = p - (d - 1) * n;
-    const int step_d = kPairThreads / n;
-    const int step_i = kPairThreads - step_d * n;
+__device__ __forceinline__ double row_loop(P px, P py, P pz, int n, int i, int d, int d_end, int G, const BoxDev& box,
+                                           const AzizDev& a, double inv_dr, bool factors, const WormDev& worm,
+                                           int slice) {
+    const double xi = px[i], yi = py[i], zi = pz[i];
+    int j = i + d;
+    if (j >= n) j -= n;
+    const int step = G % n;   // j + step < 2n, so one wrap per step
     double acc = 0.0;
-    for (; p < p_end; p += kPairThreads) {
-        int j = i + d;
-        if (j >= n) j -= n;
-        const bool ok = !(even && d == H && i >= H);
-        const double r2 = sep_r2<MODE>(px[i], py[i], pz[i], px[j], py[j], pz[j], box);
+    for (; d < d_end; d += G) {
+        const double r2 = sep_r2<MODE>(xi, yi, zi, px[j], py[j], pz[j], box);
         double v = pair_value_from_r2<MODE>(r2, a, inv_dr);
         if (factors) {
             const int lo = i < j ? i : j, hi = i < j ? j : i;
             v *= pair_factor(bead_state(worm, slice, lo), bead_state(worm, slice, hi));
         }
-        acc += ok ? v : 0.0;
-        i += step_i;
-        d += step_d;
-        if (i >= n) {
-            i -= n;
-            d += 1;
-        }
+        acc += v;
+        j += step;
+        if (j >= n) j -= n;
     }
     return acc;
 }
 
+// Persistent: the workgroups of one XCD take consecutive tile ranges (so a
+// slice is staged into one XCD's L2), each range a run of tiles grouped by
+// slice; a slice is restaged in LDS only when it changes.  One FP64 partial per
+// tile, whatever the grid.
 template <int MODE>
-__global__ __launch_bounds__(kPairThreads) void k_pair_action(const PairItem* __restrict__ items,
+__global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __restrict__ tiles,
+                                                              const int* __restrict__ ranges,
                                                               const SliceDesc* __restrict__ slices,
                                                               const WalkerDesc* __restrict__ walkers,
                                                               const double* __restrict__ pos, AzizDev az,
@@ -72,77 +66,106 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const PairItem* __
     double* lx = lds;
     double* ly = lds + lds_beads;
     double* lz = lds + 2 * lds_beads;
-
-    const PairItem it = items[blockIdx.x];
-    const SliceDesc sd = slices[it.slice];
-    const WalkerDesc& wk = walkers[sd.walker];
-    const BoxDev box = wk.box;
-    AzizDev a = az;
-    a.table_length = wk.table_length;
-    const double inv_dr = 1.0 / a.dr;
-    const int n = sd.n;
-    const double* px = pos + sd.pos;
-    const double* py = px + sd.plane;
-    const double* pz = py + sd.plane;
-    const bool in_lds = n <= lds_beads;
-    if (in_lds) {
-        for (int t = threadIdx.x; t < n; t += kPairThreads) {
-            lx[t] = px[t];
-            ly[t] = py[t];
-            lz[t] = pz[t];
+    const int r = xcd_contiguous_item(blockIdx.x, gridDim.x);
+    const int k_end = ranges[r + 1];
+    int staged = -1;
+    for (int k = ranges[r]; k < k_end; ++k) {
+        const RowTile tl = tiles[k];
+        const SliceDesc sd = slices[tl.slice];
+        const WalkerDesc& wk = walkers[sd.walker];
+        AzizDev a = az;
+        a.table_length = wk.table_length;
+        const double inv_dr = 1.0 / a.dr;
+        const int n = sd.n;
+        const double* px = pos + sd.pos;
+        const double* py = px + sd.plane;
+        const double* pz = py + sd.plane;
+        const bool in_lds = n <= lds_beads;
+        if (in_lds && staged != tl.slice) {
+            __syncthreads();   // the previous slice is no longer read
+            for (int t = threadIdx.x; t < n; t += kPairThreads) {
+                lx[t] = px[t];
+                ly[t] = py[t];
+                lz[t] = pz[t];
+            }
+            staged = tl.slice;
+            __syncthreads();
         }
+        const int R = n < kPairThreads ? n : kPairThreads;
+        const int G = kPairThreads / R;
+        const int grp = threadIdx.x / R;
+        const int i = tl.i0 + (threadIdx.x - grp * R);
+        double acc = 0.0;
+        if (grp < G && i < tl.i_end) {
+            const bool factors = worm_touches_slice(wk.worm, sd.slice);
+            const int d = tl.d0 + grp, d_end = tl.d0 + tl.dn;
+            // two copies of the loop so the LDS copy compiles to ds_read (no flat addressing)
+            acc = in_lds ? row_loop<MODE>(lx, ly, lz, n, i, d, d_end, G, wk.box, a, inv_dr, factors, wk.worm, sd.slice)
+                         : row_loop<MODE>(px, py, pz, n, i, d, d_end, G, wk.box, a, inv_dr, factors, wk.worm, sd.slice);
+        }
+        const double s = block_sum<kPairThreads>(acc, red);
+        if (threadIdx.x == 0) partials[k] = s;
     }
-    __syncthreads();
-
-    const bool factors = worm_touches_slice(wk.worm, sd.slice);
-    // two copies of the loop so the LDS copy compiles to ds_read (no flat addressing)
-    const double acc = in_lds ? pair_loop<MODE>(lx, ly, lz, n, it, box, a, inv_dr, factors, wk.worm, sd.slice)
-                              : pair_loop<MODE>(px, py, pz, n, it, box, a, inv_dr, factors, wk.worm, sd.slice);
-    const double s = block_sum<kPairThreads>(acc, red);
-    if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
 // Per-slice sums of the item partials (items are grouped by slice, in order) and
 // the per-walker potential action (src/action.cpp:462-469).  One workgroup of
-// 1024 lanes: the partials are first pulled into LDS with every load in flight,
-// then each wave reduces whole slices (lanes strided over items, fixed shuffle
-// tree) and whole walkers (lanes strided over slices), so the result depends
-// only on the work list, never on timing.
+// 1024 lanes: the partials and the slices' item ranges are first pulled into LDS
+// with every load in flight; then each lane sums the items of one slice in item
+// order, and one wave per walker sums its slices (lanes strided, fixed shuffle
+// tree), so the result depends only on the work list, never on timing.
 constexpr int kFinThreads = 1024;
-constexpr int kFinLds = 6144;   // partials staged in LDS (beyond: read from HBM)
+constexpr int kFinLds = 5120;      // partials staged in LDS (beyond: read from HBM)
+constexpr int kFinSlices = 1024;   // slice ranges / sums staged in LDS (beyond: HBM)
 __global__ __launch_bounds__(kFinThreads) void k_pair_finalize(int n_items, const SliceDesc* __restrict__ slices,
                                                               int n_slices, const WalkerDesc* __restrict__ walkers,
                                                               int n_walkers, const double* __restrict__ partials,
                                                               double* __restrict__ v_slice,
-                                                              double* __restrict__ action) {
+                                                              double* __restrict__ action,
+                                                              unsigned long long* done, unsigned long long seq) {
     __shared__ double sp[kFinLds];
+    __shared__ double vs[kFinSlices];
+    __shared__ int fi[kFinSlices + 1];
     const bool staged = n_items <= kFinLds;
+    const bool small = n_slices <= kFinSlices;
     if (staged)
         for (int t = threadIdx.x; t < n_items; t += kFinThreads) sp[t] = partials[t];
+    if (small) {
+        for (int t = threadIdx.x; t < n_slices; t += kFinThreads) fi[t] = slices[t].first_item;
+        if (threadIdx.x == 0) fi[n_slices] = n_items;
+    }
     __syncthreads();
     const double* src = staged ? sp : partials;
+    for (int sl = threadIdx.x; sl < n_slices; sl += kFinThreads) {
+        const int b = small ? fi[sl] : slices[sl].first_item;
+        const int e = small ? fi[sl + 1] : ((sl + 1 < n_slices) ? slices[sl + 1].first_item : n_items);
+        double v = 0.0;
+        for (int k = b; k < e; ++k) v += src[k];
+        v_slice[sl] = v;
+        if (small) vs[sl] = v;
+    }
+    __threadfence_block();   // v_slice is re-read below when the slices do not fit in LDS
+    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int kWaves = kFinThreads / 64;
-    for (int sl = wave; sl < n_slices; sl += kWaves) {
-        const int b = slices[sl].first_item;
-        const int e = (sl + 1 < n_slices) ? slices[sl + 1].first_item : n_items;
-        double v = 0.0;
-        for (int k = b + lane; k < e; k += 64) v += src[k];
-        v = wave_sum(v);
-        if (lane == 0) v_slice[sl] = v;
-    }
-    __threadfence_block();
-    __syncthreads();
     for (int w = wave; w < n_walkers; w += kWaves) {
         const WalkerDesc& W = walkers[w];
         const double tau_s = W.shift * W.tau;
         double tot = 0.0;
         for (int m = lane; m < W.num_slices; m += 64) {
             const double vf = (m & 1) ? W.vfactor1 : W.vfactor0;
-            tot += vf * tau_s * (0.0 + v_slice[W.first_slice + m]);
+            const double v = small ? vs[W.first_slice + m] : v_slice[W.first_slice + m];
+            tot += vf * tau_s * (0.0 + v);
         }
         tot = wave_sum(tot);
-        if (lane == 0) action[w] = tot;
+        if (lane == 0) {
+            action[w] = tot;
+            if (done) __threadfence_system();
+        }
+    }
+    if (done) {
+        __syncthreads();
+        if (threadIdx.x == 0) publish_done(done, seq);
     }
 }
 
@@ -270,10 +293,8 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
         red[2][wave] = du;
     }
     __syncthreads();
-    if (threadIdx.x < 3) {
-        const int r = threadIdx.x;
-        A.out[r] = ((red[r][0] + red[r][1]) + red[r][2]) + red[r][3];
-    }
+    if (threadIdx.x == 0)
+        for (int r = 0; r < 3; ++r) A.out[r] = ((red[r][0] + red[r][1]) + red[r][2]) + red[r][3];
     if (A.commit_in_kernel) {
         for (int k = threadIdx.x; k < A.n_beads; k += 256) {
             const VnnBead qb = A.beads ? A.beads[k] : req.beads[k];
@@ -286,6 +307,10 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
             qx[A.plane] = qb.new_pos[1];
             qx[2 * A.plane] = qb.new_pos[2];
         }
+    }
+    if (A.done) {
+        __syncthreads();
+        if (threadIdx.x == 0) publish_done(A.done, A.seq);
     }
 }
 
@@ -466,34 +491,83 @@ __global__ void k_aziz_batch(const double* __restrict__ sep, int n, AzizDev az, 
 // ---------------------------------------------------------------------------
 // Host-side launchers (C++ linkage, used by pimc_capi.hip)
 // ---------------------------------------------------------------------------
-hipError_t launch_pair_action(int mode, hipStream_t st, const PairItem* items, int n_items, const SliceDesc* slices,
-                              int n_slices, const WalkerDesc* walkers, int n_walkers, const double* pos,
-                              const AzizDev& az, int lds_beads, double* partials, double* v_slice, double* action) {
+hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, int n_tiles, const int* ranges, int G,
+                              const SliceDesc* slices, int n_slices, const WalkerDesc* walkers, int n_walkers,
+                              const double* pos, const AzizDev& az, int lds_beads, double* partials, double* v_slice,
+                              double* action, const KernelTimer* timer, bool finalize, unsigned long long* done,
+                              unsigned long long seq) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
-    if (n_items <= 0) return hipSuccess;
-    dim3 g(n_items), b(kPairThreads);
+    if (n_tiles <= 0 || G <= 0) return hipSuccess;
+    dim3 g(G), b(kPairThreads);
     const size_t lds = sizeof(double) * 3 * (size_t)lds_beads;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, g, b, lds, st, tiles, ranges, slices, walkers, pos, az, lds_beads, partials);
+    };
+    if (timer && timer->start) (void)hipEventRecord(timer->start, st);
     if (mode == 0)
-        hipLaunchKernelGGL(k_pair_action<0>, g, b, lds, st, items, slices, walkers, pos, az, lds_beads, partials);
+        go(k_pair_action<0>);
     else if (mode == 1)
-        hipLaunchKernelGGL(k_pair_action<1>, g, b, lds, st, items, slices, walkers, pos, az, lds_beads, partials);
+        go(k_pair_action<1>);
     else
-        hipLaunchKernelGGL(k_pair_action<2>, g, b, lds, st, items, slices, walkers, pos, az, lds_beads, partials);
-    hipLaunchKernelGGL(k_pair_finalize, dim3(1), dim3(kFinThreads), 0, st, n_items, slices, n_slices, walkers,
-                       n_walkers, partials, v_slice, action);
+        go(k_pair_action<2>);
+    if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
+    if (finalize)
+        hipLaunchKernelGGL(k_pair_finalize, dim3(1), dim3(kFinThreads), 0, st, n_tiles, slices, n_slices, walkers,
+                           n_walkers, partials, v_slice, action, done, seq);
     return hipGetLastError();
 }
 
-hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az) {
+int pair_grid_for(int lds_beads, int num_cus) {
+    // Four workgroups per CU even where LDS admits only three at once: the
+    // smaller per-workgroup ranges shorten the tail more than the second wave
+    // of workgroups costs (measured on the N = 2048 slices: 1.15 ms vs 1.31 ms).
+    (void)lds_beads;
+    return 4 * std::max(1, num_cus);
+}
+
+int64_t append_row_tiles(int slice_index, int n, int pairs_per_lane, std::vector<RowTile>& tiles) {
+    if (n < 2) return 0;
+    const int R = n < kPairThreads ? n : kPairThreads;
+    const int G = kPairThreads / R;
+    const int full = (n - 1) / 2;                      // offsets taken by every row
+    const int dn = std::max(1, pairs_per_lane) * G;    // offsets per tile
+    for (int i0 = 0; i0 < n; i0 += R) {
+        const int i_end = std::min(i0 + R, n);
+        for (int d0 = 1; d0 <= full; d0 += dn) tiles.push_back(RowTile{slice_index, i0, i_end, d0, std::min(dn, full - d0 + 1)});
+        if ((n & 1) == 0 && i0 < n / 2)                 // the d = n/2 pairs, once each
+            tiles.push_back(RowTile{slice_index, i0, std::min(i_end, n / 2), n / 2, 1});
+    }
+    return (int64_t)n * (n - 1) / 2;
+}
+
+void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int>& ranges) {
+    const int T = (int)tiles.size();
+    ranges.assign(G + 1, T);
+    ranges[0] = 0;
+    int64_t tot = 0;
+    for (const auto& t : tiles) tot += row_tile_pairs(t);
+    int64_t acc = 0;
+    int g = 1;
+    for (int k = 0; k < T && g < G; ++k) {
+        acc += row_tile_pairs(tiles[k]);
+        while (g < G && acc * G >= tot * g) ranges[g++] = k + 1;
+    }
+}
+
+hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
+                      const KernelTimer* timer) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (A.n_beads <= 0) return hipSuccess;
     dim3 g(A.chunks, A.n_beads), b(kVnnThreads);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, b, 0, st, A, req, az); };
+    if (timer && timer->start) (void)hipEventRecord(timer->start, st);
     if (mode == 0)
-        hipLaunchKernelGGL(k_vnn<0>, g, b, 0, st, A, req, az);
+        go(k_vnn<0>);
     else if (mode == 1)
-        hipLaunchKernelGGL(k_vnn<1>, g, b, 0, st, A, req, az);
+        go(k_vnn<1>);
     else
-        hipLaunchKernelGGL(k_vnn<2>, g, b, 0, st, A, req, az);
+        go(k_vnn<2>);
+    if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
     hipLaunchKernelGGL(k_vnn_finalize, dim3(1), dim3(256), 0, st, A, req);
     return hipGetLastError();
 }

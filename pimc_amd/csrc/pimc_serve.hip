@@ -261,6 +261,15 @@ struct Slot {
     size_t desc_bytes = 0;
 };
 
+// One packed batch, handed from pack_batch (under the lock) to fire_batch (without).
+struct BatchLaunch {
+    int slot = 0;
+    size_t o_bd = 0, o_gr = 0, bytes = 0;
+    int n_groups = 0, n_reqs = 0;
+    AzizDev az{};
+    const ServeChainDev* chains = nullptr;
+};
+
 }  // namespace
 
 struct pimc_server {
@@ -292,8 +301,13 @@ struct pimc_server {
     bool track_done = false;
     std::vector<uint64_t> done_list;   // completed request ids (load generator)
 
+    std::vector<ServeReqDev> pack_rq;
+    std::vector<VnnBead> pack_bd;
+    std::vector<ServeGroupDev> pack_gr;
+
     void run();
-    int launch_ready(std::unique_lock<std::mutex>& lk);
+    int pack_batch(int slot, BatchLaunch* L);
+    hipError_t fire_batch(const BatchLaunch& L);
     void complete(Slot& s);
     void admit_waiting();
 };
@@ -417,24 +431,23 @@ void pimc_server::admit_waiting() {
     }
 }
 
-int pimc_server::launch_ready(std::unique_lock<std::mutex>& lk) {
-    (void)lk;
-    int si = -1;
-    for (int k = 0; k < n_slots; ++k)
-        if (!slots[k].busy) {
-            si = k;
-            break;
-        }
-    if (si < 0 || ready.empty()) return 0;
+// Packs every ready request that fits (at most one in flight per chain) into
+// slot S's pinned descriptor block and marks the slot busy.  Runs under the
+// server lock; returns 1 when a batch was packed, 0 when nothing is ready.
+int pimc_server::pack_batch(int si, BatchLaunch* L) {
     Slot& S = slots[si];
+    if (ready.empty()) return 0;
     if (sync_chain_table(this, S.stream) != 0) {
         kernel_error = PIMC_E_HIP;
         kernel_error_msg = "chain table upload failed";
         return -1;
     }
-    std::vector<ServeReqDev> rq;
-    std::vector<VnnBead> bd;
-    std::vector<ServeGroupDev> gr;
+    std::vector<ServeReqDev>& rq = pack_rq;
+    std::vector<VnnBead>& bd = pack_bd;
+    std::vector<ServeGroupDev>& gr = pack_gr;
+    rq.clear();
+    bd.clear();
+    gr.clear();
     S.reqs.clear();
     while (!ready.empty() && (int)rq.size() < kMaxReqPerBatch) {
         const uint64_t id = ready.front();
@@ -470,46 +483,54 @@ int pimc_server::launch_ready(std::unique_lock<std::mutex>& lk) {
     const size_t b_rq = sizeof(ServeReqDev) * rq.size();
     const size_t b_bd = sizeof(VnnBead) * bd.size();
     const size_t b_gr = sizeof(ServeGroupDev) * gr.size();
-    const size_t o_bd = (b_rq + 255) & ~size_t(255);
-    const size_t o_gr = (o_bd + b_bd + 255) & ~size_t(255);
+    L->o_bd = (b_rq + 255) & ~size_t(255);
+    L->o_gr = (L->o_bd + b_bd + 255) & ~size_t(255);
+    L->bytes = L->o_gr + b_gr;
+    L->n_groups = (int)gr.size();
+    L->n_reqs = (int)rq.size();
+    L->slot = si;
     std::memcpy(S.h_desc, rq.data(), b_rq);
-    std::memcpy(S.h_desc + o_bd, bd.data(), b_bd);
-    std::memcpy(S.h_desc + o_gr, gr.data(), b_gr);
-    AzizDev az_l = az;
+    std::memcpy(S.h_desc + L->o_bd, bd.data(), b_bd);
+    std::memcpy(S.h_desc + L->o_gr, gr.data(), b_gr);
+    L->az = az;
     if (desc.v_mode == PIMC_V_TABLE) {
         std::string e;
-        az_l.table = cached_table(desc.aziz_year, 0, (int)(max_table_sep / ((1.0E-6) * params.rm)) + 1, &e);
+        L->az.table = cached_table(desc.aziz_year, 0, (int)(max_table_sep / ((1.0E-6) * params.rm)) + 1, &e);
     }
-    (void)hipGetLastError();
-    hipError_t e = hipMemcpyAsync(S.d_desc, S.h_desc, o_gr + b_gr, hipMemcpyHostToDevice, S.stream);
-    const ServeReqDev* d_rq = (const ServeReqDev*)S.d_desc;
-    const VnnBead* d_bd = (const VnnBead*)(S.d_desc + o_bd);
-    const ServeGroupDev* d_gr = (const ServeGroupDev*)(S.d_desc + o_gr);
-    if (e == hipSuccess) {
-        dim3 g((unsigned)gr.size()), b(kVnnThreads);
-        if (desc.v_mode == 0)
-            hipLaunchKernelGGL(k_serve_vnn<0>, g, b, 0, S.stream, d_gr, d_rq, d_bd, d_chains, az_l, kVnnChunk,
-                               S.d_partials);
-        else if (desc.v_mode == 1)
-            hipLaunchKernelGGL(k_serve_vnn<1>, g, b, 0, S.stream, d_gr, d_rq, d_bd, d_chains, az_l, kVnnChunk,
-                               S.d_partials);
-        else
-            hipLaunchKernelGGL(k_serve_vnn<2>, g, b, 0, S.stream, d_gr, d_rq, d_bd, d_chains, az_l, kVnnChunk,
-                               S.d_partials);
-        hipLaunchKernelGGL(k_serve_finalize, dim3((unsigned)rq.size()), dim3(64), 0, S.stream, d_rq, d_bd, d_chains,
-                           S.d_partials, S.d_out);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipEventRecord(S.ev, S.stream);
-    if (e != hipSuccess) {
-        kernel_error = PIMC_E_HIP;
-        kernel_error_msg = std::string("serve launch: ") + hipGetErrorString(e);
-        return -1;
-    }
+    L->chains = d_chains;
     S.busy = true;
     ++launched_batches;
     launched_reqs += (int64_t)rq.size();
     return 1;
+}
+
+// Copies a packed batch to the device and launches it on its slot's stream.
+// Runs without the server lock (HIP calls are thread safe; the slot belongs to
+// the worker thread while busy), so submitters are never held up by a launch.
+hipError_t pimc_server::fire_batch(const BatchLaunch& L) {
+    Slot& S = slots[L.slot];
+    (void)hipGetLastError();
+    hipError_t e = hipMemcpyAsync(S.d_desc, S.h_desc, L.bytes, hipMemcpyHostToDevice, S.stream);
+    const ServeReqDev* d_rq = (const ServeReqDev*)S.d_desc;
+    const VnnBead* d_bd = (const VnnBead*)(S.d_desc + L.o_bd);
+    const ServeGroupDev* d_gr = (const ServeGroupDev*)(S.d_desc + L.o_gr);
+    if (e == hipSuccess) {
+        dim3 g((unsigned)L.n_groups), b(kVnnThreads);
+        if (desc.v_mode == 0)
+            hipLaunchKernelGGL(k_serve_vnn<0>, g, b, 0, S.stream, d_gr, d_rq, d_bd, L.chains, L.az, kVnnChunk,
+                               S.d_partials);
+        else if (desc.v_mode == 1)
+            hipLaunchKernelGGL(k_serve_vnn<1>, g, b, 0, S.stream, d_gr, d_rq, d_bd, L.chains, L.az, kVnnChunk,
+                               S.d_partials);
+        else
+            hipLaunchKernelGGL(k_serve_vnn<2>, g, b, 0, S.stream, d_gr, d_rq, d_bd, L.chains, L.az, kVnnChunk,
+                               S.d_partials);
+        hipLaunchKernelGGL(k_serve_finalize, dim3((unsigned)L.n_reqs), dim3(64), 0, S.stream, d_rq, d_bd, L.chains,
+                           S.d_partials, S.d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(S.ev, S.stream);
+    return e;
 }
 
 void pimc_server::complete(Slot& S) {
@@ -529,37 +550,59 @@ void pimc_server::complete(Slot& S) {
     S.busy = false;
 }
 
+// The worker: polls the in-flight batches' events and fires new batches with the
+// lock released; takes the lock only to retire results and to pack requests.
 void pimc_server::run() {
-    std::unique_lock<std::mutex> lk(mu);
+    std::unique_lock<std::mutex> lk(mu, std::defer_lock);
     while (!stop.load()) {
-        bool progressed = false;
+        int done_k[kSlots];
+        hipError_t done_e[kSlots];
+        int nd = 0;
+        bool any_busy = false;
         for (int k = 0; k < n_slots; ++k) {
-            Slot& S = slots[k];
-            if (!S.busy) continue;
-            hipError_t q = hipEventQuery(S.ev);
-            if (q == hipSuccess) {
-                complete(S);
-                progressed = true;
-            } else if (q != hipErrorNotReady) {
-                kernel_error = PIMC_E_HIP;
-                kernel_error_msg = std::string("batch failed: ") + hipGetErrorString(q);
-                complete(S);
-                progressed = true;
+            if (!slots[k].busy) continue;
+            any_busy = true;
+            const hipError_t q = hipEventQuery(slots[k].ev);
+            if (q != hipErrorNotReady) {
+                done_k[nd] = k;
+                done_e[nd++] = q;
             }
         }
-        if (progressed) cv_done.notify_all();
-        int lr = 0;
-        while ((lr = launch_ready(lk)) > 0) progressed = true;
-        bool any_busy = false;
-        for (int k = 0; k < n_slots; ++k) any_busy |= slots[k].busy;
-        if (!progressed) {
-            if (any_busy) {
-                // batches in flight: spin briefly without the lock so submitters get in
-                lk.unlock();
-                std::this_thread::yield();
-                lk.lock();
-            } else {
+        BatchLaunch L[kSlots];
+        int nl = 0;
+        lk.lock();
+        for (int t = 0; t < nd; ++t) {
+            if (done_e[t] != hipSuccess) {
+                kernel_error = PIMC_E_HIP;
+                kernel_error_msg = std::string("batch failed: ") + hipGetErrorString(done_e[t]);
+            }
+            complete(slots[done_k[t]]);
+        }
+        if (nd) cv_done.notify_all();
+        if (!kernel_error)
+            for (int k = 0; k < n_slots; ++k) {
+                if (slots[k].busy) continue;
+                const int r = pack_batch(k, &L[nl]);
+                if (r <= 0) break;
+                ++nl;
+            }
+        if (nl == 0 && nd == 0) {
+            if (!any_busy)
                 cv_work.wait_for(lk, std::chrono::milliseconds(1));
+            lk.unlock();
+            if (any_busy) std::this_thread::yield();
+            continue;
+        }
+        lk.unlock();
+        for (int t = 0; t < nl; ++t) {
+            const hipError_t e = fire_batch(L[t]);
+            if (e != hipSuccess) {
+                lk.lock();
+                kernel_error = PIMC_E_HIP;
+                kernel_error_msg = std::string("serve launch: ") + hipGetErrorString(e);
+                complete(slots[L[t].slot]);
+                cv_done.notify_all();
+                lk.unlock();
             }
         }
     }
@@ -782,9 +825,6 @@ int pimc_server_stats(pimc_server* s, int64_t* batches, int64_t* requests, int64
 // ===========================================================================
 namespace {
 
-struct GenChain;
-std::vector<int> pending_chains_scan(const std::vector<GenChain>& gc);
-
 struct GenChain {
     std::vector<double> mirror;        // [M][N][3], kept equal to the device copy
     std::deque<Clock::time_point> arrivals;
@@ -793,13 +833,6 @@ struct GenChain {
     std::vector<int32_t> sp;
     std::vector<double> prop;
 };
-
-std::vector<int> pending_chains_scan(const std::vector<GenChain>& gc) {
-    std::vector<int> out;
-    for (int c = 0; c < (int)gc.size(); ++c)
-        if (!gc[c].inflight && !gc[c].arrivals.empty()) out.push_back(c);
-    return out;
-}
 
 }  // namespace
 
@@ -842,6 +875,7 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
     const auto t_end = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(L->duration_s));
     auto next_arrival = t0;
     int inflight = 0;
+    bool drained = false;
     auto propose = [&](int c) {
         GenChain& g = gc[c];
         const pimc_desc& d = s->chains[c].d;
@@ -880,6 +914,7 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
         }
     };
     std::unordered_map<uint64_t, int> by_id;
+    std::deque<int> runnable;   // chains with an arrival waiting and nothing in flight
     auto submit = [&](int c, Clock::time_point t_arr) -> int {
         GenChain& g = gc[c];
         propose(c);
@@ -905,6 +940,7 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
             while (next_arrival <= now) {
                 const int c = (int)(U(rng) * nc) % nc;
                 gc[c].arrivals.push_back(next_arrival);
+                if (!gc[c].inflight && gc[c].arrivals.size() == 1) runnable.push_back(c);
                 next_arrival += std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(Exp(rng)));
             }
         }
@@ -938,28 +974,32 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
             }
             g.inflight = 0;
             --inflight;
+            if (!closed && !g.arrivals.empty()) runnable.push_back(c);
             if (closed && open) {
                 r = submit(c, Clock::now());
                 if (r) return r;
             }
         }
-        if (!closed)
-            for (int c : pending_chains_scan(gc)) {
-                GenChain& g = gc[c];
-                if (!g.inflight && !g.arrivals.empty()) {
-                    const auto ta = g.arrivals.front();
-                    g.arrivals.pop_front();
-                    int r = submit(c, ta);
-                    if (r) return r;
-                }
+        while (!runnable.empty()) {
+            const int c = runnable.front();
+            runnable.pop_front();
+            GenChain& g = gc[c];
+            if (!g.inflight && !g.arrivals.empty()) {
+                const auto ta = g.arrivals.front();
+                g.arrivals.pop_front();
+                int r = submit(c, ta);
+                if (r) return r;
             }
+        }
         if (finished.empty()) std::this_thread::yield();
         if (!open && inflight == 0) break;
         if (!open && now - t_end > std::chrono::seconds(30))
             return sfail(s, PIMC_E_STATE, "synthetic load: requests still pending 30 s after the load window");
-        if (!open) {
+        if (!open && !drained) {
             // stop accepting arrivals; drop the ones not yet submitted
             for (auto& g : gc) g.arrivals.clear();
+            runnable.clear();
+            drained = true;
         }
     }
     const double secs = std::chrono::duration<double>(Clock::now() - t0).count();

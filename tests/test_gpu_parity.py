@@ -221,6 +221,13 @@ def test_aziz_batch(orc, gpu, mode):
     sep = rng.normal(size=(4000, 3)) * rng.uniform(0.0, 5.0, size=(4000, 1))
     sep[0] = [1e-8, 0, 0]
     sep[1] = [20.0, 0, 0]
+    # the hard core x = r/rm < 0.01 (r < 0.0297 A), down to the EPS bin, and the
+    # bins either side of x = 0.01 and x = D
+    rm, D = 2.9673, 1.241314
+    radii = np.concatenate([np.geomspace(3e-6, 0.0297, 30), 0.01 * rm * (1 + np.array([-1e-6, 1e-6])),
+                            D * rm * (1 + np.array([-1e-6, 1e-6]))])
+    sep[2:2 + radii.size] = 0.0
+    sep[2:2 + radii.size, 0] = radii
     v = az_g.V(sep)
     vr = np.array([az_r.V(s) for s in sep])
     # per-pair scale: |V| plus the well depth, so the zero crossing keeps a finite scale

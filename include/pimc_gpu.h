@@ -223,6 +223,34 @@ typedef struct {
  *            batches, requests per batch, seconds}; latency counts from arrival. */
 int  pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* load, double* stats10);
 
+/* ---- one Markov chain: the diagonal moves over a resident path ---------------- */
+/* StagingMove / BisectionMove / CenterOfMassMove::attemptMove (src/move.cpp:
+ * 1124-1259, 1315-1408, 1484-1661; MoveBase helpers :270-396, :467-488) on a
+ * context holding closed worldlines (every bead on, no worm).  Proposals and the
+ * Metropolis test draw on the chain's own MTRand(seed) stream in the reference's
+ * order against a host copy of the positions; every oldAction/newAction pair is
+ * one pimc_delta_action pass on the device; accepted moves stay committed on the
+ * device and rejected ones are undone as the reference undoes them.  The chain
+ * requires that nothing else moves the context's beads while it lives. */
+typedef struct pimc_chain pimc_chain;
+typedef struct pimc_chain_desc {
+    uint32_t seed;        /* MTRand(seed) */
+    int Mbar;             /* staging length (2 <= Mbar < M) */
+    int levels;           /* bisection levels b (2^b < M) */
+    int var_updates;      /* draw the length / level count per move */
+    int max_wind;         /* winding sectors sampled by staging */
+    double com_delta;     /* centre-of-mass shift width */
+} pimc_chain_desc;
+int  pimc_chain_create(pimc_ctx* ctx, const pimc_chain_desc* d, pimc_chain** out);
+void pimc_chain_destroy(pimc_chain* ch);
+const char* pimc_chain_last_error(const pimc_chain* ch);
+/* kind 0 staging, 1 bisection, 2 centre of mass; *accepted = Metropolis outcome;
+ * info3 = {attempted, last newAction - oldAction, length (stage length, levels or
+ * worldline length)}. */
+int  pimc_chain_move(pimc_chain* ch, int kind, int* accepted, double* info3);
+/* The chain's positions, [M][N][3] (equal to the device copy). */
+int  pimc_chain_positions(const pimc_chain* ch, double* beads);
+
 /* ---- seeded synthetic worldlines -------------------------------------------- */
 /* MTRand(seed) stream (include/MersenneTwister.h); slice 0 Prism::randPosition
  * (src/container.cpp:155-160); kind 0: randNorm(prev, sqrt(2 lambda tau)) + putInBC

@@ -45,6 +45,11 @@ class _LoadDesc(C.Structure):
                 ("seed", C.c_uint32)]
 
 
+class _ChainDesc(C.Structure):
+    _fields_ = [("seed", C.c_uint32), ("Mbar", C.c_int), ("levels", C.c_int), ("var_updates", C.c_int),
+                ("max_wind", C.c_int), ("com_delta", C.c_double)]
+
+
 _vp = C.c_void_p
 _u64p = C.POINTER(C.c_uint64)
 _i64p = C.POINTER(C.c_int64)
@@ -105,6 +110,11 @@ SIGNATURES = {
     "pimc_server_wait": (C.c_int, [_vp, C.c_uint64, _dp]),
     "pimc_server_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p, C.POINTER(C.c_size_t)]),
     "pimc_server_run_synthetic": (C.c_int, [_vp, C.POINTER(_LoadDesc), _dp]),
+    "pimc_chain_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pimc_chain_destroy": (None, [C.c_void_p]),
+    "pimc_chain_last_error": (C.c_char_p, [C.c_void_p]),
+    "pimc_chain_move": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]),
+    "pimc_chain_positions": (C.c_int, [C.c_void_p, _dp]),
 }
 
 _lib = None
@@ -463,6 +473,46 @@ class Server:
         keys = ["completed", "req_per_s", "pair_evals_per_s", "p50_us", "p99_us", "mean_us", "accept_frac",
                 "batches", "req_per_batch", "seconds"]
         return dict(zip(keys, o))
+
+
+class Chain:
+    """One Markov chain's Staging / Bisection / CenterOfMass moves on a Context
+    (include/pimc_gpu.h, pimc_chain): MTRand(seed) proposals and Metropolis tests
+    on the host, Delta V on the device."""
+
+    KINDS = {"staging": 0, "bisection": 1, "com": 2}
+
+    def __init__(self, ctx, seed, Mbar=8, levels=3, var_updates=True, max_wind=1, com_delta=0.1):
+        d = _ChainDesc(seed, Mbar, levels, int(bool(var_updates)), max_wind, com_delta)
+        h = C.c_void_p()
+        rc = lib().pimc_chain_create(ctx.h, C.byref(d), C.byref(h))
+        if rc != 0:
+            raise PimcError(f"pimc_chain_create failed ({rc}): {lib().pimc_last_error(ctx.h).decode()}")
+        self.h, self.ctx = h, ctx
+        self._acc = C.c_int()
+        self._info = np.zeros(3)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pimc_chain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def move(self, kind):
+        """kind: 'staging' | 'bisection' | 'com'.  Returns (accepted, info3)."""
+        rc = lib().pimc_chain_move(self.h, self.KINDS[kind], C.byref(self._acc), _p(self._info, _dp))
+        if rc != 0:
+            raise PimcError(f"chain move ({rc}): {lib().pimc_chain_last_error(self.h).decode()}")
+        return bool(self._acc.value), self._info.copy()
+
+    def positions(self):
+        b = np.zeros((self.ctx.M, self.ctx.N, 3))
+        rc = lib().pimc_chain_positions(self.h, _p(b, _dp))
+        if rc != 0:
+            raise PimcError(f"chain positions ({rc})")
+        return b
 
 
 def gen_worldlines(seed, side, M, N, lam, tau, kind=0):

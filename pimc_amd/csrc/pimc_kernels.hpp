@@ -41,6 +41,8 @@ struct WalkerDesc {
 // workgroup's 256 lanes hold one row each (n >= 256) or form 256/n groups of n
 // rows (n < 256), group g taking offsets d0 + g, d0 + g + 256/n, ...; a lane's
 // row bead stays in registers while its partner index steps through the slice.
+// For n >= 256 the d = n/2 pairs form tiles of their own (rows < n/2); for
+// n < 256 they stay in the slice's tile and the kernel masks rows >= n/2.
 struct RowTile {
     int slice;      // SliceDesc index
     int i0, i_end;  // rows
@@ -53,6 +55,7 @@ int64_t append_row_tiles(int slice_index, int n, int pairs_per_lane, std::vector
 // Splits tiles [0, T) into G contiguous ranges of about equal pair count:
 // workgroup g of the persistent launch takes tiles [ranges[g], ranges[g+1]).
 void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int>& ranges);
+// Pair evaluations of a tile (masked d = n/2 lanes included: the balancing weight).
 inline int64_t row_tile_pairs(const RowTile& t) { return (int64_t)(t.i_end - t.i0) * t.dn; }
 
 // A Vnn evaluation request for one bead: partners are the active beads of its slice.

@@ -27,7 +27,9 @@ namespace pimc {
 // ---------------------------------------------------------------------------
 // One lane: row bead i (held in registers) against partners j = (i + d) mod n for
 // d = d, d + G, ... < d_end (0 <= i < n, 1 <= d <= n/2).
-template <int MODE, typename P>
+// MASK: the offsets reach d = n/2 of an even n, whose pairs count once (rows
+// i < n/2): the lane skips d == skip_d.
+template <int MODE, bool MASK, typename P>
 __device__ __forceinline__ double row_loop(P px, P py, P pz, int n, int i, int d, int d_end, int G, const BoxDev& box,
                                            const AzizDev& a, double inv_dr, bool factors, const WormDev& worm,
                                            int slice) {
@@ -35,6 +37,7 @@ __device__ __forceinline__ double row_loop(P px, P py, P pz, int n, int i, int d
     int j = i + d;
     if (j >= n) j -= n;
     const int step = G % n;   // j + step < 2n, so one wrap per step
+    const int skip_d = (MASK && i >= (n >> 1)) ? (n >> 1) : -1;
     double acc = 0.0;
     for (; d < d_end; d += G) {
         const double r2 = sep_r2<MODE>(xi, yi, zi, px[j], py[j], pz[j], box);
@@ -43,11 +46,22 @@ __device__ __forceinline__ double row_loop(P px, P py, P pz, int n, int i, int d
             const int lo = i < j ? i : j, hi = i < j ? j : i;
             v *= pair_factor(bead_state(worm, slice, lo), bead_state(worm, slice, hi));
         }
-        acc += v;
+        if (MASK)
+            acc += (d == skip_d) ? 0.0 : v;
+        else
+            acc += v;
         j += step;
         if (j >= n) j -= n;
     }
     return acc;
+}
+
+template <int MODE, typename P>
+__device__ __forceinline__ double row_tile(P px, P py, P pz, int n, int i, int d, int d_end, int G, bool mask,
+                                           const BoxDev& box, const AzizDev& a, double inv_dr, bool factors,
+                                           const WormDev& worm, int slice) {
+    return mask ? row_loop<MODE, true>(px, py, pz, n, i, d, d_end, G, box, a, inv_dr, factors, worm, slice)
+                : row_loop<MODE, false>(px, py, pz, n, i, d, d_end, G, box, a, inv_dr, factors, worm, slice);
 }
 
 // Persistent: the workgroups of one XCD take consecutive tile ranges (so a
@@ -67,11 +81,19 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
     double* ly = lds + lds_beads;
     double* lz = lds + 2 * lds_beads;
     const int r = xcd_contiguous_item(blockIdx.x, gridDim.x);
-    const int k_end = ranges[r + 1];
+    const int k_begin = ranges[r], k_end = ranges[r + 1];
+    if (k_begin >= k_end) return;
     int staged = -1;
-    for (int k = ranges[r]; k < k_end; ++k) {
-        const RowTile tl = tiles[k];
-        const SliceDesc sd = slices[tl.slice];
+    RowTile tl = tiles[k_begin];
+    SliceDesc sd = slices[tl.slice];
+    for (int k = k_begin; k < k_end; ++k) {
+        // the next tile's descriptors load while this tile computes
+        RowTile tl_next = tl;
+        SliceDesc sd_next = sd;
+        if (k + 1 < k_end) {
+            tl_next = tiles[k + 1];
+            sd_next = slices[tl_next.slice];
+        }
         const WalkerDesc& wk = walkers[sd.walker];
         AzizDev a = az;
         a.table_length = wk.table_length;
@@ -99,12 +121,17 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
         if (grp < G && i < tl.i_end) {
             const bool factors = worm_touches_slice(wk.worm, sd.slice);
             const int d = tl.d0 + grp, d_end = tl.d0 + tl.dn;
+            const bool mask = ((n & 1) == 0) && d_end > (n >> 1) && tl.i_end > (n >> 1);
             // two copies of the loop so the LDS copy compiles to ds_read (no flat addressing)
-            acc = in_lds ? row_loop<MODE>(lx, ly, lz, n, i, d, d_end, G, wk.box, a, inv_dr, factors, wk.worm, sd.slice)
-                         : row_loop<MODE>(px, py, pz, n, i, d, d_end, G, wk.box, a, inv_dr, factors, wk.worm, sd.slice);
+            acc = in_lds ? row_tile<MODE>(lx, ly, lz, n, i, d, d_end, G, mask, wk.box, a, inv_dr, factors, wk.worm,
+                                          sd.slice)
+                         : row_tile<MODE>(px, py, pz, n, i, d, d_end, G, mask, wk.box, a, inv_dr, factors, wk.worm,
+                                          sd.slice);
         }
         const double s = block_sum<kPairThreads>(acc, red);
         if (threadIdx.x == 0) partials[k] = s;
+        tl = tl_next;
+        sd = sd_next;
     }
 }
 
@@ -529,8 +556,15 @@ int64_t append_row_tiles(int slice_index, int n, int pairs_per_lane, std::vector
     if (n < 2) return 0;
     const int R = n < kPairThreads ? n : kPairThreads;
     const int G = kPairThreads / R;
-    const int full = (n - 1) / 2;                      // offsets taken by every row
     const int dn = std::max(1, pairs_per_lane) * G;    // offsets per tile
+    if (n < kPairThreads) {
+        // one row block: offsets 1..n/2 in one run, d = n/2 of an even n masked to
+        // rows i < n/2 inside the kernel (a tile of its own would mostly idle)
+        const int dmax = n / 2;
+        for (int d0 = 1; d0 <= dmax; d0 += dn) tiles.push_back(RowTile{slice_index, 0, n, d0, std::min(dn, dmax - d0 + 1)});
+        return (int64_t)n * (n - 1) / 2;
+    }
+    const int full = (n - 1) / 2;                      // offsets taken by every row
     for (int i0 = 0; i0 < n; i0 += R) {
         const int i_end = std::min(i0 + R, n);
         for (int d0 = 1; d0 <= full; d0 += dn) tiles.push_back(RowTile{slice_index, i0, i_end, d0, std::min(dn, full - d0 + 1)});

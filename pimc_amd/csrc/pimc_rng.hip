@@ -1,9 +1,7 @@
-// pimc_rng.hip -- host-side MTRand (include/MersenneTwister.h) and the seeded
-// worldline generator used to build the measured workloads.
+// pimc_rng.hip -- the seeded worldline generator used to build the measured
+// workloads, on the MTRand stream of pimc_mtrand.hpp.
 //
-// The stream is MT19937 with init_genrand seeding; rand() = u/4294967295,
-// randExc() = u/4294967296, randNorm = Marsaglia polar keeping the first
-// variate (:290-363).  Slice 0 is Prism::randPosition (src/container.cpp:155-160);
+// Slice 0 is Prism::randPosition (src/container.cpp:155-160);
 // kind 0 adds randNorm(prev, sqrt(2 lambda tau)) per axis and putInBC, kind 1
 // closes a Gaussian bridge from slice 0 back to slice 0 with the staging
 // construction (src/move.cpp:270-291).
@@ -12,57 +10,12 @@
 #include <vector>
 
 #include "../../include/pimc_gpu.h"
+#include "pimc_mtrand.hpp"
 
 #pragma STDC FP_CONTRACT OFF
 
-namespace {
-
-struct Mt {
-    static constexpr int N = 624, M = 397;
-    uint32_t s[N];
-    int left = 0, next = 0;
-    explicit Mt(uint32_t seed) {
-        s[0] = seed;
-        for (int i = 1; i < N; ++i) s[i] = 1812433253u * (s[i - 1] ^ (s[i - 1] >> 30)) + (uint32_t)i;
-        reload();
-    }
-    static uint32_t twist(uint32_t m, uint32_t a, uint32_t b) {
-        return m ^ (((a & 0x80000000u) | (b & 0x7fffffffu)) >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
-    }
-    void reload() {
-        int i = 0;
-        for (; i < N - M; ++i) s[i] = twist(s[i + M], s[i], s[i + 1]);
-        for (; i < N - 1; ++i) s[i] = twist(s[i + M - N], s[i], s[i + 1]);
-        s[N - 1] = twist(s[M - 1], s[N - 1], s[0]);
-        left = N;
-        next = 0;
-    }
-    uint32_t u32() {
-        if (left == 0) reload();
-        --left;
-        uint32_t y = s[next++];
-        y ^= (y >> 11);
-        y ^= (y << 7) & 0x9d2c5680u;
-        y ^= (y << 15) & 0xefc60000u;
-        return y ^ (y >> 18);
-    }
-    double rand() { return double(u32()) * (1.0 / 4294967295.0); }
-    double randExc() { return double(u32()) * (1.0 / 4294967296.0); }
-    double randNorm(double mean, double sd) {
-        double x, y, r;
-        do {
-            x = 2.0 * rand() - 1.0;
-            y = 2.0 * rand() - 1.0;
-            r = x * x + y * y;
-        } while (r >= 1.0 || r == 0.0);
-        double f = std::sqrt(-2.0 * std::log(r) / r);
-        return mean + x * f * sd;
-    }
-};
-
-inline double put_in_bc(double r, double L) { return r - L * std::floor(r * (1.0 / L) + 0.5); }
-
-}  // namespace
+using pimc::put_in_bc;
+using Mt = pimc::MtRand;
 
 extern "C" int pimc_gen_worldlines(uint32_t seed, double side, int M, int N, double lambda, double tau, int kind,
                                    double* beads) {

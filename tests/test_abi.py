@@ -47,7 +47,7 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
 
 def test_package_exports_the_mirror_classes():
     import pimc_amd
-    for name in ("Context", "Aziz", "Batch", "Server", "PimcError", "gen_worldlines",
+    for name in ("Context", "Aziz", "Batch", "Server", "Chain", "PimcError", "gen_worldlines",
                  "device_synchronize", "copy_probe", "V_FP32", "V_FP64", "V_TABLE"):
         assert hasattr(pimc_amd, name), name
     assert set(pimc_amd.__all__) <= set(dir(pimc_amd))

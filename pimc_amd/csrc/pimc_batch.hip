@@ -153,7 +153,7 @@ int build_work(pimc_batch* b) {
             s.first_item = (int)items.size();
             s.pos = b->pos_off[w] + (int64_t)m * N;
             s.plane = (int64_t)M * N;
-            b->total_pairs += append_row_tiles(si, N, b->pairs_per_lane, items);
+            b->total_pairs += append_row_tiles(si, s, b->pairs_per_lane, items);
         }
     }
     b->n_items = (int)items.size();

@@ -406,7 +406,7 @@ static int rebuild_work(pimc_ctx* c) {
         sl[m].first_item = (int)tiles.size();
         sl[m].pos = (int64_t)m * c->desc.num_alloc;
         sl[m].plane = c->plane;
-        append_row_tiles(m, n, ppl, tiles);
+        append_row_tiles(m, sl[m], ppl, tiles);
     }
     c->n_tiles = (int)tiles.size();
     c->n_groups = std::max(1, std::min(G0, c->n_tiles));

@@ -43,15 +43,22 @@ struct WalkerDesc {
 // row bead stays in registers while its partner index steps through the slice.
 // For n >= 256 the d = n/2 pairs form tiles of their own (rows < n/2); for
 // n < 256 they stay in the slice's tile and the kernel masks rows >= n/2.
+// A tile carries its slice's fields, so a workgroup reaches the positions and
+// the walker in one dependent load after the tile's.
 struct RowTile {
-    int slice;      // SliceDesc index
+    int slice;      // SliceDesc index (the partial's slice)
     int i0, i_end;  // rows
     int d0, dn;     // partner offsets
+    int n;          // active beads on the slice
+    int m;          // slice index within its walker
+    int walker;     // WalkerDesc index
+    int64_t pos;    // offset of x of bead 0 in the SoA position array
+    int64_t plane;  // distance between the x, y and z planes
 };
 
-// Appends the tiles of slice `slice_index` (n active beads) with about
+// Appends the tiles of slice `slice_index` (described by sd) with about
 // pairs_per_lane pairs per lane each; returns the slice's pair count n(n-1)/2.
-int64_t append_row_tiles(int slice_index, int n, int pairs_per_lane, std::vector<RowTile>& tiles);
+int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lane, std::vector<RowTile>& tiles);
 // Splits tiles [0, T) into G contiguous ranges of about equal pair count:
 // workgroup g of the persistent launch takes tiles [ranges[g], ranges[g+1]).
 void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int>& ranges);

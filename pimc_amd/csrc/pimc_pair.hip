@@ -71,7 +71,6 @@ __device__ __forceinline__ double row_tile(P px, P py, P pz, int n, int i, int d
 template <int MODE>
 __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __restrict__ tiles,
                                                               const int* __restrict__ ranges,
-                                                              const SliceDesc* __restrict__ slices,
                                                               const WalkerDesc* __restrict__ walkers,
                                                               const double* __restrict__ pos, AzizDev az,
                                                               int lds_beads, double* __restrict__ partials) {
@@ -85,23 +84,18 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
     if (k_begin >= k_end) return;
     int staged = -1;
     RowTile tl = tiles[k_begin];
-    SliceDesc sd = slices[tl.slice];
     for (int k = k_begin; k < k_end; ++k) {
-        // the next tile's descriptors load while this tile computes
-        RowTile tl_next = tl;
-        SliceDesc sd_next = sd;
-        if (k + 1 < k_end) {
-            tl_next = tiles[k + 1];
-            sd_next = slices[tl_next.slice];
-        }
-        const WalkerDesc& wk = walkers[sd.walker];
+        // the next tile's descriptor loads while this tile computes (clamped index:
+        // a plain uniform load, no select of the struct)
+        const RowTile tl_next = tiles[k + 1 < k_end ? k + 1 : k];
+        const WalkerDesc& wk = walkers[tl.walker];
         AzizDev a = az;
         a.table_length = wk.table_length;
         const double inv_dr = 1.0 / a.dr;
-        const int n = sd.n;
-        const double* px = pos + sd.pos;
-        const double* py = px + sd.plane;
-        const double* pz = py + sd.plane;
+        const int n = tl.n;
+        const double* px = pos + tl.pos;
+        const double* py = px + tl.plane;
+        const double* pz = py + tl.plane;
         const bool in_lds = n <= lds_beads;
         if (in_lds && staged != tl.slice) {
             __syncthreads();   // the previous slice is no longer read
@@ -119,19 +113,18 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
         const int i = tl.i0 + (threadIdx.x - grp * R);
         double acc = 0.0;
         if (grp < G && i < tl.i_end) {
-            const bool factors = worm_touches_slice(wk.worm, sd.slice);
+            const bool factors = worm_touches_slice(wk.worm, tl.m);
             const int d = tl.d0 + grp, d_end = tl.d0 + tl.dn;
             const bool mask = ((n & 1) == 0) && d_end > (n >> 1) && tl.i_end > (n >> 1);
             // two copies of the loop so the LDS copy compiles to ds_read (no flat addressing)
             acc = in_lds ? row_tile<MODE>(lx, ly, lz, n, i, d, d_end, G, mask, wk.box, a, inv_dr, factors, wk.worm,
-                                          sd.slice)
+                                          tl.m)
                          : row_tile<MODE>(px, py, pz, n, i, d, d_end, G, mask, wk.box, a, inv_dr, factors, wk.worm,
-                                          sd.slice);
+                                          tl.m);
         }
         const double s = block_sum<kPairThreads>(acc, red);
         if (threadIdx.x == 0) partials[k] = s;
         tl = tl_next;
-        sd = sd_next;
     }
 }
 
@@ -528,7 +521,7 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
     dim3 g(G), b(kPairThreads);
     const size_t lds = sizeof(double) * 3 * (size_t)lds_beads;
     auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, g, b, lds, st, tiles, ranges, slices, walkers, pos, az, lds_beads, partials);
+        hipLaunchKernelGGL(kern, g, b, lds, st, tiles, ranges, walkers, pos, az, lds_beads, partials);
     };
     if (timer && timer->start) (void)hipEventRecord(timer->start, st);
     if (mode == 0)
@@ -552,8 +545,12 @@ int pair_grid_for(int lds_beads, int num_cus) {
     return 4 * std::max(1, num_cus);
 }
 
-int64_t append_row_tiles(int slice_index, int n, int pairs_per_lane, std::vector<RowTile>& tiles) {
+int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lane, std::vector<RowTile>& tiles) {
+    const int n = sd.n;
     if (n < 2) return 0;
+    auto tile = [&](int i0, int i_end, int d0, int dn) {
+        tiles.push_back(RowTile{slice_index, i0, i_end, d0, dn, n, sd.slice, sd.walker, sd.pos, sd.plane});
+    };
     const int R = n < kPairThreads ? n : kPairThreads;
     const int G = kPairThreads / R;
     const int dn = std::max(1, pairs_per_lane) * G;    // offsets per tile
@@ -561,15 +558,15 @@ int64_t append_row_tiles(int slice_index, int n, int pairs_per_lane, std::vector
         // one row block: offsets 1..n/2 in one run, d = n/2 of an even n masked to
         // rows i < n/2 inside the kernel (a tile of its own would mostly idle)
         const int dmax = n / 2;
-        for (int d0 = 1; d0 <= dmax; d0 += dn) tiles.push_back(RowTile{slice_index, 0, n, d0, std::min(dn, dmax - d0 + 1)});
+        for (int d0 = 1; d0 <= dmax; d0 += dn) tile(0, n, d0, std::min(dn, dmax - d0 + 1));
         return (int64_t)n * (n - 1) / 2;
     }
     const int full = (n - 1) / 2;                      // offsets taken by every row
     for (int i0 = 0; i0 < n; i0 += R) {
         const int i_end = std::min(i0 + R, n);
-        for (int d0 = 1; d0 <= full; d0 += dn) tiles.push_back(RowTile{slice_index, i0, i_end, d0, std::min(dn, full - d0 + 1)});
+        for (int d0 = 1; d0 <= full; d0 += dn) tile(i0, i_end, d0, std::min(dn, full - d0 + 1));
         if ((n & 1) == 0 && i0 < n / 2)                 // the d = n/2 pairs, once each
-            tiles.push_back(RowTile{slice_index, i0, std::min(i_end, n / 2), n / 2, 1});
+            tile(i0, std::min(i_end, n / 2), n / 2, 1);
     }
     return (int64_t)n * (n - 1) / 2;
 }

@@ -160,6 +160,12 @@ def main():
     dv_us_total = sum(sum(lat[s]) for s in segs)
     plain_dv_rate = dv_pairs_total / (dv_us_total * 1e-6)
     line["plain_delta_v_pair_evals_per_s"] = plain_dv_rate
+    # the same calls from a C++ caller (no Python in the loop), as the reference's moves make them
+    cxx = cxx_client(N, M, tau)
+    if cxx:
+        line["delta_v_cxx"] = cxx
+        plain_dv_rate = sum(pairs_dv[s] for s in segs) / (sum(cxx[f"dv{s}_mean_us"] for s in segs) * 1e-6)
+        line["plain_delta_v_pair_evals_per_s_cxx"] = plain_dv_rate
     if "batched" in legs:
         line["batched"] = batched_leg(pimc_amd, value)
     if "served" in legs:
@@ -170,6 +176,22 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(beads, L, M, N, tau, LAMBDA, pairs_full, pairs_dv)
     print(json.dumps(line))
+
+
+def cxx_client(N, M, tau, requests=2000):
+    """pimc_amd/dv_client: p50/p99 of pimc_delta_action and pimc_total_action from C++."""
+    import subprocess
+    exe = os.path.join(ROOT, "pimc_amd", "dv_client")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe, str(N), str(M), str(tau), "139853", str(requests)], capture_output=True, text=True,
+                           timeout=120)
+    except subprocess.TimeoutExpired:
+        return None
+    if r.returncode != 0:
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def batched_leg(pimc_amd, plain_value):
@@ -226,12 +248,13 @@ def served_leg(pimc_amd, plain_dv_rate, seconds):
         tau = 1.0 / (M * T)
         b = pimc_amd.api.gen_worldlines(SEED0 + 10000 + k, L, M, N, LAMBDA, tau, 1)
         srv.add_chain(L, M, N, tau, LAMBDA, L, b)
+    threads = 4   # client threads submitting (the chains split among them)
     out = {"workload": "1024 chains, N in [32,512], M in [64,512], segment moves of 4..64 slices + 10% CoM, "
-                       "Poisson arrivals", "rates": {}}
+                       "Poisson arrivals", "client_threads": threads, "rates": {}}
     for rate in (2e3, 2e4, 2e5):
-        st = srv.run_synthetic(1024, rate=rate, duration_s=seconds)
+        st = srv.run_synthetic(1024, rate=rate, duration_s=seconds, threads=threads)
         out["rates"][f"{int(rate)}"] = {k: float(v) for k, v in st.items()}
-    st = srv.run_synthetic(1024, rate=0.0, duration_s=seconds)
+    st = srv.run_synthetic(1024, rate=0.0, duration_s=seconds, threads=threads)
     out["closed_loop"] = {k: float(v) for k, v in st.items()}
     best = max([out["closed_loop"]] + list(out["rates"].values()), key=lambda d: d["pair_evals_per_s"])
     out["pair_evals_per_s"] = best["pair_evals_per_s"]

@@ -218,6 +218,7 @@ typedef struct {
     double com_sigma;   /* Gaussian CoM shift per axis [A] */
     double lambda;      /* segment proposal width sqrt(lambda tau) */
     uint32_t seed;
+    int threads;        /* client threads submitting (chains split among them); 0 = 1 */
 } pimc_load_desc;
 /* stats10 = {completed, req/s, pair evals/s, p50_us, p99_us, mean_us, accept frac,
  *            batches, requests per batch, seconds}; latency counts from arrival. */

@@ -42,7 +42,7 @@ class _ServerDesc(C.Structure):
 class _LoadDesc(C.Structure):
     _fields_ = [("n_chains", C.c_int), ("rate", C.c_double), ("duration_s", C.c_double), ("seg_min", C.c_int),
                 ("seg_max", C.c_int), ("p_com", C.c_double), ("com_sigma", C.c_double), ("lam", C.c_double),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("threads", C.c_int)]
 
 
 class _ChainDesc(C.Structure):
@@ -466,8 +466,8 @@ class Server:
         return dict(batches=b.value, requests=r.value, pair_evals=p.value, free_bytes=f.value)
 
     def run_synthetic(self, n_chains, rate, duration_s, seg=(4, 64), p_com=0.1, com_sigma=0.1, lam=6.0596,
-                      seed=139853):
-        d = _LoadDesc(n_chains, rate, duration_s, seg[0], seg[1], p_com, com_sigma, lam, seed)
+                      seed=139853, threads=1):
+        d = _LoadDesc(n_chains, rate, duration_s, seg[0], seg[1], p_com, com_sigma, lam, seed, threads)
         o = np.zeros(10)
         self._check(lib().pimc_server_run_synthetic(self.h, C.byref(d), _p(o, _dp)), "run_synthetic")
         keys = ["completed", "req_per_s", "pair_evals_per_s", "p50_us", "p99_us", "mean_us", "accept_frac",

@@ -49,8 +49,20 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_dv_client()
     _build_adapter_check()
     return LIB
+
+
+def _build_dv_client():
+    """pimc_amd/dv_client: a C++ caller timing pimc_delta_action / pimc_total_action."""
+    src = os.path.join(SRC, "dv_client.cpp")
+    out = os.path.join(HERE, "dv_client")
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(HERE, "..", "include"), src, "-o", out,
+           "-L", HERE, "-lpimc_gpu", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"dv_client build failed:\n{r.stderr}")
 
 
 def _build_adapter_check():

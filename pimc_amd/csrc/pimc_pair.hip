@@ -221,64 +221,107 @@ __device__ void cell_update_bead(int m, int i, int c_new, int nalloc, int num_sl
 
 
 
+// One moved bead of a Vnn request: its slice, old and new positions and cells.
+struct VnnSubject {
+    int m, me;
+    double ox, oy, oz, nx, ny, nz;
+    int c_old, c_new, s1;
+    bool has_new, factors, me_on, use_cells;
+    const double *px, *py, *pz;
+};
+
+__device__ __forceinline__ VnnSubject vnn_subject(const VnnArgs& A, const VnnBead& q) {
+    VnnSubject v;
+    v.m = q.slice;
+    v.me = q.ptcl;
+    v.px = A.pos + (int64_t)v.m * A.nalloc;
+    v.py = v.px + A.plane;
+    v.pz = v.py + A.plane;
+    const bool has_old = q.flags & 1;
+    v.has_new = q.flags & 2;
+    v.ox = has_old ? q.old_pos[0] : v.px[v.me];
+    v.oy = has_old ? q.old_pos[1] : v.py[v.me];
+    v.oz = has_old ? q.old_pos[2] : v.pz[v.me];
+    v.nx = q.new_pos[0];
+    v.ny = q.new_pos[1];
+    v.nz = q.new_pos[2];
+    v.use_cells = A.box.ngrid >= 4;
+    v.c_old = v.c_new = 0;
+    if (v.use_cells) {
+        v.c_old = has_old ? cell_of(v.ox, v.oy, v.oz, A.box) : A.cell[v.m * A.nalloc + v.me];
+        v.c_new = cell_of(v.nx, v.ny, v.nz, A.box);
+    }
+    v.s1 = bead_state(A.worm, v.m, v.me);
+    v.factors = worm_touches_slice(A.worm, v.m);
+    v.me_on = A.on[v.m * A.nalloc + v.me] != 0;
+    return v;
+}
+
+// Partner j's terms of Vnn(bead) at the old and at the new position
+// (src/action.cpp:1044-1060 over the interaction list, src/lookuptable.cpp:464-516).
+template <int MODE>
+__device__ __forceinline__ void vnn_partner(const VnnArgs& A, const VnnSubject& v, int j, const AzizDev& az,
+                                            double inv_dr, double& acc_o, double& acc_n) {
+    const bool on = (A.on[v.m * A.nalloc + j] != 0) && (j != v.me) && v.me_on;
+    const double xj = v.px[j], yj = v.py[j], zj = v.pz[j];
+    double f = 1.0;
+    if (v.factors) f = pair_factor(v.s1, bead_state(A.worm, v.m, j));
+    int cj = 0;
+    if (v.use_cells) cj = A.cell[v.m * A.nalloc + j];
+    {
+        const double r2 = sep_r2<MODE>(v.ox, v.oy, v.oz, xj, yj, zj, A.box);
+        const bool ok = on && (r2 < A.box.rc2) && (!v.use_cells || cells_adjacent(v.c_old, cj, A.box.ngrid));
+        const double val = pair_value_from_r2<MODE>(r2, az, inv_dr);
+        acc_o += ok ? f * val : 0.0;
+    }
+    if (v.has_new) {
+        const double r2 = sep_r2<MODE>(v.nx, v.ny, v.nz, xj, yj, zj, A.box);
+        const bool ok = on && (r2 < A.box.rc2) && (!v.use_cells || cells_adjacent(v.c_new, cj, A.box.ngrid));
+        const double val = pair_value_from_r2<MODE>(r2, az, inv_dr);
+        acc_n += ok ? f * val : 0.0;
+    }
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kVnnThreads) void k_vnn(VnnArgs A, VnnInline req, AzizDev az) {
-    __shared__ double red[kVnnThreads / 64];
+    static_assert(kVnnThreads == 256, "four waves per Vnn workgroup");
+    __shared__ double red[kVnnThreads / 64][2];
     const int b = blockIdx.y;
     const int c = blockIdx.x;
     const VnnBead q = A.beads ? A.beads[b] : req.beads[b];
-    const int m = q.slice;
-    const int me = q.ptcl;
-    const BoxDev box = A.box;
+    const VnnSubject v = vnn_subject(A, q);
     const double inv_dr = 1.0 / az.dr;
-    const double* px = A.pos + (int64_t)m * A.nalloc;
-    const double* py = px + A.plane;
-    const double* pz = py + A.plane;
-    const bool has_old = q.flags & 1;
-    const bool has_new = q.flags & 2;
-    double ox = q.old_pos[0], oy = q.old_pos[1], oz = q.old_pos[2];
-    if (!has_old) {
-        ox = px[me];
-        oy = py[me];
-        oz = pz[me];
-    }
-    const double nx = q.new_pos[0], ny = q.new_pos[1], nz = q.new_pos[2];
-    const bool use_cells = box.ngrid >= 4;
-    int c_old = 0, c_new = 0;
-    if (use_cells) {
-        c_old = has_old ? cell_of(ox, oy, oz, box) : A.cell[m * A.nalloc + me];
-        c_new = cell_of(nx, ny, nz, box);
-    }
-    const int s1 = bead_state(A.worm, m, me);
-    const bool factors = worm_touches_slice(A.worm, m);
-    const bool me_on = A.on[m * A.nalloc + me] != 0;
     double acc_o = 0.0, acc_n = 0.0;
     const int j_end = min((c + 1) * A.chunk, A.nalloc);
-    for (int j = c * A.chunk + threadIdx.x; j < j_end; j += kVnnThreads) {
-        const bool on = (A.on[m * A.nalloc + j] != 0) && (j != me) && me_on;
-        const double xj = px[j], yj = py[j], zj = pz[j];
-        double f = 1.0;
-        if (factors) f = pair_factor(s1, bead_state(A.worm, m, j));
-        int cj = 0;
-        if (use_cells) cj = A.cell[m * A.nalloc + j];
-        {
-            const double r2 = sep_r2<MODE>(ox, oy, oz, xj, yj, zj, box);
-            const bool ok = on && (r2 < box.rc2) && (!use_cells || cells_adjacent(c_old, cj, box.ngrid));
-            const double v = pair_value_from_r2<MODE>(r2, az, inv_dr);
-            acc_o += ok ? f * v : 0.0;
-        }
-        if (has_new) {
-            const double r2 = sep_r2<MODE>(nx, ny, nz, xj, yj, zj, box);
-            const bool ok = on && (r2 < box.rc2) && (!use_cells || cells_adjacent(c_new, cj, box.ngrid));
-            const double v = pair_value_from_r2<MODE>(r2, az, inv_dr);
-            acc_n += ok ? f * v : 0.0;
-        }
+    for (int j = c * A.chunk + threadIdx.x; j < j_end; j += kVnnThreads) vnn_partner<MODE>(A, v, j, az, inv_dr, acc_o, acc_n);
+    // both sums in one pass: wave trees, then the four waves in order
+    acc_o = wave_sum(acc_o);
+    acc_n = wave_sum(acc_n);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[wave][0] = acc_o;
+        red[wave][1] = acc_n;
     }
-    const double so = block_sum<kVnnThreads>(acc_o, red);
-    const double sn = block_sum<kVnnThreads>(acc_n, red);
-    if (threadIdx.x == 0) {
-        A.partials[((int64_t)b * A.chunks + c) * 2 + 0] = so;
-        A.partials[((int64_t)b * A.chunks + c) * 2 + 1] = sn;
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int r = threadIdx.x;
+        A.partials[((int64_t)b * A.chunks + c) * 2 + r] = ((red[0][r] + red[1][r]) + red[2][r]) + red[3][r];
+    }
+}
+
+// The committed bead updates of a request, one lane per bead (the beads sit on
+// distinct slices): Path::updateBead with LookupTable::updateBead.
+__device__ __forceinline__ void vnn_commit(const VnnArgs& A, const VnnInline& req, int nthreads) {
+    for (int k = threadIdx.x; k < A.n_beads; k += nthreads) {
+        const VnnBead qb = A.beads ? A.beads[k] : req.beads[k];
+        if (!(qb.flags & 4)) continue;
+        const int mm = qb.slice, ii = qb.ptcl;
+        const int cn = cell_of(qb.new_pos[0], qb.new_pos[1], qb.new_pos[2], A.box);
+        cell_update_bead(mm, ii, cn, A.nalloc, A.num_slices, A.cell, A.label, A.numlab, A.hash);
+        double* qx = A.pos + (int64_t)mm * A.nalloc + ii;
+        qx[0] = qb.new_pos[0];
+        qx[A.plane] = qb.new_pos[1];
+        qx[2 * A.plane] = qb.new_pos[2];
     }
 }
 
@@ -315,19 +358,7 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
     __syncthreads();
     if (threadIdx.x == 0)
         for (int r = 0; r < 3; ++r) A.out[r] = ((red[r][0] + red[r][1]) + red[r][2]) + red[r][3];
-    if (A.commit_in_kernel) {
-        for (int k = threadIdx.x; k < A.n_beads; k += 256) {
-            const VnnBead qb = A.beads ? A.beads[k] : req.beads[k];
-            if (!(qb.flags & 4)) continue;
-            const int mm = qb.slice, ii = qb.ptcl;
-            const int cn = cell_of(qb.new_pos[0], qb.new_pos[1], qb.new_pos[2], A.box);
-            cell_update_bead(mm, ii, cn, A.nalloc, A.num_slices, A.cell, A.label, A.numlab, A.hash);
-            double* qx = A.pos + (int64_t)mm * A.nalloc + ii;
-            qx[0] = qb.new_pos[0];
-            qx[A.plane] = qb.new_pos[1];
-            qx[2 * A.plane] = qb.new_pos[2];
-        }
-    }
+    if (A.commit_in_kernel) vnn_commit(A, req, 256);
     if (A.done) {
         __syncthreads();
         if (threadIdx.x == 0) publish_done(A.done, A.seq);

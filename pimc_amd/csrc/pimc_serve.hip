@@ -245,6 +245,7 @@ struct Request {
     std::vector<VnnBead> beads;
     Clock::time_point t_submit, t_done;
     bool done = false;
+    int tag = -1;                     // load-generator thread that submitted it (-1: none)
     double out[4] = {0, 0, 0, 0};
 };
 
@@ -299,7 +300,7 @@ struct pimc_server {
     int kernel_error = 0;
     std::string kernel_error_msg;
     bool track_done = false;
-    std::vector<uint64_t> done_list;   // completed request ids (load generator)
+    std::vector<std::vector<uint64_t>> done_lists;   // completed request ids per load-generator thread
 
     std::vector<ServeReqDev> pack_rq;
     std::vector<VnnBead> pack_bd;
@@ -540,7 +541,7 @@ void pimc_server::complete(Slot& S) {
         for (int t = 0; t < 4; ++t) R.out[t] = S.h_out[4 * k + t];
         R.done = true;
         R.t_done = now;
-        if (track_done) done_list.push_back(R.id);
+        if (track_done && R.tag >= 0 && R.tag < (int)done_lists.size()) done_lists[R.tag].push_back(R.id);
         Chain& C = chains[R.chain];
         C.busy = false;
         if (!C.queue.empty() && C.queue.front() == R.id) C.queue.pop_front();
@@ -737,8 +738,10 @@ int pimc_server_download(pimc_server* s, int chain, double* beads) {
     return PIMC_OK;
 }
 
-int pimc_server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos, int n,
-                       int shift, const double vfactor[2], int commit_mode, double u, uint64_t* req_id) {
+}  // extern "C"
+
+static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos, int n,
+                         int shift, const double vfactor[2], int commit_mode, double u, int tag, uint64_t* req_id) {
     if (!s || !slice_ptcl || !new_pos || n < 1 || !req_id || shift < 1 || commit_mode < 0 || commit_mode > 2)
         return PIMC_E_ARG;
     std::lock_guard<std::mutex> g(s->mu);
@@ -774,6 +777,7 @@ int pimc_server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, con
         }
     }
     R.t_submit = Clock::now();
+    R.tag = tag;
     const uint64_t id = R.id;
     s->reqs.emplace(id, std::move(R));
     c.queue.push_back(id);
@@ -781,6 +785,13 @@ int pimc_server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, con
     *req_id = id;
     s->cv_work.notify_one();
     return PIMC_OK;
+}
+
+extern "C" {
+
+int pimc_server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos, int n,
+                       int shift, const double vfactor[2], int commit_mode, double u, uint64_t* req_id) {
+    return server_submit(s, chain, slice_ptcl, new_pos, n, shift, vfactor, commit_mode, u, -1, req_id);
 }
 
 // out5 = {A_old, A_new, dU, accepted, latency_us}; releases the request.
@@ -822,6 +833,9 @@ int pimc_server_stats(pimc_server* s, int64_t* batches, int64_t* requests, int64
 
 // ===========================================================================
 // Synthetic load: Poisson arrivals of Delta V requests over the resident chains.
+// T generator threads, thread t owning the chains c = t (mod T), each with its
+// own random stream, arrival process (rate / T) and completion list, so
+// submissions are not limited by one host thread.
 // ===========================================================================
 namespace {
 
@@ -834,48 +848,28 @@ struct GenChain {
     std::vector<double> prop;
 };
 
-}  // namespace
+struct GenStats {
+    int64_t completed = 0, accepted = 0, pairs = 0;
+    std::vector<double> lat;
+    int err = 0;
+};
 
-extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L, double* stats) {
-    if (!s || !L || !stats || L->n_chains < 1 || L->duration_s <= 0 || L->seg_min < 2 || L->seg_max < L->seg_min)
-        return PIMC_E_ARG;
-    const int nc = L->n_chains;
-    std::vector<GenChain> gc(nc);
-    {
-        std::lock_guard<std::mutex> g(s->mu);
-        if (nc > (int)s->chains.size()) return sfail(s, PIMC_E_ARG, "more load chains than server chains");
-        s->track_done = true;
-        s->done_list.clear();
-    }
-    struct TrackOff {
-        pimc_server* s;
-        ~TrackOff() {
-            std::lock_guard<std::mutex> g(s->mu);
-            s->track_done = false;
-            s->done_list.clear();
-        }
-    } track_off{s};
-    for (int c = 0; c < nc; ++c) {
-        const pimc_desc& d = s->chains[c].d;
-        gc[c].mirror.resize(3 * (size_t)d.num_slices * d.num_alloc);
-        int r = pimc_server_download(s, c, gc[c].mirror.data());
-        if (r) return r;
-    }
-    std::mt19937_64 rng(L->seed);
+void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vector<GenChain>& gc,
+                Clock::time_point t0, Clock::time_point t_end, GenStats& st) {
+    std::mt19937_64 rng(L->seed + 7919ull * (uint64_t)t);
     std::uniform_real_distribution<double> U(0.0, 1.0);
     std::normal_distribution<double> G(0.0, 1.0);
-    std::exponential_distribution<double> Exp(L->rate > 0 ? L->rate : 1.0);
     const bool closed = L->rate <= 0;
-    std::vector<double> lat;
-    lat.reserve(1 << 20);
-    int64_t completed = 0, accepted = 0, pairs = 0;
-    int64_t b0 = 0, r0 = 0;
-    pimc_server_stats(s, &b0, &r0, nullptr, nullptr);
-    const auto t0 = Clock::now();
-    const auto t_end = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(L->duration_s));
+    std::exponential_distribution<double> Exp(closed ? 1.0 : L->rate / T);
+    std::vector<int> mine;
+    for (int c = t; c < (int)gc.size(); c += T) mine.push_back(c);
+    if (mine.empty()) return;
+    st.lat.reserve(1 << 18);
     auto next_arrival = t0;
     int inflight = 0;
     bool drained = false;
+    std::unordered_map<uint64_t, int> by_id;
+    std::deque<int> runnable;   // own chains with an arrival waiting and nothing in flight
     auto propose = [&](int c) {
         GenChain& g = gc[c];
         const pimc_desc& d = s->chains[c].d;
@@ -890,8 +884,8 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
             for (int m = 0; m < M; ++m) {
                 g.sp.push_back(m);
                 g.sp.push_back(p);
-                for (int t = 0; t < 3; ++t) {
-                    double x = g.mirror[3 * ((size_t)m * N + p) + t] + sh[t];
+                for (int k = 0; k < 3; ++k) {
+                    double x = g.mirror[3 * ((size_t)m * N + p) + k] + sh[k];
                     x -= d.side * std::floor(x * (1.0 / d.side) + 0.5);
                     g.prop.push_back(x);
                 }
@@ -905,22 +899,20 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
                 const int m = (m0 + k) % M;
                 g.sp.push_back(m);
                 g.sp.push_back(p);
-                for (int t = 0; t < 3; ++t) {
-                    double x = g.mirror[3 * ((size_t)m * N + p) + t] + sd * G(rng);
+                for (int a = 0; a < 3; ++a) {
+                    double x = g.mirror[3 * ((size_t)m * N + p) + a] + sd * G(rng);
                     x -= d.side * std::floor(x * (1.0 / d.side) + 0.5);
                     g.prop.push_back(x);
                 }
             }
         }
     };
-    std::unordered_map<uint64_t, int> by_id;
-    std::deque<int> runnable;   // chains with an arrival waiting and nothing in flight
     auto submit = [&](int c, Clock::time_point t_arr) -> int {
         GenChain& g = gc[c];
         propose(c);
         uint64_t id = 0;
         const double vf[2] = {1.0, 1.0};
-        int r = pimc_server_submit(s, c, g.sp.data(), g.prop.data(), (int)g.sp.size() / 2, 1, vf, 1, U(rng), &id);
+        int r = server_submit(s, c, g.sp.data(), g.prop.data(), (int)g.sp.size() / 2, 1, vf, 1, U(rng), t, &id);
         if (r) return r;
         g.inflight = id;
         g.t_arrival = t_arr;
@@ -929,27 +921,29 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
         return 0;
     };
     if (closed)
-        for (int c = 0; c < nc; ++c) {
-            int r = submit(c, Clock::now());
-            if (r) return r;
-        }
+        for (int c : mine)
+            if ((st.err = submit(c, Clock::now()))) return;
+    std::vector<uint64_t> finished;
     while (true) {
         const auto now = Clock::now();
         const bool open = now < t_end;
         if (!closed && open) {
             while (next_arrival <= now) {
-                const int c = (int)(U(rng) * nc) % nc;
+                const int c = mine[(size_t)(U(rng) * mine.size()) % mine.size()];
                 gc[c].arrivals.push_back(next_arrival);
                 if (!gc[c].inflight && gc[c].arrivals.size() == 1) runnable.push_back(c);
                 next_arrival += std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(Exp(rng)));
             }
         }
-        // completions, in the order the server finished them
-        std::vector<uint64_t> finished;
+        // completions of this thread's requests, in the order the server finished them
+        finished.clear();
         {
             std::lock_guard<std::mutex> g(s->mu);
-            finished.swap(s->done_list);
-            if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
+            finished.swap(s->done_lists[t]);
+            if (s->kernel_error) {
+                st.err = sfail(s, s->kernel_error, s->kernel_error_msg);
+                return;
+            }
         }
         for (uint64_t id : finished) {
             auto it = by_id.find(id);
@@ -958,27 +952,24 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
             by_id.erase(it);
             GenChain& g = gc[c];
             double o[5];
-            int r = pimc_server_wait(s, id, o);
-            if (r) return r;
+            if ((st.err = pimc_server_wait(s, id, o))) return;
             const auto t_done = Clock::now();
-            lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
-            ++completed;
+            st.lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
+            ++st.completed;
             const pimc_desc& d = s->chains[c].d;
-            pairs += 2LL * ((int64_t)g.sp.size() / 2) * (d.num_alloc - 1);
+            st.pairs += 2LL * ((int64_t)g.sp.size() / 2) * (d.num_alloc - 1);
             if (o[3] > 0.5) {
-                ++accepted;
+                ++st.accepted;
                 for (size_t k = 0; k < g.sp.size() / 2; ++k) {
                     const size_t bi = (size_t)g.sp[2 * k] * d.num_alloc + g.sp[2 * k + 1];
-                    for (int t = 0; t < 3; ++t) g.mirror[3 * bi + t] = g.prop[3 * k + t];
+                    for (int a = 0; a < 3; ++a) g.mirror[3 * bi + a] = g.prop[3 * k + a];
                 }
             }
             g.inflight = 0;
             --inflight;
             if (!closed && !g.arrivals.empty()) runnable.push_back(c);
-            if (closed && open) {
-                r = submit(c, Clock::now());
-                if (r) return r;
-            }
+            if (closed && open)
+                if ((st.err = submit(c, Clock::now()))) return;
         }
         while (!runnable.empty()) {
             const int c = runnable.front();
@@ -987,24 +978,74 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
             if (!g.inflight && !g.arrivals.empty()) {
                 const auto ta = g.arrivals.front();
                 g.arrivals.pop_front();
-                int r = submit(c, ta);
-                if (r) return r;
+                if ((st.err = submit(c, ta))) return;
             }
         }
         if (finished.empty()) std::this_thread::yield();
         if (!open && inflight == 0) break;
-        if (!open && now - t_end > std::chrono::seconds(30))
-            return sfail(s, PIMC_E_STATE, "synthetic load: requests still pending 30 s after the load window");
+        if (!open && now - t_end > std::chrono::seconds(30)) {
+            st.err = sfail(s, PIMC_E_STATE, "synthetic load: requests still pending 30 s after the load window");
+            return;
+        }
         if (!open && !drained) {
             // stop accepting arrivals; drop the ones not yet submitted
-            for (auto& g : gc) g.arrivals.clear();
+            for (int c : mine) gc[c].arrivals.clear();
             runnable.clear();
             drained = true;
         }
     }
+}
+
+}  // namespace
+
+extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L, double* stats) {
+    if (!s || !L || !stats || L->n_chains < 1 || L->duration_s <= 0 || L->seg_min < 2 || L->seg_max < L->seg_min ||
+        L->threads < 0 || L->threads > 64)
+        return PIMC_E_ARG;
+    const int nc = L->n_chains;
+    const int T = std::max(1, std::min(L->threads, nc));
+    std::vector<GenChain> gc(nc);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (nc > (int)s->chains.size()) return sfail(s, PIMC_E_ARG, "more load chains than server chains");
+        s->done_lists.assign(T, {});
+        s->track_done = true;
+    }
+    struct TrackOff {
+        pimc_server* s;
+        ~TrackOff() {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->track_done = false;
+            s->done_lists.clear();
+        }
+    } track_off{s};
+    for (int c = 0; c < nc; ++c) {
+        const pimc_desc& d = s->chains[c].d;
+        gc[c].mirror.resize(3 * (size_t)d.num_slices * d.num_alloc);
+        int r = pimc_server_download(s, c, gc[c].mirror.data());
+        if (r) return r;
+    }
+    int64_t b0 = 0, r0 = 0;
+    pimc_server_stats(s, &b0, &r0, nullptr, nullptr);
+    const auto t0 = Clock::now();
+    const auto t_end = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(L->duration_s));
+    std::vector<GenStats> st(T);
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back(gen_thread, s, L, t, T, std::ref(gc), t0, t_end, std::ref(st[t]));
+    for (auto& x : th) x.join();
+    for (auto& x : st)
+        if (x.err) return x.err;
     const double secs = std::chrono::duration<double>(Clock::now() - t0).count();
     int64_t b1 = 0, r1 = 0;
     pimc_server_stats(s, &b1, &r1, nullptr, nullptr);
+    std::vector<double> lat;
+    int64_t completed = 0, accepted = 0, pairs = 0;
+    for (auto& x : st) {
+        lat.insert(lat.end(), x.lat.begin(), x.lat.end());
+        completed += x.completed;
+        accepted += x.accepted;
+        pairs += x.pairs;
+    }
     std::sort(lat.begin(), lat.end());
     auto pct = [&](double q) { return lat.empty() ? 0.0 : lat[std::min(lat.size() - 1, (size_t)(q * (lat.size() - 1)))]; };
     double mean = 0;

@@ -168,3 +168,12 @@ def test_server_synthetic_load(orc):
     assert st["completed"] > 100 and st["p99_us"] >= st["p50_us"] > 0
     st2 = srv.run_synthetic(32, rate=0.0, duration_s=0.3)
     assert st2["completed"] > 100 and st2["req_per_batch"] >= 1.0
+    # several client threads (chains split among them), open and closed loop
+    st3 = srv.run_synthetic(32, rate=20000.0, duration_s=0.3, threads=4)
+    assert st3["completed"] > 100 and st3["p99_us"] >= st3["p50_us"] > 0
+    st4 = srv.run_synthetic(32, rate=0.0, duration_s=0.3, threads=4)
+    assert st4["completed"] > 100 and 0.0 < st4["accept_frac"] < 1.0
+    # the chains come back whole after concurrent clients
+    for c in (0, 7, 31):
+        b = srv.download(c)
+        assert np.all(np.isfinite(b))

@@ -65,6 +65,15 @@ int  pimc_ctx_create(const pimc_desc* desc, pimc_ctx** out);
 void pimc_ctx_destroy(pimc_ctx* ctx);
 const char* pimc_last_error(const pimc_ctx* ctx);   /* NULL ctx: last global error */
 int  pimc_set_v_mode(pimc_ctx* ctx, int v_mode);
+/* LocalAction's factors (src/setup.cpp:1232-1256): VFactor[2] (used where a call
+ * passes vfactor = NULL), gradVFactor[2] of the GSF / Li-Broughton correction
+ * and the estimator period.  With a finite gradVFactor the total action adds
+ * gradVFactor tau^3 lambda gradVSquared(slice) per slice (src/action.cpp:456-472),
+ * every bead action at shift 1 adds gradVFactor tau^3 lambda gradVnnSquared(bead)
+ * (:487-620, :1282-1337), and pimc_energy uses the corrected tau and lambda
+ * derivatives (:751-802).  Primitive action: {1,1}, {0,0}, 1 (the default);
+ * GSF: {2/3, 4/3}, {0, 2/9}, 2; Li-Broughton: {1,1}, {1/12, 1/12}, 2. */
+int  pimc_set_action_factors(pimc_ctx* ctx, const double vfactor[2], const double gradvfactor[2], int period);
 
 /* Upload a worldline configuration.  Replaces the Path constructor
  * (src/path.cpp:28-129) / loadState + leftPack + LookupTable::updateGrid

@@ -131,6 +131,7 @@ struct Box {
 // (include/potential.h:163-260).
 // ---------------------------------------------------------------------------
 struct Aziz {
+    int year = 1979;
     double rm = 0, A = 0, epsilon = 0, alpha = 0, beta = 0, D = 0, C6 = 0, C8 = 0, C10 = 0;
     double dr = 0;
     int tableLength = 0;
@@ -210,7 +211,8 @@ struct Aziz {
         double T4 = -(C6 * ix6 + C8 * ix8 + C10 * ix10) * d2F(x);
         return ((epsilon / (rm * rm)) * (T1 + T2 + T3 + T4));
     }
-    void init(int year, double maxSep, double rc, bool derivs) {
+    void init(int year_, double maxSep, double rc, bool derivs) {
+        year = year_;
         setYear(year);
         // initLookupTable((1.0E-6)*rm, maxSep): accumulated r += dr (potential.h:163-183)
         dr = (1.0E-6) * rm;
@@ -416,6 +418,8 @@ struct State {
     Lookup lookup;
     int shift = 1;
     double VFactor[2] = {1.0, 1.0};
+    double gradVFactor[2] = {0.0, 0.0};   // GSF / Li-Broughton correction (setup.cpp:1232-1256)
+    int period = 1;
     std::array<int, kNPCFSEP> sepHist{};
     double dSep = 0;
 
@@ -540,10 +544,66 @@ struct State {
         for (double f : extFactors) totVext += f * 0.0;
         return (totVext + totVint);
     }
-    double actionBead(const Bead& b) {
-        // action.cpp:487-514 (primitive: no gradient correction)
+    double gradVSquared(int slice) const {
+        // action.cpp:1188-1223: sum over the slice's beads of |sum_j gradV(r1 - r2)|^2
+        // (the free external potential has gradV = 0)
+        double totF2 = 0.0;
+        int n = numBeadsAtSlice[slice];
+        Bead b1{slice, 0}, b2{slice, 0};
+        for (b1[1] = 0; b1[1] < n; b1[1]++) {
+            Vec F{0.0, 0.0, 0.0};
+            for (b2[1] = 0; b2[1] < n; b2[1]++) {
+                if (!(b1 == b2)) {
+                    Vec g = aziz.gradV(getSeparation(b1, b2));
+                    for (int i = 0; i < 3; i++) F[i] += g[i];
+                }
+            }
+            Vec Fext{0.0, 0.0, 0.0};
+            for (int i = 0; i < 3; i++) F[i] += Fext[i];
+            totF2 += dot3(F, F);
+        }
+        return totF2;
+    }
+    double gradVnnSquared(const Bead& b1) const {
+        // action.cpp:1282-1337, over the interaction list left by the preceding Vnn(b1)
+        double totF2 = 0.0;
+        if (beadOn(b1)) {
+            const Vec Fext1{0.0, 0.0, 0.0};
+            Vec Fint1{0.0, 0.0, 0.0};
+            for (int n = 0; n < lookup.numBeads; n++) {
+                const Bead& b2 = lookup.beadList[n];
+                if (!(b1 == b2)) {
+                    const Vec Fint2 = aziz.gradV(lookup.beadSep[n]);
+                    for (int i = 0; i < 3; i++) Fint1[i] -= Fint2[i];
+                    const Vec Fext2{0.0, 0.0, 0.0};
+                    Vec Fint3{0.0, 0.0, 0.0};
+                    for (int m = 0; m < lookup.numBeads; m++) {
+                        const Bead& b3 = lookup.beadList[m];
+                        if (!(b3 == b2) && !(b3 == b1)) {
+                            Vec g = aziz.gradV(getSeparation(b2, b3));
+                            for (int i = 0; i < 3; i++) Fint3[i] += g[i];
+                        }
+                    }
+                    totF2 += dot3(Fint2, Fint2) + 2.0 * dot3(Fext2, Fint2) + 2.0 * dot3(Fint2, Fint3);
+                }
+            }
+            totF2 += dot3(Fext1, Fext1) + 2.0 * dot3(Fext1, Fint1) + dot3(Fint1, Fint1);
+        }
+        return totF2;
+    }
+    // the correction of potentialAction(bead) (action.cpp:497-503): at shift 1 only
+    double correction(const Bead& b) const {
         int eo = b[0] % 2;
-        return VFactor[eo] * tauShift() * Vnn(b);
+        if ((shift == 1) && (gradVFactor[eo] > kEPS))
+            return gradVFactor[eo] * tauShift() * tauShift() * tauShift() * lambda * gradVnnSquared(b);
+        return 0.0;
+    }
+    double actionBead(const Bead& b) {
+        // action.cpp:487-514
+        int eo = b[0] % 2;
+        double bareU = VFactor[eo] * tauShift() * Vnn(b);
+        double corU = beadOn(b) ? correction(b) : 0.0;
+        return bareU + corU;
     }
     double actionList(const std::vector<Bead>& beadsIn) {
         // action.cpp:572-620
@@ -558,7 +618,8 @@ struct State {
         for (size_t i = 0; i < active.size(); ++i) {
             int eo = active[i][0] % 2;
             double bareU = VFactor[eo] * tauShift() * Vnn(active[i], ext[i]);
-            totU += bareU + 0.0;
+            double corU = correction(active[i]);   // uses the list Vnn just built
+            totU += bareU + corU;
         }
         return totU;
     }
@@ -583,6 +644,8 @@ struct State {
             s += v[0];
             s += v[1];
             totU += VFactor[eo] * tauShift() * s;
+            if (gradVFactor[eo] > kEPS)
+                totU += gradVFactor[eo] * tauShift() * tauShift() * tauShift() * lambda * gradVSquared(slice);
         }
         return totU;
     }
@@ -1103,6 +1166,21 @@ double orc_action_list(void* p, const int32_t* b, int n) {
 void orc_update_bead(void* p, int slice, int ptcl, const double* pos3) {
     static_cast<State*>(p)->updateBead(Bead{slice, ptcl}, Vec{pos3[0], pos3[1], pos3[2]});
 }
+void orc_state_set_correction(void* p, double gvf0, double gvf1, int period) {
+    State* st = static_cast<State*>(p);
+    st->gradVFactor[0] = gvf0;
+    st->gradVFactor[1] = gvf1;
+    st->period = period < 1 ? 1 : period;
+    if ((gvf0 > kEPS || gvf1 > kEPS) && st->aziz.tdV.empty())
+        st->aziz.init(st->aziz.year, st->box.maxSep, st->rc, true);   // gradV needs lookupdVdr
+}
+double orc_gradV2_slice(void* p, int slice) { return static_cast<State*>(p)->gradVSquared(slice); }
+double orc_gradVnn2(void* p, int slice, int ptcl) {
+    State* st = static_cast<State*>(p);
+    Bead b{slice, ptcl};
+    st->lookup.updateInteractionList(*st, b);
+    return st->gradVnnSquared(b);
+}
 
 void orc_energy(void* p, double* out) {
     // EnergyEstimator::accumulate (estimator.cpp:940-1029), PIMC, primitive/local action
@@ -1127,15 +1205,22 @@ void orc_energy(void* p, double* out) {
     totK *= kinNorm;
     double t1 = 0.0, t2 = 0.0;
     std::array<double, 2> totVop{0.0, 0.0};
-    const int period = 1;
+    const int period = st->period;
+    const double tau = st->tauShift();
     for (int slice = 0; slice < nts; slice++) {
-        t1 += 1.0 * 0.0;  // derivPotentialActionLambda: no gradient correction
         int eo = slice % 2;
+        const bool cor = st->gradVFactor[eo] > kEPS;
+        const double g2 = cor ? st->gradVSquared(slice) : 0.0;
+        // derivPotentialActionLambda (action.cpp:795-802)
+        t1 += 1.0 * (cor ? st->gradVFactor[eo] * tau * tau * tau * g2 : 0.0);
         auto v = st->Vslice(slice);
         double s = 0.0;
         s += v[0];
         s += v[1];
-        t2 += 1.0 * (st->VFactor[eo] * s);
+        // derivPotentialActionTau (action.cpp:751-767)
+        double dU = st->VFactor[eo] * s;
+        if (cor) dU += 3.0 * st->gradVFactor[eo] * tau * tau * st->lambda * g2;
+        t2 += 1.0 * dU;
         if (!(slice % period)) {
             auto w = st->Vslice(slice);
             totVop[0] += 1.0 * w[0];

@@ -76,6 +76,11 @@ double orc_action_bead(void* st, int slice, int ptcl);  /* potentialAction(bead)
 double orc_action_range(void* st, const int32_t* start2, const int32_t* end2); /* :522-570 */
 double orc_action_list(void* st, const int32_t* beads2, int n); /* potentialAction(vector) :572-620 */
 void   orc_update_bead(void* st, int slice, int ptcl, const double* pos3); /* Path::updateBead path.cpp:197-200 */
+/* GSF / Li-Broughton correction factors gradVFactor[2] and the estimator period
+ * (src/setup.cpp:1232-1256); builds the dV/dr table when a factor is finite. */
+void   orc_state_set_correction(void* st, double gvf0, double gvf1, int period);
+double orc_gradV2_slice(void* st, int slice);             /* gradVSquared(slice) action.cpp:1188-1223 */
+double orc_gradVnn2(void* st, int slice, int ptcl);       /* gradVnnSquared(bead) action.cpp:1282-1337 */
 /* EnergyEstimator::accumulate (estimator.cpp:940-1029), primitive PIMC:
  * out = {K, V, V_ext, V_int, E, K/N, V/N, E/N} for one sample. */
 void   orc_energy(void* st, double* out8);

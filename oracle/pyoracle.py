@@ -69,6 +69,9 @@ def lib():
             "orc_action_range": (C.c_double, [C.c_void_p, ip, ip]),
             "orc_action_list": (C.c_double, [C.c_void_p, ip, C.c_int]),
             "orc_update_bead": (None, [C.c_void_p, C.c_int, C.c_int, dp]),
+            "orc_state_set_correction": (None, [C.c_void_p, C.c_double, C.c_double, C.c_int]),
+            "orc_gradV2_slice": (C.c_double, [C.c_void_p, C.c_int]),
+            "orc_gradVnn2": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
             "orc_energy": (None, [C.c_void_p, dp]),
             "orc_ssf": (None, [C.c_void_p, dp, C.c_int, dp]),
             "orc_isf": (None, [C.c_void_p, dp, C.c_int, dp]),
@@ -233,6 +236,16 @@ class State:
 
     def set_action(self, shift=1, vfactor=(1.0, 1.0)):
         lib().orc_state_set_action(self.h, shift, vfactor[0], vfactor[1])
+
+    def set_correction(self, gradvfactor=(0.0, 0.0), period=1):
+        """GSF / Li-Broughton gradVFactor[2] and estimator period (src/setup.cpp:1232-1256)."""
+        lib().orc_state_set_correction(self.h, gradvfactor[0], gradvfactor[1], period)
+
+    def gradV2_slice(self, slice_):
+        return lib().orc_gradV2_slice(self.h, slice_)
+
+    def gradVnn2(self, slice_, ptcl):
+        return lib().orc_gradVnn2(self.h, slice_, ptcl)
 
     def total_action(self):
         return lib().orc_total_action(self.h)

@@ -110,6 +110,7 @@ SIGNATURES = {
     "pimc_server_wait": (C.c_int, [_vp, C.c_uint64, _dp]),
     "pimc_server_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p, C.POINTER(C.c_size_t)]),
     "pimc_server_run_synthetic": (C.c_int, [_vp, C.POINTER(_LoadDesc), _dp]),
+    "pimc_set_action_factors": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
     "pimc_chain_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pimc_chain_destroy": (None, [C.c_void_p]),
     "pimc_chain_last_error": (C.c_char_p, [C.c_void_p]),
@@ -160,7 +161,6 @@ class Context:
             raise PimcError(f"pimc_ctx_create: {lib().pimc_last_error(None).decode()} ({rc_})")
         self.h = h
         self.M, self.N, self.side = num_slices, num_alloc, side
-        self._vf1 = np.ones(2)          # default vfactor
         self._out = np.zeros(4)         # reused result buffer of the per-move calls
 
     def close(self):
@@ -174,6 +174,11 @@ class Context:
     def _check(self, rc, what):
         if rc != 0:
             raise PimcError(f"{what}: {lib().pimc_last_error(self.h).decode()} ({rc})")
+
+    def set_action_factors(self, vfactor=(1.0, 1.0), gradvfactor=(0.0, 0.0), period=1):
+        """LocalAction VFactor, gradVFactor (GSF / Li-Broughton correction) and estimator period."""
+        vf, gf = _f64(vfactor), _f64(gradvfactor)
+        self._check(lib().pimc_set_action_factors(self.h, _p(vf, _dp), _p(gf, _dp), period), "set_action_factors")
 
     def set_v_mode(self, mode):
         self._check(lib().pimc_set_v_mode(self.h, mode), "set_v_mode")
@@ -198,11 +203,13 @@ class Context:
         p = _f64(pos, (-1, 3))
         self._check(lib().pimc_update_beads(self.h, _p(sp, _ip), _p(p, _dp), sp.shape[0]), "update_beads")
 
-    def potentialAction(self, shift=1, vfactor=(1.0, 1.0), with_slices=False):
-        vf = self._vf1 if vfactor == (1.0, 1.0) else _f64(vfactor)
+    def potentialAction(self, shift=1, vfactor=None, with_slices=False):
+        """vfactor None: the context's VFactor (set_action_factors; {1, 1} by default)."""
+        vf = None if vfactor is None else _f64(vfactor)
         vs = np.zeros((self.M, 2)) if with_slices else None
-        self._check(lib().pimc_total_action(self.h, shift, vf.ctypes.data, self._out.ctypes.data,
-                                            None if vs is None else vs.ctypes.data), "total_action")
+        self._check(lib().pimc_total_action(self.h, shift, None if vf is None else vf.ctypes.data,
+                                            self._out.ctypes.data, None if vs is None else vs.ctypes.data),
+                    "total_action")
         a = float(self._out[0])
         return (a, vs) if with_slices else a
 
@@ -218,34 +225,35 @@ class Context:
         self._check(lib().pimc_energy(self.h, _p(nx, _ip), _p(o, _dp)), "energy")
         return dict(zip(["K", "V", "V_ext", "V_int", "E", "K/N", "V/N", "E/N"], o))
 
-    def action_list(self, slice_ptcl, shift=1, vfactor=(1.0, 1.0), per_bead=False):
+    def action_list(self, slice_ptcl, shift=1, vfactor=None, per_bead=False):
         sp = _i32(slice_ptcl, (-1, 2))
-        vf = _f64(vfactor)
+        vf = None if vfactor is None else _f64(vfactor)
         a = np.zeros(1)
         pb = np.zeros(sp.shape[0]) if per_bead else None
         self._check(lib().pimc_action_list(self.h, _p(sp, _ip), sp.shape[0], shift, _p(vf, _dp), _p(a, _dp),
                                            _p(pb, _dp)), "action_list")
         return (a[0], pb) if per_bead else a[0]
 
-    def action_range(self, start, end, next_links=None, shift=1, vfactor=(1.0, 1.0)):
+    def action_range(self, start, end, next_links=None, shift=1, vfactor=None):
         s, e = _i32(start), _i32(end)
         nx = None if next_links is None else _i32(next_links)
-        vf = _f64(vfactor)
+        vf = None if vfactor is None else _f64(vfactor)
         a = np.zeros(1)
         self._check(lib().pimc_action_range(self.h, _p(s, _ip), _p(e, _ip), _p(nx, _ip), shift, _p(vf, _dp),
                                             _p(a, _dp)), "action_range")
         return a[0]
 
-    def delta_action(self, slice_ptcl, new_pos, old_pos=None, shift=1, vfactor=(1.0, 1.0), commit=False):
+    def delta_action(self, slice_ptcl, new_pos, old_pos=None, shift=1, vfactor=None, commit=False):
         sp = _i32(slice_ptcl, (-1, 2))
         npos = _f64(new_pos, (-1, 3))
         opos = None if old_pos is None else _f64(old_pos, (-1, 3))
         if npos.shape[0] != sp.shape[0] or (opos is not None and opos.shape[0] != sp.shape[0]):
             raise PimcError("delta_action: bead list and positions differ in length")
-        vf = self._vf1 if vfactor == (1.0, 1.0) else _f64(vfactor)
+        vf = None if vfactor is None else _f64(vfactor)
         o = self._out
         self._check(lib().pimc_delta_action(self.h, sp.ctypes.data, None if opos is None else opos.ctypes.data,
-                                            npos.ctypes.data, sp.shape[0], shift, vf.ctypes.data, int(bool(commit)),
+                                            npos.ctypes.data, sp.shape[0], shift,
+                                            None if vf is None else vf.ctypes.data, int(bool(commit)),
                                             o.ctypes.data), "delta_action")
         return o[:3].copy()
 

@@ -159,7 +159,10 @@ AzizDev make_aziz_dev(const AzizParams& p, double max_sep) {
     a.C6 = p.C6;
     a.C8 = p.C8;
     a.C10 = p.C10;
+    a.eps_rm = p.eps / p.rm;
+    a.inv_rm = 1.0 / p.rm;
     a.table = nullptr;
+    a.dtable = nullptr;
     return a;
 }
 
@@ -445,6 +448,11 @@ static AzizDev ctx_aziz(pimc_ctx* c, int* rc) {
         std::string err;
         a.table = table_cache().get(c->desc.aziz_year, 0, a.table_length, &err);
         if (!a.table) *rc = fail(c, PIMC_E_NOMEM, err);
+        if (c->correction() && !*rc) {
+            // gradV gathers lookupdVdr (include/potential.h:996-1002)
+            a.dtable = table_cache().get(c->desc.aziz_year, 1, a.table_length, &err);
+            if (!a.dtable) *rc = fail(c, PIMC_E_NOMEM, err);
+        }
     }
     return a;
 }
@@ -455,10 +463,29 @@ static void fill_walker(pimc_ctx* c, WalkerDesc* w, int shift, const double vf[2
     w->num_slices = c->desc.num_slices;
     w->first_slice = 0;
     w->shift = shift;
-    w->vfactor0 = vf ? vf[0] : 1.0;
-    w->vfactor1 = vf ? vf[1] : 1.0;
+    w->vfactor0 = vf ? vf[0] : c->vf_default[0];
+    w->vfactor1 = vf ? vf[1] : c->vf_default[1];
     w->tau = c->desc.tau;
+    w->gvf0 = c->gvf[0];
+    w->gvf1 = c->gvf[1];
+    w->lambda = c->desc.lambda;
     w->worm = c->worm;
+}
+
+// gradVSquared(slice) for every slice into c->d_g2part (partials per slice).
+static int run_gradv2(pimc_ctx* c, const AzizDev& az) {
+    const int M = c->desc.num_slices, N = c->desc.num_alloc;
+    const int B = (N + 255) / 256;
+    if (!c->d_g2part || c->g2_blocks < B) {
+        int r;
+        if ((r = ctx_dalloc(c, &c->d_g2part, (size_t)M * B))) return r;
+        if (!c->d_g2slice && (r = ctx_dalloc(c, &c->d_g2slice, (size_t)M))) return r;
+    }
+    int blocks = 0;
+    HIP_OR_FAIL(c, launch_gradv2(c->desc.v_mode, c->stream, c->d_pos, c->plane, N, M, c->d_nbs, c->box, az,
+                                 c->lds_beads, c->d_g2part, &blocks));
+    c->g2_blocks = blocks;
+    return PIMC_OK;
 }
 
 extern "C" {
@@ -689,11 +716,14 @@ int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* a
         c->walker_valid = true;
     }
     if (c->n_tiles > 0) {
+        const bool cor = c->correction();
+        if (cor && (r = run_gradv2(c, az))) return r;
         const KernelTimer kt = c->kernel_timer(0);
         const unsigned long long seq = ++c->seq;
         HIP_OR_FAIL(c, launch_pair_action(c->desc.v_mode, c->stream, c->d_tiles, c->n_tiles, c->d_ranges, c->n_groups,
                                           c->d_slices, M, c->d_walker, 1, c->d_pos, az, c->lds_beads, c->d_partials,
-                                          c->d_vslice, c->d_map, &kt, true, c->d_done(), seq));
+                                          c->d_vslice, c->d_map, &kt, true, c->d_done(), seq,
+                                          cor ? c->d_g2part : nullptr, c->g2_blocks, c->d_g2slice));
         if (v_slices) {
             HIP_OR_FAIL(c, hipMemcpyAsync(c->h_stage, c->d_vslice, sizeof(double) * M, hipMemcpyDeviceToHost,
                                           c->stream));
@@ -744,7 +774,7 @@ int pimc_energy(pimc_ctx* c, const int32_t* next_links, double* out8) {
         c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)d_next),
                             c->dev_allocs.end());
     }
-    // EnergyEstimator::accumulate (src/estimator.cpp:940-1029), primitive action
+    // EnergyEstimator::accumulate (src/estimator.cpp:940-1029) with the context's action factors
     long on = 0;
     for (auto v : c->h_on) on += v;
     const int Np = (int)(on / M);
@@ -754,17 +784,33 @@ int pimc_energy(pimc_ctx* c, const int32_t* next_links, double* out8) {
     const double kinNorm = (1.0 / (4.0 * lambda * tau)) / (tau * M);
     const double classical = (0.5 * 3 / tau) * Np;
     double totK = -link_sum * kinNorm;
-    double t2 = 0.0, vop0 = 0.0, vop1 = 0.0;
-    for (int m = 0; m < M; ++m) {
-        t2 += (vs[2 * m] + vs[2 * m + 1]);
-        vop0 += vs[2 * m];
-        vop1 += vs[2 * m + 1];
+    // gradVSquared(slice) from the same pass when the action carries the correction
+    std::vector<double> g2(M, 0.0);
+    if (c->correction()) {
+        HIP_OR_FAIL(c, hipMemcpy(g2.data(), c->d_g2slice, sizeof(double) * M, hipMemcpyDeviceToHost));
     }
+    // derivPotentialActionLambda / Tau (src/action.cpp:751-802) and the operator
+    // V on every period-th slice
+    double t1 = 0.0, t2 = 0.0, vop0 = 0.0, vop1 = 0.0;
+    for (int m = 0; m < M; ++m) {
+        const int eo = m % 2;
+        const double gvf = c->gvf[eo];
+        const bool cor = gvf > kCorrEps;
+        t1 += cor ? gvf * tau * tau * tau * g2[m] : 0.0;
+        double dU = c->vf_default[eo] * (vs[2 * m] + vs[2 * m + 1]);
+        if (cor) dU += 3.0 * gvf * tau * tau * lambda * g2[m];
+        t2 += dU;
+        if (!(m % c->period)) {
+            vop0 += vs[2 * m];
+            vop1 += vs[2 * m + 1];
+        }
+    }
+    t1 *= lambda / (tau * M);
     t2 /= 1.0 * M;
-    vop0 /= M;
-    vop1 /= M;
-    totK += classical;
-    const double totV = t2 + tailV;
+    vop0 /= (M / c->period);
+    vop1 /= (M / c->period);
+    totK += (classical + t1);
+    const double totV = t2 - t1 + tailV;
     vop1 += tailV;
     out8[0] = totK;
     out8[1] = totV;
@@ -801,8 +847,24 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
     A.box = c->box;
     A.worm = c->worm;
     A.shift = shift;
-    A.vfactor0 = vf ? vf[0] : 1.0;
-    A.vfactor1 = vf ? vf[1] : 1.0;
+    A.vfactor0 = vf ? vf[0] : c->vf_default[0];
+    A.vfactor1 = vf ? vf[1] : c->vf_default[1];
+    // gradient correction of the bead actions (shift 1 only, as the reference)
+    const bool cor = c->correction() && shift == 1;
+    A.gnn_part = nullptr;
+    A.gnn_chunks = (c->desc.num_alloc + kGnnGroup - 1) / kGnnGroup;
+    A.gvf0 = c->gvf[0];
+    A.gvf1 = c->gvf[1];
+    A.lambda = c->desc.lambda;
+    if (cor) {
+        const size_t need = (size_t)c->cap_req * 2 * A.gnn_chunks * 4;
+        if ((size_t)c->cap_gnn < need) {
+            int r2;
+            if ((r2 = ctx_dalloc(c, &c->d_gnn_part, need))) return r2;
+            c->cap_gnn = (int)need;
+        }
+        A.gnn_part = c->d_gnn_part;
+    }
     A.tau = c->desc.tau;
     A.partials = c->d_vnn_partials;
     A.per_bead = c->d_per_bead;
@@ -817,7 +879,8 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
                                       c->stream));
     }
     const KernelTimer kt = c->kernel_timer(1);
-    HIP_OR_FAIL(c, launch_vnn(c->desc.v_mode, c->stream, A, req, az, &kt));
+    const int gnn_lds = c->desc.num_alloc <= kLdsBeads ? c->desc.num_alloc : 0;
+    HIP_OR_FAIL(c, launch_vnn(c->desc.v_mode, c->stream, A, req, az, &kt, cor ? c->d_gnn_part : nullptr, gnn_lds));
     if (per_bead) {
         HIP_OR_FAIL(c, hipMemcpyAsync(c->h_per_bead, c->d_per_bead, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
                                       c->stream));
@@ -986,6 +1049,21 @@ int pimc_interaction_list(pimc_ctx* c, int slice, int ptcl, int32_t* list2, doub
     (void)hipFree(d_cnt);
     if (e != hipSuccess) return fail(c, PIMC_E_HIP, std::string("interaction list: ") + hipGetErrorString(e));
     *count = cnt;
+    return PIMC_OK;
+}
+
+int pimc_set_action_factors(pimc_ctx* c, const double vfactor[2], const double gradvfactor[2], int period) {
+    if (!c || !vfactor || !gradvfactor) return fail(c, PIMC_E_ARG, "null argument");
+    if (period < 1) return fail(c, PIMC_E_ARG, "period must be >= 1");
+    if (!(gradvfactor[0] >= 0.0) || !(gradvfactor[1] >= 0.0)) return fail(c, PIMC_E_ARG, "negative gradVFactor");
+    if ((gradvfactor[0] > kCorrEps || gradvfactor[1] > kCorrEps) && c->desc.num_alloc > 16384)
+        return fail(c, PIMC_E_ARG, "the gradient correction holds an interaction list of at most 16384 beads");
+    c->vf_default[0] = vfactor[0];
+    c->vf_default[1] = vfactor[1];
+    c->gvf[0] = gradvfactor[0];
+    c->gvf[1] = gradvfactor[1];
+    c->period = period;
+    c->walker_valid = false;
     return PIMC_OK;
 }
 

@@ -115,8 +115,8 @@ struct pimc_chain {
     // One device pass: {A_old, A_new} of the listed beads (old: device copy, or
     // `old` when given), new positions `np`; commit applies them on the device.
     int delta(int n, const double* old, int shift, bool commit, double out3[3]) {
-        const double vf[2] = {1.0, 1.0};
-        int r = pimc_delta_action(ctx, sp.data(), old, np.data(), n, shift, vf, commit ? 1 : 0, out3);
+        // the context's action factors (VFactor and any gradient correction)
+        int r = pimc_delta_action(ctx, sp.data(), old, np.data(), n, shift, nullptr, commit ? 1 : 0, out3);
         if (r) return fail(r, std::string("delta action: ") + pimc_last_error(ctx));
         return PIMC_OK;
     }

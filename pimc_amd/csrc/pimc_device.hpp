@@ -36,7 +36,10 @@ struct AzizDev {
     double eC6d, eC8d, eC10d;   // epsilon * C6, C8, C10
     // FP64 closed form
     double eps, A, alpha, beta, D, C6, C8, C10;
+    double eps_rm;         // epsilon / rm (dV/dr = (epsilon/rm) dV/dx)
+    double inv_rm;         // 1 / rm
     const double* table;   // device lookupV (PIMC_V_TABLE)
+    const double* dtable;  // device lookupdVdr (PIMC_V_TABLE, gradient terms)
 };
 
 struct BoxDev {
@@ -135,6 +138,81 @@ __device__ __forceinline__ double aziz_v64(int k, const AzizDev& a) {
         F = exp(-u * u);
     }
     return a.eps * (rep - (a.C6 * ix6 + a.C8 * ix8 + a.C10 * ix10) * F);
+}
+
+// valuedVdr at the bin radius (src/potential.cpp:1847-1873): FP64 closed form.
+// Requires k_eps <= k < table_length.
+__device__ __forceinline__ double aziz_dvdr64(int k, const AzizDev& a) {
+    const double x = (double)k * a.s;
+    const double T1 = a.A * (-a.alpha + 2.0 * a.beta * x) * exp(-a.alpha * x + a.beta * x * x);
+    if (k < a.k_hc) return a.eps_rm * T1;
+    const double ix = 1.0 / x;
+    const double ix2 = ix * ix;
+    const double ix6 = ix2 * ix2 * ix2;
+    const double ix7 = ix6 * ix, ix8 = ix6 * ix2, ix9 = ix8 * ix, ix10 = ix8 * ix2, ix11 = ix10 * ix;
+    double F = 1.0, dF = 0.0;
+    if (k < a.k_d) {
+        const double t = a.D * ix - 1.0;
+        const double e = exp(-t * t);
+        F = e;
+        dF = 2.0 * a.D * ix2 * t * e;
+    }
+    const double T2 = (6.0 * a.C6 * ix7 + 8.0 * a.C8 * ix9 + 10.0 * a.C10 * ix11) * F;
+    const double T3 = -(a.C6 * ix6 + a.C8 * ix8 + a.C10 * ix10) * dF;
+    return a.eps_rm * (T1 + T2 + T3);
+}
+
+// valuedVdr at the bin radius with the FP32 transcendental units (split
+// exponents, FP64 around them), the counterpart of aziz_vmix.
+__device__ __forceinline__ double aziz_dvdr_mix(int k, const AzizDev& a) {
+    const double kd = (double)k;
+    const double x = kd * a.s;
+    // eps*A*exp(-alpha x + beta x^2) = 2^(c0 + c1 k + c2 k^2)
+    const double rep = split_exp2(fma(fma(a.rep_c2, kd, a.rep_c1), kd, a.rep_c0));
+    const double T1 = rep * (-a.alpha + 2.0 * a.beta * x);   // epsilon folded into rep
+    double ix = (double)__builtin_amdgcn_rcpf((float)x);
+    ix = ix * fma(-x, ix, 2.0);
+    ix = ix * fma(-x, ix, 2.0);                  // two Newton steps: FP64 accurate
+    const double ix2 = ix * ix;
+    const double ix6 = ix2 * ix2 * ix2;
+    double F = 1.0, dF = 0.0;
+    if (k < a.k_d) {
+        const double t = fma(a.D, ix, -1.0);
+        const double e = split_exp2(-1.4426950408889634074 * t * t);
+        F = e;
+        dF = 2.0 * a.D * ix2 * t * e;
+    }
+    // epsilon * [ (6 C6 x^-7 + 8 C8 x^-9 + 10 C10 x^-11) F - (C6 x^-6 + C8 x^-8 + C10 x^-10) dF ]
+    const double pd = ix6 * ix * fma(fma(10.0 * a.eC10d, ix2, 8.0 * a.eC8d), ix2, 6.0 * a.eC6d);
+    const double pv = ix6 * fma(fma(a.eC10d, ix2, a.eC8d), ix2, a.eC6d);
+    const double att = fma(pd, F, -pv * dF);
+    // below x = 0.01 the damping factor and its derivative underflow to 0 (see aziz_vmix)
+    return (T1 + att) * a.inv_rm;
+}
+
+// The gradient factor g of gradV(sep) = g * sep, g = lookupdVdr[k] / |sep|
+// (include/potential.h:996-1002 with direct(), :249-260); 0 outside the table.
+template <int MODE>
+__device__ __forceinline__ double grad_factor_from_r2(double r2, const AzizDev& a, double inv_dr) {
+    int k;
+    double rn;
+    if (MODE == 0) {
+        float rs;
+        k = bin_fast(r2, inv_dr, &rs);
+        rn = __builtin_sqrt(r2);
+    } else {
+        k = bin_exact(r2, a.dr, &rn);
+    }
+    const bool valid = (unsigned)(k - a.k_eps) < (unsigned)(a.table_length - a.k_eps);
+    const int kk = valid ? k : a.k_hc;   // keep the arithmetic finite on masked lanes
+    double d;
+    if (MODE == 0)
+        d = aziz_dvdr_mix(kk, a);
+    else if (MODE == 1)
+        d = aziz_dvdr64(kk, a);
+    else
+        d = valid ? a.dtable[k] : 0.0;
+    return valid ? d / rn : 0.0;
 }
 
 // Pair value for separation components (dx,dy,dz) already minimum-imaged.

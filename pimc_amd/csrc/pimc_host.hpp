@@ -67,6 +67,17 @@ struct pimc_ctx {
     unsigned long long* d_done() { return reinterpret_cast<unsigned long long*>(d_map + kDoneSlot); }
     int wait_done(unsigned long long s);   // spin, then fall back to a stream sync
     int lds_beads = 0;            // slice positions staged in LDS (0: read from HBM)
+    // action factors (LocalAction VFactor, gradVFactor, estimator period); the
+    // per-call vfactor argument overrides VFactor when given
+    double vf_default[2] = {1.0, 1.0};
+    double gvf[2] = {0.0, 0.0};
+    int period = 1;
+    bool correction() const { return gvf[0] > pimc::kCorrEps || gvf[1] > pimc::kCorrEps; }
+    double* d_g2part = nullptr;   // gradVSquared partials [M][g2_blocks]
+    double* d_g2slice = nullptr;  // gradVSquared(slice) [M]
+    int g2_blocks = 0;
+    double* d_gnn_part = nullptr; // gradVnnSquared partials [cap_req][2][chunks][4]
+    int cap_gnn = 0;
     pimc::WalkerDesc walker_cache{};
     bool walker_valid = false;
     int* d_hist = nullptr;

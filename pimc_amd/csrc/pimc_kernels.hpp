@@ -32,8 +32,12 @@ struct WalkerDesc {
     int first_slice;   // index of its slice 0 in the SliceDesc array
     int shift;
     double vfactor0, vfactor1, tau;
+    double gvf0, gvf1, lambda;   // gradient correction gradVFactor[2] and lambda (0: none)
     WormDev worm;
 };
+
+// A gradVFactor counts as finite above EPS (src/action.cpp:466, :500).
+constexpr double kCorrEps = 1.0e-7;
 
 // Every i<j pair of a slice with n beads is visited once as (i, j = (i + d) mod n)
 // with offsets 1 <= d <= (n-1)/2 for all rows i, plus, for even n, d = n/2 for
@@ -100,7 +104,13 @@ struct VnnArgs {
     double* out;       // {A_old, A_new, dU} (host-mapped pinned memory)
     unsigned long long* done;   // host-mapped: set to seq once out is written (or null)
     unsigned long long seq;
+    // gradient correction: gradVnnSquared partials [n_beads][2][gnn_chunks][4] (null: none)
+    const double* gnn_part;
+    int gnn_chunks;
+    double gvf0, gvf1, lambda;
 };
+
+constexpr int kGnnGroup = 16;   // bead2 per three-body workgroup (16 lanes each)
 
 // Requests of up to kVnnInline beads travel in the kernel arguments.
 constexpr int kVnnInline = 32;
@@ -147,15 +157,22 @@ __device__ __forceinline__ int xcd_contiguous_item(int b, int G) {
 // The persistent row-tile pair kernel over G workgroups, then (finalize != 0)
 // the single-workgroup slice and walker sums.
 // done (host-mapped, or null) is set to seq after action has been written.
+// g2_part (or null): gradVSquared partials [n_slices][g2_blocks] whose slice sums
+// (written to g2_slice) enter the walkers' actions with the gradient correction.
 hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, int n_tiles, const int* ranges, int G,
                               const SliceDesc* slices, int n_slices, const WalkerDesc* walkers, int n_walkers,
                               const double* pos, const AzizDev& az, int lds_beads, double* partials, double* v_slice,
                               double* action, const KernelTimer* timer = nullptr, bool finalize = true,
-                              unsigned long long* done = nullptr, unsigned long long seq = 0);
+                              unsigned long long* done = nullptr, unsigned long long seq = 0,
+                              const double* g2_part = nullptr, int g2_blocks = 0, double* g2_slice = nullptr);
 // Workgroups of the persistent pair kernel for a slice staging of lds_beads beads.
 int pair_grid_for(int lds_beads, int num_cus);
 hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
-                      const KernelTimer* timer = nullptr);
+                      const KernelTimer* timer = nullptr, double* gnn_part = nullptr, int lds_beads = 0);
+// gradVSquared(slice) partials [M][blocks_per_slice] for every slice.
+hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
+                         const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,
+                         int* blocks_per_slice);
 hipError_t launch_update_beads(hipStream_t st, const int* sp, const double* p3, int n, double* pos, int64_t plane,
                                int nalloc, int num_slices, int* cell, int* label, int* numlab, int* hash,
                                const BoxDev& box);

@@ -142,9 +142,12 @@ __global__ __launch_bounds__(kFinThreads) void k_pair_finalize(int n_items, cons
                                                               int n_walkers, const double* __restrict__ partials,
                                                               double* __restrict__ v_slice,
                                                               double* __restrict__ action,
-                                                              unsigned long long* done, unsigned long long seq) {
+                                                              unsigned long long* done, unsigned long long seq,
+                                                              const double* __restrict__ g2_part, int g2_blocks,
+                                                              double* __restrict__ g2_slice) {
     __shared__ double sp[kFinLds];
     __shared__ double vs[kFinSlices];
+    __shared__ double gs[kFinSlices];
     __shared__ int fi[kFinSlices + 1];
     const bool staged = n_items <= kFinLds;
     const bool small = n_slices <= kFinSlices;
@@ -163,6 +166,12 @@ __global__ __launch_bounds__(kFinThreads) void k_pair_finalize(int n_items, cons
         for (int k = b; k < e; ++k) v += src[k];
         v_slice[sl] = v;
         if (small) vs[sl] = v;
+        if (g2_part) {
+            double g = 0.0;
+            for (int b2 = 0; b2 < g2_blocks; ++b2) g += g2_part[(int64_t)sl * g2_blocks + b2];
+            g2_slice[sl] = g;
+            if (small) gs[sl] = g;
+        }
     }
     __threadfence_block();   // v_slice is re-read below when the slices do not fit in LDS
     __syncthreads();
@@ -175,7 +184,14 @@ __global__ __launch_bounds__(kFinThreads) void k_pair_finalize(int n_items, cons
         for (int m = lane; m < W.num_slices; m += 64) {
             const double vf = (m & 1) ? W.vfactor1 : W.vfactor0;
             const double v = small ? vs[W.first_slice + m] : v_slice[W.first_slice + m];
-            tot += vf * tau_s * (0.0 + v);
+            double u = vf * tau_s * (0.0 + v);
+            // potentialAction() correction (src/action.cpp:465-467)
+            const double gvf = (m & 1) ? W.gvf1 : W.gvf0;
+            if (g2_part && gvf > kCorrEps) {
+                const double g = small ? gs[W.first_slice + m] : g2_slice[W.first_slice + m];
+                u += gvf * tau_s * tau_s * tau_s * W.lambda * g;
+            }
+            tot += u;
         }
         tot = wave_sum(tot);
         if (lane == 0) {
@@ -325,6 +341,205 @@ __device__ __forceinline__ void vnn_commit(const VnnArgs& A, const VnnInline& re
     }
 }
 
+// ---------------------------------------------------------------------------
+// Gradient correction (GSF / Li-Broughton): gradVSquared per slice and the
+// three-body gradVnnSquared of moved beads.
+// ---------------------------------------------------------------------------
+// |F_i|^2 with F_i = sum_{j != i} gradV(r_i - r_j) over the slice's n beads, j in order.
+template <int MODE, typename P>
+__device__ __forceinline__ double force2(P px, P py, P pz, int n, int i, const BoxDev& box, const AzizDev& a,
+                                         double inv_dr) {
+    const double xi = px[i], yi = py[i], zi = pz[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int j = 0; j < n; ++j) {
+        if (j == i) continue;
+        double dx, dy, dz;
+        if (MODE == 0) {
+            dx = min_image_fast(xi - px[j], box.L, box.invL);
+            dy = min_image_fast(yi - py[j], box.L, box.invL);
+            dz = min_image_fast(zi - pz[j], box.L, box.invL);
+        } else {
+            dx = min_image_exact(xi - px[j], box.L, box.invL);
+            dy = min_image_exact(yi - py[j], box.L, box.invL);
+            dz = min_image_exact(zi - pz[j], box.L, box.invL);
+        }
+        const double g = grad_factor_from_r2<MODE>(norm2_exact(dx, dy, dz), a, inv_dr);
+        fx += g * dx;
+        fy += g * dy;
+        fz += g * dz;
+    }
+    return norm2_exact(fx, fy, fz);
+}
+
+// gradVSquared(slice) (src/action.cpp:1188-1223): block (b, m) takes rows
+// 256 b .. 256 b + 255 of slice m; one partial per block, summed in block order.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gradv2(const double* __restrict__ pos, int64_t plane, int nalloc,
+                                                const int* __restrict__ nbs, BoxDev box, AzizDev az, int lds_beads,
+                                                double* __restrict__ part) {
+    extern __shared__ double lds[];
+    __shared__ double red[4];
+    double* lx = lds;
+    double* ly = lds + lds_beads;
+    double* lz = lds + 2 * lds_beads;
+    const int m = blockIdx.y, b = blockIdx.x;
+    const int n = nbs[m];
+    const double* px = pos + (int64_t)m * nalloc;
+    const double* py = px + plane;
+    const double* pz = py + plane;
+    const bool in_lds = n <= lds_beads;
+    if (in_lds)
+        for (int t = threadIdx.x; t < n; t += 256) {
+            lx[t] = px[t];
+            ly[t] = py[t];
+            lz[t] = pz[t];
+        }
+    __syncthreads();
+    const double inv_dr = 1.0 / az.dr;
+    const int i = b * 256 + threadIdx.x;
+    double f2 = 0.0;
+    if (i < n)
+        f2 = in_lds ? force2<MODE>(lx, ly, lz, n, i, box, az, inv_dr) : force2<MODE>(px, py, pz, n, i, box, az, inv_dr);
+    const double s = block_sum<256>(f2, red);
+    if (threadIdx.x == 0) part[(int64_t)m * gridDim.x + b] = s;
+}
+
+__device__ __forceinline__ double group16_sum(double v) {
+    v += __shfl_xor(v, 8, 16);
+    v += __shfl_xor(v, 4, 16);
+    v += __shfl_xor(v, 2, 16);
+    v += __shfl_xor(v, 1, 16);
+    return v;
+}
+
+// gradVnnSquared(bead1) (src/action.cpp:1282-1337, external gradV = 0) at the
+// old (y even) or new (y odd) position of request bead y/2.  Every workgroup
+// builds the bead's interaction list (the Vnn partner set, in index order) in
+// LDS; workgroup c takes list entries bead2 = 16c .. 16c+15, 16 lanes each over
+// bead3, and writes {sum of |F2|^2 + 2 F2.F3, F1} with F1 = -sum F2.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gradvnn(VnnArgs A, VnnInline req, AzizDev az, int lds_beads,
+                                                 double* __restrict__ part) {
+    extern __shared__ double lds[];   // x | y | z (lds_beads each), then the list (int[nalloc])
+    __shared__ int wcount[4];
+    __shared__ int list_n;
+    __shared__ double red[4][4];
+    const int c = blockIdx.x, bead = blockIdx.y >> 1, which = blockIdx.y & 1;
+    const VnnBead q = A.beads ? A.beads[bead] : req.beads[bead];
+    const VnnSubject v = vnn_subject(A, q);
+    double* out = part + (((int64_t)bead * 2 + which) * A.gnn_chunks + c) * 4;
+    if (which == 1 && !v.has_new) {
+        if (threadIdx.x < 4) out[threadIdx.x] = 0.0;
+        return;
+    }
+    const int N = A.nalloc;
+    const bool in_lds = N <= lds_beads;
+    double* lx = lds;
+    double* ly = lds + lds_beads;
+    double* lz = lds + 2 * lds_beads;
+    int* list = reinterpret_cast<int*>(lds + 3 * lds_beads);
+    const double* px = in_lds ? lx : v.px;
+    const double* py = in_lds ? ly : v.py;
+    const double* pz = in_lds ? lz : v.pz;
+    if (in_lds)
+        for (int t = threadIdx.x; t < N; t += 256) {
+            lx[t] = v.px[t];
+            ly[t] = v.py[t];
+            lz[t] = v.pz[t];
+        }
+    const double x1 = which ? v.nx : v.ox, y1 = which ? v.ny : v.oy, z1 = which ? v.nz : v.oz;
+    const int c1 = which ? v.c_new : v.c_old;
+    // interaction list in partner order: on, not itself, r^2 < rc^2, neighbouring cell
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int count = 0;
+    for (int j0 = 0; j0 < N; j0 += 256) {
+        const int j = j0 + threadIdx.x;
+        bool keep = false;
+        if (j < N) {
+            const double r2 = sep_r2<MODE>(x1, y1, z1, v.px[j], v.py[j], v.pz[j], A.box);
+            keep = (A.on[v.m * N + j] != 0) && (j != v.me) && v.me_on && (r2 < A.box.rc2) &&
+                   (!v.use_cells || cells_adjacent(c1, A.cell[v.m * N + j], A.box.ngrid));
+        }
+        const unsigned long long ball = __ballot(keep);
+        if (lane == 0) wcount[wave] = __popcll(ball);
+        __syncthreads();
+        int off = count;
+        for (int w = 0; w < wave; ++w) off += wcount[w];
+        if (keep) list[off + __popcll(ball & ((1ull << lane) - 1ull))] = j;
+        count += wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) list_n = count;
+    __syncthreads();
+    const int nl = list_n;
+    const double inv_dr = 1.0 / az.dr;
+    const int grp = threadIdx.x >> 4, gl = threadIdx.x & 15;
+    const int n2 = c * kGnnGroup + grp;
+    double f3x = 0.0, f3y = 0.0, f3z = 0.0;
+    double x2 = 0.0, y2 = 0.0, z2 = 0.0;
+    if (n2 < nl) {
+        const int j2 = list[n2];
+        x2 = px[j2];
+        y2 = py[j2];
+        z2 = pz[j2];
+        for (int m3 = gl; m3 < nl; m3 += 16) {
+            if (m3 == n2) continue;
+            const int j3 = list[m3];
+            // Path::getSeparation(bead2, bead3) = r2 - r3, in the bin mode's arithmetic
+            double dx, dy, dz;
+            if (MODE == 0) {
+                dx = min_image_fast(x2 - px[j3], A.box.L, A.box.invL);
+                dy = min_image_fast(y2 - py[j3], A.box.L, A.box.invL);
+                dz = min_image_fast(z2 - pz[j3], A.box.L, A.box.invL);
+            } else {
+                dx = min_image_exact(x2 - px[j3], A.box.L, A.box.invL);
+                dy = min_image_exact(y2 - py[j3], A.box.L, A.box.invL);
+                dz = min_image_exact(z2 - pz[j3], A.box.L, A.box.invL);
+            }
+            const double g = grad_factor_from_r2<MODE>(norm2_exact(dx, dy, dz), az, inv_dr);
+            f3x += g * dx;
+            f3y += g * dy;
+            f3z += g * dz;
+        }
+    }
+    f3x = group16_sum(f3x);
+    f3y = group16_sum(f3y);
+    f3z = group16_sum(f3z);
+    double term = 0.0, f1x = 0.0, f1y = 0.0, f1z = 0.0;
+    if (n2 < nl && gl == 0) {
+        // beadSep = Path::getSeparation(bead2, bead1) = r2 - r1
+        double dx, dy, dz;
+        if (MODE == 0) {
+            dx = min_image_fast(x2 - x1, A.box.L, A.box.invL);
+            dy = min_image_fast(y2 - y1, A.box.L, A.box.invL);
+            dz = min_image_fast(z2 - z1, A.box.L, A.box.invL);
+        } else {
+            dx = min_image_exact(x2 - x1, A.box.L, A.box.invL);
+            dy = min_image_exact(y2 - y1, A.box.L, A.box.invL);
+            dz = min_image_exact(z2 - z1, A.box.L, A.box.invL);
+        }
+        const double g = grad_factor_from_r2<MODE>(norm2_exact(dx, dy, dz), az, inv_dr);
+        const double f2x = g * dx, f2y = g * dy, f2z = g * dz;
+        term = norm2_exact(f2x, f2y, f2z) + 2.0 * 0.0 + 2.0 * ((f2x * f3x + f2y * f3y) + f2z * f3z);
+        f1x = -f2x;
+        f1y = -f2y;
+        f1z = -f2z;
+    }
+    term = wave_sum(term);
+    f1x = wave_sum(f1x);
+    f1y = wave_sum(f1y);
+    f1z = wave_sum(f1z);
+    if (lane == 0) {
+        red[wave][0] = term;
+        red[wave][1] = f1x;
+        red[wave][2] = f1y;
+        red[wave][3] = f1z;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) out[threadIdx.x] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) +
+                                            red[3][threadIdx.x];
+}
+
 // Per-bead Vnn (chunks in order), the two actions and dU (a fixed shuffle tree
 // over beads), then the committed bead updates, one lane per bead (the beads sit
 // on distinct slices).
@@ -342,9 +557,26 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
         A.per_bead[2 * k + 1] = vn;
         const int slice = A.beads ? A.beads[k].slice : req.beads[k].slice;
         const double co = ((slice & 1) ? A.vfactor1 : A.vfactor0) * tau_s;
-        ao += co * vo;
-        an += co * vn;
-        du += co * (vn - vo);
+        // potentialAction(bead) correction at shift 1 (src/action.cpp:497-503):
+        // gradVnnSquared = sum of the chunks' three-body terms + |F1|^2
+        double cor[2] = {0.0, 0.0};
+        const double gvf = (slice & 1) ? A.gvf1 : A.gvf0;
+        if (A.gnn_part && A.shift == 1 && gvf > kCorrEps)
+            for (int w = 0; w < 2; ++w) {
+                const double* g = A.gnn_part + ((int64_t)k * 2 + w) * A.gnn_chunks * 4;
+                double t = 0.0, f1x = 0.0, f1y = 0.0, f1z = 0.0;
+                for (int cc = 0; cc < A.gnn_chunks; ++cc) {
+                    t += g[4 * cc + 0];
+                    f1x += g[4 * cc + 1];
+                    f1y += g[4 * cc + 2];
+                    f1z += g[4 * cc + 3];
+                }
+                t += 0.0 + 2.0 * 0.0 + norm2_exact(f1x, f1y, f1z);
+                cor[w] = gvf * tau_s * tau_s * tau_s * A.lambda * t;
+            }
+        ao += co * vo + cor[0];
+        an += co * vn + cor[1];
+        du += co * (vn - vo) + (cor[1] - cor[0]);
     }
     ao = wave_sum(ao);
     an = wave_sum(an);
@@ -546,7 +778,7 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
                               const SliceDesc* slices, int n_slices, const WalkerDesc* walkers, int n_walkers,
                               const double* pos, const AzizDev& az, int lds_beads, double* partials, double* v_slice,
                               double* action, const KernelTimer* timer, bool finalize, unsigned long long* done,
-                              unsigned long long seq) {
+                              unsigned long long seq, const double* g2_part, int g2_blocks, double* g2_slice) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (n_tiles <= 0 || G <= 0) return hipSuccess;
     dim3 g(G), b(kPairThreads);
@@ -564,7 +796,7 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
     if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
     if (finalize)
         hipLaunchKernelGGL(k_pair_finalize, dim3(1), dim3(kFinThreads), 0, st, n_tiles, slices, n_slices, walkers,
-                           n_walkers, partials, v_slice, action, done, seq);
+                           n_walkers, partials, v_slice, action, done, seq, g2_part, g2_blocks, g2_slice);
     return hipGetLastError();
 }
 
@@ -617,7 +849,7 @@ void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int
 }
 
 hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
-                      const KernelTimer* timer) {
+                      const KernelTimer* timer, double* gnn_part, int lds_beads) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (A.n_beads <= 0) return hipSuccess;
     dim3 g(A.chunks, A.n_beads), b(kVnnThreads);
@@ -630,7 +862,35 @@ hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInlin
     else
         go(k_vnn<2>);
     if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
+    if (gnn_part) {
+        VnnArgs B = A;
+        const size_t lds = sizeof(double) * 3 * (size_t)lds_beads + sizeof(int) * (size_t)A.nalloc;
+        dim3 gg(A.gnn_chunks, 2 * A.n_beads);
+        if (mode == 0)
+            hipLaunchKernelGGL(k_gradvnn<0>, gg, dim3(256), lds, st, B, req, az, lds_beads, gnn_part);
+        else if (mode == 1)
+            hipLaunchKernelGGL(k_gradvnn<1>, gg, dim3(256), lds, st, B, req, az, lds_beads, gnn_part);
+        else
+            hipLaunchKernelGGL(k_gradvnn<2>, gg, dim3(256), lds, st, B, req, az, lds_beads, gnn_part);
+    }
     hipLaunchKernelGGL(k_vnn_finalize, dim3(1), dim3(256), 0, st, A, req);
+    return hipGetLastError();
+}
+
+hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
+                         const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,
+                         int* blocks_per_slice) {
+    (void)hipGetLastError();
+    const int B = (nalloc + 255) / 256;
+    *blocks_per_slice = B;
+    dim3 g(B, num_slices), b(256);
+    const size_t lds = sizeof(double) * 3 * (size_t)lds_beads;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_gradv2<0>, g, b, lds, st, pos, plane, nalloc, nbs, box, az, lds_beads, part);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_gradv2<1>, g, b, lds, st, pos, plane, nalloc, nbs, box, az, lds_beads, part);
+    else
+        hipLaunchKernelGGL(k_gradv2<2>, g, b, lds, st, pos, plane, nalloc, nbs, box, az, lds_beads, part);
     return hipGetLastError();
 }
 

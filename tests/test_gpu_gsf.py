@@ -1,0 +1,111 @@
+"""GPU parity of the GSF / Li-Broughton gradient correction against the oracle
+(src/action.cpp:456-806, src/setup.cpp:1232-1256) on identical seeded worldlines.
+
+Tolerances: total action, bead actions, dU and energy rel <= 1e-6 in the FP32
+mode (FP32 transcendentals in V and dV/dr), 1e-9 in the FP64 closed form, 1e-11
+in the table mode (per-pair values from the host's tables, summation order
+differs); Markov runs: identical decisions and bitwise-equal worldlines."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DENSITY, LAMBDA, SEED0 = 0.02198, 6.0596, 139853
+TOL = {0: 1e-6, 1: 1e-9, 2: 1e-11}
+GSF = dict(vfactor=(2.0 / 3.0, 4.0 / 3.0), gradvfactor=(0.0, 2.0 / 9.0), period=2)
+LB = dict(vfactor=(1.0, 1.0), gradvfactor=(1.0 / 12.0, 1.0 / 12.0), period=2)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import pimc_amd
+    return pimc_amd
+
+
+def pair(orc, gpu, N, M, tau, rc_frac, mode, action, kind=1, seed=SEED0):
+    L = orc.box_side(N, DENSITY)
+    beads = orc.gen_worldlines(seed, L, M, N, LAMBDA, tau, kind)
+    ref = orc.State(L, M, N, tau, LAMBDA, rc_frac * L)
+    ref.set_beads(beads)
+    ref.set_action(1, action["vfactor"])
+    ref.set_correction(action["gradvfactor"], action["period"])
+    ctx = gpu.Context(L, M, N, tau, LAMBDA, rc_frac * L, v_mode=mode)
+    ctx.upload_path(beads)
+    ctx.set_action_factors(action["vfactor"], action["gradvfactor"], action["period"])
+    return ref, ctx, beads, L
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("name,action", [("gsf", GSF), ("li_broughton", LB)])
+def test_total_action_and_energy(orc, gpu, mode, name, action):
+    ref, ctx, _, _ = pair(orc, gpu, 32, 40, 0.01, 1.0, mode, action)
+    a, b = ctx.potentialAction(), ref.total_action()
+    print(f"{name} mode {mode}: total action rel err {abs(a - b) / abs(b):.2e}")
+    assert abs(a - b) <= TOL[mode] * abs(b)
+    eg, er = ctx.energy(), ref.energy()
+    for k in ("K", "V", "V_int", "E", "E/N"):
+        assert abs(eg[k] - er[k]) <= TOL[mode] * max(abs(er[k]), 1.0), (k, eg[k], er[k])
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("N,M,rc_frac", [(24, 32, 1.0), (256, 24, 0.25)])
+def test_bead_actions_and_delta(orc, gpu, mode, N, M, rc_frac):
+    """potentialAction(vector) at shift 1 and a segment move's dU, with and
+    without a cutoff (rc = L/4: 4 cells per side, lists restricted to r < rc)."""
+    ref, ctx, beads, L = pair(orc, gpu, N, M, 0.01, rc_frac, mode, GSF)
+    rng = np.random.default_rng(4)
+    scale = abs(ref.total_action())
+    for trial in range(4):
+        seg = int(rng.integers(3, 9))
+        start, p = int(rng.integers(M)), int(rng.integers(N))
+        sp = np.array([[(start + k) % M, p] for k in range(seg)], dtype=np.int32)
+        a_ref = ref.action_list(sp)
+        a_gpu = ctx.action_list(sp)
+        assert abs(a_gpu - a_ref) <= TOL[mode] * scale, (trial, a_gpu, a_ref)
+        new = beads[sp[:, 0], sp[:, 1]] + rng.normal(scale=0.3, size=(seg, 3))
+        new -= L * np.floor(new / L + 0.5)
+        out = ctx.delta_action(sp, new)
+        for k in range(seg):
+            ref.update_bead(int(sp[k, 0]), int(sp[k, 1]), new[k])
+        a_new = ref.action_list(sp)
+        for k in range(seg):
+            ref.update_bead(int(sp[k, 0]), int(sp[k, 1]), beads[sp[k, 0], sp[k, 1]])
+        assert abs(out[0] - a_ref) <= TOL[mode] * scale
+        assert abs(out[1] - a_new) <= TOL[mode] * scale
+        assert abs(out[2] - (a_new - a_ref)) <= TOL[mode] * scale
+
+
+def test_shift_above_one_has_no_correction(orc, gpu):
+    ref, ctx, _, _ = pair(orc, gpu, 24, 32, 0.01, 1.0, 2, GSF)
+    sp = np.array([[3, 5], [5, 5], [7, 5]], dtype=np.int32)   # odd slices: gradVFactor = 2/9
+    ref.set_action(2, GSF["vfactor"])
+    r2 = ref.action_list(sp)
+    ref.set_action(1, GSF["vfactor"])
+    r1 = ref.action_list(sp)
+    a1, a2 = ctx.action_list(sp, shift=1), ctx.action_list(sp, shift=2)
+    assert abs(a2 - r2) <= 1e-11 * abs(r2)
+    assert abs(a1 - r1) <= 1e-11 * abs(r1)
+    # the bare part scales with shift tau, so a1 - a2/2 is the shift-1 correction
+    # alone (gradVnnSquared carries cross terms: its sign is not fixed)
+    cor_ref = r1 - r2 / 2.0
+    assert cor_ref != 0.0
+    assert abs((a1 - a2 / 2.0) - cor_ref) <= 1e-9 * abs(cor_ref)
+
+
+def test_gsf_markov_run_matches_oracle(orc, gpu):
+    ref, ctx, _, _ = pair(orc, gpu, 16, 40, 0.01, 1.0, 2, GSF)
+    seed, Mbar, levels, com_delta = 4242, 8, 3, 0.5
+    moves = orc.Moves(ref, Mbar, levels, True, 1, com_delta)
+    rng = orc.MTRand(seed)
+    chain = gpu.Chain(ctx, seed, Mbar, levels, True, 1, com_delta)
+    kinds = ("staging", "bisection", "com")
+    n_acc = 0
+    for k in range(450):
+        kind = kinds[k % 3]
+        acc_r, info_r = getattr(moves, kind)(rng)
+        acc_g, info_g = chain.move(kind)
+        assert bool(acc_r) == acc_g, f"move {k} ({kind}): oracle {acc_r} device {acc_g}"
+        assert abs(info_g[1] - info_r[1]) <= 1e-9 * (1.0 + abs(info_r[1]))
+        n_acc += acc_g
+    print(f"GSF Markov run: {n_acc}/450 accepted, decisions identical")
+    assert np.array_equal(chain.positions(), ref.beads())

@@ -40,8 +40,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--isf-reps", type=int, default=5)
-    ap.add_argument("--legs", default="plain,batched,served",
-                    help="comma list of plain, batched, served (plain always runs)")
+    ap.add_argument("--legs", default="plain,gsf,batched,served",
+                    help="comma list of plain, gsf, batched, served (plain always runs)")
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (HBM counter scaling)")
@@ -166,6 +166,8 @@ def main():
         line["delta_v_cxx"] = cxx
         plain_dv_rate = sum(pairs_dv[s] for s in segs) / (sum(cxx[f"dv{s}_mean_us"] for s in segs) * 1e-6)
         line["plain_delta_v_pair_evals_per_s_cxx"] = plain_dv_rate
+    if "gsf" in legs:
+        line["gsf"] = gsf_leg(pimc_amd, beads, L, M, N, tau)
     if "batched" in legs:
         line["batched"] = batched_leg(pimc_amd, value)
     if "served" in legs:
@@ -178,20 +180,52 @@ def main():
     print(json.dumps(line))
 
 
-def cxx_client(N, M, tau, requests=2000):
+def cxx_client(N, M, tau, requests=2000, action="primitive"):
     """pimc_amd/dv_client: p50/p99 of pimc_delta_action and pimc_total_action from C++."""
     import subprocess
     exe = os.path.join(ROOT, "pimc_amd", "dv_client")
     if not os.path.exists(exe):
         return None
+    cmd = [exe, str(N), str(M), str(tau), "139853", str(requests)] + (["gsf"] if action == "gsf" else [])
     try:
-        r = subprocess.run([exe, str(N), str(M), str(tau), "139853", str(requests)], capture_output=True, text=True,
-                           timeout=120)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     except subprocess.TimeoutExpired:
         return None
     if r.returncode != 0:
         return None
     return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def gsf_leg(pimc_amd, beads, L, M, N, tau, reps=50):
+    """The same walker under the GSF action (VFactor {2/3, 4/3}, gradVFactor
+    {0, 2/9}): the full action adds gradVSquared per slice (N(N-1) gradV pairs per
+    slice), every shift-1 bead action the three-body gradVnnSquared."""
+    from pimc_amd.workloads import LAMBDA
+    ctx = pimc_amd.Context(L, M, N, tau, LAMBDA, L, v_mode=pimc_amd.V_FP32)
+    ctx.upload_path(beads)
+    ctx.set_action_factors((2.0 / 3.0, 4.0 / 3.0), (0.0, 2.0 / 9.0), 2)
+    for _ in range(5):
+        ctx.potentialAction()
+    ctx.set_timing(True)
+    pimc_amd.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.potentialAction()
+    pimc_amd.device_synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    pa_ms, pa_n = ctx.kernel_time("pair_action")
+    g2_ms, g2_n = ctx.kernel_time("gradv2")
+    ctx.set_timing(False)
+    grad_pairs = M * N * (N - 1)
+    out = {"action": "gsf", "total_action_wall_us": wall * 1e6,
+           "pair_kernel_us": pa_ms / max(pa_n, 1) * 1e3, "gradv2_kernel_us": g2_ms / max(g2_n, 1) * 1e3,
+           "gradv2_pair_evals": grad_pairs,
+           "gradv2_pair_evals_per_s": grad_pairs / (g2_ms / max(g2_n, 1) * 1e-3) if g2_n else None}
+    ctx.close()
+    cxx = cxx_client(N, M, tau, requests=500, action="gsf")
+    if cxx:
+        out["cxx"] = cxx
+    return out
 
 
 def batched_leg(pimc_amd, plain_value):

@@ -273,9 +273,10 @@ int pimc_gen_worldlines(uint32_t seed, double side, int M, int N, double lambda,
 /* When enabled, every launch of a hot kernel is bracketed by events (from a
  * pre-created pool) on the stream it runs on; pimc_kernel_time returns the summed
  * duration [ms] and the launch count for name in
- * {"pair_action","delta_action","rho","ssf","isf"}.  "pair_action" and
- * "delta_action" are the pair kernels alone (k_pair_action, k_vnn), without their
- * reductions. */
+ * {"pair_action","delta_action","rho","ssf","isf","gradv2","gradvnn"}.
+ * "pair_action" and "delta_action" are the pair kernels alone (k_pair_action,
+ * k_vnn), without their reductions; "gradv2" and "gradvnn" the gradient
+ * correction's slice and three-body kernels. */
 int pimc_set_timing(pimc_ctx* ctx, int enable);
 int pimc_kernel_time(pimc_ctx* ctx, const char* name, double* total_ms, long* count);
 int pimc_device_synchronize(void);

@@ -3,13 +3,16 @@
 // slices: old + new Vnn of the s-1 moved beads) and of pimc_total_action on
 // seeded worldlines, printed as one JSON object.
 //
-//   dv_client N M tau seed requests
+//   dv_client N M tau seed requests [gsf]
+// "gsf" selects the GSF action (VFactor {2/3, 4/3}, gradVFactor {0, 2/9}): the
+// bead actions then carry the three-body gradient correction.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "../../include/pimc_gpu.h"
@@ -27,6 +30,7 @@ int main(int argc, char** argv) {
     const double tau = argc > 3 ? std::atof(argv[3]) : 0.004;
     const unsigned seed = argc > 4 ? (unsigned)std::atol(argv[4]) : 139853u;
     const int R = argc > 5 ? std::atoi(argv[5]) : 2000;
+    const bool gsf = argc > 6 && std::string(argv[6]) == "gsf";
     const double lambda = 6.0596, density = 0.02198;
     const double side = std::pow(N / density, 1.0 / 3.0);
     std::vector<double> beads(3 * (size_t)M * N);
@@ -41,7 +45,11 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "upload: %s\n", pimc_last_error(c));
         return 1;
     }
-    const double vf[2] = {1.0, 1.0};
+    if (gsf) {
+        const double vf_gsf[2] = {2.0 / 3.0, 4.0 / 3.0}, gvf[2] = {0.0, 2.0 / 9.0};
+        if (pimc_set_action_factors(c, vf_gsf, gvf, 2) != PIMC_OK) return 1;
+    }
+    const double* vf = nullptr;   // the context's factors
     double a = 0.0;
     std::vector<double> t_full;
     for (int k = 0; k < R + 50; ++k) {
@@ -51,8 +59,9 @@ int main(int argc, char** argv) {
     }
     std::mt19937_64 rng(seed);
     std::normal_distribution<double> G(0.0, std::sqrt(lambda * tau));
-    std::printf("{\"N\": %d, \"M\": %d, \"requests\": %d, \"total_action_p50_us\": %.3f, \"total_action_p99_us\": %.3f",
-                N, M, R, pct(t_full, 0.5), pct(t_full, 0.99));
+    std::printf("{\"N\": %d, \"M\": %d, \"action\": \"%s\", \"requests\": %d, \"total_action_p50_us\": %.3f, "
+                "\"total_action_p99_us\": %.3f",
+                N, M, gsf ? "gsf" : "primitive", R, pct(t_full, 0.5), pct(t_full, 0.99));
     for (int seg : {8, 16, 32}) {
         // requests drawn before timing
         std::vector<std::vector<int32_t>> sps(R + 50);

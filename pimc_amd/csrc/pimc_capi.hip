@@ -235,7 +235,7 @@ const double* cached_table(int year, int what, int length, std::string* err) {
 // ===========================================================================
 // timing
 // ===========================================================================
-static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf", "isf"};
+static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf", "isf", "gradv2", "gradvnn"};
 
 constexpr int kTimingSlots = 2048;
 
@@ -482,8 +482,10 @@ static int run_gradv2(pimc_ctx* c, const AzizDev& az) {
         if (!c->d_g2slice && (r = ctx_dalloc(c, &c->d_g2slice, (size_t)M))) return r;
     }
     int blocks = 0;
+    c->timed_begin(5);
     HIP_OR_FAIL(c, launch_gradv2(c->desc.v_mode, c->stream, c->d_pos, c->plane, N, M, c->d_nbs, c->box, az,
                                  c->lds_beads, c->d_g2part, &blocks));
+    c->timed_end();
     c->g2_blocks = blocks;
     return PIMC_OK;
 }
@@ -879,8 +881,10 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
                                       c->stream));
     }
     const KernelTimer kt = c->kernel_timer(1);
+    const KernelTimer kg = cor ? c->kernel_timer(6) : KernelTimer{};
     const int gnn_lds = c->desc.num_alloc <= kLdsBeads ? c->desc.num_alloc : 0;
-    HIP_OR_FAIL(c, launch_vnn(c->desc.v_mode, c->stream, A, req, az, &kt, cor ? c->d_gnn_part : nullptr, gnn_lds));
+    HIP_OR_FAIL(c, launch_vnn(c->desc.v_mode, c->stream, A, req, az, &kt, cor ? c->d_gnn_part : nullptr, gnn_lds,
+                              &kg));
     if (per_bead) {
         HIP_OR_FAIL(c, hipMemcpyAsync(c->h_per_bead, c->d_per_bead, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
                                       c->stream));
@@ -1077,7 +1081,7 @@ int pimc_set_timing(pimc_ctx* c, int enable) {
 int pimc_kernel_time(pimc_ctx* c, const char* name, double* total_ms, long* count) {
     if (!c || !name) return fail(c, PIMC_E_ARG, "null argument");
     c->resolve_timing();
-    for (int k = 0; k < 5; ++k)
+    for (int k = 0; k < pimc_ctx::kKinds; ++k)
         if (std::strcmp(name, kKernelNames[k]) == 0) {
             if (total_ms) *total_ms = c->kernel_ms[k];
             if (count) *count = c->kernel_count[k];

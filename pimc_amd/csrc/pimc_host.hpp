@@ -122,8 +122,9 @@ struct pimc_ctx {
     std::vector<hipEvent_t> ev_pool;   // 2 per slot
     std::vector<Pending> pending;
     int ev_next = 0;
-    double kernel_ms[5] = {0, 0, 0, 0, 0};
-    long kernel_count[5] = {0, 0, 0, 0, 0};
+    static constexpr int kKinds = 7;   // pair_action, delta_action, rho, ssf, isf, gradv2, gradvnn
+    double kernel_ms[kKinds] = {0, 0, 0, 0, 0, 0, 0};
+    long kernel_count[kKinds] = {0, 0, 0, 0, 0, 0, 0};
     int timed_slot(int kind);          // -1 when timing is off
     void timed_begin(int kind);
     void timed_end();

@@ -168,7 +168,8 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
 // Workgroups of the persistent pair kernel for a slice staging of lds_beads beads.
 int pair_grid_for(int lds_beads, int num_cus);
 hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
-                      const KernelTimer* timer = nullptr, double* gnn_part = nullptr, int lds_beads = 0);
+                      const KernelTimer* timer = nullptr, double* gnn_part = nullptr, int lds_beads = 0,
+                      const KernelTimer* gnn_timer = nullptr);
 // gradVSquared(slice) partials [M][blocks_per_slice] for every slice.
 hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,

@@ -849,7 +849,7 @@ void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int
 }
 
 hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
-                      const KernelTimer* timer, double* gnn_part, int lds_beads) {
+                      const KernelTimer* timer, double* gnn_part, int lds_beads, const KernelTimer* gnn_timer) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (A.n_beads <= 0) return hipSuccess;
     dim3 g(A.chunks, A.n_beads), b(kVnnThreads);
@@ -866,12 +866,14 @@ hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInlin
         VnnArgs B = A;
         const size_t lds = sizeof(double) * 3 * (size_t)lds_beads + sizeof(int) * (size_t)A.nalloc;
         dim3 gg(A.gnn_chunks, 2 * A.n_beads);
+        if (gnn_timer && gnn_timer->start) (void)hipEventRecord(gnn_timer->start, st);
         if (mode == 0)
             hipLaunchKernelGGL(k_gradvnn<0>, gg, dim3(256), lds, st, B, req, az, lds_beads, gnn_part);
         else if (mode == 1)
             hipLaunchKernelGGL(k_gradvnn<1>, gg, dim3(256), lds, st, B, req, az, lds_beads, gnn_part);
         else
             hipLaunchKernelGGL(k_gradvnn<2>, gg, dim3(256), lds, st, B, req, az, lds_beads, gnn_part);
+        if (gnn_timer && gnn_timer->stop) (void)hipEventRecord(gnn_timer->stop, st);
     }
     hipLaunchKernelGGL(k_vnn_finalize, dim3(1), dim3(256), 0, st, A, req);
     return hipGetLastError();

@@ -113,6 +113,18 @@ int pimc_energy(pimc_ctx* ctx, const int32_t* next_links, double* out8);
  * per_bead [n] (may be NULL) receives each bead's Vnn. */
 int pimc_action_list(pimc_ctx* ctx, const int32_t* slice_ptcl, int n, int shift,
                      const double vfactor[2], double* action, double* per_bead);
+/* The two parts of the bead actions that the worm moves take apart:
+ *   bare [n] -- LocalAction::barePotentialAction(beads, values) (src/action.cpp:647-689):
+ *               vfactor[slice%2]*shift*tau*Vnn(bead), 0 for a bead that is off;
+ *   cor  [n] -- LocalAction::potentialActionCorrection(bead) (src/action.cpp:696-716):
+ *               gradVFactor[slice%2]*(shift*tau)^3*lambda*gradVnnSquared(bead) at
+ *               ANY shift, 0 where gradVFactor is not finite (set by
+ *               pimc_set_action_factors) and for a bead that is off (the reference's
+ *               callers walk next links, which reach only beads that are on).
+ * The range form potentialActionCorrection(start, end) (:723-738) is the sum of
+ * cor over the beads from start to end.  Either output may be NULL. */
+int pimc_action_parts(pimc_ctx* ctx, const int32_t* slice_ptcl, int n, int shift,
+                      const double vfactor[2], double* bare, double* cor);
 /* ActionBase/LocalAction::potentialAction(start, end) (src/action.cpp:384-399, :522-570):
  * walks next links from start to end inclusive; next_links as in pimc_energy. */
 int pimc_action_range(pimc_ctx* ctx, const int32_t start[2], const int32_t end[2],

@@ -598,6 +598,26 @@ struct State {
             return gradVFactor[eo] * tauShift() * tauShift() * tauShift() * lambda * gradVnnSquared(b);
         return 0.0;
     }
+    // barePotentialAction(vector, values) (action.cpp:647-689): VFactor tau Vnn per
+    // active bead, 0 for the others, in list order
+    void bareActionList(const std::vector<Bead>& beadsIn, std::vector<double>& values) {
+        values.assign(beadsIn.size(), 0.0);
+        for (size_t i = 0; i < beadsIn.size(); ++i) {
+            const Bead& b = beadsIn[i];
+            if (!beadOn(b)) continue;
+            double ext = factor(getState(b)) * 0.0;
+            values[i] = VFactor[b[0] % 2] * tauShift() * Vnn(b, ext);
+        }
+    }
+    // potentialActionCorrection(bead) (action.cpp:700-716): any shift
+    double actionCorrection(const Bead& b) {
+        int eo = b[0] % 2;
+        if (gradVFactor[eo] > kEPS) {
+            lookup.updateInteractionList(*this, b);
+            return gradVFactor[eo] * tauShift() * tauShift() * tauShift() * lambda * gradVnnSquared(b);
+        }
+        return 0.0;
+    }
     double actionBead(const Bead& b) {
         // action.cpp:487-514
         int eo = b[0] % 2;
@@ -1175,6 +1195,16 @@ void orc_state_set_correction(void* p, double gvf0, double gvf1, int period) {
         st->aziz.init(st->aziz.year, st->box.maxSep, st->rc, true);   // gradV needs lookupdVdr
 }
 double orc_gradV2_slice(void* p, int slice) { return static_cast<State*>(p)->gradVSquared(slice); }
+void orc_bare_action_list(void* p, const int32_t* b, int n, double* out) {
+    std::vector<Bead> list;
+    for (int i = 0; i < n; i++) list.push_back(Bead{b[2 * i], b[2 * i + 1]});
+    std::vector<double> v;
+    static_cast<State*>(p)->bareActionList(list, v);
+    for (int i = 0; i < n; i++) out[i] = v[i];
+}
+double orc_action_correction(void* p, int slice, int ptcl) {
+    return static_cast<State*>(p)->actionCorrection(Bead{slice, ptcl});
+}
 double orc_gradVnn2(void* p, int slice, int ptcl) {
     State* st = static_cast<State*>(p);
     Bead b{slice, ptcl};

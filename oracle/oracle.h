@@ -81,6 +81,8 @@ void   orc_update_bead(void* st, int slice, int ptcl, const double* pos3); /* Pa
 void   orc_state_set_correction(void* st, double gvf0, double gvf1, int period);
 double orc_gradV2_slice(void* st, int slice);             /* gradVSquared(slice) action.cpp:1188-1223 */
 double orc_gradVnn2(void* st, int slice, int ptcl);       /* gradVnnSquared(bead) action.cpp:1282-1337 */
+void   orc_bare_action_list(void* st, const int32_t* beads2, int n, double* out); /* barePotentialAction :647-689 */
+double orc_action_correction(void* st, int slice, int ptcl); /* potentialActionCorrection(bead) :700-716 */
 /* EnergyEstimator::accumulate (estimator.cpp:940-1029), primitive PIMC:
  * out = {K, V, V_ext, V_int, E, K/N, V/N, E/N} for one sample. */
 void   orc_energy(void* st, double* out8);

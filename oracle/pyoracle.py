@@ -72,6 +72,8 @@ def lib():
             "orc_state_set_correction": (None, [C.c_void_p, C.c_double, C.c_double, C.c_int]),
             "orc_gradV2_slice": (C.c_double, [C.c_void_p, C.c_int]),
             "orc_gradVnn2": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
+            "orc_bare_action_list": (None, [C.c_void_p, ip, C.c_int, dp]),
+            "orc_action_correction": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
             "orc_energy": (None, [C.c_void_p, dp]),
             "orc_ssf": (None, [C.c_void_p, dp, C.c_int, dp]),
             "orc_isf": (None, [C.c_void_p, dp, C.c_int, dp]),
@@ -246,6 +248,15 @@ class State:
 
     def gradVnn2(self, slice_, ptcl):
         return lib().orc_gradVnn2(self.h, slice_, ptcl)
+
+    def bare_action_list(self, beads2):
+        b = np.ascontiguousarray(beads2, dtype=np.int32).reshape(-1, 2)
+        o = np.zeros(b.shape[0])
+        lib().orc_bare_action_list(self.h, _p(b, ip), b.shape[0], _p(o, dp))
+        return o
+
+    def action_correction(self, slice_, ptcl):
+        return lib().orc_action_correction(self.h, slice_, ptcl)
 
     def total_action(self):
         return lib().orc_total_action(self.h)

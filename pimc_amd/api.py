@@ -68,6 +68,7 @@ SIGNATURES = {
     "pimc_potential_slices": (C.c_int, [C.c_void_p, _dp, _ip]),
     "pimc_energy": (C.c_int, [C.c_void_p, _ip, _dp]),
     "pimc_action_list": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp]),
+    "pimc_action_parts": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp]),
     "pimc_action_range": (C.c_int, [C.c_void_p, _ip, _ip, _ip, C.c_int, _dp, _dp]),
     "pimc_delta_action": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                     C.c_int, C.c_void_p]),
@@ -233,6 +234,25 @@ class Context:
         self._check(lib().pimc_action_list(self.h, _p(sp, _ip), sp.shape[0], shift, _p(vf, _dp), _p(a, _dp),
                                            _p(pb, _dp)), "action_list")
         return (a[0], pb) if per_bead else a[0]
+
+    def action_parts(self, slice_ptcl, shift=1, vfactor=None):
+        """(bare [n], correction [n]) per bead: barePotentialAction and
+        potentialActionCorrection (src/action.cpp:647-716)."""
+        sp = _i32(slice_ptcl, (-1, 2))
+        vf = None if vfactor is None else _f64(vfactor)
+        bare, cor = np.zeros(sp.shape[0]), np.zeros(sp.shape[0])
+        self._check(lib().pimc_action_parts(self.h, _p(sp, _ip), sp.shape[0], shift, _p(vf, _dp), _p(bare, _dp),
+                                            _p(cor, _dp)), "action_parts")
+        return bare, cor
+
+    def barePotentialAction(self, slice_ptcl, shift=1):
+        """LocalAction::barePotentialAction(beads, values) (src/action.cpp:647-689)."""
+        return self.action_parts(slice_ptcl, shift)[0]
+
+    def potentialActionCorrection(self, slice_ptcl, shift=1):
+        """LocalAction::potentialActionCorrection summed over the listed beads
+        (src/action.cpp:696-738; a start..end range is its bead list)."""
+        return float(sum(self.action_parts(slice_ptcl, shift)[1].tolist()))   # in bead order
 
     def action_range(self, start, end, next_links=None, shift=1, vfactor=None):
         s, e = _i32(start), _i32(end)

@@ -657,6 +657,8 @@ static int ensure_req(pimc_ctx* c, int n) {
     if ((r = ctx_halloc(c, &c->h_req_pos, 3 * (size_t)cap))) return r;
     if ((r = ctx_dalloc(c, &c->d_per_bead, 2 * (size_t)cap))) return r;
     if ((r = ctx_halloc(c, &c->h_per_bead, 2 * (size_t)cap))) return r;
+    if ((r = ctx_dalloc(c, &c->d_per_bead_cor, 2 * (size_t)cap))) return r;
+    if ((r = ctx_halloc(c, &c->h_per_bead_cor, 2 * (size_t)cap))) return r;
     const int chunks = (c->desc.num_alloc + kVnnChunk - 1) / kVnnChunk;
     if ((r = ctx_dalloc(c, &c->d_vnn_partials, 2 * (size_t)cap * chunks))) return r;
     c->cap_req = cap;
@@ -826,7 +828,7 @@ int pimc_energy(pimc_ctx* c, const int32_t* next_links, double* out8) {
 }
 
 static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commit_in_kernel, double* out3,
-                   double* per_bead) {
+                   double* per_bead, double* cor_out = nullptr) {
     int r;
     AzizDev az = ctx_aziz(c, &r);
     if (r) return r;
@@ -852,7 +854,9 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
     A.vfactor0 = vf ? vf[0] : c->vf_default[0];
     A.vfactor1 = vf ? vf[1] : c->vf_default[1];
     // gradient correction of the bead actions (shift 1 only, as the reference)
-    const bool cor = c->correction() && shift == 1;
+    const bool cor = c->correction() && (shift == 1 || cor_out);
+    A.cor_any_shift = cor_out ? 1 : 0;
+    A.per_bead_cor = cor_out ? c->d_per_bead_cor : nullptr;
     A.gnn_part = nullptr;
     A.gnn_chunks = (c->desc.num_alloc + kGnnGroup - 1) / kGnnGroup;
     A.gvf0 = c->gvf[0];
@@ -885,10 +889,16 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
     const int gnn_lds = c->desc.num_alloc <= kLdsBeads ? c->desc.num_alloc : 0;
     HIP_OR_FAIL(c, launch_vnn(c->desc.v_mode, c->stream, A, req, az, &kt, cor ? c->d_gnn_part : nullptr, gnn_lds,
                               &kg));
-    if (per_bead) {
-        HIP_OR_FAIL(c, hipMemcpyAsync(c->h_per_bead, c->d_per_bead, sizeof(double) * 2 * n, hipMemcpyDeviceToHost,
-                                      c->stream));
+    if (per_bead || cor_out) {
+        if (per_bead)
+            HIP_OR_FAIL(c, hipMemcpyAsync(c->h_per_bead, c->d_per_bead, sizeof(double) * 2 * n,
+                                          hipMemcpyDeviceToHost, c->stream));
+        if (cor_out)
+            HIP_OR_FAIL(c, hipMemcpyAsync(c->h_per_bead_cor, c->d_per_bead_cor, sizeof(double) * 2 * n,
+                                          hipMemcpyDeviceToHost, c->stream));
         HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+        if (cor_out)
+            for (int k = 0; k < n; ++k) cor_out[k] = cor ? c->h_per_bead_cor[2 * k] : 0.0;
     } else if (c->wait_done(A.seq) != 0) {
         return fail(c, PIMC_E_HIP, "delta action: completion not published");
     }
@@ -924,6 +934,35 @@ int pimc_action_list(pimc_ctx* c, const int32_t* sp, int n, int shift, const dou
     *action = out3[0];
     if (per_bead)
         for (int k = 0; k < n; ++k) per_bead[k] = pb[2 * k];
+    return PIMC_OK;
+}
+
+int pimc_action_parts(pimc_ctx* c, const int32_t* sp, int n, int shift, const double vfactor[2], double* bare,
+                      double* cor) {
+    if (!c) return fail(c, PIMC_E_ARG, "null context");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    if (shift < 1) return fail(c, PIMC_E_ARG, "shift must be >= 1");
+    int r;
+    if ((r = check_beads(c, sp, n))) return r;
+    if (n == 0 || (!bare && !cor)) return PIMC_OK;
+    if ((r = ensure_req(c, n))) return r;
+    for (int k = 0; k < n; ++k) {
+        VnnBead& q = c->h_req_beads[k];
+        q.walker = 0;
+        q.slice = sp[2 * k];
+        q.ptcl = sp[2 * k + 1];
+        q.flags = 0;
+        for (int d = 0; d < 3; ++d) q.old_pos[d] = q.new_pos[d] = 0.0;
+    }
+    std::vector<double> pb(2 * (size_t)n), cb(cor ? (size_t)n : 0);
+    if ((r = run_vnn(c, n, shift, vfactor, false, nullptr, pb.data(), cor ? cb.data() : nullptr))) return r;
+    const double tau_s = shift * c->desc.tau;
+    if (bare)
+        for (int k = 0; k < n; ++k) {
+            const double vf = vfactor ? vfactor[sp[2 * k] & 1] : c->vf_default[sp[2 * k] & 1];
+            bare[k] = vf * tau_s * pb[2 * k];
+        }
+    if (cor) std::memcpy(cor, cb.data(), sizeof(double) * n);
     return PIMC_OK;
 }
 

@@ -92,6 +92,8 @@ struct pimc_ctx {
     double* h_req_pos = nullptr;
     double* d_per_bead = nullptr;
     double* h_per_bead = nullptr;
+    double* d_per_bead_cor = nullptr;   // per-bead corrections [cap_req][2]
+    double* h_per_bead_cor = nullptr;
     double* d_vnn_partials = nullptr;
     // structure factors
     int cap_q = 0;

@@ -108,6 +108,8 @@ struct VnnArgs {
     const double* gnn_part;
     int gnn_chunks;
     double gvf0, gvf1, lambda;
+    int cor_any_shift;     // evaluate the correction at any shift (potentialActionCorrection)
+    double* per_bead_cor;  // [n_beads][2] corrections at the old / new position, or null
 };
 
 constexpr int kGnnGroup = 16;   // bead2 per three-body workgroup (16 lanes each)

@@ -561,7 +561,7 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
         // gradVnnSquared = sum of the chunks' three-body terms + |F1|^2
         double cor[2] = {0.0, 0.0};
         const double gvf = (slice & 1) ? A.gvf1 : A.gvf0;
-        if (A.gnn_part && A.shift == 1 && gvf > kCorrEps)
+        if (A.gnn_part && (A.shift == 1 || A.cor_any_shift) && gvf > kCorrEps)
             for (int w = 0; w < 2; ++w) {
                 const double* g = A.gnn_part + ((int64_t)k * 2 + w) * A.gnn_chunks * 4;
                 double t = 0.0, f1x = 0.0, f1y = 0.0, f1z = 0.0;
@@ -571,9 +571,15 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
                     f1y += g[4 * cc + 2];
                     f1z += g[4 * cc + 3];
                 }
-                t += 0.0 + 2.0 * 0.0 + norm2_exact(f1x, f1y, f1z);
+                t += norm2_exact(f1x, f1y, f1z);
                 cor[w] = gvf * tau_s * tau_s * tau_s * A.lambda * t;
             }
+        if (A.per_bead_cor) {
+            A.per_bead_cor[2 * k + 0] = cor[0];
+            A.per_bead_cor[2 * k + 1] = cor[1];
+        }
+        // the bead actions carry the correction at shift 1 only (src/action.cpp:500)
+        if (A.shift != 1) cor[0] = cor[1] = 0.0;
         ao += co * vo + cor[0];
         an += co * vn + cor[1];
         du += co * (vn - vo) + (cor[1] - cor[0]);

@@ -109,3 +109,31 @@ def test_gsf_markov_run_matches_oracle(orc, gpu):
         n_acc += acc_g
     print(f"GSF Markov run: {n_acc}/450 accepted, decisions identical")
     assert np.array_equal(chain.positions(), ref.beads())
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("shift", [1, 2])
+@pytest.mark.parametrize("name,action", [("gsf", GSF), ("li_broughton", LB)])
+def test_bare_action_and_correction_parts(orc, gpu, mode, shift, name, action):
+    """barePotentialAction(beads, values) and potentialActionCorrection(bead)
+    (src/action.cpp:647-716) per bead, at shift 1 and 2; at shift 1 their sum is
+    the bead action the moves use."""
+    ref, ctx, _, _ = pair(orc, gpu, 24, 32, 0.01, 1.0, mode, action)
+    ref.set_action(shift, action["vfactor"])
+    rng = np.random.default_rng(9)
+    sp = np.array([[int(rng.integers(32)), int(rng.integers(24))] for _ in range(12)], dtype=np.int32)
+    bare, cor = ctx.action_parts(sp, shift=shift)
+    bare_ref = np.array(ref.bare_action_list(sp))
+    cor_ref = np.array([ref.action_correction(int(m), int(p)) for m, p in sp])
+    scale = np.abs(bare_ref).max()
+    assert np.all(np.abs(bare - bare_ref) <= TOL[mode] * scale), (bare, bare_ref)
+    cscale = max(np.abs(cor_ref).max(), 1e-300)
+    assert np.any(cor_ref != 0.0)
+    assert np.all(np.abs(cor - cor_ref) <= TOL[mode] * cscale), (cor, cor_ref)
+    # even slices carry no GSF correction (gradVFactor[0] = 0)
+    if name == "gsf":
+        assert np.all(cor[sp[:, 0] % 2 == 0] == 0.0)
+    if shift == 1:
+        total = ctx.action_list(sp, shift=1)
+        assert abs(total - (bare.sum() + cor.sum())) <= 1e-12 * abs(total)
+    assert ctx.potentialActionCorrection(sp, shift=shift) == pytest.approx(cor.sum(), rel=1e-14, abs=0)

@@ -104,3 +104,33 @@ def test_energy_is_the_tau_lambda_derivative(orc):
     # the operator estimator averages the even slices only (period 2)
     vop = np.mean([st.V_slice(m)[1] for m in range(0, M, 2)]) + tail
     assert np.isclose(st.energy()["V_int"], vop, rtol=1e-12)
+
+
+@pytest.mark.parametrize("shift", [1, 2, 3])
+def test_bare_list_and_correction(orc, shift):
+    """barePotentialAction(beads, values) (src/action.cpp:647-689) is
+    VFactor*shift*tau times a numpy pair sum of the table V over the slice;
+    potentialActionCorrection (:696-716) scales as (shift*tau)^3 and, at shift 1,
+    is the bead action minus its bare part (the bead action's correction being
+    pinned by the checkMove invariant above)."""
+    N, M, tau = 10, 8, 0.02
+    st, L = make(orc, N, M, tau, seed=13)
+    az = st.aziz
+    b = st.beads()
+    rng = np.random.default_rng(shift)
+    sp = np.array([[int(rng.integers(M)), int(rng.integers(N))] for _ in range(10)], dtype=np.int32)
+    st.set_action(shift, GSF["vfactor"])
+    bare = np.array(st.bare_action_list(sp))
+    cor = np.array([st.action_correction(int(m), int(p)) for m, p in sp])
+    for k, (m, p) in enumerate(sp):
+        seps = b[m, p] - np.delete(b[m], p, axis=0)
+        seps -= L * np.floor(seps / L + 0.5)
+        vnn = sum(az.V(s) for s in seps)
+        assert np.isclose(bare[k], GSF["vfactor"][m % 2] * shift * tau * vnn, rtol=1e-12, atol=0)
+    st.set_action(1, GSF["vfactor"])
+    cor1 = np.array([st.action_correction(int(m), int(p)) for m, p in sp])
+    assert np.allclose(cor, shift ** 3 * cor1, rtol=1e-12, atol=0)
+    assert np.all(cor1[sp[:, 0] % 2 == 0] == 0.0) and np.any(cor1 != 0.0)
+    bare1 = np.array(st.bare_action_list(sp))
+    for k, (m, p) in enumerate(sp):
+        assert np.isclose(bare1[k] + cor1[k], st.action_bead(int(m), int(p)), rtol=1e-13, atol=0)

@@ -40,8 +40,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--isf-reps", type=int, default=5)
-    ap.add_argument("--legs", default="plain,gsf,batched,served",
-                    help="comma list of plain, gsf, batched, served (plain always runs)")
+    ap.add_argument("--legs", default="plain,gsf,virial,batched,served",
+                    help="comma list of plain, gsf, virial, batched, served (plain always runs)")
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (HBM counter scaling)")
@@ -168,6 +168,8 @@ def main():
         line["plain_delta_v_pair_evals_per_s_cxx"] = plain_dv_rate
     if "gsf" in legs:
         line["gsf"] = gsf_leg(pimc_amd, beads, L, M, N, tau)
+    if "virial" in legs:
+        line["virial"] = virial_leg(pimc_amd, beads, L, M, N, tau, not args.no_cpu_baseline)
     if "batched" in legs:
         line["batched"] = batched_leg(pimc_amd, value)
     if "served" in legs:
@@ -225,6 +227,52 @@ def gsf_leg(pimc_amd, beads, L, M, N, tau, reps=50):
     cxx = cxx_client(N, M, tau, requests=500, action="gsf")
     if cxx:
         out["cxx"] = cxx
+    return out
+
+
+def virial_leg(pimc_amd, beads, L, M, N, tau, cpu, window=5, reps=20):
+    """VirialEnergyEstimator::accumulate on the same walker under the GSF action
+    (virial window 5): V(slice) from the pair kernel plus the virial kernel's
+    N(N-1) ordered pairs per slice (gradV on every slice, the T-matrix on the
+    odd slices where gradVFactor = 2/9)."""
+    from pimc_amd.workloads import LAMBDA
+    ctx = pimc_amd.Context(L, M, N, tau, LAMBDA, L, v_mode=pimc_amd.V_FP32)
+    ctx.upload_path(beads)
+    ctx.set_action_factors((2.0 / 3.0, 4.0 / 3.0), (0.0, 2.0 / 9.0), 2)
+    for _ in range(3):
+        ctx.virial_energy(window)
+    ctx.set_timing(True)
+    pimc_amd.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e = ctx.virial_energy(window)
+    wall = (time.perf_counter() - t0) / reps
+    v_ms, v_n = ctx.kernel_time("virial")
+    pa_ms, pa_n = ctx.kernel_time("pair_action")
+    ctx.set_timing(False)
+    ctx.close()
+    pairs = M * N * (N - 1)
+    k_s = v_ms / max(v_n, 1) * 1e-3
+    out = {"action": "gsf", "window": window, "estimator_wall_us": wall * 1e6,
+           "virial_kernel_us": k_s * 1e6, "pair_kernel_us": pa_ms / max(pa_n, 1) * 1e3,
+           "ordered_pairs_per_launch": pairs, "virial_pair_evals_per_s": pairs / k_s if v_n else None,
+           "E": e["E"], "E_th": e["E_th"]}
+    if cpu:
+        # the oracle's accumulate on the first 24 slices of the same worldlines
+        # (periodic over those 24), one call, single thread
+        from oracle import pyoracle as orc
+        Ms = 24
+        st = orc.State(L, Ms, N, tau, LAMBDA, L)
+        st.set_beads(np.ascontiguousarray(beads[:Ms]))
+        st.set_action(1, (2.0 / 3.0, 4.0 / 3.0))
+        st.set_correction((0.0, 2.0 / 9.0), 2)
+        st.virial_energy(window)            # builds the derivative tables
+        t0 = time.perf_counter()
+        st.virial_energy(window)
+        t_cpu = time.perf_counter() - t0
+        out["cpu_port"] = {"sample": f"{Ms} slices of N={N}, one accumulate", "seconds": t_cpu, "cores": 1,
+                           "virial_pair_evals_per_s": Ms * N * (N - 1) / t_cpu,
+                           "estimator_us_scaled_to_M": t_cpu * M / Ms * 1e6}
     return out
 
 

@@ -106,6 +106,28 @@ int pimc_potential_slices(pimc_ctx* ctx, double* v_slices, int32_t* sep_hist);
  * action (src/estimator.cpp:940-1029): out8 = {K, V, V_ext, V_int, E, K/N, V/N, E/N}.
  * next_links [M][N_alloc][2] may be NULL (periodic closed worldlines). */
 int pimc_energy(pimc_ctx* ctx, const int32_t* next_links, double* out8);
+/* The virial pair terms of LocalAction for every slice (src/action.cpp:1446-1786,
+ * :776-787; ActionBase virtuals include/action.h:75-76, :183-194), with the
+ * context's action factors (pimc_set_action_factors) at shift 1:
+ *   out6 [M][6] = {rDOTgradUterm1, rDOTgradUterm2, deltadotgradUterm1,
+ *                  deltadotgradUterm2, virialKinCorrection,
+ *                  secondderivPotentialActionTau}.
+ * Every pair of a slice counts (no cutoff).  The T-matrix of the second terms
+ * weights its transverse part with |gradV|, as the reference does.  window is
+ * virialWindow (>= 1, the reference's default 5); next_links as in pimc_energy
+ * (NULL: periodic closed worldlines); every active bead must link to an active
+ * bead (PIMC_E_ARG otherwise).  FP64 in every mode (PIMC_V_MIXED evaluates dV/dr
+ * and d2V/dr2 in closed form like PIMC_V_FP64; PIMC_V_TABLE gathers lookupdVdr
+ * and lookupd2Vdr2). */
+int pimc_virial_terms(pimc_ctx* ctx, const int32_t* next_links, int window, double* out6);
+/* VirialEnergyEstimator::accumulate (src/estimator.cpp:1086-1251): the 19 values
+ * one call adds, in the reference's column order
+ *   K_op K_cv V_op V_cv E E_mu K_op/N K_cv/N V_op/N V_cv/N E/N EEcv*Beta^2
+ *   Ecv*Beta dEdB CvCov1 CvCov2 CvCov3 E_th P
+ * mu is the chemical potential of E_mu.  As in the reference, the term the
+ * estimator files under the unregistered key "cVCov2" lands on K_op (index 0)
+ * and CvCov2 stays 0 (src/estimator.cpp:1244). */
+int pimc_virial_energy(pimc_ctx* ctx, const int32_t* next_links, int window, double mu, double* out19);
 
 /* LocalAction::potentialAction(std::vector<beadLocator>) (src/action.cpp:572-620):
  * sum over active listed beads of vfactor[slice%2]*shift*tau*Vnn(bead)

@@ -609,6 +609,124 @@ struct State {
             values[i] = VFactor[b[0] % 2] * tauShift() * Vnn(b, ext);
         }
     }
+    // Virial pair terms (action.cpp:1446-1805; the free external potential has
+    // gradV = grad2V = 0).  The T-matrix weights the transverse part with
+    // dV = |gradV| (a magnitude, not the signed dV/dr) as the reference does.
+    void tMatAdd(const Vec& rDiff, double rmag, double dV, double d2V, double tMat[3][3]) const {
+        for (int a = 0; a < 3; a++)
+            for (int c = 0; c < 3; c++) {
+                tMat[a][c] += rDiff[a] * rDiff[c] * d2V / (rmag * rmag) - rDiff[a] * rDiff[c] * dV / std::pow(rmag, 3);
+                if (a == c) tMat[a][c] += dV / rmag;
+            }
+    }
+    Vec sliceForce(const Bead& b1) const {
+        Vec gVi{0.0, 0.0, 0.0};
+        Bead b2{b1[0], 0};
+        for (b2[1] = 0; b2[1] < numBeadsAtSlice[b1[0]]; b2[1]++)
+            if (!(b1 == b2)) {
+                Vec g = aziz.gradV(getSeparation(b1, b2));
+                for (int i = 0; i < 3; i++) gVi[i] += g[i];
+            }
+        return gVi;   // + externalPtr->gradV(path(bead1)) = 0
+    }
+    // gVdotT = T gV for bead1 (the bead2 loop of rDOTgradUterm2, :1530-1566)
+    Vec forceDotT(const Bead& b1) const {
+        Vec gV{0.0, 0.0, 0.0};
+        double tMat[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        Bead b2{b1[0], 0};
+        for (b2[1] = 0; b2[1] < numBeadsAtSlice[b1[0]]; b2[1]++) {
+            Vec rDiff = getSeparation(b1, b2);
+            double rmag = std::sqrt(dot3(rDiff, rDiff));
+            if (!(b1 == b2)) {
+                Vec gVi = aziz.gradV(rDiff);
+                double dVi = std::sqrt(dot3(gVi, gVi));
+                double g2Vi = aziz.grad2V(rDiff);
+                double dV = dVi + 0.0, d2V = g2Vi + 0.0;
+                tMatAdd(rDiff, rmag, dV, d2V, tMat);
+                for (int i = 0; i < 3; i++) gV[i] += gVi[i];
+            }
+        }
+        Vec out{0.0, 0.0, 0.0};
+        for (int r = 0; r < 3; r++) {
+            double ip = 0.0;
+            for (int c = 0; c < 3; c++) ip += tMat[r][c] * gV[c];
+            out[r] += ip;
+        }
+        return out;
+    }
+    // pos1 - centre of the worldline window, put in the box (:1621-1645)
+    Vec windowDelta(const Bead& b1, int W) const {
+        Vec runMore{0, 0, 0}, runLess{0, 0, 0}, COM{0, 0, 0};
+        const Vec pos1 = pos(b1);
+        Bead nOld = b1, pOld = b1;
+        for (int gamma = 0; gamma < W; gamma++) {
+            Bead bn = next(b1, gamma), bp = prev(b1, gamma);
+            Vec sm = getSeparation(bn, nOld), sl = getSeparation(bp, pOld);
+            for (int i = 0; i < 3; i++) {
+                runMore[i] += sm[i];
+                runLess[i] += sl[i];
+                COM[i] += (pos1[i] + runMore[i]) + (pos1[i] + runLess[i]);
+            }
+            nOld = bn;
+            pOld = bp;
+        }
+        Vec delta;
+        for (int i = 0; i < 3; i++) {
+            COM[i] /= (2.0 * W);
+            delta[i] = pos1[i] - COM[i];
+        }
+        box.putInBC(delta);
+        return delta;
+    }
+    double rDOTgradUterm1(int slice) const {   // :1446-1482
+        double rDotgV = 0.0;
+        Bead b1{slice, 0};
+        for (b1[1] = 0; b1[1] < numBeadsAtSlice[slice]; b1[1]++) rDotgV += dot3(sliceForce(b1), pos(b1));
+        return VFactor[slice % 2] * tauShift() * rDotgV;
+    }
+    double rDOTgradUterm2(int slice) const {   // :1494-1574
+        const int eo = slice % 2;
+        double term2 = 0.0;
+        if (gradVFactor[eo] > kEPS) {
+            Bead b1{slice, 0};
+            for (b1[1] = 0; b1[1] < numBeadsAtSlice[slice]; b1[1]++) term2 += dot3(forceDotT(b1), pos(b1));
+            term2 *= (2.0 * gradVFactor[eo] * std::pow(tauShift(), 3) * lambda);
+        }
+        return term2;
+    }
+    double deltadotgradUterm1(int slice, int W) const {   // :1587-1653
+        double rDotgV = 0.0;
+        Bead b1{slice, 0};
+        for (b1[1] = 0; b1[1] < numBeadsAtSlice[slice]; b1[1]++)
+            rDotgV += dot3(sliceForce(b1), windowDelta(b1, W));
+        return VFactor[slice % 2] * tauShift() * rDotgV;
+    }
+    double deltadotgradUterm2(int slice, int W) const {   // :1665-1762
+        const int eo = slice % 2;
+        double term2 = 0.0;
+        if (gradVFactor[eo] > kEPS) {
+            Bead b1{slice, 0};
+            for (b1[1] = 0; b1[1] < numBeadsAtSlice[slice]; b1[1]++)
+                term2 += dot3(forceDotT(b1), windowDelta(b1, W));
+            term2 *= (2.0 * gradVFactor[eo] * std::pow(tauShift(), 3) * lambda);
+        }
+        return term2;
+    }
+    double virialKinCorrection(int slice) const {   // :1769-1786
+        const int eo = slice % 2;
+        double vKC = 0.0;
+        if (gradVFactor[eo] > kEPS) {
+            vKC += gradVSquared(slice);
+            vKC *= gradVFactor[eo] * std::pow(tauShift(), 3) * lambda;
+        }
+        return vKC;
+    }
+    double secondDerivTau(int slice) const {   // :776-787
+        const int eo = slice % 2;
+        double d2U = 0.0;
+        if (gradVFactor[eo] > kEPS) d2U += 6.0 * gradVFactor[eo] * tauShift() * lambda * gradVSquared(slice);
+        return d2U;
+    }
     // potentialActionCorrection(bead) (action.cpp:700-716): any shift
     double actionCorrection(const Bead& b) {
         int eo = b[0] % 2;
@@ -1204,6 +1322,117 @@ void orc_bare_action_list(void* p, const int32_t* b, int n, double* out) {
 }
 double orc_action_correction(void* p, int slice, int ptcl) {
     return static_cast<State*>(p)->actionCorrection(Bead{slice, ptcl});
+}
+void orc_virial_terms(void* p, int W, double* out) {
+    State* st = static_cast<State*>(p);
+    if (st->aziz.tdV.empty()) st->aziz.init(st->aziz.year, st->box.maxSep, st->rc, true);
+    for (int m = 0; m < st->M; m++) {
+        double* o = out + 6 * (size_t)m;
+        o[0] = st->rDOTgradUterm1(m);
+        o[1] = st->rDOTgradUterm2(m);
+        o[2] = st->deltadotgradUterm1(m, W);
+        o[3] = st->deltadotgradUterm2(m, W);
+        o[4] = st->virialKinCorrection(m);
+        o[5] = st->secondDerivTau(m);
+    }
+}
+void orc_virial_energy(void* p, int W, double mu, double* out) {
+    // VirialEnergyEstimator::accumulate (estimator.cpp:1086-1251), one call
+    State* st = static_cast<State*>(p);
+    if (st->aziz.tdV.empty()) st->aziz.init(st->aziz.year, st->box.maxSep, st->rc, true);
+    for (int i = 0; i < 19; i++) out[i] = 0.0;
+    double totVop = 0.0, thermE = 0.0, totEcv = 0.0, Kcv = 0.0, T2 = 0.0, T3 = 0.0, T4 = 0.0, virKinTerm = 0.0;
+    const int numParticles = st->trueNumParticles();
+    const int nts = st->M;
+    const double tau = st->tau, lambda = st->lambda;
+    const double volume = st->box.volume;
+    const double beta = 1.0 * nts * tau;
+    const double tailV = (1.0 * numParticles * numParticles / volume) * st->aziz.tailV;
+    const double thermTerm1 = (0.5 * kNDIM / tau) * numParticles;
+    const double T1 = 0.5 * kNDIM * numParticles / (1.0 * W * tau);
+    const double exchangeNorm = 1.0 / (4.0 * W * std::pow(tau, 2) * lambda * nts);
+    double Pressure = kNDIM * numParticles;
+    double P2 = 0.0, P3 = 0.0;
+    Bead b1;
+    for (int slice = 0; slice < nts; slice++) {
+        b1[0] = slice;
+        for (int ptcl = 0; ptcl < st->numBeadsAtSlice[slice]; ptcl++) {
+            b1[1] = ptcl;
+            Vec vel2 = st->getVelocity(b1);
+            Bead nOld = b1;
+            Vec vel1{0, 0, 0};
+            for (int gamma = 1; gamma <= W; gamma++) {
+                Bead bn = st->next(b1, gamma);
+                Vec s = st->getSeparation(bn, nOld);
+                for (int i = 0; i < 3; i++) vel1[i] += s[i];
+                nOld = bn;
+            }
+            T2 -= dot3(vel1, vel2);
+            thermE -= dot3(vel2, vel2);
+        }
+    }
+    P2 = thermE;
+    T2 *= exchangeNorm;
+    P2 /= (2.0 * lambda * tau * nts);
+    Pressure += P2;
+    double T5 = 0.0;
+    for (int slice = 0; slice < nts; slice++) {
+        const int eo = slice % 2;
+        T3 += st->deltadotgradUterm1(slice, W);
+        T4 += st->deltadotgradUterm2(slice, W);
+        // derivPotentialActionTau (action.cpp:751-765)
+        auto v = st->Vslice(slice);
+        double dU = st->VFactor[eo] * (v[0] + v[1]);
+        if (st->gradVFactor[eo] > kEPS)
+            dU += 3.0 * st->gradVFactor[eo] * st->tauShift() * st->tauShift() * lambda * st->gradVSquared(slice);
+        T5 += dU;
+        virKinTerm += st->virialKinCorrection(slice);
+        if (eo == 0) totVop += v[0] + v[1];
+        P3 += st->rDOTgradUterm1(slice) + st->rDOTgradUterm2(slice);
+    }
+    P3 *= (1.0 / (2.0 * nts));
+    Pressure -= P3;
+    Pressure /= (kNDIM * tau * volume);
+    totVop /= (0.5 * nts);
+    totVop += tailV;
+    T3 /= (2.0 * beta);
+    T4 /= (1.0 * beta);
+    T5 /= (1.0 * nts);
+    virKinTerm /= (0.5 * beta);
+    thermE *= (1.0 / (4.0 * lambda * tau)) / (tau * nts);
+    thermE += thermTerm1;
+    thermE += T5;
+    thermE += tailV;
+    totEcv = T1 + T2 + T3 + T4 + T5 + tailV;
+    Kcv = T1 + T2 + T3 + T4 + virKinTerm;
+    double dEdB = (-1.0 * T1 - 2.0 * T2 + 2.0 * T4) / tau;
+    for (int slice = 0; slice < nts; slice++) dEdB += st->secondDerivTau(slice) / (1.0 * nts);
+    dEdB *= beta * beta / (1.0 * nts);
+    // columns: K_op K_cv V_op V_cv E E_mu K_op/N K_cv/N V_op/N V_cv/N E/N
+    //          EEcv*Beta^2 Ecv*Beta dEdB CvCov1 CvCov2 CvCov3 E_th P
+    out[0] += totEcv - totVop;
+    out[1] += Kcv;
+    out[2] += totVop;
+    out[3] += totEcv - Kcv;
+    out[4] += totEcv;
+    out[5] += totEcv - mu * numParticles;
+    if (numParticles > 0) {
+        out[6] += (totEcv - totVop) / (1.0 * numParticles);
+        out[7] += Kcv / (1.0 * numParticles);
+        out[8] += totVop / (1.0 * numParticles);
+        out[9] += (totEcv - Kcv) / (1.0 * numParticles);
+        out[10] += totEcv / (1.0 * numParticles);
+    }
+    out[11] += totEcv * thermE * beta * beta;
+    out[12] += totEcv * beta;
+    out[13] += dEdB;
+    out[14] += totEcv * thermE * beta * beta * totEcv * beta;
+    // estIndex["cVCov2"]: a key the constructor never registered; std::map
+    // inserts it with index 0, so this lands on K_op and CvCov2 stays 0
+    out[0] += totEcv * beta * dEdB;
+    out[16] += totEcv * thermE * beta * beta * dEdB;
+    out[17] += thermE;
+    out[18] += Pressure;
 }
 double orc_gradVnn2(void* p, int slice, int ptcl) {
     State* st = static_cast<State*>(p);

@@ -83,6 +83,12 @@ double orc_gradV2_slice(void* st, int slice);             /* gradVSquared(slice)
 double orc_gradVnn2(void* st, int slice, int ptcl);       /* gradVnnSquared(bead) action.cpp:1282-1337 */
 void   orc_bare_action_list(void* st, const int32_t* beads2, int n, double* out); /* barePotentialAction :647-689 */
 double orc_action_correction(void* st, int slice, int ptcl); /* potentialActionCorrection(bead) :700-716 */
+/* Virial pair terms per slice (action.cpp:1446-1786, :776-787), out [M][6] =
+ * {rDOTgradUterm1, rDOTgradUterm2, deltadotgradUterm1, deltadotgradUterm2,
+ *  virialKinCorrection, secondderivPotentialActionTau}; W = virial window. */
+void   orc_virial_terms(void* st, int W, double* out);
+/* VirialEnergyEstimator::accumulate (estimator.cpp:1086-1251): the 19 columns of one call. */
+void   orc_virial_energy(void* st, int W, double mu, double* out19);
 /* EnergyEstimator::accumulate (estimator.cpp:940-1029), primitive PIMC:
  * out = {K, V, V_ext, V_int, E, K/N, V/N, E/N} for one sample. */
 void   orc_energy(void* st, double* out8);

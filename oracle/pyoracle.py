@@ -19,6 +19,10 @@ up = C.POINTER(C.c_uint32)
 bp = C.POINTER(C.c_uint8)
 
 
+VIRIAL_COLUMNS = ("K_op", "K_cv", "V_op", "V_cv", "E", "E_mu", "K_op/N", "K_cv/N", "V_op/N", "V_cv/N", "E/N",
+                  "EEcv*Beta^2", "Ecv*Beta", "dEdB", "CvCov1", "CvCov2", "CvCov3", "E_th", "P")
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -75,6 +79,8 @@ def lib():
             "orc_bare_action_list": (None, [C.c_void_p, ip, C.c_int, dp]),
             "orc_action_correction": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
             "orc_energy": (None, [C.c_void_p, dp]),
+            "orc_virial_terms": (None, [C.c_void_p, C.c_int, dp]),
+            "orc_virial_energy": (None, [C.c_void_p, C.c_int, C.c_double, dp]),
             "orc_ssf": (None, [C.c_void_p, dp, C.c_int, dp]),
             "orc_isf": (None, [C.c_void_p, dp, C.c_int, dp]),
             "orc_grid_dims": (None, [C.c_void_p, ip, ip]),
@@ -293,6 +299,19 @@ class State:
         o = np.zeros(8)
         lib().orc_energy(self.h, _p(o, dp))
         return dict(zip(["K", "V", "V_ext", "V_int", "E", "K/N", "V/N", "E/N"], o))
+
+    def virial_terms(self, window=5):
+        """[M][6] = rDOTgradUterm1, rDOTgradUterm2, deltadotgradUterm1,
+        deltadotgradUterm2, virialKinCorrection, secondderivPotentialActionTau."""
+        o = np.zeros((self.M, 6))
+        lib().orc_virial_terms(self.h, window, _p(o, dp))
+        return o
+
+    def virial_energy(self, window=5, mu=0.0):
+        """VirialEnergyEstimator::accumulate, one call (src/estimator.cpp:1086-1251)."""
+        o = np.zeros(19)
+        lib().orc_virial_energy(self.h, window, mu, _p(o, dp))
+        return dict(zip(VIRIAL_COLUMNS, o))
 
     def ssf(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)

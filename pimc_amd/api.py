@@ -18,6 +18,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpimc_gpu.so")
 
 V_FP32, V_FP64, V_TABLE = 0, 1, 2
+VIRIAL_COLUMNS = ("K_op", "K_cv", "V_op", "V_cv", "E", "E_mu", "K_op/N", "K_cv/N", "V_op/N", "V_cv/N", "E/N",
+                  "EEcv*Beta^2", "Ecv*Beta", "dEdB", "CvCov1", "CvCov2", "CvCov3", "E_th", "P")
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -67,6 +69,8 @@ SIGNATURES = {
     "pimc_total_action": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pimc_potential_slices": (C.c_int, [C.c_void_p, _dp, _ip]),
     "pimc_energy": (C.c_int, [C.c_void_p, _ip, _dp]),
+    "pimc_virial_terms": (C.c_int, [C.c_void_p, _ip, C.c_int, _dp]),
+    "pimc_virial_energy": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_double, _dp]),
     "pimc_action_list": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp]),
     "pimc_action_parts": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp]),
     "pimc_action_range": (C.c_int, [C.c_void_p, _ip, _ip, _ip, C.c_int, _dp, _dp]),
@@ -225,6 +229,22 @@ class Context:
         nx = None if next_links is None else _i32(next_links)
         self._check(lib().pimc_energy(self.h, _p(nx, _ip), _p(o, _dp)), "energy")
         return dict(zip(["K", "V", "V_ext", "V_int", "E", "K/N", "V/N", "E/N"], o))
+
+    def virial_terms(self, window=5, next_links=None):
+        """[M][6]: rDOTgradUterm1, rDOTgradUterm2, deltadotgradUterm1,
+        deltadotgradUterm2, virialKinCorrection, secondderivPotentialActionTau
+        (src/action.cpp:1446-1786, :776-787)."""
+        o = np.zeros((self.M, 6))
+        nx = None if next_links is None else _i32(next_links)
+        self._check(lib().pimc_virial_terms(self.h, _p(nx, _ip), window, _p(o, _dp)), "virial_terms")
+        return o
+
+    def virial_energy(self, window=5, mu=0.0, next_links=None):
+        """VirialEnergyEstimator::accumulate, one call (src/estimator.cpp:1086-1251)."""
+        o = np.zeros(19)
+        nx = None if next_links is None else _i32(next_links)
+        self._check(lib().pimc_virial_energy(self.h, _p(nx, _ip), window, mu, _p(o, _dp)), "virial_energy")
+        return dict(zip(VIRIAL_COLUMNS, o))
 
     def action_list(self, slice_ptcl, shift=1, vfactor=None, per_bead=False):
         sp = _i32(slice_ptcl, (-1, 2))

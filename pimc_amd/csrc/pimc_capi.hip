@@ -163,6 +163,7 @@ AzizDev make_aziz_dev(const AzizParams& p, double max_sep) {
     a.inv_rm = 1.0 / p.rm;
     a.table = nullptr;
     a.dtable = nullptr;
+    a.d2table = nullptr;
     return a;
 }
 
@@ -235,7 +236,8 @@ const double* cached_table(int year, int what, int length, std::string* err) {
 // ===========================================================================
 // timing
 // ===========================================================================
-static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf", "isf", "gradv2", "gradvnn"};
+static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf", "isf", "gradv2", "gradvnn",
+                                     "virial"};
 
 constexpr int kTimingSlots = 2048;
 
@@ -746,6 +748,204 @@ int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* a
         if (v_slices)
             for (int m = 0; m < 2 * M; ++m) v_slices[m] = 0.0;
     }
+    return PIMC_OK;
+}
+
+// Per-slice virial sums (pimc_virial.hip), summed over each slice's workgroups
+// in order: S [M][kVirialTerms].
+static int run_virial(pimc_ctx* c, const int32_t* next_links, int window, std::vector<double>& S) {
+    const int M = c->desc.num_slices, N = c->desc.num_alloc;
+    if (window < 1) return fail(c, PIMC_E_ARG, "virial window must be >= 1");
+    int r;
+    AzizDev az = ctx_aziz(c, &r);
+    if (r) return r;
+    if (c->desc.v_mode == PIMC_V_TABLE) {
+        // gradV and grad2V gather lookupdVdr / lookupd2Vdr2 (include/potential.h:996-1017)
+        std::string err;
+        az.dtable = cached_table(c->desc.aziz_year, 1, az.table_length, &err);
+        if (az.dtable) az.d2table = cached_table(c->desc.aziz_year, 2, az.table_length, &err);
+        if (!az.dtable || !az.d2table) return fail(c, PIMC_E_NOMEM, err);
+    }
+    // links: next as given, prev as its inverse; every active bead must link to
+    // an active bead (closed worldlines, the estimator's diagonal sector)
+    std::vector<int32_t> prev;
+    if (next_links) {
+        prev.assign(2 * (size_t)M * N, -1);
+        for (int m = 0; m < M; ++m)
+            for (int i = 0; i < c->h_nbs[m]; ++i) {
+                const size_t t = (size_t)m * N + i;
+                const int a = next_links[2 * t], b = next_links[2 * t + 1];
+                if (a < 0 || a >= M || b < 0 || b >= c->h_nbs[a])
+                    return fail(c, PIMC_E_ARG, "virial estimator: a next link leaves the closed worldlines");
+                const size_t u = (size_t)a * N + b;
+                if (prev[2 * u] != -1) return fail(c, PIMC_E_ARG, "virial estimator: next links are not a permutation");
+                prev[2 * u] = m;
+                prev[2 * u + 1] = i;
+            }
+    }
+    int* d_links = nullptr;
+    if (next_links) {
+        if ((r = ctx_dalloc(c, &d_links, 4 * (size_t)M * N))) return r;
+        HIP_OR_FAIL(c, hipMemcpyAsync(d_links, next_links, sizeof(int) * 2 * (size_t)M * N, hipMemcpyHostToDevice,
+                                      c->stream));
+        HIP_OR_FAIL(c, hipMemcpyAsync(d_links + 2 * (size_t)M * N, prev.data(), sizeof(int) * 2 * (size_t)M * N,
+                                      hipMemcpyHostToDevice, c->stream));
+    }
+    const int B = (N + 255) / 256;
+    const size_t need = (size_t)M * B * kVirialTerms;
+    if (c->cap_virial < need) {
+        if ((r = ctx_dalloc(c, &c->d_virial, need))) return r;
+        c->cap_virial = need;
+    }
+    VirialArgs A{};
+    A.pos = c->d_pos;
+    A.plane = c->plane;
+    A.nalloc = N;
+    A.num_slices = M;
+    A.nbs = c->d_nbs;
+    A.next = d_links;
+    A.prev = d_links ? d_links + 2 * (size_t)M * N : nullptr;
+    A.window = window;
+    A.t_even = c->gvf[0] > kCorrEps;
+    A.t_odd = c->gvf[1] > kCorrEps;
+    A.lds_beads = N <= kLdsBeads ? N : 0;
+    A.box = c->box;
+    A.part = c->d_virial;
+    int blocks = 0;
+    c->timed_begin(7);
+    HIP_OR_FAIL(c, launch_virial(c->desc.v_mode, c->stream, A, az, &blocks));
+    c->timed_end();
+    std::vector<double> part(need);
+    HIP_OR_FAIL(c, hipMemcpyAsync(part.data(), c->d_virial, sizeof(double) * need, hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    if (d_links) {
+        (void)hipFree(d_links);
+        c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)d_links),
+                            c->dev_allocs.end());
+    }
+    S.assign((size_t)M * kVirialTerms, 0.0);
+    for (int m = 0; m < M; ++m)
+        for (int b = 0; b < blocks; ++b)
+            for (int q = 0; q < kVirialTerms; ++q)
+                S[(size_t)m * kVirialTerms + q] += part[((size_t)m * blocks + b) * kVirialTerms + q];
+    return PIMC_OK;
+}
+
+// The LocalAction virial virtuals per slice from the slice sums S.
+static void virial_slice_terms(const pimc_ctx* c, const std::vector<double>& S, double* out6) {
+    const double tau = c->desc.tau, lambda = c->desc.lambda;
+    for (int m = 0; m < c->desc.num_slices; ++m) {
+        const double* s = &S[(size_t)m * kVirialTerms];
+        const int eo = m % 2;
+        const double gvf = c->gvf[eo];
+        const bool cor = gvf > kCorrEps;
+        const double t2 = 2.0 * gvf * (tau * tau * tau) * lambda;
+        double* o = out6 + 6 * (size_t)m;
+        o[0] = c->vf_default[eo] * tau * s[0];                          // rDOTgradUterm1 (:1446-1482)
+        o[1] = cor ? s[1] * t2 : 0.0;                                   // rDOTgradUterm2 (:1494-1574)
+        o[2] = c->vf_default[eo] * tau * s[2];                          // deltadotgradUterm1 (:1587-1653)
+        o[3] = cor ? s[3] * t2 : 0.0;                                   // deltadotgradUterm2 (:1665-1762)
+        o[4] = cor ? s[4] * (gvf * (tau * tau * tau) * lambda) : 0.0;   // virialKinCorrection (:1769-1786)
+        o[5] = cor ? 6.0 * gvf * tau * lambda * s[4] : 0.0;             // secondderivPotentialActionTau (:776-787)
+    }
+}
+
+int pimc_virial_terms(pimc_ctx* c, const int32_t* next_links, int window, double* out6) {
+    if (!c || !out6) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    std::vector<double> S;
+    int r = run_virial(c, next_links, window, S);
+    if (r) return r;
+    virial_slice_terms(c, S, out6);
+    return PIMC_OK;
+}
+
+int pimc_virial_energy(pimc_ctx* c, const int32_t* next_links, int window, double mu, double* out19) {
+    if (!c || !out19) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    const int M = c->desc.num_slices;
+    std::vector<double> vs(2 * (size_t)M), terms(6 * (size_t)M), S;
+    int r = pimc_potential_slices(c, vs.data(), nullptr);
+    if (r) return r;
+    if ((r = run_virial(c, next_links, window, S))) return r;
+    virial_slice_terms(c, S, terms.data());
+    // VirialEnergyEstimator::accumulate (src/estimator.cpp:1086-1251), one call
+    long on = 0;
+    for (auto v : c->h_on) on += v;
+    const int Np = (int)(on / M);
+    const double tau = c->desc.tau, lambda = c->desc.lambda;
+    const double volume = c->desc.side * c->desc.side * c->desc.side;
+    const double beta = 1.0 * M * tau;
+    const double tailV = (1.0 * Np * Np / volume) * c->tailV;
+    const double thermTerm1 = (0.5 * 3 / tau) * Np;
+    const double T1 = 0.5 * 3 * Np / (1.0 * window * tau);
+    const double exchangeNorm = 1.0 / (4.0 * window * (tau * tau) * lambda * M);
+    double T2 = 0.0, thermE = 0.0, T3 = 0.0, T4 = 0.0, T5 = 0.0, virKin = 0.0, totVop = 0.0, P3 = 0.0, d2U = 0.0;
+    for (int m = 0; m < M; ++m) {
+        const double* s = &S[(size_t)m * kVirialTerms];
+        T2 += s[5];
+        thermE += s[6];
+        const double* o = &terms[6 * (size_t)m];
+        const int eo = m % 2;
+        T3 += o[2];
+        T4 += o[3];
+        // derivPotentialActionTau (src/action.cpp:751-765)
+        double dU = c->vf_default[eo] * (vs[2 * m] + vs[2 * m + 1]);
+        if (c->gvf[eo] > kCorrEps) dU += 3.0 * c->gvf[eo] * tau * tau * lambda * s[4];
+        T5 += dU;
+        virKin += o[4];
+        if (eo == 0) totVop += vs[2 * m] + vs[2 * m + 1];
+        P3 += o[0] + o[1];
+        d2U += o[5] / (1.0 * M);
+    }
+    double P2 = thermE;
+    T2 *= exchangeNorm;
+    P2 /= (2.0 * lambda * tau * M);
+    double Pressure = 3.0 * Np + P2;
+    P3 *= (1.0 / (2.0 * M));
+    Pressure -= P3;
+    Pressure /= (3.0 * tau * volume);
+    totVop /= (0.5 * M);
+    totVop += tailV;
+    T3 /= (2.0 * beta);
+    T4 /= (1.0 * beta);
+    T5 /= (1.0 * M);
+    virKin /= (0.5 * beta);
+    thermE *= (1.0 / (4.0 * lambda * tau)) / (tau * M);
+    thermE += thermTerm1;
+    thermE += T5;
+    thermE += tailV;
+    const double totEcv = T1 + T2 + T3 + T4 + T5 + tailV;
+    const double Kcv = T1 + T2 + T3 + T4 + virKin;
+    double dEdB = (-1.0 * T1 - 2.0 * T2 + 2.0 * T4) / tau;
+    dEdB += d2U;
+    dEdB *= beta * beta / (1.0 * M);
+    for (int k = 0; k < 19; ++k) out19[k] = 0.0;
+    out19[0] = totEcv - totVop;
+    out19[1] = Kcv;
+    out19[2] = totVop;
+    out19[3] = totEcv - Kcv;
+    out19[4] = totEcv;
+    out19[5] = totEcv - mu * Np;
+    if (Np > 0) {
+        out19[6] = (totEcv - totVop) / (1.0 * Np);
+        out19[7] = Kcv / (1.0 * Np);
+        out19[8] = totVop / (1.0 * Np);
+        out19[9] = (totEcv - Kcv) / (1.0 * Np);
+        out19[10] = totEcv / (1.0 * Np);
+    }
+    out19[11] = totEcv * thermE * beta * beta;
+    out19[12] = totEcv * beta;
+    out19[13] = dEdB;
+    out19[14] = totEcv * thermE * beta * beta * totEcv * beta;
+    // the reference adds this term under the key "cVCov2", which its constructor
+    // never registered: std::map inserts it at index 0, K_op (src/estimator.cpp:1244)
+    out19[0] += totEcv * beta * dEdB;
+    out19[15] = 0.0;
+    out19[16] = totEcv * thermE * beta * beta * dEdB;
+    out19[17] = thermE;
+    out19[18] = Pressure;
     return PIMC_OK;
 }
 

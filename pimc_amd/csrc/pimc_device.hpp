@@ -40,6 +40,7 @@ struct AzizDev {
     double inv_rm;         // 1 / rm
     const double* table;   // device lookupV (PIMC_V_TABLE)
     const double* dtable;  // device lookupdVdr (PIMC_V_TABLE, gradient terms)
+    const double* d2table; // device lookupd2Vdr2 (PIMC_V_TABLE, virial T-matrix)
 };
 
 struct BoxDev {
@@ -160,6 +161,34 @@ __device__ __forceinline__ double aziz_dvdr64(int k, const AzizDev& a) {
     const double T2 = (6.0 * a.C6 * ix7 + 8.0 * a.C8 * ix9 + 10.0 * a.C10 * ix11) * F;
     const double T3 = -(a.C6 * ix6 + a.C8 * ix8 + a.C10 * ix10) * dF;
     return a.eps_rm * (T1 + T2 + T3);
+}
+
+// valued2Vdr2 at the bin radius (src/potential.cpp:1880-1909): FP64 closed
+// form, including the reference's epsilon/rm (not epsilon/rm^2) factor of the
+// hard-core branch.  Requires k_eps <= k < table_length.
+__device__ __forceinline__ double aziz_d2vdr2_64(int k, const AzizDev& a) {
+    const double x = (double)k * a.s;
+    const double ab = (a.alpha - 2.0 * a.beta * x) * (a.alpha - 2.0 * a.beta * x);
+    const double T1 = a.A * (2.0 * a.beta + ab) * exp(-a.alpha * x + a.beta * x * x);
+    if (k < a.k_hc) return a.eps_rm * T1;
+    const double ix = 1.0 / x;
+    const double ix2 = ix * ix;
+    const double ix6 = ix2 * ix2 * ix2;
+    const double ix7 = ix6 * ix, ix8 = ix6 * ix2, ix9 = ix8 * ix, ix10 = ix8 * ix2, ix11 = ix10 * ix;
+    const double ix12 = ix11 * ix;
+    double F = 1.0, dF = 0.0, d2F = 0.0;
+    if (k < a.k_d) {
+        const double t = a.D * ix - 1.0;
+        const double e = exp(-t * t);
+        F = e;
+        dF = 2.0 * a.D * ix2 * t * e;
+        const double D = a.D;
+        d2F = 2.0 * D * ix2 * ix * (2.0 * D * D * D * ix2 * ix - 4.0 * D * D * ix2 - D * ix + 2.0) * e;
+    }
+    const double T2 = -(42.0 * a.C6 * ix8 + 72.0 * a.C8 * ix10 + 110.0 * a.C10 * ix12) * F;
+    const double T3 = 2.0 * (6.0 * a.C6 * ix7 + 8.0 * a.C8 * ix9 + 10.0 * a.C10 * ix11) * dF;
+    const double T4 = -(a.C6 * ix6 + a.C8 * ix8 + a.C10 * ix10) * d2F;
+    return a.eps_rm * a.inv_rm * (T1 + T2 + T3 + T4);
 }
 
 // valuedVdr at the bin radius with the FP32 transcendental units (split

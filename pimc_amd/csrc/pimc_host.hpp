@@ -92,6 +92,8 @@ struct pimc_ctx {
     double* h_req_pos = nullptr;
     double* d_per_bead = nullptr;
     double* h_per_bead = nullptr;
+    double* d_virial = nullptr;         // virial per-workgroup sums [M][blocks][7]
+    size_t cap_virial = 0;
     double* d_per_bead_cor = nullptr;   // per-bead corrections [cap_req][2]
     double* h_per_bead_cor = nullptr;
     double* d_vnn_partials = nullptr;
@@ -124,9 +126,9 @@ struct pimc_ctx {
     std::vector<hipEvent_t> ev_pool;   // 2 per slot
     std::vector<Pending> pending;
     int ev_next = 0;
-    static constexpr int kKinds = 7;   // pair_action, delta_action, rho, ssf, isf, gradv2, gradvnn
-    double kernel_ms[kKinds] = {0, 0, 0, 0, 0, 0, 0};
-    long kernel_count[kKinds] = {0, 0, 0, 0, 0, 0, 0};
+    static constexpr int kKinds = 8;   // pair_action, delta_action, rho, ssf, isf, gradv2, gradvnn, virial
+    double kernel_ms[kKinds] = {0, 0, 0, 0, 0, 0, 0, 0};
+    long kernel_count[kKinds] = {0, 0, 0, 0, 0, 0, 0, 0};
     int timed_slot(int kind);          // -1 when timing is off
     void timed_begin(int kind);
     void timed_end();

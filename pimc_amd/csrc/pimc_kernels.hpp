@@ -69,6 +69,23 @@ void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int
 // Pair evaluations of a tile (masked d = n/2 lanes included: the balancing weight).
 inline int64_t row_tile_pairs(const RowTile& t) { return (int64_t)(t.i_end - t.i0) * t.dn; }
 
+// Virial pair terms of a closed path (pimc_virial.hip): per-bead terms
+// F.r, (T F).r, F.delta, (T F).delta, |F|^2, -vel1.vel2, -vel2.vel2.
+constexpr int kVirialTerms = 7;
+struct VirialArgs {
+    const double* pos;
+    int64_t plane;
+    int nalloc, num_slices;
+    const int* nbs;      // numBeadsAtSlice [M]
+    const int* next;     // next links [M][nalloc][2], or null: (m+1 mod M, i)
+    const int* prev;     // prev links [M][nalloc][2], or null: (m-1 mod M, i)
+    int window;          // virialWindow >= 1
+    int t_even, t_odd;   // evaluate the T-matrix on even / odd slices (finite gradVFactor)
+    int lds_beads;       // slices with at most this many beads are staged in LDS
+    BoxDev box;
+    double* part;        // [M][blocks][kVirialTerms]
+};
+
 // A Vnn evaluation request for one bead: partners are the active beads of its slice.
 struct VnnBead {
     int walker;
@@ -176,6 +193,7 @@ hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInlin
 hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,
                          int* blocks_per_slice);
+hipError_t launch_virial(int mode, hipStream_t st, const VirialArgs& A, const AzizDev& az, int* blocks_per_slice);
 hipError_t launch_update_beads(hipStream_t st, const int* sp, const double* p3, int n, double* pos, int64_t plane,
                                int nalloc, int num_slices, int* cell, int* label, int* numlab, int* hash,
                                const BoxDev& box);

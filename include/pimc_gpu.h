@@ -295,6 +295,56 @@ int  pimc_chain_move(pimc_chain* ch, int kind, int* accepted, double* info3);
 /* The chain's positions, [M][N][3] (equal to the device copy). */
 int  pimc_chain_positions(const pimc_chain* ch, double* beads);
 
+/* ---- state files ------------------------------------------------------------- */
+/* The reference's text state file (PathIntegralMonteCarlo::saveState, src/pimc.cpp:
+ * 926-986, written with the stream operators of include/common.h:244-272):
+ * the worldline count, the acceptance and estimator counter lines ("%16d\t%16d"),
+ * then beads (16 significant digits), nextLink, prevLink and worm.beads as
+ * "(0,M-1) x (0,N-1)" arrays, then the MTRand save words.  Host only.  Links are
+ * {slice, ptcl} with {-1,-1} for XXX.  The HDF5 variant (useHDF5State) is not
+ * read: no HDF5 library is present.  Errors: pimc_state_last_error(). */
+enum { PIMC_MT_SAVE = 625 };   /* MTRand::SAVE = N + 1 words (include/MersenneTwister.h:80) */
+typedef struct pimc_state pimc_state;
+const char* pimc_state_last_error(void);
+/* Parse a state file (the stream extractors of include/common.h:276-397 with
+ * loadState's skip to the first '(' line, src/pimc.cpp:1131-1144). */
+int  pimc_state_read(const char* path, pimc_state** out);
+int  pimc_state_parse(const char* text, size_t len, pimc_state** out);
+/* An empty state of M x N beads (links XXX, worm 0, no counters, no random words). */
+int  pimc_state_create(int num_slices, int num_alloc, pimc_state** out);
+void pimc_state_destroy(pimc_state* st);
+/* Array extents (slices, columns) and the first line's worldline count. */
+int  pimc_state_dims(const pimc_state* st, int* num_slices, int* num_alloc, int* num_worldlines);
+/* Copy the arrays out / in: beads [M][N][3], next / prev [M][N][2], worm [M][N];
+ * any pointer may be NULL; pimc_state_set rejects links outside the arrays. */
+int  pimc_state_get(const pimc_state* st, double* beads, int32_t* next, int32_t* prev, uint32_t* worm);
+int  pimc_state_set(pimc_state* st, const double* beads, const int32_t* next, const int32_t* prev,
+                    const uint32_t* worm);
+/* Counter lines as {a, b} pairs in file order (the totals first): *count pairs
+ * exist, up to cap are copied to pairs; set_pairs (set_count pairs) replaces them. */
+int  pimc_state_counters(pimc_state* st, int64_t* pairs, int cap, int* count, const int64_t* set_pairs,
+                         int set_count);
+/* MTRand::save words [PIMC_MT_SAVE] (uint32 is unsigned long in the reference):
+ * *have = 0 when the file carried none; set_words replaces them. */
+int  pimc_state_random(pimc_state* st, uint64_t* words, int* have, const uint64_t* set_words);
+/* Path::leftPack (src/path.cpp:145-189): active beads moved to the left of each
+ * slice, their links and their neighbours' links rewritten.  saveState calls it
+ * before writing.  A worm end's missing neighbour (link XXX) is left alone. */
+int  pimc_state_left_pack(pimc_state* st);
+/* The file text, byte for byte as saveState writes it; *len excludes the NUL. */
+int  pimc_state_format(const pimc_state* st, char* buf, size_t cap, size_t* len);
+int  pimc_state_write(const pimc_state* st, const char* path);
+/* loadState's path set-up (src/pimc.cpp:1150-1253) for a run of num_slices
+ * slices in a cube of side `side`: the file's arrays when it has num_slices
+ * slices, else the classical path from slice 0 of every active worldline
+ * (loadClassicalState, :1006-1027; periodic links, inactive beads unlinked);
+ * then every bead put inside the box and the active beads counted per slice.
+ * Outputs have num_worldlines columns: beads [M][NW][3], bead_on [M][NW],
+ * num_beads [M], next / prev [M][NW][2] -- ready for pimc_upload_path,
+ * pimc_energy and pimc_virial_*.  The worm is reset (no head or tail). */
+int  pimc_state_to_path(const pimc_state* st, int num_slices, double side, double* beads, uint8_t* bead_on,
+                        int32_t* num_beads, int32_t* next, int32_t* prev);
+
 /* ---- seeded synthetic worldlines -------------------------------------------- */
 /* MTRand(seed) stream (include/MersenneTwister.h); slice 0 Prism::randPosition
  * (src/container.cpp:155-160); kind 0: randNorm(prev, sqrt(2 lambda tau)) + putInBC

@@ -1323,6 +1323,46 @@ void orc_bare_action_list(void* p, const int32_t* b, int n, double* out) {
 double orc_action_correction(void* p, int slice, int ptcl) {
     return static_cast<State*>(p)->actionCorrection(Bead{slice, ptcl});
 }
+void orc_left_pack(int M, int N, double* beads, int32_t* next, int32_t* prev, uint32_t* worm) {
+    // Path::leftPack (path.cpp:145-189) on raw [M][N] arrays; links {slice, ptcl},
+    // XXX = {-1,-1}
+    auto at = [N](int m, int i) { return size_t(m) * N + i; };
+    for (int m = 0; m < M; m++) {
+        for (int i = 0; i < N; i++) {
+            if (worm[at(m, i)]) continue;
+            bool found = false;
+            int j;
+            for (j = i + 1; j < N; j++)
+                if (worm[at(m, j)]) {
+                    found = true;
+                    break;
+                }
+            if (!found) continue;
+            size_t b1 = at(m, i), b2 = at(m, j);
+            for (int d = 0; d < 3; d++) beads[3 * b1 + d] = beads[3 * b2 + d];
+            worm[b1] = 1;   // worm.addBead(bead1)
+            worm[b2] = 0;   // worm.delBead(bead2)
+            next[2 * b1] = next[2 * b2];
+            next[2 * b1 + 1] = next[2 * b2 + 1];
+            prev[2 * b1] = prev[2 * b2];
+            prev[2 * b1 + 1] = prev[2 * b2 + 1];
+            // next(prev(bead2)) = bead1; prev(next(bead2)) = bead1 (skipped where
+            // the link is XXX: the reference would index with XXX there)
+            int pm = prev[2 * b2], pp = prev[2 * b2 + 1];
+            if (!(pm == -1 && pp == -1)) {
+                next[2 * at(pm, pp)] = m;
+                next[2 * at(pm, pp) + 1] = i;
+            }
+            int nm = next[2 * b2], np_ = next[2 * b2 + 1];
+            if (!(nm == -1 && np_ == -1)) {
+                prev[2 * at(nm, np_)] = m;
+                prev[2 * at(nm, np_) + 1] = i;
+            }
+            next[2 * b2] = next[2 * b2 + 1] = -1;
+            prev[2 * b2] = prev[2 * b2 + 1] = -1;
+        }
+    }
+}
 void orc_virial_terms(void* p, int W, double* out) {
     State* st = static_cast<State*>(p);
     if (st->aziz.tdV.empty()) st->aziz.init(st->aziz.year, st->box.maxSep, st->rc, true);

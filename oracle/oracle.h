@@ -86,6 +86,8 @@ double orc_action_correction(void* st, int slice, int ptcl); /* potentialActionC
 /* Virial pair terms per slice (action.cpp:1446-1786, :776-787), out [M][6] =
  * {rDOTgradUterm1, rDOTgradUterm2, deltadotgradUterm1, deltadotgradUterm2,
  *  virialKinCorrection, secondderivPotentialActionTau}; W = virial window. */
+/* Path::leftPack (path.cpp:145-189) on raw arrays beads [M][N][3], next/prev [M][N][2], worm [M][N]. */
+void   orc_left_pack(int M, int N, double* beads, int32_t* next, int32_t* prev, uint32_t* worm);
 void   orc_virial_terms(void* st, int W, double* out);
 /* VirialEnergyEstimator::accumulate (estimator.cpp:1086-1251): the 19 columns of one call. */
 void   orc_virial_energy(void* st, int W, double mu, double* out19);

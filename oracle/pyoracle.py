@@ -19,6 +19,17 @@ up = C.POINTER(C.c_uint32)
 bp = C.POINTER(C.c_uint8)
 
 
+def left_pack(beads, nxt, prv, worm):
+    """Path::leftPack (src/path.cpp:145-189) on copies of the arrays."""
+    b = np.array(beads, dtype=np.float64, order="C")
+    n = np.array(nxt, dtype=np.int32, order="C")
+    p = np.array(prv, dtype=np.int32, order="C")
+    w = np.array(worm, dtype=np.uint32, order="C")
+    M, N = w.shape
+    lib().orc_left_pack(M, N, _p(b, dp), _p(n, ip), _p(p, ip), w.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return b, n, p, w
+
+
 VIRIAL_COLUMNS = ("K_op", "K_cv", "V_op", "V_cv", "E", "E_mu", "K_op/N", "K_cv/N", "V_op/N", "V_cv/N", "E/N",
                   "EEcv*Beta^2", "Ecv*Beta", "dEdB", "CvCov1", "CvCov2", "CvCov3", "E_th", "P")
 
@@ -80,6 +91,7 @@ def lib():
             "orc_action_correction": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
             "orc_energy": (None, [C.c_void_p, dp]),
             "orc_virial_terms": (None, [C.c_void_p, C.c_int, dp]),
+            "orc_left_pack": (None, [C.c_int, C.c_int, dp, ip, ip, C.POINTER(C.c_uint32)]),
             "orc_virial_energy": (None, [C.c_void_p, C.c_int, C.c_double, dp]),
             "orc_ssf": (None, [C.c_void_p, dp, C.c_int, dp]),
             "orc_isf": (None, [C.c_void_p, dp, C.c_int, dp]),

@@ -121,6 +121,21 @@ SIGNATURES = {
     "pimc_chain_last_error": (C.c_char_p, [C.c_void_p]),
     "pimc_chain_move": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]),
     "pimc_chain_positions": (C.c_int, [C.c_void_p, _dp]),
+    "pimc_state_last_error": (C.c_char_p, []),
+    "pimc_state_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "pimc_state_parse": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "pimc_state_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "pimc_state_destroy": (None, [C.c_void_p]),
+    "pimc_state_dims": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    "pimc_state_get": (C.c_int, [C.c_void_p, _dp, _ip, _ip, C.POINTER(C.c_uint32)]),
+    "pimc_state_set": (C.c_int, [C.c_void_p, _dp, _ip, _ip, C.POINTER(C.c_uint32)]),
+    "pimc_state_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int, _ip, C.POINTER(C.c_int64),
+                                      C.c_int]),
+    "pimc_state_random": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), _ip, C.POINTER(C.c_uint64)]),
+    "pimc_state_left_pack": (C.c_int, [C.c_void_p]),
+    "pimc_state_format": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "pimc_state_write": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "pimc_state_to_path": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _dp, _bp, _ip, _ip, _ip]),
 }
 
 _lib = None
@@ -193,6 +208,17 @@ class Context:
         on = None if bead_on is None else np.ascontiguousarray(bead_on, dtype=np.uint8)
         nb = None if num_beads is None else _i32(num_beads)
         self._check(lib().pimc_upload_path(self.h, _p(b, _dp), _p(on, _bp), _p(nb, _ip)), "upload_path")
+
+    def load_state(self, state):
+        """loadState into this context (src/pimc.cpp:1108-1258): the path from a
+        StateFile, uploaded with its active-bead mask and counts, the worm reset.
+        Returns the path dict (next / prev links for energy and virial calls)."""
+        p = state.to_path(self.M, self.side)
+        if p["beads"].shape[1] != self.N:
+            raise PimcError(f"load_state: the state has {p['beads'].shape[1]} worldlines, the context {self.N}")
+        self.upload_path(p["beads"], p["bead_on"], p["num_beads"])
+        self.set_worm()
+        return p
 
     def set_worm(self, head=(-1, -1), tail=(-1, -1), special1=(-1, -1), special2=(-1, -1)):
         arrs = [_i32(x) for x in (head, tail, special1, special2)]
@@ -561,6 +587,119 @@ class Chain:
         if rc != 0:
             raise PimcError(f"chain positions ({rc})")
         return b
+
+
+class StateFile:
+    """The reference's text state file (include/pimc_gpu.h, pimc_state):
+    PathIntegralMonteCarlo::saveState / loadState, src/pimc.cpp:926-986,
+    :1108-1258.  Host code in the library."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @staticmethod
+    def _make(rc, h, what):
+        if rc != 0:
+            raise PimcError(f"{what}: {lib().pimc_state_last_error().decode()} ({rc})")
+        return StateFile(h)
+
+    @classmethod
+    def read(cls, path):
+        h = C.c_void_p()
+        return cls._make(lib().pimc_state_read(os.fsencode(path), C.byref(h)), h, "state read")
+
+    @classmethod
+    def parse(cls, text):
+        b = text.encode() if isinstance(text, str) else bytes(text)
+        h = C.c_void_p()
+        return cls._make(lib().pimc_state_parse(b, len(b), C.byref(h)), h, "state parse")
+
+    @classmethod
+    def create(cls, num_slices, num_alloc):
+        h = C.c_void_p()
+        return cls._make(lib().pimc_state_create(num_slices, num_alloc, C.byref(h)), h, "state create")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pimc_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise PimcError(f"{what}: {lib().pimc_state_last_error().decode()} ({rc})")
+
+    def dims(self):
+        m, n, w = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().pimc_state_dims(self.h, C.byref(m), C.byref(n), C.byref(w)), "state dims")
+        return m.value, n.value, w.value
+
+    def arrays(self):
+        M, N, _ = self.dims()
+        b, nx, pv = np.zeros((M, N, 3)), np.zeros((M, N, 2), np.int32), np.zeros((M, N, 2), np.int32)
+        w = np.zeros((M, N), np.uint32)
+        self._check(lib().pimc_state_get(self.h, _p(b, _dp), _p(nx, _ip), _p(pv, _ip),
+                                         w.ctypes.data_as(C.POINTER(C.c_uint32))), "state get")
+        return {"beads": b, "next": nx, "prev": pv, "worm": w}
+
+    def set(self, beads=None, next=None, prev=None, worm=None):
+        M, N, _ = self.dims()
+        b = None if beads is None else _f64(beads, (M, N, 3))
+        nx = None if next is None else _i32(next, (M, N, 2))
+        pv = None if prev is None else _i32(prev, (M, N, 2))
+        w = None if worm is None else np.ascontiguousarray(worm, dtype=np.uint32).reshape(M, N)
+        self._check(lib().pimc_state_set(self.h, _p(b, _dp), _p(nx, _ip), _p(pv, _ip),
+                                         None if w is None else w.ctypes.data_as(C.POINTER(C.c_uint32))),
+                    "state set")
+
+    def counters(self):
+        n = C.c_int()
+        self._check(lib().pimc_state_counters(self.h, None, 0, C.byref(n), None, -1), "state counters")
+        a = np.zeros((n.value, 2), np.int64)
+        self._check(lib().pimc_state_counters(self.h, a.ctypes.data_as(C.POINTER(C.c_int64)), n.value, None,
+                                              None, -1), "state counters")
+        return a
+
+    def set_counters(self, pairs):
+        a = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+        self._check(lib().pimc_state_counters(self.h, None, 0, None, a.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              a.shape[0]), "state counters")
+
+    def random(self):
+        w, have = np.zeros(625, np.uint64), C.c_int()
+        self._check(lib().pimc_state_random(self.h, w.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(have),
+                                            None), "state random")
+        return w if have.value else None
+
+    def set_random(self, words):
+        w = np.ascontiguousarray(words, dtype=np.uint64).reshape(625)
+        self._check(lib().pimc_state_random(self.h, None, None, w.ctypes.data_as(C.POINTER(C.c_uint64))),
+                    "state random")
+
+    def left_pack(self):
+        self._check(lib().pimc_state_left_pack(self.h), "leftPack")
+
+    def text(self):
+        n = C.c_size_t()
+        self._check(lib().pimc_state_format(self.h, None, 0, C.byref(n)), "state format")
+        buf = C.create_string_buffer(n.value + 1)
+        self._check(lib().pimc_state_format(self.h, buf, n.value + 1, None), "state format")
+        return buf.raw[:n.value].decode()
+
+    def write(self, path):
+        self._check(lib().pimc_state_write(self.h, os.fsencode(path)), "state write")
+
+    def to_path(self, num_slices, side):
+        """loadState's set-up for a run of num_slices slices in a cube of side `side`."""
+        _, _, W = self.dims()
+        M = num_slices
+        b, on = np.zeros((M, W, 3)), np.zeros((M, W), np.uint8)
+        nb, nx, pv = np.zeros(M, np.int32), np.zeros((M, W, 2), np.int32), np.zeros((M, W, 2), np.int32)
+        self._check(lib().pimc_state_to_path(self.h, M, side, _p(b, _dp), _p(on, _bp), _p(nb, _ip), _p(nx, _ip),
+                                             _p(pv, _ip)), "state to path")
+        return {"beads": b, "bead_on": on, "num_beads": nb, "next": nx, "prev": pv}
 
 
 def gen_worldlines(seed, side, M, N, lam, tau, kind=0):

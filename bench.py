@@ -40,8 +40,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--isf-reps", type=int, default=5)
-    ap.add_argument("--legs", default="plain,gsf,virial,batched,served",
-                    help="comma list of plain, gsf, virial, batched, served (plain always runs)")
+    ap.add_argument("--legs", default="plain,gsf,virial,chain,batched,served",
+                    help="comma list of plain, gsf, virial, chain, batched, served (plain always runs)")
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (HBM counter scaling)")
@@ -170,6 +170,8 @@ def main():
         line["gsf"] = gsf_leg(pimc_amd, beads, L, M, N, tau)
     if "virial" in legs:
         line["virial"] = virial_leg(pimc_amd, beads, L, M, N, tau, not args.no_cpu_baseline)
+    if "chain" in legs:
+        line["chain"] = chain_leg(pimc_amd, beads, L, M, N, tau, not args.no_cpu_baseline)
     if "batched" in legs:
         line["batched"] = batched_leg(pimc_amd, value)
     if "served" in legs:
@@ -273,6 +275,53 @@ def virial_leg(pimc_amd, beads, L, M, N, tau, cpu, window=5, reps=20):
         out["cpu_port"] = {"sample": f"{Ms} slices of N={N}, one accumulate", "seconds": t_cpu, "cores": 1,
                            "virial_pair_evals_per_s": Ms * N * (N - 1) / t_cpu,
                            "estimator_us_scaled_to_M": t_cpu * M / Ms * 1e6}
+    return out
+
+
+def chain_leg(pimc_amd, beads, L, M, N, tau, cpu, moves=3000):
+    """One Markov chain on the same walker: Staging (Mbar 16), Bisection (4
+    levels) and CenterOfMass (0.1 A) moves in rotation, primitive action.
+    Host-driven (proposals on the host, one device Delta V pass per move) against
+    device-resident (pimc_chain_run_device: the whole run in one launch)."""
+    from pimc_amd.workloads import LAMBDA
+    kinds = [k % 3 for k in range(moves)]
+    ctx = pimc_amd.Context(L, M, N, tau, LAMBDA, L, v_mode=pimc_amd.V_FP32)
+    ctx.upload_path(beads)
+    ch = pimc_amd.Chain(ctx, 4357, 16, 4, True, 1, 0.1)
+    host_moves = min(600, moves)
+    names = ("staging", "bisection", "com")
+    for k in range(30):
+        ch.move(names[k % 3])
+    pimc_amd.device_synchronize()
+    t0 = time.perf_counter()
+    for k in range(host_moves):
+        ch.move(names[k % 3])
+    t_host = time.perf_counter() - t0
+    ch.run_device(kinds[:60])
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    acc, info, near = ch.run_device(kinds)
+    t_dev = time.perf_counter() - t0
+    k_ms, k_n = ctx.kernel_time("chain_run")
+    ctx.set_timing(False)
+    ctx.close()
+    out = {"moves": moves, "mix": "staging / bisection / centre of mass in rotation",
+           "host_driven_moves_per_s": host_moves / t_host,
+           "device_resident_moves_per_s": moves / t_dev, "device_kernel_ms": k_ms / max(k_n, 1),
+           "device_us_per_move": t_dev / moves * 1e6, "accept_frac": float(np.mean(acc)),
+           "near_threshold_decisions": near}
+    if cpu:
+        from oracle import pyoracle as orc
+        st = orc.State(L, M, N, tau, LAMBDA, L)
+        st.set_beads(beads)
+        mv = orc.Moves(st, 16, 4, True, 1, 0.1)
+        rng = orc.MTRand(4357)
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 5.0:
+            getattr(mv, names[n % 3])(rng)
+            n += 1
+        out["cpu_port"] = {"moves_per_s": n / (time.perf_counter() - t0), "moves": n, "cores": 1,
+                           "sample": "the oracle's moves on the same walker for 5 s, single thread"}
     return out
 
 

@@ -294,6 +294,18 @@ const char* pimc_chain_last_error(const pimc_chain* ch);
 int  pimc_chain_move(pimc_chain* ch, int kind, int* accepted, double* info3);
 /* The chain's positions, [M][N][3] (equal to the device copy). */
 int  pimc_chain_positions(const pimc_chain* ch, double* beads);
+/* n_moves moves of kinds[k] (0 staging, 1 bisection, 2 centre of mass) in one
+ * device launch: the proposals, the Metropolis draws and the commits run on the
+ * device from the chain's MTRand state, which continues the host stream and is
+ * handed back afterwards (pimc_chain_move may follow).  accepted [n_moves] and
+ * info3 [n_moves][3] as pimc_chain_move (either may be NULL).  The device's
+ * log / sqrt / exp / pow may differ from the host's in the last ulp, so a
+ * proposal can too; *near_threshold (may be NULL) counts the decisions whose
+ * uniform deviate lay within a relative 1e-10 of the acceptance probability,
+ * the only ones such a difference could flip.  Requires M <= 512 and an action
+ * without the gradient correction (PIMC_E_UNSUPPORTED otherwise). */
+int  pimc_chain_run_device(pimc_chain* ch, int n_moves, const int* kinds, int* accepted, double* info3,
+                           int* near_threshold);
 
 /* ---- state files ------------------------------------------------------------- */
 /* The reference's text state file (PathIntegralMonteCarlo::saveState, src/pimc.cpp:
@@ -357,7 +369,7 @@ int pimc_gen_worldlines(uint32_t seed, double side, int M, int N, double lambda,
 /* When enabled, every launch of a hot kernel is bracketed by events (from a
  * pre-created pool) on the stream it runs on; pimc_kernel_time returns the summed
  * duration [ms] and the launch count for name in
- * {"pair_action","delta_action","rho","ssf","isf","gradv2","gradvnn"}.
+ * {"pair_action","delta_action","rho","ssf","isf","gradv2","gradvnn","virial","chain_run"}.
  * "pair_action" and "delta_action" are the pair kernels alone (k_pair_action,
  * k_vnn), without their reductions; "gradv2" and "gradvnn" the gradient
  * correction's slice and three-body kernels. */

@@ -121,6 +121,7 @@ SIGNATURES = {
     "pimc_chain_last_error": (C.c_char_p, [C.c_void_p]),
     "pimc_chain_move": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]),
     "pimc_chain_positions": (C.c_int, [C.c_void_p, _dp]),
+    "pimc_chain_run_device": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip]),
     "pimc_state_last_error": (C.c_char_p, []),
     "pimc_state_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "pimc_state_parse": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -580,6 +581,19 @@ class Chain:
         if rc != 0:
             raise PimcError(f"chain move ({rc}): {lib().pimc_chain_last_error(self.h).decode()}")
         return bool(self._acc.value), self._info.copy()
+
+    def run_device(self, kinds):
+        """The moves of `kinds` (names or 0/1/2) in one device launch.  Returns
+        (accepted [n], info [n][3], near_threshold)."""
+        k = np.array([self.KINDS[x] if isinstance(x, str) else int(x) for x in kinds], dtype=np.int32)
+        acc = np.zeros(k.shape[0], np.int32)
+        info = np.zeros((k.shape[0], 3))
+        near = C.c_int()
+        rc = lib().pimc_chain_run_device(self.h, k.shape[0], _p(k, _ip), _p(acc, _ip), _p(info, _dp),
+                                         C.byref(near))
+        if rc != 0:
+            raise PimcError(f"chain run_device ({rc}): {lib().pimc_chain_last_error(self.h).decode()}")
+        return acc.astype(bool), info, near.value
 
     def positions(self):
         b = np.zeros((self.ctx.M, self.ctx.N, 3))

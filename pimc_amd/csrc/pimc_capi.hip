@@ -237,7 +237,7 @@ const double* cached_table(int year, int what, int length, std::string* err) {
 // timing
 // ===========================================================================
 static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf", "isf", "gradv2", "gradvnn",
-                                     "virial"};
+                                     "virial", "chain_run"};
 
 constexpr int kTimingSlots = 2048;
 
@@ -458,6 +458,11 @@ static AzizDev ctx_aziz(pimc_ctx* c, int* rc) {
     }
     return a;
 }
+
+// The context's Aziz constants with its tables resolved (for the chain driver).
+namespace pimc {
+AzizDev context_aziz(pimc_ctx* c, int* rc) { return ctx_aziz(c, rc); }
+}  // namespace pimc
 
 static void fill_walker(pimc_ctx* c, WalkerDesc* w, int shift, const double vf[2]) {
     w->box = c->box;

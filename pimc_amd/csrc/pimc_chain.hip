@@ -42,6 +42,19 @@ struct pimc_chain {
     // scratch
     std::vector<int32_t> sp;
     std::vector<double> np, op;
+    // device-resident runs: MTRand state, winding table, per-move outputs
+    uint32_t* d_mt = nullptr;
+    int* d_wind = nullptr;
+    int* d_kinds = nullptr;
+    int* d_acc = nullptr;
+    double* d_info = nullptr;
+    int* d_near = nullptr;
+    int cap_moves = 0;
+    ~pimc_chain() {
+        for (void* p : {(void*)d_mt, (void*)d_wind, (void*)d_kinds, (void*)d_acc, (void*)d_info, (void*)d_near})
+            if (p) (void)hipFree(p);
+    }
+    int run_device(int n_moves, const int* kinds, int* accepted, double* info, int* near);
 
     double* P(int m, int i) { return pos.data() + 3 * ((size_t)m * N + i); }
     int fail(int code, const std::string& m) {
@@ -306,6 +319,104 @@ int pimc_chain::com(int* acc, double* info) {
     return device_set(beads, back);
 }
 
+// The moves of `kinds` inside one kernel (pimc_dchain.hip) on the chain's own
+// stream state: the host's MTRand state goes up before and comes back after,
+// so host moves and device runs continue one stream.
+int pimc_chain::run_device(int n_moves, const int* kinds, int* accepted, double* info, int* near) {
+    if (n_moves < 0 || (n_moves > 0 && !kinds)) return fail(PIMC_E_ARG, "bad move list");
+    for (int k = 0; k < n_moves; ++k)
+        if (kinds[k] < 0 || kinds[k] > 2) return fail(PIMC_E_ARG, "move kind must be 0, 1 or 2");
+    if (M > pimc::kChainMaxBeads) return fail(PIMC_E_UNSUPPORTED, "device runs hold at most 512 slices");
+    const int num_wind = (int)cumrho0.size();
+    if (num_wind > pimc::kChainMaxWind) return fail(PIMC_E_UNSUPPORTED, "device runs sample at most 343 windings");
+    if (ctx->correction())
+        return fail(PIMC_E_UNSUPPORTED, "device runs evaluate the action without the gradient correction");
+    if (n_moves == 0) {
+        if (near) *near = 0;
+        return PIMC_OK;
+    }
+    auto chk = [&](hipError_t e, const char* what) {
+        return e == hipSuccess ? PIMC_OK : fail(PIMC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    int r;
+    if (!d_mt) {
+        if ((r = chk(hipMalloc(&d_mt, sizeof(uint32_t) * (MtRand::N + 2)), "hipMalloc"))) return r;
+        if ((r = chk(hipMalloc(&d_wind, sizeof(int) * wind.size()), "hipMalloc"))) return r;
+        if ((r = chk(hipMalloc(&d_near, sizeof(int)), "hipMalloc"))) return r;
+        if ((r = chk(hipMemcpy(d_wind, wind.data(), sizeof(int) * wind.size(), hipMemcpyHostToDevice), "upload"))) return r;
+    }
+    if (cap_moves < n_moves) {
+        for (void* p : {(void*)d_kinds, (void*)d_acc, (void*)d_info})
+            if (p) (void)hipFree(p);
+        d_kinds = d_acc = nullptr;
+        d_info = nullptr;
+        if ((r = chk(hipMalloc(&d_kinds, sizeof(int) * n_moves), "hipMalloc"))) return r;
+        if ((r = chk(hipMalloc(&d_acc, sizeof(int) * n_moves), "hipMalloc"))) return r;
+        if ((r = chk(hipMalloc(&d_info, sizeof(double) * 3 * n_moves), "hipMalloc"))) return r;
+        cap_moves = n_moves;
+    }
+    uint32_t mt[MtRand::N + 2];
+    std::memcpy(mt, rng.s, sizeof(uint32_t) * MtRand::N);
+    mt[MtRand::N] = (uint32_t)rng.left;
+    mt[MtRand::N + 1] = (uint32_t)rng.next;
+    hipStream_t st = ctx->stream;
+    if ((r = chk(hipMemcpyAsync(d_mt, mt, sizeof(mt), hipMemcpyHostToDevice, st), "upload"))) return r;
+    if ((r = chk(hipMemcpyAsync(d_kinds, kinds, sizeof(int) * n_moves, hipMemcpyHostToDevice, st), "upload")))
+        return r;
+    pimc::AzizDev az = pimc::context_aziz(ctx, &r);
+    if (r) return fail(r, pimc_last_error(ctx));
+    pimc::ChainRunArgs A{};
+    A.pos = ctx->d_pos;
+    A.plane = ctx->plane;
+    A.nalloc = N;
+    A.num_slices = M;
+    A.cell = ctx->d_cell;
+    A.label = ctx->d_label;
+    A.numlab = ctx->d_numlab;
+    A.hash = ctx->d_hash;
+    A.box = ctx->box;
+    A.side = side;
+    A.tau = tau;
+    A.lambda = lambda;
+    A.vf0 = ctx->vf_default[0];
+    A.vf1 = ctx->vf_default[1];
+    A.sqrtLambdaTau = sqrtLambdaTau;
+    A.sqrt2LambdaTau = sqrt2LambdaTau;
+    A.com_delta = d.com_delta;
+    A.Mbar = d.Mbar;
+    A.levels = d.levels;
+    A.var_updates = d.var_updates;
+    A.num_wind = num_wind;
+    A.wind = d_wind;
+    A.mt = d_mt;
+    A.kinds = d_kinds;
+    A.n_moves = n_moves;
+    A.accepted = d_acc;
+    A.info = d_info;
+    A.near = d_near;
+    ctx->timed_begin(8);
+    if ((r = chk(pimc::launch_chain_run(ctx->desc.v_mode, st, A, az), "chain kernel"))) return r;
+    ctx->timed_end();
+    if ((r = chk(hipMemcpyAsync(mt, d_mt, sizeof(mt), hipMemcpyDeviceToHost, st), "download"))) return r;
+    if (accepted &&
+        (r = chk(hipMemcpyAsync(accepted, d_acc, sizeof(int) * n_moves, hipMemcpyDeviceToHost, st), "download")))
+        return r;
+    if (info &&
+        (r = chk(hipMemcpyAsync(info, d_info, sizeof(double) * 3 * n_moves, hipMemcpyDeviceToHost, st), "download")))
+        return r;
+    int nr = 0;
+    if ((r = chk(hipMemcpyAsync(&nr, d_near, sizeof(int), hipMemcpyDeviceToHost, st), "download"))) return r;
+    if ((r = chk(hipStreamSynchronize(st), "chain kernel"))) return r;
+    std::memcpy(rng.s, mt, sizeof(uint32_t) * MtRand::N);
+    rng.left = (int)mt[MtRand::N];
+    rng.next = (int)mt[MtRand::N + 1];
+    if (near) *near = nr;
+    // the host copy follows the device path
+    r = pimc_download_path(ctx, pos.data());
+    if (r) return fail(r, pimc_last_error(ctx));
+    return PIMC_OK;
+}
+
 extern "C" {
 
 int pimc_chain_create(pimc_ctx* ctx, const pimc_chain_desc* d, pimc_chain** out) {
@@ -360,6 +471,12 @@ int pimc_chain_create(pimc_ctx* ctx, const pimc_chain_desc* d, pimc_chain** out)
 void pimc_chain_destroy(pimc_chain* ch) { delete ch; }
 
 const char* pimc_chain_last_error(const pimc_chain* ch) { return ch ? ch->err.c_str() : ""; }
+
+int pimc_chain_run_device(pimc_chain* ch, int n_moves, const int* kinds, int* accepted, double* info3,
+                          int* near_threshold) {
+    if (!ch) return PIMC_E_ARG;
+    return ch->run_device(n_moves, kinds, accepted, info3, near_threshold);
+}
 
 int pimc_chain_move(pimc_chain* ch, int kind, int* accepted, double* info3) {
     if (!ch || !accepted) return PIMC_E_ARG;

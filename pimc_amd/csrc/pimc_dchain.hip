@@ -1,0 +1,465 @@
+// pimc_dchain.hip -- device-resident Markov chain: Staging, Bisection and
+// CenterOfMass moves (src/move.cpp:1124-1259, 1315-1408, 1484-1661) run inside
+// one kernel, many moves per launch, with no host round trip per move.
+//
+// One workgroup per chain.  Lane 0 replays the chain's MTRand stream (state in
+// LDS, include/MersenneTwister.h) in the reference's draw order and builds each
+// move's proposal; the workgroup then evaluates the move's old and new
+// Vnn sums over every partner of the moved beads (the pimc_delta_action
+// contract) and lane 0 takes the Metropolis decision; accepted moves are
+// written to the resident path and its cell lists, rejected CenterOfMass moves
+// are shifted back as the reference shifts them.  The transcendental units
+// (log, sqrt, exp, pow) are the device's, so a proposal can differ from the
+// host's in the last ulp; a decision whose uniform deviate lies within a
+// relative 1e-10 of its acceptance probability is counted (guard band) so a
+// caller can tell when such a difference could have flipped it.
+#include <hip/hip_runtime.h>
+
+#include "pimc_kernels.hpp"
+
+namespace pimc {
+
+namespace {
+
+constexpr int kMt = 624, kMtM = 397;
+
+struct MtLds {
+    uint32_t s[kMt];
+    int left, next;
+};
+
+__device__ __forceinline__ uint32_t mt_twist(uint32_t m, uint32_t a, uint32_t b) {
+    return m ^ (((a & 0x80000000u) | (b & 0x7fffffffu)) >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
+}
+
+// MTRand::reload (include/MersenneTwister.h:271-288), lane 0.
+__device__ void mt_reload(MtLds& g) {
+    int i = 0;
+    for (; i < kMt - kMtM; ++i) g.s[i] = mt_twist(g.s[i + kMtM], g.s[i], g.s[i + 1]);
+    for (; i < kMt - 1; ++i) g.s[i] = mt_twist(g.s[i + kMtM - kMt], g.s[i], g.s[i + 1]);
+    g.s[kMt - 1] = mt_twist(g.s[kMtM - 1], g.s[kMt - 1], g.s[0]);
+    g.left = kMt;
+    g.next = 0;
+}
+
+__device__ uint32_t mt_u32(MtLds& g) {
+    if (g.left == 0) mt_reload(g);
+    --g.left;
+    uint32_t y = g.s[g.next++];
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    return y ^ (y >> 18);
+}
+
+__device__ uint32_t mt_randint(MtLds& g, uint32_t n) {
+    uint32_t used = n;
+    used |= used >> 1;
+    used |= used >> 2;
+    used |= used >> 4;
+    used |= used >> 8;
+    used |= used >> 16;
+    uint32_t i;
+    do {
+        i = mt_u32(g) & used;
+    } while (i > n);
+    return i;
+}
+
+__device__ double mt_rand(MtLds& g) {
+#pragma clang fp contract(off)
+    return double(mt_u32(g)) * (1.0 / 4294967295.0);
+}
+
+__device__ double mt_randnorm(MtLds& g, double mean, double sd) {
+#pragma clang fp contract(off)
+    double x, y, r;
+    do {
+        x = 2.0 * mt_rand(g) - 1.0;
+        y = 2.0 * mt_rand(g) - 1.0;
+        r = x * x + y * y;
+    } while (r >= 1.0 || r == 0.0);
+    const double f = sqrt(-2.0 * log(r) / r);
+    return mean + x * f * sd;
+}
+
+__device__ __forceinline__ double pib(double r, double L) {
+#pragma clang fp contract(off)
+    return r - L * floor(r * (1.0 / L) + 0.5);
+}
+
+struct ChainLds {
+    MtLds mt;
+    // the move's bead list: entry k is bead (slice[k], ptcl) with weight co[k] =
+    // VFactor * shift * tau and positions oldp / newp [at[k]]
+    int at[kChainMaxBeads];
+    int slice[kChainMaxBeads];
+    double co[kChainMaxBeads];
+    double oldp[kChainMaxBeads][3];
+    double newp[kChainMaxBeads][3];
+    // bisection: current positions of the segment (offset o = bead s0 + o)
+    double seg[kChainMaxBeads + 2][3];
+    unsigned char include[kChainMaxBeads];
+    double cumrho[kChainMaxWind];
+    double red[4][2];
+    int nb, ptcl, accept, done, near;
+    double ao, an;
+};
+
+__device__ __forceinline__ void load_pos(const ChainRunArgs& A, int m, int i, double r[3]) {
+    const double* p = A.pos + (int64_t)m * A.nalloc + i;
+    r[0] = p[0];
+    r[1] = p[A.plane];
+    r[2] = p[2 * A.plane];
+}
+
+// The workgroup's pass over the bead list: sum over beads of co * Vnn at the
+// old and at the new position (src/action.cpp:1026-1060 with the interaction
+// list's cutoff and cells, src/lookuptable.cpp:464-516; every bead on).
+template <int MODE>
+__device__ void delta_pass(const ChainRunArgs& A, const AzizDev& az, ChainLds& L) {
+    const double inv_dr = 1.0 / az.dr;
+    const bool use_cells = A.box.ngrid >= 4;
+    double to = 0.0, tn = 0.0;
+    for (int k = 0; k < L.nb; ++k) {
+        const int m = L.slice[k], me = L.ptcl;
+        const double* px = A.pos + (int64_t)m * A.nalloc;
+        const double* py = px + A.plane;
+        const double* pz = py + A.plane;
+        const int a = L.at[k];
+        const double ox = L.oldp[a][0], oy = L.oldp[a][1], oz = L.oldp[a][2];
+        const double nx = L.newp[a][0], ny = L.newp[a][1], nz = L.newp[a][2];
+        int c_old = 0, c_new = 0;
+        if (use_cells) {
+            c_old = cell_of(ox, oy, oz, A.box);
+            c_new = cell_of(nx, ny, nz, A.box);
+        }
+        double so = 0.0, sn = 0.0;
+        for (int j = threadIdx.x; j < A.nalloc; j += kChainThreads) {
+            if (j == me) continue;
+            const double xj = px[j], yj = py[j], zj = pz[j];
+            const int cj = use_cells ? A.cell[m * A.nalloc + j] : 0;
+            const double r2o = sep_r2<MODE>(ox, oy, oz, xj, yj, zj, A.box);
+            const double r2n = sep_r2<MODE>(nx, ny, nz, xj, yj, zj, A.box);
+            const bool oko = (r2o < A.box.rc2) && (!use_cells || cells_adjacent(c_old, cj, A.box.ngrid));
+            const bool okn = (r2n < A.box.rc2) && (!use_cells || cells_adjacent(c_new, cj, A.box.ngrid));
+            const double vo = pair_value_from_r2<MODE>(r2o, az, inv_dr);
+            const double vn = pair_value_from_r2<MODE>(r2n, az, inv_dr);
+            so += oko ? vo : 0.0;
+            sn += okn ? vn : 0.0;
+        }
+        to += L.co[k] * so;
+        tn += L.co[k] * sn;
+    }
+    to = wave_sum(to);
+    tn = wave_sum(tn);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        L.red[wave][0] = to;
+        L.red[wave][1] = tn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        L.ao = ((L.red[0][0] + L.red[1][0]) + L.red[2][0]) + L.red[3][0];
+        L.an = ((L.red[0][1] + L.red[1][1]) + L.red[2][1]) + L.red[3][1];
+    }
+    __syncthreads();
+}
+
+// Write list entries [k0, nb) to the path, with the cell-list update of
+// Path::updateBead; the beads sit on distinct slices, one lane each.
+__device__ void commit_list(const ChainRunArgs& A, ChainLds& L, int k0) {
+    for (int k = k0 + (int)threadIdx.x; k < L.nb; k += kChainThreads) {
+        const int m = L.slice[k], i = L.ptcl;
+        const double* r = L.newp[L.at[k]];
+        double* q = A.pos + (int64_t)m * A.nalloc + i;
+        q[0] = r[0];
+        q[A.plane] = r[1];
+        q[2 * A.plane] = r[2];
+        const int cn = cell_of(r[0], r[1], r[2], A.box);
+        cell_update_bead(m, i, cn, A.nalloc, A.num_slices, A.cell, A.label, A.numlab, A.hash);
+    }
+    __syncthreads();
+}
+
+__device__ bool metropolis(ChainLds& L, double arg) {
+    const double u = mt_rand(L.mt);
+    const double e = exp(arg);
+    if (e <= 1.0 && fabs(u - e) <= 1.0e-10 * e) ++L.near;   // u <= 1: only e <= 1 can be close
+    return u < e;
+}
+
+// rho0 over Mlen slices (src/action.cpp:119-127)
+__device__ double rho0(const ChainRunArgs& A, const double v[3], int Mlen) {
+#pragma clang fp contract(off)
+    const double norm = pow(4.0 * 3.14159265358979323846 * A.lambda * Mlen * A.tau, -0.5 * 3);
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    return norm * exp(-v2 / (4.0 * A.lambda * Mlen * A.tau));
+}
+
+// StagingMove::attemptMove (src/move.cpp:1315-1408), lane 0 proposes.
+template <int MODE>
+__device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* acc, double* info) {
+#pragma clang fp contract(off)
+    const int M = A.num_slices;
+    if (threadIdx.x == 0) {
+        int stageLength = A.Mbar;
+        const int s0 = (int)mt_randint(L.mt, M - 1);
+        const int s1 = (int)mt_randint(L.mt, A.nalloc - 1);
+        if (A.var_updates) stageLength = 2 + (int)mt_randint(L.mt, A.Mbar - 2);
+        const int e0 = (s0 + stageLength) % M;
+        info[0] = 1.0;
+        info[2] = stageLength;
+        // MoveBase::sampleWindingSector (src/move.cpp:359-396)
+        double ps[3], pe[3];
+        load_pos(A, s0, s1, ps);
+        load_pos(A, e0, s1, pe);
+        const double vel[3] = {pe[0] - ps[0], pe[1] - ps[1], pe[2] - ps[2]};
+        L.cumrho[0] = rho0(A, vel, stageLength);
+        double total = L.cumrho[0];
+        for (int n = 1; n < A.num_wind; ++n) {
+            double vw[3];
+            for (int k = 0; k < 3; ++k) vw[k] = vel[k] + A.wind[3 * n + k] * A.side;
+            const double c = rho0(A, vw, stageLength);
+            total += c;
+            L.cumrho[n] = L.cumrho[n - 1] + c;
+        }
+        for (int n = 0; n < A.num_wind; ++n) L.cumrho[n] /= total;
+        const double u = mt_rand(L.mt);
+        int idx = 0;
+        while (idx < A.num_wind && L.cumrho[idx] < u) ++idx;   // std::lower_bound
+        if (idx == A.num_wind) idx = A.num_wind - 1;   // cumrho ends at 1 and u <= 1: not reached
+        int w[3];
+        for (int k = 0; k < 3; ++k) w[k] = A.wind[3 * idx + k];
+        // the start bead and the stageLength - 1 moved beads
+        L.nb = stageLength;
+        L.ptcl = s1;
+        const double co = A.tau;
+        for (int k = 0; k < stageLength; ++k) {
+            const int m = (s0 + k) % M;
+            L.at[k] = k;
+            L.slice[k] = m;
+            L.co[k] = ((m & 1) ? A.vf1 : A.vf0) * co;
+            load_pos(A, m, s1, L.oldp[k]);
+        }
+        for (int d = 0; d < 3; ++d) L.newp[0][d] = L.oldp[0][d];
+        // MoveBase::newStagingPosition, winding form (src/move.cpp:306-346)
+        for (int k = 0; k < stageLength - 1; ++k) {
+            const double f1 = 1.0 * (stageLength - k - 1);
+            const double f2 = 1.0 / (1.0 * (stageLength - k));
+            const double sq = A.sqrt2LambdaTau * sqrt(f1 * f2);
+            const double* npv = L.newp[k];
+            double r[3];
+            for (int a = 0; a < 3; ++a) r[a] = (pe[a] + A.side * w[a]) - npv[a];
+            for (int a = 0; a < 3; ++a) r[a] *= f2;
+            for (int a = 0; a < 3; ++a) r[a] += npv[a];
+            for (int a = 0; a < 3; ++a) r[a] = mt_randnorm(L.mt, r[a], sq);
+            for (int a = 0; a < 3; ++a) {
+                while (r[a] < -0.5 * A.side) {
+                    w[a]++;
+                    r[a] += A.side;
+                }
+                while (r[a] >= 0.5 * A.side) {
+                    w[a]--;
+                    r[a] -= A.side;
+                }
+            }
+            for (int a = 0; a < 3; ++a) L.newp[k + 1][a] = pib(r[a], A.side);
+        }
+    }
+    __syncthreads();
+    delta_pass<MODE>(A, az, L);
+    if (threadIdx.x == 0) {
+        const double dlt = L.an - L.ao;
+        info[1] = dlt;
+        L.accept = metropolis(L, -dlt) ? 1 : 0;
+        *acc = L.accept;
+    }
+    __syncthreads();
+    if (L.accept) commit_list(A, L, 1);
+}
+
+// BisectionMove::attemptMove, local action (src/move.cpp:1484-1661).
+template <int MODE>
+__device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* acc, double* info) {
+#pragma clang fp contract(off)
+    const int M = A.num_slices;
+    __shared__ int s_levels, s_active, s_s0;
+    __shared__ double s_old_delta;
+    if (threadIdx.x == 0) {
+        int numLevels = A.levels;
+        if (A.var_updates) numLevels = 1 + (int)mt_randint(L.mt, A.levels - 1);
+        const int numActive = (1 << numLevels) - 1;
+        const int s0 = (int)mt_randint(L.mt, M - 1);
+        const int s1 = (int)mt_randint(L.mt, A.nalloc - 1);
+        info[0] = 1.0;
+        info[2] = numLevels;
+        s_levels = numLevels;
+        s_active = numActive;
+        s_s0 = s0;
+        s_old_delta = 0.0;
+        L.ptcl = s1;
+        L.done = 0;
+        for (int o = 0; o <= numActive + 1; ++o) load_pos(A, (s0 + o) % M, s1, L.seg[o]);
+        for (int n = 0; n < numActive; ++n) L.include[n] = 1;
+    }
+    __syncthreads();
+    for (int level = s_levels; level > 0; --level) {
+        const int shift = 1 << (level - 1);
+        if (threadIdx.x == 0) {
+            const int s0 = s_s0, numActive = s_active;
+            const double delta = A.sqrtLambdaTau * sqrt(1.0 * shift);
+            const double co_s = shift * A.tau;
+            int nb = 0;
+            // this level's beads: proposals (src/move.cpp:467-488) in order
+            for (int k = 1; k * shift < numActive + 1; ++k) {
+                const int n = k * shift - 1;
+                if (!L.include[n]) continue;
+                const int o = n + 1;
+                for (int d = 0; d < 3; ++d) L.oldp[n][d] = L.seg[o][d];
+                double r[3];
+                for (int d = 0; d < 3; ++d) r[d] = pib(L.seg[o + shift][d] - L.seg[o - shift][d], A.side);
+                for (int d = 0; d < 3; ++d) r[d] *= 0.5;
+                for (int d = 0; d < 3; ++d) r[d] += L.seg[o - shift][d];
+                for (int d = 0; d < 3; ++d) r[d] = mt_randnorm(L.mt, r[d], delta);
+                for (int d = 0; d < 3; ++d) r[d] = pib(r[d], A.side);
+                for (int d = 0; d < 3; ++d) {
+                    L.newp[n][d] = r[d];
+                    L.seg[o][d] = r[d];
+                }
+                L.include[n] = 0;
+                const int m = (s0 + o) % M;
+                L.at[nb] = n;
+                L.slice[nb] = m;
+                L.co[nb] = ((m & 1) ? A.vf1 : A.vf0) * co_s;
+                ++nb;
+            }
+            // at level 1 the earlier levels' beads join (newAction and oldAction over them)
+            if (level == 1)
+                for (int k = 1; k * shift < numActive + 1; ++k) {
+                    const int n = k * shift - 1;
+                    if ((n + 1) % 2 == 1) continue;   // this level's beads: already listed
+                    const int o = n + 1, m = (s0 + o) % M;
+                    L.at[nb] = n;
+                    L.slice[nb] = m;
+                    L.co[nb] = ((m & 1) ? A.vf1 : A.vf0) * co_s;
+                    ++nb;
+                }
+            L.nb = nb;
+        }
+        __syncthreads();
+        delta_pass<MODE>(A, az, L);
+        if (threadIdx.x == 0) {
+            const double dlt = L.an - L.ao;
+            info[1] = dlt;
+            if (metropolis(L, -dlt + s_old_delta)) {
+                if (level == 1) {
+                    *acc = 1;
+                    // every bead of the segment takes its proposal
+                    for (int n = 0; n < s_active; ++n) {
+                        L.at[n] = n;
+                        L.slice[n] = (s_s0 + n + 1) % M;
+                    }
+                    L.nb = s_active;
+                    L.accept = 1;
+                }
+            } else {
+                L.accept = 0;
+                L.done = 1;   // undoMove: the path still holds every original position
+            }
+            s_old_delta = dlt;
+        }
+        __syncthreads();
+        if (L.done) break;
+    }
+    if (L.accept && !L.done) commit_list(A, L, 0);
+}
+
+// CenterOfMassMove::attemptMove for a closed worldline (src/move.cpp:1124-1259).
+template <int MODE>
+__device__ void com(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* acc, double* info) {
+#pragma clang fp contract(off)
+    const int M = A.num_slices;
+    __shared__ double sh[3];
+    if (threadIdx.x == 0) {
+        const int p = (int)mt_randint(L.mt, A.nalloc - 1);
+        info[0] = 1.0;
+        info[2] = M;
+        for (int k = 0; k < 3; ++k) sh[k] = A.com_delta * (-0.5 + mt_rand(L.mt));
+        L.ptcl = p;
+        L.nb = M;
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < M; m += kChainThreads) {
+        L.at[m] = m;
+        L.slice[m] = m;
+        L.co[m] = ((m & 1) ? A.vf1 : A.vf0) * A.tau;
+        load_pos(A, m, L.ptcl, L.oldp[m]);
+        for (int k = 0; k < 3; ++k) L.newp[m][k] = pib(L.oldp[m][k] + sh[k], A.side);
+    }
+    __syncthreads();
+    delta_pass<MODE>(A, az, L);
+    if (threadIdx.x == 0) {
+        const double dlt = L.an - L.ao;
+        info[1] = dlt;
+        L.accept = metropolis(L, -dlt) ? 1 : 0;
+        *acc = L.accept;
+    }
+    __syncthreads();
+    if (!L.accept) {
+        // the reference shifts the worldline back (it does not restore saved positions)
+        for (int m = threadIdx.x; m < M; m += kChainThreads)
+            for (int k = 0; k < 3; ++k) L.newp[m][k] = pib(L.newp[m][k] - sh[k], A.side);
+        __syncthreads();
+    }
+    commit_list(A, L, 0);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kChainThreads) void k_chain_run(ChainRunArgs A, AzizDev az) {
+    __shared__ ChainLds L;
+    for (int t = threadIdx.x; t < kMt; t += kChainThreads) L.mt.s[t] = A.mt[t];
+    if (threadIdx.x == 0) {
+        L.mt.left = (int)A.mt[kMt];
+        L.mt.next = (int)A.mt[kMt + 1];
+        L.near = 0;
+    }
+    __syncthreads();
+    for (int mv = 0; mv < A.n_moves; ++mv) {
+        const int kind = A.kinds[mv];
+        int* acc = A.accepted + mv;
+        double* info = A.info + 3 * mv;
+        if (threadIdx.x == 0) {
+            *acc = 0;
+            L.accept = 0;
+        }
+        if (kind == 0)
+            staging<MODE>(A, az, L, acc, info);
+        else if (kind == 1)
+            bisection<MODE>(A, az, L, acc, info);
+        else
+            com<MODE>(A, az, L, acc, info);
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < kMt; t += kChainThreads) A.mt[t] = L.mt.s[t];
+    if (threadIdx.x == 0) {
+        A.mt[kMt] = (uint32_t)L.mt.left;
+        A.mt[kMt + 1] = (uint32_t)L.mt.next;
+        A.near[0] = L.near;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_chain_run(int mode, hipStream_t st, const ChainRunArgs& A, const AzizDev& az) {
+    (void)hipGetLastError();
+    if (mode == 1)
+        hipLaunchKernelGGL(k_chain_run<1>, dim3(1), dim3(kChainThreads), 0, st, A, az);
+    else if (mode == 2)
+        hipLaunchKernelGGL(k_chain_run<2>, dim3(1), dim3(kChainThreads), 0, st, A, az);
+    else
+        hipLaunchKernelGGL(k_chain_run<0>, dim3(1), dim3(kChainThreads), 0, st, A, az);
+    return hipGetLastError();
+}
+
+}  // namespace pimc

@@ -335,6 +335,34 @@ __device__ __forceinline__ bool worm_touches_slice(const WormDev& w, int slice) 
     return w.head[0] == slice || w.tail[0] == slice || w.sp1[0] == slice || w.sp2[0] == slice;
 }
 
+// Path::updateBead -> LookupTable::updateBead (src/lookuptable.cpp:355-393, :430-458):
+// bead (m, i) moves to cell c_new; the label table keeps the reference's
+// swap-with-last order.
+__device__ __forceinline__ void cell_update_bead(int m, int i, int c_new, int nalloc, int num_slices, int* cell, int* label,
+                                 int* numlab, int* hash) {
+    const int bi = m * nalloc + i;
+    const int c_old = cell[bi];
+    if (c_new == c_old || c_old < 0) return;  // unchanged cell, or bead not in the table
+    const int64_t hb_old = ((int64_t)c_old * num_slices + m) * nalloc;
+    const int lab = label[bi];
+    const int last = numlab[c_old * num_slices + m] - 1;
+    if (lab < last) {
+        const int moved = hash[hb_old + last];
+        hash[hb_old + last] = -1;
+        hash[hb_old + lab] = moved;
+        label[m * nalloc + moved] = lab;
+    } else {
+        hash[hb_old + last] = -1;
+    }
+    numlab[c_old * num_slices + m] = last;
+    const int64_t hb_new = ((int64_t)c_new * num_slices + m) * nalloc;
+    const int nl = numlab[c_new * num_slices + m];
+    hash[hb_new + nl] = i;
+    label[bi] = nl;
+    numlab[c_new * num_slices + m] = nl + 1;
+    cell[bi] = c_new;
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic wave64 / workgroup FP64 reductions (fixed shuffle tree).
 // ---------------------------------------------------------------------------

@@ -126,9 +126,10 @@ struct pimc_ctx {
     std::vector<hipEvent_t> ev_pool;   // 2 per slot
     std::vector<Pending> pending;
     int ev_next = 0;
-    static constexpr int kKinds = 8;   // pair_action, delta_action, rho, ssf, isf, gradv2, gradvnn, virial
-    double kernel_ms[kKinds] = {0, 0, 0, 0, 0, 0, 0, 0};
-    long kernel_count[kKinds] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // pair_action, delta_action, rho, ssf, isf, gradv2, gradvnn, virial, chain_run
+    static constexpr int kKinds = 9;
+    double kernel_ms[kKinds] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long kernel_count[kKinds] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int timed_slot(int kind);          // -1 when timing is off
     void timed_begin(int kind);
     void timed_end();
@@ -138,3 +139,8 @@ struct pimc_ctx {
 
     ~pimc_ctx();
 };
+
+namespace pimc {
+// The context's Aziz constants with the table pointers its mode needs.
+AzizDev context_aziz(pimc_ctx* c, int* rc);
+}  // namespace pimc

@@ -193,6 +193,28 @@ hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInlin
 hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,
                          int* blocks_per_slice);
+// Device-resident chain (pimc_dchain.hip): one workgroup runs n_moves moves.
+constexpr int kChainThreads = 256;
+constexpr int kChainMaxBeads = 512;   // beads one move may list (M for CenterOfMass)
+constexpr int kChainMaxWind = 343;    // winding sectors, (2 max_wind + 1)^3 with max_wind <= 3
+struct ChainRunArgs {
+    double* pos;
+    int64_t plane;
+    int nalloc, num_slices;
+    int *cell, *label, *numlab, *hash;
+    BoxDev box;
+    double side, tau, lambda, vf0, vf1;
+    double sqrtLambdaTau, sqrt2LambdaTau, com_delta;
+    int Mbar, levels, var_updates, num_wind;
+    const int* wind;     // [num_wind][3]
+    uint32_t* mt;        // MTRand state [624], left, next (read and written back)
+    const int* kinds;    // [n_moves]: 0 staging, 1 bisection, 2 centre of mass
+    int n_moves;
+    int* accepted;       // [n_moves]
+    double* info;        // [n_moves][3]: attempted, last Delta, length
+    int* near;           // decisions inside the guard band
+};
+hipError_t launch_chain_run(int mode, hipStream_t st, const ChainRunArgs& A, const AzizDev& az);
 hipError_t launch_virial(int mode, hipStream_t st, const VirialArgs& A, const AzizDev& az, int* blocks_per_slice);
 hipError_t launch_update_beads(hipStream_t st, const int* sp, const double* p3, int n, double* pos, int64_t plane,
                                int nalloc, int num_slices, int* cell, int* label, int* numlab, int* hash,

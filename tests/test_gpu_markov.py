@@ -84,3 +84,38 @@ def test_markov_energy_independent_seeds(orc, gpu):
     print(f"E/N oracle {mr:.4f} +- {sr:.4f}  device {mg:.4f} +- {sg:.4f}  ({len(e_r)} samples)")
     assert sr > 0 and sg > 0
     assert abs(mr - mg) <= 5.0 * np.hypot(sr, sg)
+
+
+@pytest.mark.parametrize("mode,rc_frac,N,M", [(2, 1.0, 16, 40), (2, 0.25, 128, 32), (1, 0.5, 64, 48)])
+def test_device_resident_run_matches_oracle(orc, gpu, mode, rc_frac, N, M):
+    """pimc_chain_run_device: proposals, draws and decisions on the device in
+    one launch, continuing the host stream (host moves before and after).  The
+    decisions and stage lengths equal the oracle's; dU and the final positions
+    agree to 1e-9 (the device's log / exp may differ from the host's in the
+    last ulp); decisions inside the 1e-10 guard band are counted."""
+    ref, ctx, L = make_pair(orc, gpu, N, M, 0.01, rc_frac, kind=1, mode=mode)
+    seed, Mbar, levels, com_delta = 99, 8, 3, 0.5
+    moves = orc.Moves(ref, Mbar, levels, True, 1, com_delta)
+    rng = orc.MTRand(seed)
+    chain = gpu.Chain(ctx, seed, Mbar, levels, True, 1, com_delta)
+    kinds = ("staging", "bisection", "com")
+    seq = [kinds[k % 3] for k in range(900)]
+    for kind in seq[:30]:                           # host-driven first
+        acc_r, _ = getattr(moves, kind)(rng)
+        acc_g, _ = chain.move(kind)
+        assert bool(acc_r) == acc_g
+    acc_d, info_d, near = chain.run_device(seq[30:870])
+    for k, kind in enumerate(seq[30:870]):
+        acc_r, info_r = getattr(moves, kind)(rng)
+        assert bool(acc_r) == bool(acc_d[k]), f"device move {k} ({kind}): oracle {acc_r} device {acc_d[k]}"
+        assert info_d[k, 2] == info_r[2] and info_d[k, 0] == 1.0
+        assert abs(info_d[k, 1] - info_r[1]) <= 1e-9 * (1.0 + abs(info_r[1])), (k, kind, info_r, info_d[k])
+    for kind in seq[870:]:                          # and the stream continues on the host
+        acc_r, _ = getattr(moves, kind)(rng)
+        acc_g, _ = chain.move(kind)
+        assert bool(acc_r) == acc_g
+    print(f"device run mode {mode} N={N} M={M}: {int(acc_d.sum())}/840 accepted, {near} near the threshold")
+    assert 0 < acc_d.sum() < 840
+    assert np.allclose(ctx.download_path(), ref.beads(), rtol=0, atol=1e-9)
+    a_g, a_r = ctx.potentialAction(), ref.total_action()
+    assert abs(a_g - a_r) <= 1e-9 * abs(a_r)

@@ -305,7 +305,27 @@ def chain_leg(pimc_amd, beads, L, M, N, tau, cpu, moves=3000):
     k_ms, k_n = ctx.kernel_time("chain_run")
     ctx.set_timing(False)
     ctx.close()
+    # an ensemble of independent chains of the same size, one workgroup each, one launch
+    from pimc_amd.workloads import SEED0
+    n_ens, ens_moves = 256, 600
+    ectx, echains = [], []
+    for c in range(n_ens):
+        e = pimc_amd.Context(L, M, N, tau, LAMBDA, L, v_mode=pimc_amd.V_FP32)
+        e.upload_path(pimc_amd.gen_worldlines(SEED0 + 100 + c, L, M, N, LAMBDA, tau, 1))
+        ectx.append(e)
+        echains.append(pimc_amd.Chain(e, 5000 + c, 16, 4, True, 1, 0.1))
+    pimc_amd.run_chains_device(echains, kinds[:30])
+    t0 = time.perf_counter()
+    e_acc, _, e_near = pimc_amd.run_chains_device(echains, kinds[:ens_moves])
+    t_ens = time.perf_counter() - t0
+    for ch in echains:
+        ch.close()
+    for e in ectx:
+        e.close()
     out = {"moves": moves, "mix": "staging / bisection / centre of mass in rotation",
+           "ensemble": {"chains": n_ens, "moves_per_chain": ens_moves, "seconds": t_ens,
+                        "moves_per_s": n_ens * ens_moves / t_ens, "accept_frac": float(np.mean(e_acc)),
+                        "near_threshold_decisions": int(np.sum(e_near))},
            "host_driven_moves_per_s": host_moves / t_host,
            "device_resident_moves_per_s": moves / t_dev, "device_kernel_ms": k_ms / max(k_n, 1),
            "device_us_per_move": t_dev / moves * 1e6, "accept_frac": float(np.mean(acc)),

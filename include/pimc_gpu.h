@@ -306,6 +306,13 @@ int  pimc_chain_positions(const pimc_chain* ch, double* beads);
  * without the gradient correction (PIMC_E_UNSUPPORTED otherwise). */
 int  pimc_chain_run_device(pimc_chain* ch, int n_moves, const int* kinds, int* accepted, double* info3,
                            int* near_threshold);
+/* The same for n_chains chains (each on its own context, one pair-term mode) in
+ * one launch, one workgroup per chain: independent chains, the reference's
+ * process-per-seed ensemble (src/pdrive.cpp:41,63), sharing the GPU.  kinds
+ * [n_moves] is every chain's move list; accepted [n_chains][n_moves], info3
+ * [n_chains][n_moves][3], near_threshold [n_chains] (each may be NULL). */
+int  pimc_chain_run_device_many(pimc_chain* const* chains, int n_chains, int n_moves, const int* kinds,
+                                int* accepted, double* info3, int* near_threshold);
 
 /* ---- state files ------------------------------------------------------------- */
 /* The reference's text state file (PathIntegralMonteCarlo::saveState, src/pimc.cpp:

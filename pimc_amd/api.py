@@ -122,6 +122,7 @@ SIGNATURES = {
     "pimc_chain_move": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _dp]),
     "pimc_chain_positions": (C.c_int, [C.c_void_p, _dp]),
     "pimc_chain_run_device": (C.c_int, [C.c_void_p, C.c_int, _ip, _ip, _dp, _ip]),
+    "pimc_chain_run_device_many": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int, _ip, _ip, _dp, _ip]),
     "pimc_state_last_error": (C.c_char_p, []),
     "pimc_state_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "pimc_state_parse": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -714,6 +715,23 @@ class StateFile:
         self._check(lib().pimc_state_to_path(self.h, M, side, _p(b, _dp), _p(on, _bp), _p(nb, _ip), _p(nx, _ip),
                                              _p(pv, _ip)), "state to path")
         return {"beads": b, "bead_on": on, "num_beads": nb, "next": nx, "prev": pv}
+
+
+def run_chains_device(chains, kinds):
+    """pimc_chain_run_device_many: every chain's moves of `kinds` in one launch.
+    Returns (accepted [n][k], info [n][k][3], near_threshold [n])."""
+    k = np.array([Chain.KINDS[x] if isinstance(x, str) else int(x) for x in kinds], dtype=np.int32)
+    n = len(chains)
+    hs = (C.c_void_p * n)(*[c.h.value for c in chains])
+    acc = np.zeros((n, k.shape[0]), np.int32)
+    info = np.zeros((n, k.shape[0], 3))
+    near = np.zeros(n, np.int32)
+    rc = lib().pimc_chain_run_device_many(hs, n, k.shape[0], _p(k, _ip), _p(acc, _ip), _p(info, _dp),
+                                          _p(near, _ip))
+    if rc != 0:
+        msgs = {lib().pimc_chain_last_error(c.h).decode() for c in chains} - {""}
+        raise PimcError(f"run_chains_device ({rc}): {'; '.join(sorted(msgs))}")
+    return acc.astype(bool), info, near
 
 
 def gen_worldlines(seed, side, M, N, lam, tau, kind=0):

@@ -685,9 +685,16 @@ int pimc_update_beads(pimc_ctx* c, const int32_t* sp, const double* pos, int n) 
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_req_sp, c->h_req_sp, sizeof(int) * 2 * n, hipMemcpyHostToDevice, c->stream));
     HIP_OR_FAIL(c, hipMemcpyAsync(c->d_req_pos, c->h_req_pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
                                   c->stream));
+    // beads on distinct slices touch disjoint cell tables: one lane each
+    std::vector<char> seen(c->desc.num_slices, 0);
+    bool distinct = true;
+    for (int k = 0; k < n && distinct; ++k) {
+        if (seen[sp[2 * k]]) distinct = false;
+        seen[sp[2 * k]] = 1;
+    }
     HIP_OR_FAIL(c, launch_update_beads(c->stream, c->d_req_sp, c->d_req_pos, n, c->d_pos, c->plane,
                                        c->desc.num_alloc, c->desc.num_slices, c->d_cell, c->d_label, c->d_numlab,
-                                       c->d_hash, c->box));
+                                       c->d_hash, c->box, distinct));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     return PIMC_OK;
 }

@@ -54,7 +54,9 @@ struct pimc_chain {
         for (void* p : {(void*)d_mt, (void*)d_wind, (void*)d_kinds, (void*)d_acc, (void*)d_info, (void*)d_near})
             if (p) (void)hipFree(p);
     }
-    int run_device(int n_moves, const int* kinds, int* accepted, double* info, int* near);
+    int check_device(int n_moves, const int* kinds);
+    int device_args(int n_moves, const int* kinds, pimc::ChainRunArgs* A, pimc::AzizDev* az);
+    int device_results(int n_moves, int* accepted, double* info, int* near);
 
     double* P(int m, int i) { return pos.data() + 3 * ((size_t)m * N + i); }
     int fail(int code, const std::string& m) {
@@ -319,22 +321,23 @@ int pimc_chain::com(int* acc, double* info) {
     return device_set(beads, back);
 }
 
-// The moves of `kinds` inside one kernel (pimc_dchain.hip) on the chain's own
-// stream state: the host's MTRand state goes up before and comes back after,
-// so host moves and device runs continue one stream.
-int pimc_chain::run_device(int n_moves, const int* kinds, int* accepted, double* info, int* near) {
+// Device-resident runs (pimc_dchain.hip): every chain's moves of `kinds` in
+// one launch, one workgroup per chain, on the chains' own stream states: each
+// host MTRand state goes up before and comes back after, so host moves and
+// device runs continue one stream per chain.
+int pimc_chain::check_device(int n_moves, const int* kinds) {
     if (n_moves < 0 || (n_moves > 0 && !kinds)) return fail(PIMC_E_ARG, "bad move list");
     for (int k = 0; k < n_moves; ++k)
         if (kinds[k] < 0 || kinds[k] > 2) return fail(PIMC_E_ARG, "move kind must be 0, 1 or 2");
     if (M > pimc::kChainMaxBeads) return fail(PIMC_E_UNSUPPORTED, "device runs hold at most 512 slices");
-    const int num_wind = (int)cumrho0.size();
-    if (num_wind > pimc::kChainMaxWind) return fail(PIMC_E_UNSUPPORTED, "device runs sample at most 343 windings");
+    if ((int)cumrho0.size() > pimc::kChainMaxWind)
+        return fail(PIMC_E_UNSUPPORTED, "device runs sample at most 343 windings");
     if (ctx->correction())
         return fail(PIMC_E_UNSUPPORTED, "device runs evaluate the action without the gradient correction");
-    if (n_moves == 0) {
-        if (near) *near = 0;
-        return PIMC_OK;
-    }
+    return PIMC_OK;
+}
+
+int pimc_chain::device_args(int n_moves, const int* kinds, pimc::ChainRunArgs* A, pimc::AzizDev* az) {
     auto chk = [&](hipError_t e, const char* what) {
         return e == hipSuccess ? PIMC_OK : fail(PIMC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
     };
@@ -343,7 +346,8 @@ int pimc_chain::run_device(int n_moves, const int* kinds, int* accepted, double*
         if ((r = chk(hipMalloc(&d_mt, sizeof(uint32_t) * (MtRand::N + 2)), "hipMalloc"))) return r;
         if ((r = chk(hipMalloc(&d_wind, sizeof(int) * wind.size()), "hipMalloc"))) return r;
         if ((r = chk(hipMalloc(&d_near, sizeof(int)), "hipMalloc"))) return r;
-        if ((r = chk(hipMemcpy(d_wind, wind.data(), sizeof(int) * wind.size(), hipMemcpyHostToDevice), "upload"))) return r;
+        if ((r = chk(hipMemcpy(d_wind, wind.data(), sizeof(int) * wind.size(), hipMemcpyHostToDevice), "upload")))
+            return r;
     }
     if (cap_moves < n_moves) {
         for (void* p : {(void*)d_kinds, (void*)d_acc, (void*)d_info})
@@ -359,54 +363,55 @@ int pimc_chain::run_device(int n_moves, const int* kinds, int* accepted, double*
     std::memcpy(mt, rng.s, sizeof(uint32_t) * MtRand::N);
     mt[MtRand::N] = (uint32_t)rng.left;
     mt[MtRand::N + 1] = (uint32_t)rng.next;
-    hipStream_t st = ctx->stream;
-    if ((r = chk(hipMemcpyAsync(d_mt, mt, sizeof(mt), hipMemcpyHostToDevice, st), "upload"))) return r;
-    if ((r = chk(hipMemcpyAsync(d_kinds, kinds, sizeof(int) * n_moves, hipMemcpyHostToDevice, st), "upload")))
-        return r;
-    pimc::AzizDev az = pimc::context_aziz(ctx, &r);
+    if ((r = chk(hipMemcpy(d_mt, mt, sizeof(mt), hipMemcpyHostToDevice), "upload"))) return r;
+    if ((r = chk(hipMemcpy(d_kinds, kinds, sizeof(int) * n_moves, hipMemcpyHostToDevice), "upload"))) return r;
+    *az = pimc::context_aziz(ctx, &r);
     if (r) return fail(r, pimc_last_error(ctx));
-    pimc::ChainRunArgs A{};
-    A.pos = ctx->d_pos;
-    A.plane = ctx->plane;
-    A.nalloc = N;
-    A.num_slices = M;
-    A.cell = ctx->d_cell;
-    A.label = ctx->d_label;
-    A.numlab = ctx->d_numlab;
-    A.hash = ctx->d_hash;
-    A.box = ctx->box;
-    A.side = side;
-    A.tau = tau;
-    A.lambda = lambda;
-    A.vf0 = ctx->vf_default[0];
-    A.vf1 = ctx->vf_default[1];
-    A.sqrtLambdaTau = sqrtLambdaTau;
-    A.sqrt2LambdaTau = sqrt2LambdaTau;
-    A.com_delta = d.com_delta;
-    A.Mbar = d.Mbar;
-    A.levels = d.levels;
-    A.var_updates = d.var_updates;
-    A.num_wind = num_wind;
-    A.wind = d_wind;
-    A.mt = d_mt;
-    A.kinds = d_kinds;
-    A.n_moves = n_moves;
-    A.accepted = d_acc;
-    A.info = d_info;
-    A.near = d_near;
-    ctx->timed_begin(8);
-    if ((r = chk(pimc::launch_chain_run(ctx->desc.v_mode, st, A, az), "chain kernel"))) return r;
-    ctx->timed_end();
-    if ((r = chk(hipMemcpyAsync(mt, d_mt, sizeof(mt), hipMemcpyDeviceToHost, st), "download"))) return r;
-    if (accepted &&
-        (r = chk(hipMemcpyAsync(accepted, d_acc, sizeof(int) * n_moves, hipMemcpyDeviceToHost, st), "download")))
+    *A = pimc::ChainRunArgs{};
+    A->pos = ctx->d_pos;
+    A->plane = ctx->plane;
+    A->nalloc = N;
+    A->num_slices = M;
+    A->cell = ctx->d_cell;
+    A->label = ctx->d_label;
+    A->numlab = ctx->d_numlab;
+    A->hash = ctx->d_hash;
+    A->box = ctx->box;
+    A->side = side;
+    A->tau = tau;
+    A->lambda = lambda;
+    A->vf0 = ctx->vf_default[0];
+    A->vf1 = ctx->vf_default[1];
+    A->sqrtLambdaTau = sqrtLambdaTau;
+    A->sqrt2LambdaTau = sqrt2LambdaTau;
+    A->com_delta = d.com_delta;
+    A->Mbar = d.Mbar;
+    A->levels = d.levels;
+    A->var_updates = d.var_updates;
+    A->num_wind = (int)cumrho0.size();
+    A->wind = d_wind;
+    A->mt = d_mt;
+    A->kinds = d_kinds;
+    A->n_moves = n_moves;
+    A->accepted = d_acc;
+    A->info = d_info;
+    A->near = d_near;
+    return PIMC_OK;
+}
+
+int pimc_chain::device_results(int n_moves, int* accepted, double* info, int* near) {
+    auto chk = [&](hipError_t e, const char* what) {
+        return e == hipSuccess ? PIMC_OK : fail(PIMC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    int r;
+    uint32_t mt[MtRand::N + 2];
+    if ((r = chk(hipMemcpy(mt, d_mt, sizeof(mt), hipMemcpyDeviceToHost), "download"))) return r;
+    if (accepted && (r = chk(hipMemcpy(accepted, d_acc, sizeof(int) * n_moves, hipMemcpyDeviceToHost), "download")))
         return r;
-    if (info &&
-        (r = chk(hipMemcpyAsync(info, d_info, sizeof(double) * 3 * n_moves, hipMemcpyDeviceToHost, st), "download")))
+    if (info && (r = chk(hipMemcpy(info, d_info, sizeof(double) * 3 * n_moves, hipMemcpyDeviceToHost), "download")))
         return r;
     int nr = 0;
-    if ((r = chk(hipMemcpyAsync(&nr, d_near, sizeof(int), hipMemcpyDeviceToHost, st), "download"))) return r;
-    if ((r = chk(hipStreamSynchronize(st), "chain kernel"))) return r;
+    if ((r = chk(hipMemcpy(&nr, d_near, sizeof(int), hipMemcpyDeviceToHost), "download"))) return r;
     std::memcpy(rng.s, mt, sizeof(uint32_t) * MtRand::N);
     rng.left = (int)mt[MtRand::N];
     rng.next = (int)mt[MtRand::N + 1];
@@ -414,6 +419,58 @@ int pimc_chain::run_device(int n_moves, const int* kinds, int* accepted, double*
     // the host copy follows the device path
     r = pimc_download_path(ctx, pos.data());
     if (r) return fail(r, pimc_last_error(ctx));
+    return PIMC_OK;
+}
+
+static int run_chains_on_device(pimc_chain* const* chains, int n, int n_moves, const int* kinds, int* accepted,
+                                double* info, int* near) {
+    if (!chains || n < 1) return PIMC_E_ARG;
+    for (int c = 0; c < n; ++c) {
+        if (!chains[c]) return PIMC_E_ARG;
+        int r = chains[c]->check_device(n_moves, kinds);
+        if (r) return r;
+        if (chains[c]->ctx->desc.v_mode != chains[0]->ctx->desc.v_mode)
+            return chains[c]->fail(PIMC_E_ARG, "chains of one run share the pair-term mode");
+        for (int e = 0; e < c; ++e)
+            if (chains[e]->ctx == chains[c]->ctx) return chains[c]->fail(PIMC_E_ARG, "two chains on one context");
+    }
+    if (n_moves == 0) {
+        if (near)
+            for (int c = 0; c < n; ++c) near[c] = 0;
+        return PIMC_OK;
+    }
+    // every context's queued work first: the launch reads and writes their paths
+    for (int c = 0; c < n; ++c)
+        if (hipStreamSynchronize(chains[c]->ctx->stream) != hipSuccess)
+            return chains[c]->fail(PIMC_E_HIP, "stream synchronisation failed");
+    std::vector<pimc::ChainRunArgs> args(n);
+    std::vector<pimc::AzizDev> azs(n);
+    for (int c = 0; c < n; ++c) {
+        int r = chains[c]->device_args(n_moves, kinds, &args[c], &azs[c]);
+        if (r) return r;
+    }
+    pimc_chain* lead = chains[0];
+    pimc::ChainRunArgs* d_args = nullptr;
+    pimc::AzizDev* d_azs = nullptr;
+    hipStream_t st = lead->ctx->stream;
+    hipError_t e = hipMalloc(&d_args, sizeof(pimc::ChainRunArgs) * n);
+    if (e == hipSuccess) e = hipMalloc(&d_azs, sizeof(pimc::AzizDev) * n);
+    if (e == hipSuccess) e = hipMemcpy(d_args, args.data(), sizeof(pimc::ChainRunArgs) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_azs, azs.data(), sizeof(pimc::AzizDev) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        lead->ctx->timed_begin(8);
+        e = pimc::launch_chain_run(lead->ctx->desc.v_mode, st, n, d_args, d_azs);
+        lead->ctx->timed_end();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (d_args) (void)hipFree(d_args);
+    if (d_azs) (void)hipFree(d_azs);
+    if (e != hipSuccess) return lead->fail(PIMC_E_HIP, std::string("chain kernel: ") + hipGetErrorString(e));
+    for (int c = 0; c < n; ++c) {
+        int r = chains[c]->device_results(n_moves, accepted ? accepted + (size_t)c * n_moves : nullptr,
+                                          info ? info + (size_t)c * 3 * n_moves : nullptr, near ? near + c : nullptr);
+        if (r) return r;
+    }
     return PIMC_OK;
 }
 
@@ -475,7 +532,12 @@ const char* pimc_chain_last_error(const pimc_chain* ch) { return ch ? ch->err.c_
 int pimc_chain_run_device(pimc_chain* ch, int n_moves, const int* kinds, int* accepted, double* info3,
                           int* near_threshold) {
     if (!ch) return PIMC_E_ARG;
-    return ch->run_device(n_moves, kinds, accepted, info3, near_threshold);
+    return run_chains_on_device(&ch, 1, n_moves, kinds, accepted, info3, near_threshold);
+}
+
+int pimc_chain_run_device_many(pimc_chain* const* chains, int n_chains, int n_moves, const int* kinds,
+                               int* accepted, double* info3, int* near_threshold) {
+    return run_chains_on_device(chains, n_chains, n_moves, kinds, accepted, info3, near_threshold);
 }
 
 int pimc_chain_move(pimc_chain* ch, int kind, int* accepted, double* info3) {

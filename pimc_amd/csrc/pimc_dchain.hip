@@ -2,7 +2,8 @@
 // CenterOfMass moves (src/move.cpp:1124-1259, 1315-1408, 1484-1661) run inside
 // one kernel, many moves per launch, with no host round trip per move.
 //
-// One workgroup per chain.  Lane 0 replays the chain's MTRand stream (state in
+// One workgroup per chain (a launch may run many chains, one workgroup each,
+// each with its own arguments).  Lane 0 replays the chain's MTRand stream (state in
 // LDS, include/MersenneTwister.h) in the reference's draw order and builds each
 // move's proposal; the workgroup then evaluates the move's old and new
 // Vnn sums over every partner of the moved beads (the pimc_delta_action
@@ -202,6 +203,8 @@ template <int MODE>
 __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* acc, double* info) {
 #pragma clang fp contract(off)
     const int M = A.num_slices;
+    __shared__ int s_s0, s_s1, s_len;
+    __shared__ double s_vel[3];
     if (threadIdx.x == 0) {
         int stageLength = A.Mbar;
         const int s0 = (int)mt_randint(L.mt, M - 1);
@@ -210,17 +213,31 @@ __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, i
         const int e0 = (s0 + stageLength) % M;
         info[0] = 1.0;
         info[2] = stageLength;
-        // MoveBase::sampleWindingSector (src/move.cpp:359-396)
         double ps[3], pe[3];
         load_pos(A, s0, s1, ps);
         load_pos(A, e0, s1, pe);
-        const double vel[3] = {pe[0] - ps[0], pe[1] - ps[1], pe[2] - ps[2]};
-        L.cumrho[0] = rho0(A, vel, stageLength);
+        for (int k = 0; k < 3; ++k) s_vel[k] = pe[k] - ps[k];
+        s_s0 = s0;
+        s_s1 = s1;
+        s_len = stageLength;
+    }
+    __syncthreads();
+    // MoveBase::sampleWindingSector (src/move.cpp:359-396): the sector weights
+    // on the lanes, their running sum on lane 0 in the reference's order
+    for (int n = threadIdx.x; n < A.num_wind; n += kChainThreads) {
+        double vw[3];
+        for (int k = 0; k < 3; ++k) vw[k] = n == 0 ? s_vel[k] : s_vel[k] + A.wind[3 * n + k] * A.side;
+        L.cumrho[n] = rho0(A, vw, s_len);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int stageLength = s_len, s0 = s_s0, s1 = s_s1;
+        const int e0 = (s0 + stageLength) % M;
+        double pe[3];
+        load_pos(A, e0, s1, pe);
         double total = L.cumrho[0];
         for (int n = 1; n < A.num_wind; ++n) {
-            double vw[3];
-            for (int k = 0; k < 3; ++k) vw[k] = vel[k] + A.wind[3 * n + k] * A.side;
-            const double c = rho0(A, vw, stageLength);
+            const double c = L.cumrho[n];
             total += c;
             L.cumrho[n] = L.cumrho[n - 1] + c;
         }
@@ -416,8 +433,11 @@ __device__ void com(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* 
 }
 
 template <int MODE>
-__global__ __launch_bounds__(kChainThreads) void k_chain_run(ChainRunArgs A, AzizDev az) {
+__global__ __launch_bounds__(kChainThreads) void k_chain_run(const ChainRunArgs* __restrict__ args,
+                                                             const AzizDev* __restrict__ azs) {
     __shared__ ChainLds L;
+    const ChainRunArgs A = args[blockIdx.x];
+    const AzizDev az = azs[blockIdx.x];
     for (int t = threadIdx.x; t < kMt; t += kChainThreads) L.mt.s[t] = A.mt[t];
     if (threadIdx.x == 0) {
         L.mt.left = (int)A.mt[kMt];
@@ -451,14 +471,14 @@ __global__ __launch_bounds__(kChainThreads) void k_chain_run(ChainRunArgs A, Azi
 
 }  // namespace
 
-hipError_t launch_chain_run(int mode, hipStream_t st, const ChainRunArgs& A, const AzizDev& az) {
+hipError_t launch_chain_run(int mode, hipStream_t st, int n_chains, const ChainRunArgs* args, const AzizDev* azs) {
     (void)hipGetLastError();
     if (mode == 1)
-        hipLaunchKernelGGL(k_chain_run<1>, dim3(1), dim3(kChainThreads), 0, st, A, az);
+        hipLaunchKernelGGL(k_chain_run<1>, dim3(n_chains), dim3(kChainThreads), 0, st, args, azs);
     else if (mode == 2)
-        hipLaunchKernelGGL(k_chain_run<2>, dim3(1), dim3(kChainThreads), 0, st, A, az);
+        hipLaunchKernelGGL(k_chain_run<2>, dim3(n_chains), dim3(kChainThreads), 0, st, args, azs);
     else
-        hipLaunchKernelGGL(k_chain_run<0>, dim3(1), dim3(kChainThreads), 0, st, A, az);
+        hipLaunchKernelGGL(k_chain_run<0>, dim3(n_chains), dim3(kChainThreads), 0, st, args, azs);
     return hipGetLastError();
 }
 

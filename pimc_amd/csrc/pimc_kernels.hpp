@@ -214,11 +214,12 @@ struct ChainRunArgs {
     double* info;        // [n_moves][3]: attempted, last Delta, length
     int* near;           // decisions inside the guard band
 };
-hipError_t launch_chain_run(int mode, hipStream_t st, const ChainRunArgs& A, const AzizDev& az);
+// args / azs: device arrays, one entry per chain (workgroup).
+hipError_t launch_chain_run(int mode, hipStream_t st, int n_chains, const ChainRunArgs* args, const AzizDev* azs);
 hipError_t launch_virial(int mode, hipStream_t st, const VirialArgs& A, const AzizDev& az, int* blocks_per_slice);
 hipError_t launch_update_beads(hipStream_t st, const int* sp, const double* p3, int n, double* pos, int64_t plane,
                                int nalloc, int num_slices, int* cell, int* label, int* numlab, int* hash,
-                               const BoxDev& box);
+                               const BoxDev& box, bool distinct);
 hipError_t launch_cell_build(hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
                              const uint8_t* on, int* cell, int* label, int* numlab, int* hash, int cells,
                              const BoxDev& box);

@@ -578,11 +578,19 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
 // ---------------------------------------------------------------------------
 // Sequential bead updates (any slices, list order) -- Path::updateBead.
 // ---------------------------------------------------------------------------
+// distinct != 0: the beads sit on distinct slices (their cell tables are
+// disjoint), one lane each; otherwise one lane in list order.
 __global__ void k_update_beads(const int* __restrict__ sp, const double* __restrict__ p3, int n, double* pos,
                                int64_t plane, int nalloc, int num_slices, int* cell, int* label, int* numlab,
-                               int* hash, BoxDev box) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int k = 0; k < n; ++k) {
+                               int* hash, BoxDev box, int distinct) {
+    int k0 = 0, step = 1;
+    if (distinct) {
+        k0 = blockIdx.x * blockDim.x + threadIdx.x;
+        step = gridDim.x * blockDim.x;
+    } else if (blockIdx.x != 0 || threadIdx.x != 0) {
+        return;
+    }
+    for (int k = k0; k < n; k += step) {
         const int m = sp[2 * k], i = sp[2 * k + 1];
         const int cn = cell_of(p3[3 * k], p3[3 * k + 1], p3[3 * k + 2], box);
         cell_update_bead(m, i, cn, nalloc, num_slices, cell, label, numlab, hash);
@@ -876,10 +884,11 @@ hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t pl
 
 hipError_t launch_update_beads(hipStream_t st, const int* sp, const double* p3, int n, double* pos, int64_t plane,
                                int nalloc, int num_slices, int* cell, int* label, int* numlab, int* hash,
-                               const BoxDev& box) {
+                               const BoxDev& box, bool distinct) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
-    hipLaunchKernelGGL(k_update_beads, dim3(1), dim3(64), 0, st, sp, p3, n, pos, plane, nalloc, num_slices, cell,
-                       label, numlab, hash, box);
+    const int grid = distinct ? (n + 255) / 256 : 1;
+    hipLaunchKernelGGL(k_update_beads, dim3(grid), dim3(distinct ? 256 : 64), 0, st, sp, p3, n, pos, plane, nalloc,
+                       num_slices, cell, label, numlab, hash, box, distinct ? 1 : 0);
     return hipGetLastError();
 }
 

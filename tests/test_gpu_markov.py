@@ -119,3 +119,42 @@ def test_device_resident_run_matches_oracle(orc, gpu, mode, rc_frac, N, M):
     assert np.allclose(ctx.download_path(), ref.beads(), rtol=0, atol=1e-9)
     a_g, a_r = ctx.potentialAction(), ref.total_action()
     assert abs(a_g - a_r) <= 1e-9 * abs(a_r)
+
+
+def test_many_chains_one_launch(orc, gpu):
+    """pimc_chain_run_device_many: six independent chains (their own worldlines
+    and seeds, uneven N and M) in one launch equal the same chains run one at a
+    time (bitwise: one workgroup each, same arithmetic), and the first equals the
+    oracle's replay decision for decision."""
+    cfg = [(16, 40), (24, 32), (32, 48), (16, 64), (40, 24), (20, 36)]
+    kinds = [k % 3 for k in range(240)]
+    runs = []
+    for rep in range(2):
+        chains, ctxs = [], []
+        for c, (N, M) in enumerate(cfg):
+            ref, ctx, L = make_pair(orc, gpu, N, M, 0.01, 1.0, kind=1, mode=2, seed_beads=SEED0 + c)
+            ctxs.append((ref, ctx))
+            chains.append(gpu.Chain(ctx, 1000 + c, 6, 2, True, 1, 0.4))
+        if rep == 0:
+            acc, info, near = gpu.run_chains_device(chains, kinds)
+        else:
+            acc = np.zeros((len(cfg), len(kinds)), bool)
+            info = np.zeros((len(cfg), len(kinds), 3))
+            for c, ch in enumerate(chains):
+                a, i, _ = ch.run_device(kinds)
+                acc[c], info[c] = a, i
+        runs.append((acc, info, [ctx.download_path() for _, ctx in ctxs], ctxs))
+    (a0, i0, p0, ctxs), (a1, i1, p1, _) = runs
+    assert np.array_equal(a0, a1) and np.array_equal(i0, i1)
+    for x, y in zip(p0, p1):
+        assert np.array_equal(x, y)
+    ref = ctxs[0][0]
+    moves = orc.Moves(ref, 6, 2, True, 1, 0.4)
+    rng = orc.MTRand(1000)
+    names = ("staging", "bisection", "com")
+    for k, kd in enumerate(kinds):
+        acc_r, info_r = getattr(moves, names[kd])(rng)
+        assert bool(acc_r) == bool(a0[0, k]), k
+        assert abs(i0[0, k, 1] - info_r[1]) <= 1e-9 * (1.0 + abs(info_r[1]))
+    assert np.allclose(p0[0], ref.beads(), rtol=0, atol=1e-9)
+    print(f"6 chains x 240 moves in one launch; acceptance {a0.mean():.3f}")

@@ -23,37 +23,79 @@ namespace pimc {
 namespace {
 
 constexpr int kMt = 624, kMtM = 397;
+constexpr int kPolarBeads = 64;   // beads whose Gaussian deviates are drawn per round
 
+// The MTRand state (include/MersenneTwister.h:76-419) in LDS: the current
+// 624-word block and, once refilled, the next one (MTRand::reload of the
+// current block, computed ahead of need by the whole workgroup).  Lane 0 draws
+// through a cursor that keeps the block index and position in registers.
 struct MtLds {
-    uint32_t s[kMt];
-    int left, next;
+    uint32_t s[2][kMt];
+    int cur, next, have_next;
 };
 
 __device__ __forceinline__ uint32_t mt_twist(uint32_t m, uint32_t a, uint32_t b) {
     return m ^ (((a & 0x80000000u) | (b & 0x7fffffffu)) >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
 }
 
-// MTRand::reload (include/MersenneTwister.h:271-288), lane 0.
-__device__ void mt_reload(MtLds& g) {
-    int i = 0;
-    for (; i < kMt - kMtM; ++i) g.s[i] = mt_twist(g.s[i + kMtM], g.s[i], g.s[i + 1]);
-    for (; i < kMt - 1; ++i) g.s[i] = mt_twist(g.s[i + kMtM - kMt], g.s[i], g.s[i + 1]);
-    g.s[kMt - 1] = mt_twist(g.s[kMtM - 1], g.s[kMt - 1], g.s[0]);
-    g.left = kMt;
-    g.next = 0;
+// MTRand::reload (include/MersenneTwister.h:271-288) out of place: dst = the
+// block that follows src.  Its three ranges depend on earlier ones only, so the
+// workgroup computes each range in parallel.
+__device__ void mt_refill(MtLds& g) {
+    __syncthreads();
+    if (g.have_next) return;
+    const uint32_t* src = g.s[g.cur];
+    uint32_t* dst = g.s[g.cur ^ 1];
+    for (int i = threadIdx.x; i < kMt - kMtM; i += kChainThreads) dst[i] = mt_twist(src[i + kMtM], src[i], src[i + 1]);
+    __syncthreads();
+    for (int i = kMt - kMtM + threadIdx.x; i < 2 * (kMt - kMtM); i += kChainThreads)
+        dst[i] = mt_twist(dst[i + kMtM - kMt], src[i], src[i + 1]);
+    __syncthreads();
+    for (int i = 2 * (kMt - kMtM) + threadIdx.x; i < kMt - 1; i += kChainThreads)
+        dst[i] = mt_twist(dst[i + kMtM - kMt], src[i], src[i + 1]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        dst[kMt - 1] = mt_twist(dst[kMtM - 1], src[kMt - 1], dst[0]);
+        g.have_next = 1;
+    }
+    __syncthreads();
 }
 
-__device__ uint32_t mt_u32(MtLds& g) {
-    if (g.left == 0) mt_reload(g);
-    --g.left;
-    uint32_t y = g.s[g.next++];
+// Lane 0's cursor over the stream.
+struct MtCur {
+    MtLds* g;
+    int cur, next, have_next;
+    __device__ explicit MtCur(MtLds& m) : g(&m), cur(m.cur), next(m.next), have_next(m.have_next) {}
+    __device__ void close() {
+        g->cur = cur;
+        g->next = next;
+        g->have_next = have_next;
+    }
+};
+
+__device__ uint32_t mt_u32(MtCur& c) {
+    if (c.next == kMt) {
+        if (!c.have_next) {
+            // a section drew past the prepared block: one reload on this lane
+            const uint32_t* src = c.g->s[c.cur];
+            uint32_t* dst = c.g->s[c.cur ^ 1];
+            int i = 0;
+            for (; i < kMt - kMtM; ++i) dst[i] = mt_twist(src[i + kMtM], src[i], src[i + 1]);
+            for (; i < kMt - 1; ++i) dst[i] = mt_twist(dst[i + kMtM - kMt], src[i], src[i + 1]);
+            dst[kMt - 1] = mt_twist(dst[kMtM - 1], src[kMt - 1], dst[0]);
+        }
+        c.cur ^= 1;
+        c.next = 0;
+        c.have_next = 0;
+    }
+    uint32_t y = c.g->s[c.cur][c.next++];
     y ^= (y >> 11);
     y ^= (y << 7) & 0x9d2c5680u;
     y ^= (y << 15) & 0xefc60000u;
     return y ^ (y >> 18);
 }
 
-__device__ uint32_t mt_randint(MtLds& g, uint32_t n) {
+__device__ uint32_t mt_randint(MtCur& g, uint32_t n) {
     uint32_t used = n;
     used |= used >> 1;
     used |= used >> 2;
@@ -67,21 +109,29 @@ __device__ uint32_t mt_randint(MtLds& g, uint32_t n) {
     return i;
 }
 
-__device__ double mt_rand(MtLds& g) {
+__device__ double mt_rand(MtCur& g) {
 #pragma clang fp contract(off)
     return double(mt_u32(g)) * (1.0 / 4294967295.0);
 }
 
-__device__ double mt_randnorm(MtLds& g, double mean, double sd) {
+// MTRand::randNorm (include/MersenneTwister.h:349-363) split in two: lane 0 draws
+// the accepted polar pairs (x, r) of cnt calls in stream order; then every lane
+// forms x * sqrt(-2 log r / r), so a caller's mean + (x f) sd is
+// randNorm(mean, sd) to the bit.
+__device__ void draw_polar(MtLds& m, int cnt, double* gx, double* gr) {
 #pragma clang fp contract(off)
-    double x, y, r;
-    do {
-        x = 2.0 * mt_rand(g) - 1.0;
-        y = 2.0 * mt_rand(g) - 1.0;
-        r = x * x + y * y;
-    } while (r >= 1.0 || r == 0.0);
-    const double f = sqrt(-2.0 * log(r) / r);
-    return mean + x * f * sd;
+    MtCur g(m);
+    for (int c = 0; c < cnt; ++c) {
+        double x, y, r;
+        do {
+            x = 2.0 * mt_rand(g) - 1.0;
+            y = 2.0 * mt_rand(g) - 1.0;
+            r = x * x + y * y;
+        } while (r >= 1.0 || r == 0.0);
+        gx[c] = x;
+        gr[c] = r;
+    }
+    g.close();
 }
 
 __device__ __forceinline__ double pib(double r, double L) {
@@ -102,7 +152,11 @@ struct ChainLds {
     double seg[kChainMaxBeads + 2][3];
     unsigned char include[kChainMaxBeads];
     double cumrho[kChainMaxWind];
-    double red[4][2];
+    int c_old[kChainMaxBeads], c_new[kChainMaxBeads];
+    // Marsaglia polar pairs of up to kPolarBeads beads: x (then x f) and r
+    double gx[3 * kPolarBeads], gr[3 * kPolarBeads];
+    int lvl_cnt;
+    double red[kChainThreads / 64][2];
     int nb, ptcl, accept, done, near;
     double ao, an;
 };
@@ -121,36 +175,43 @@ template <int MODE>
 __device__ void delta_pass(const ChainRunArgs& A, const AzizDev& az, ChainLds& L) {
     const double inv_dr = 1.0 / az.dr;
     const bool use_cells = A.box.ngrid >= 4;
-    double to = 0.0, tn = 0.0;
-    for (int k = 0; k < L.nb; ++k) {
-        const int m = L.slice[k], me = L.ptcl;
-        const double* px = A.pos + (int64_t)m * A.nalloc;
-        const double* py = px + A.plane;
-        const double* pz = py + A.plane;
-        const int a = L.at[k];
-        const double ox = L.oldp[a][0], oy = L.oldp[a][1], oz = L.oldp[a][2];
-        const double nx = L.newp[a][0], ny = L.newp[a][1], nz = L.newp[a][2];
-        int c_old = 0, c_new = 0;
-        if (use_cells) {
-            c_old = cell_of(ox, oy, oz, A.box);
-            c_new = cell_of(nx, ny, nz, A.box);
+    const int N = A.nalloc, nb = L.nb, me = L.ptcl;
+    if (use_cells) {
+        for (int k = threadIdx.x; k < nb; k += kChainThreads) {
+            const int a = L.at[k];
+            L.c_old[k] = cell_of(L.oldp[a][0], L.oldp[a][1], L.oldp[a][2], A.box);
+            L.c_new[k] = cell_of(L.newp[a][0], L.newp[a][1], L.newp[a][2], A.box);
         }
-        double so = 0.0, sn = 0.0;
-        for (int j = threadIdx.x; j < A.nalloc; j += kChainThreads) {
-            if (j == me) continue;
-            const double xj = px[j], yj = py[j], zj = pz[j];
-            const int cj = use_cells ? A.cell[m * A.nalloc + j] : 0;
-            const double r2o = sep_r2<MODE>(ox, oy, oz, xj, yj, zj, A.box);
-            const double r2n = sep_r2<MODE>(nx, ny, nz, xj, yj, zj, A.box);
-            const bool oko = (r2o < A.box.rc2) && (!use_cells || cells_adjacent(c_old, cj, A.box.ngrid));
-            const bool okn = (r2n < A.box.rc2) && (!use_cells || cells_adjacent(c_new, cj, A.box.ngrid));
+        __syncthreads();
+    }
+    // (bead k, partner j) pairs over every lane: t = k N + j
+    double to = 0.0, tn = 0.0;
+    const int qk = kChainThreads / N, rj = kChainThreads % N;
+    int k = (int)threadIdx.x / N, j = (int)threadIdx.x % N;
+    for (int t = threadIdx.x; t < nb * N; t += kChainThreads) {
+        if (j != me) {
+            const int m = L.slice[k], a = L.at[k];
+            const double* px = A.pos + (int64_t)m * N;
+            const double xj = px[j], yj = px[j + A.plane], zj = px[j + 2 * A.plane];
+            const double r2o = sep_r2<MODE>(L.oldp[a][0], L.oldp[a][1], L.oldp[a][2], xj, yj, zj, A.box);
+            const double r2n = sep_r2<MODE>(L.newp[a][0], L.newp[a][1], L.newp[a][2], xj, yj, zj, A.box);
+            bool oko = r2o < A.box.rc2, okn = r2n < A.box.rc2;
+            if (use_cells) {
+                const int cj = A.cell[m * N + j];
+                oko = oko && cells_adjacent(L.c_old[k], cj, A.box.ngrid);
+                okn = okn && cells_adjacent(L.c_new[k], cj, A.box.ngrid);
+            }
             const double vo = pair_value_from_r2<MODE>(r2o, az, inv_dr);
             const double vn = pair_value_from_r2<MODE>(r2n, az, inv_dr);
-            so += oko ? vo : 0.0;
-            sn += okn ? vn : 0.0;
+            to += oko ? L.co[k] * vo : 0.0;
+            tn += okn ? L.co[k] * vn : 0.0;
         }
-        to += L.co[k] * so;
-        tn += L.co[k] * sn;
+        j += rj;
+        k += qk;
+        if (j >= N) {
+            j -= N;
+            ++k;
+        }
     }
     to = wave_sum(to);
     tn = wave_sum(tn);
@@ -161,8 +222,13 @@ __device__ void delta_pass(const ChainRunArgs& A, const AzizDev& az, ChainLds& L
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        L.ao = ((L.red[0][0] + L.red[1][0]) + L.red[2][0]) + L.red[3][0];
-        L.an = ((L.red[0][1] + L.red[1][1]) + L.red[2][1]) + L.red[3][1];
+        double so = 0.0, sn = 0.0;
+        for (int w = 0; w < kChainThreads / 64; ++w) {
+            so += L.red[w][0];
+            sn += L.red[w][1];
+        }
+        L.ao = so;
+        L.an = sn;
     }
     __syncthreads();
 }
@@ -184,7 +250,9 @@ __device__ void commit_list(const ChainRunArgs& A, ChainLds& L, int k0) {
 }
 
 __device__ bool metropolis(ChainLds& L, double arg) {
-    const double u = mt_rand(L.mt);
+    MtCur g(L.mt);
+    const double u = mt_rand(g);
+    g.close();
     const double e = exp(arg);
     if (e <= 1.0 && fabs(u - e) <= 1.0e-10 * e) ++L.near;   // u <= 1: only e <= 1 can be close
     return u < e;
@@ -203,13 +271,15 @@ template <int MODE>
 __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* acc, double* info) {
 #pragma clang fp contract(off)
     const int M = A.num_slices;
-    __shared__ int s_s0, s_s1, s_len;
-    __shared__ double s_vel[3];
+    __shared__ int s_s0, s_s1, s_len, s_w[3];
+    __shared__ double s_vel[3], s_pe[3];
     if (threadIdx.x == 0) {
+        MtCur g(L.mt);
         int stageLength = A.Mbar;
-        const int s0 = (int)mt_randint(L.mt, M - 1);
-        const int s1 = (int)mt_randint(L.mt, A.nalloc - 1);
-        if (A.var_updates) stageLength = 2 + (int)mt_randint(L.mt, A.Mbar - 2);
+        const int s0 = (int)mt_randint(g, M - 1);
+        const int s1 = (int)mt_randint(g, A.nalloc - 1);
+        if (A.var_updates) stageLength = 2 + (int)mt_randint(g, A.Mbar - 2);
+        g.close();
         const int e0 = (s0 + stageLength) % M;
         info[0] = 1.0;
         info[2] = stageLength;
@@ -242,12 +312,16 @@ __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, i
             L.cumrho[n] = L.cumrho[n - 1] + c;
         }
         for (int n = 0; n < A.num_wind; ++n) L.cumrho[n] /= total;
-        const double u = mt_rand(L.mt);
+        MtCur g(L.mt);
+        const double u = mt_rand(g);
+        g.close();
         int idx = 0;
         while (idx < A.num_wind && L.cumrho[idx] < u) ++idx;   // std::lower_bound
         if (idx == A.num_wind) idx = A.num_wind - 1;   // cumrho ends at 1 and u <= 1: not reached
-        int w[3];
-        for (int k = 0; k < 3; ++k) w[k] = A.wind[3 * idx + k];
+        for (int k = 0; k < 3; ++k) {
+            s_w[k] = A.wind[3 * idx + k];
+            s_pe[k] = pe[k];
+        }
         // the start bead and the stageLength - 1 moved beads
         L.nb = stageLength;
         L.ptcl = s1;
@@ -260,31 +334,46 @@ __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, i
             load_pos(A, m, s1, L.oldp[k]);
         }
         for (int d = 0; d < 3; ++d) L.newp[0][d] = L.oldp[0][d];
-        // MoveBase::newStagingPosition, winding form (src/move.cpp:306-346)
-        for (int k = 0; k < stageLength - 1; ++k) {
-            const double f1 = 1.0 * (stageLength - k - 1);
-            const double f2 = 1.0 / (1.0 * (stageLength - k));
-            const double sq = A.sqrt2LambdaTau * sqrt(f1 * f2);
-            const double* npv = L.newp[k];
-            double r[3];
-            for (int a = 0; a < 3; ++a) r[a] = (pe[a] + A.side * w[a]) - npv[a];
-            for (int a = 0; a < 3; ++a) r[a] *= f2;
-            for (int a = 0; a < 3; ++a) r[a] += npv[a];
-            for (int a = 0; a < 3; ++a) r[a] = mt_randnorm(L.mt, r[a], sq);
-            for (int a = 0; a < 3; ++a) {
-                while (r[a] < -0.5 * A.side) {
-                    w[a]++;
-                    r[a] += A.side;
-                }
-                while (r[a] >= 0.5 * A.side) {
-                    w[a]--;
-                    r[a] -= A.side;
-                }
-            }
-            for (int a = 0; a < 3; ++a) L.newp[k + 1][a] = pib(r[a], A.side);
-        }
     }
     __syncthreads();
+    // MoveBase::newStagingPosition, winding form (src/move.cpp:306-346): each
+    // bead starts from the previous bead's new position, so lane 0 composes
+    // them in order; the Gaussian deviates come from the split randNorm
+    const int nm = s_len - 1;
+    for (int kb = 0; kb < nm; kb += kPolarBeads) {
+        const int cnt = nm - kb < kPolarBeads ? nm - kb : kPolarBeads;
+        mt_refill(L.mt);
+        if (threadIdx.x == 0) draw_polar(L.mt, 3 * cnt, L.gx, L.gr);
+        __syncthreads();
+        for (int c = threadIdx.x; c < 3 * cnt; c += kChainThreads) L.gx[c] = L.gx[c] * sqrt(-2.0 * log(L.gr[c]) / L.gr[c]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int stageLength = s_len;
+            for (int k = kb; k < kb + cnt; ++k) {
+                const double f1 = 1.0 * (stageLength - k - 1);
+                const double f2 = 1.0 / (1.0 * (stageLength - k));
+                const double sq = A.sqrt2LambdaTau * sqrt(f1 * f2);
+                const double* npv = L.newp[k];
+                double r[3];
+                for (int a = 0; a < 3; ++a) r[a] = (s_pe[a] + A.side * s_w[a]) - npv[a];
+                for (int a = 0; a < 3; ++a) r[a] *= f2;
+                for (int a = 0; a < 3; ++a) r[a] += npv[a];
+                for (int a = 0; a < 3; ++a) r[a] = r[a] + L.gx[3 * (k - kb) + a] * sq;
+                for (int a = 0; a < 3; ++a) {
+                    while (r[a] < -0.5 * A.side) {
+                        s_w[a]++;
+                        r[a] += A.side;
+                    }
+                    while (r[a] >= 0.5 * A.side) {
+                        s_w[a]--;
+                        r[a] -= A.side;
+                    }
+                }
+                for (int a = 0; a < 3; ++a) L.newp[k + 1][a] = pib(r[a], A.side);
+            }
+        }
+        __syncthreads();
+    }
     delta_pass<MODE>(A, az, L);
     if (threadIdx.x == 0) {
         const double dlt = L.an - L.ao;
@@ -304,11 +393,13 @@ __device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L,
     __shared__ int s_levels, s_active, s_s0;
     __shared__ double s_old_delta;
     if (threadIdx.x == 0) {
+        MtCur g(L.mt);
         int numLevels = A.levels;
-        if (A.var_updates) numLevels = 1 + (int)mt_randint(L.mt, A.levels - 1);
+        if (A.var_updates) numLevels = 1 + (int)mt_randint(g, A.levels - 1);
         const int numActive = (1 << numLevels) - 1;
-        const int s0 = (int)mt_randint(L.mt, M - 1);
-        const int s1 = (int)mt_randint(L.mt, A.nalloc - 1);
+        const int s0 = (int)mt_randint(g, M - 1);
+        const int s1 = (int)mt_randint(g, A.nalloc - 1);
+        g.close();
         info[0] = 1.0;
         info[2] = numLevels;
         s_levels = numLevels;
@@ -325,25 +416,14 @@ __device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L,
         const int shift = 1 << (level - 1);
         if (threadIdx.x == 0) {
             const int s0 = s_s0, numActive = s_active;
-            const double delta = A.sqrtLambdaTau * sqrt(1.0 * shift);
             const double co_s = shift * A.tau;
             int nb = 0;
-            // this level's beads: proposals (src/move.cpp:467-488) in order
+            // this level's beads in order (src/move.cpp:1513-1535)
             for (int k = 1; k * shift < numActive + 1; ++k) {
                 const int n = k * shift - 1;
                 if (!L.include[n]) continue;
                 const int o = n + 1;
                 for (int d = 0; d < 3; ++d) L.oldp[n][d] = L.seg[o][d];
-                double r[3];
-                for (int d = 0; d < 3; ++d) r[d] = pib(L.seg[o + shift][d] - L.seg[o - shift][d], A.side);
-                for (int d = 0; d < 3; ++d) r[d] *= 0.5;
-                for (int d = 0; d < 3; ++d) r[d] += L.seg[o - shift][d];
-                for (int d = 0; d < 3; ++d) r[d] = mt_randnorm(L.mt, r[d], delta);
-                for (int d = 0; d < 3; ++d) r[d] = pib(r[d], A.side);
-                for (int d = 0; d < 3; ++d) {
-                    L.newp[n][d] = r[d];
-                    L.seg[o][d] = r[d];
-                }
                 L.include[n] = 0;
                 const int m = (s0 + o) % M;
                 L.at[nb] = n;
@@ -351,6 +431,7 @@ __device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L,
                 L.co[nb] = ((m & 1) ? A.vf1 : A.vf0) * co_s;
                 ++nb;
             }
+            L.lvl_cnt = nb;
             // at level 1 the earlier levels' beads join (newAction and oldAction over them)
             if (level == 1)
                 for (int k = 1; k * shift < numActive + 1; ++k) {
@@ -365,6 +446,36 @@ __device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L,
             L.nb = nb;
         }
         __syncthreads();
+        // their proposals (src/move.cpp:467-488): the deviates in stream order,
+        // then one lane per bead; a bead's neighbours at +-shift belong to
+        // earlier levels, so the beads of a level are independent
+        {
+            const double delta = A.sqrtLambdaTau * sqrt(1.0 * shift);
+            const int nl = L.lvl_cnt;
+            for (int kb = 0; kb < nl; kb += kPolarBeads) {
+                const int cnt = nl - kb < kPolarBeads ? nl - kb : kPolarBeads;
+                mt_refill(L.mt);
+                if (threadIdx.x == 0) draw_polar(L.mt, 3 * cnt, L.gx, L.gr);
+                __syncthreads();
+                for (int c = threadIdx.x; c < 3 * cnt; c += kChainThreads)
+                    L.gx[c] = L.gx[c] * sqrt(-2.0 * log(L.gr[c]) / L.gr[c]);
+                __syncthreads();
+                for (int t = threadIdx.x; t < cnt; t += kChainThreads) {
+                    const int n = L.at[kb + t], o = n + 1;
+                    double r[3];
+                    for (int d = 0; d < 3; ++d) r[d] = pib(L.seg[o + shift][d] - L.seg[o - shift][d], A.side);
+                    for (int d = 0; d < 3; ++d) r[d] *= 0.5;
+                    for (int d = 0; d < 3; ++d) r[d] += L.seg[o - shift][d];
+                    for (int d = 0; d < 3; ++d) r[d] = r[d] + L.gx[3 * t + d] * delta;
+                    for (int d = 0; d < 3; ++d) r[d] = pib(r[d], A.side);
+                    for (int d = 0; d < 3; ++d) {
+                        L.newp[n][d] = r[d];
+                        L.seg[o][d] = r[d];
+                    }
+                }
+                __syncthreads();
+            }
+        }
         delta_pass<MODE>(A, az, L);
         if (threadIdx.x == 0) {
             const double dlt = L.an - L.ao;
@@ -399,10 +510,12 @@ __device__ void com(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* 
     const int M = A.num_slices;
     __shared__ double sh[3];
     if (threadIdx.x == 0) {
-        const int p = (int)mt_randint(L.mt, A.nalloc - 1);
+        MtCur g(L.mt);
+        const int p = (int)mt_randint(g, A.nalloc - 1);
         info[0] = 1.0;
         info[2] = M;
-        for (int k = 0; k < 3; ++k) sh[k] = A.com_delta * (-0.5 + mt_rand(L.mt));
+        for (int k = 0; k < 3; ++k) sh[k] = A.com_delta * (-0.5 + mt_rand(g));
+        g.close();
         L.ptcl = p;
         L.nb = M;
     }
@@ -438,10 +551,11 @@ __global__ __launch_bounds__(kChainThreads) void k_chain_run(const ChainRunArgs*
     __shared__ ChainLds L;
     const ChainRunArgs A = args[blockIdx.x];
     const AzizDev az = azs[blockIdx.x];
-    for (int t = threadIdx.x; t < kMt; t += kChainThreads) L.mt.s[t] = A.mt[t];
+    for (int t = threadIdx.x; t < kMt; t += kChainThreads) L.mt.s[0][t] = A.mt[t];
     if (threadIdx.x == 0) {
-        L.mt.left = (int)A.mt[kMt];
+        L.mt.cur = 0;
         L.mt.next = (int)A.mt[kMt + 1];
+        L.mt.have_next = 0;
         L.near = 0;
     }
     __syncthreads();
@@ -449,6 +563,7 @@ __global__ __launch_bounds__(kChainThreads) void k_chain_run(const ChainRunArgs*
         const int kind = A.kinds[mv];
         int* acc = A.accepted + mv;
         double* info = A.info + 3 * mv;
+        mt_refill(L.mt);
         if (threadIdx.x == 0) {
             *acc = 0;
             L.accept = 0;
@@ -461,9 +576,9 @@ __global__ __launch_bounds__(kChainThreads) void k_chain_run(const ChainRunArgs*
             com<MODE>(A, az, L, acc, info);
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < kMt; t += kChainThreads) A.mt[t] = L.mt.s[t];
+    for (int t = threadIdx.x; t < kMt; t += kChainThreads) A.mt[t] = L.mt.s[L.mt.cur][t];
     if (threadIdx.x == 0) {
-        A.mt[kMt] = (uint32_t)L.mt.left;
+        A.mt[kMt] = (uint32_t)(kMt - L.mt.next);
         A.mt[kMt + 1] = (uint32_t)L.mt.next;
         A.near[0] = L.near;
     }

@@ -194,7 +194,7 @@ hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t pl
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,
                          int* blocks_per_slice);
 // Device-resident chain (pimc_dchain.hip): one workgroup runs n_moves moves.
-constexpr int kChainThreads = 256;
+constexpr int kChainThreads = 768;   // 12 waves: three per SIMD
 constexpr int kChainMaxBeads = 512;   // beads one move may list (M for CenterOfMass)
 constexpr int kChainMaxWind = 343;    // winding sectors, (2 max_wind + 1)^3 with max_wind <= 3
 struct ChainRunArgs {

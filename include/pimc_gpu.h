@@ -302,8 +302,11 @@ int  pimc_chain_positions(const pimc_chain* ch, double* beads);
  * log / sqrt / exp / pow may differ from the host's in the last ulp, so a
  * proposal can too; *near_threshold (may be NULL) counts the decisions whose
  * uniform deviate lay within a relative 1e-10 of the acceptance probability,
- * the only ones such a difference could flip.  Requires M <= 512 and an action
- * without the gradient correction (PIMC_E_UNSUPPORTED otherwise). */
+ * the only ones such a difference could flip.  With the gradient correction
+ * (pimc_set_action_factors) every shift-1 bead action adds gradVFactor tau^3
+ * lambda gradVnnSquared at the old and the new position, as pimc_delta_action
+ * does.  Requires M <= 512 (and N <= 8192 with the correction;
+ * PIMC_E_UNSUPPORTED otherwise). */
 int  pimc_chain_run_device(pimc_chain* ch, int n_moves, const int* kinds, int* accepted, double* info3,
                            int* near_threshold);
 /* The same for n_chains chains (each on its own context, one pair-term mode) in

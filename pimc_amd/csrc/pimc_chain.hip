@@ -332,8 +332,8 @@ int pimc_chain::check_device(int n_moves, const int* kinds) {
     if (M > pimc::kChainMaxBeads) return fail(PIMC_E_UNSUPPORTED, "device runs hold at most 512 slices");
     if ((int)cumrho0.size() > pimc::kChainMaxWind)
         return fail(PIMC_E_UNSUPPORTED, "device runs sample at most 343 windings");
-    if (ctx->correction())
-        return fail(PIMC_E_UNSUPPORTED, "device runs evaluate the action without the gradient correction");
+    if (ctx->correction() && N > pimc::kChainMaxList)
+        return fail(PIMC_E_UNSUPPORTED, "device runs with the gradient correction hold at most 8192 beads per slice");
     return PIMC_OK;
 }
 
@@ -382,6 +382,8 @@ int pimc_chain::device_args(int n_moves, const int* kinds, pimc::ChainRunArgs* A
     A->lambda = lambda;
     A->vf0 = ctx->vf_default[0];
     A->vf1 = ctx->vf_default[1];
+    A->gvf0 = ctx->correction() ? ctx->gvf[0] : 0.0;
+    A->gvf1 = ctx->correction() ? ctx->gvf[1] : 0.0;
     A->sqrtLambdaTau = sqrtLambdaTau;
     A->sqrt2LambdaTau = sqrt2LambdaTau;
     A->com_delta = d.com_delta;
@@ -431,6 +433,8 @@ static int run_chains_on_device(pimc_chain* const* chains, int n, int n_moves, c
         if (r) return r;
         if (chains[c]->ctx->desc.v_mode != chains[0]->ctx->desc.v_mode)
             return chains[c]->fail(PIMC_E_ARG, "chains of one run share the pair-term mode");
+        if (chains[c]->ctx->correction() != chains[0]->ctx->correction())
+            return chains[c]->fail(PIMC_E_ARG, "chains of one run all carry the gradient correction or none does");
         for (int e = 0; e < c; ++e)
             if (chains[e]->ctx == chains[c]->ctx) return chains[c]->fail(PIMC_E_ARG, "two chains on one context");
     }
@@ -459,7 +463,7 @@ static int run_chains_on_device(pimc_chain* const* chains, int n, int n_moves, c
     if (e == hipSuccess) e = hipMemcpy(d_azs, azs.data(), sizeof(pimc::AzizDev) * n, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         lead->ctx->timed_begin(8);
-        e = pimc::launch_chain_run(lead->ctx->desc.v_mode, st, n, d_args, d_azs);
+        e = pimc::launch_chain_run(lead->ctx->desc.v_mode, lead->ctx->correction(), st, n, d_args, d_azs);
         lead->ctx->timed_end();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(st);

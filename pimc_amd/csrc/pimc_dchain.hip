@@ -46,12 +46,12 @@ __device__ void mt_refill(MtLds& g) {
     if (g.have_next) return;
     const uint32_t* src = g.s[g.cur];
     uint32_t* dst = g.s[g.cur ^ 1];
-    for (int i = threadIdx.x; i < kMt - kMtM; i += kChainThreads) dst[i] = mt_twist(src[i + kMtM], src[i], src[i + 1]);
+    for (int i = threadIdx.x; i < kMt - kMtM; i += (int)blockDim.x) dst[i] = mt_twist(src[i + kMtM], src[i], src[i + 1]);
     __syncthreads();
-    for (int i = kMt - kMtM + threadIdx.x; i < 2 * (kMt - kMtM); i += kChainThreads)
+    for (int i = kMt - kMtM + threadIdx.x; i < 2 * (kMt - kMtM); i += (int)blockDim.x)
         dst[i] = mt_twist(dst[i + kMtM - kMt], src[i], src[i + 1]);
     __syncthreads();
-    for (int i = 2 * (kMt - kMtM) + threadIdx.x; i < kMt - 1; i += kChainThreads)
+    for (int i = 2 * (kMt - kMtM) + threadIdx.x; i < kMt - 1; i += (int)blockDim.x)
         dst[i] = mt_twist(dst[i + kMtM - kMt], src[i], src[i + 1]);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -153,10 +153,12 @@ struct ChainLds {
     unsigned char include[kChainMaxBeads];
     double cumrho[kChainMaxWind];
     int c_old[kChainMaxBeads], c_new[kChainMaxBeads];
+    uint32_t inlist[kChainMaxList / 32];   // the correction's interaction list, one bit per bead
+    double red4[kChainThreads / 64][4];
     // Marsaglia polar pairs of up to kPolarBeads beads: x (then x f) and r
     double gx[3 * kPolarBeads], gr[3 * kPolarBeads];
     int lvl_cnt;
-    double red[kChainThreads / 64][2];
+    double red[kChainThreads / 64][2];   // up to kChainThreads lanes
     int nb, ptcl, accept, done, near;
     double ao, an;
 };
@@ -177,7 +179,7 @@ __device__ void delta_pass(const ChainRunArgs& A, const AzizDev& az, ChainLds& L
     const bool use_cells = A.box.ngrid >= 4;
     const int N = A.nalloc, nb = L.nb, me = L.ptcl;
     if (use_cells) {
-        for (int k = threadIdx.x; k < nb; k += kChainThreads) {
+        for (int k = threadIdx.x; k < nb; k += (int)blockDim.x) {
             const int a = L.at[k];
             L.c_old[k] = cell_of(L.oldp[a][0], L.oldp[a][1], L.oldp[a][2], A.box);
             L.c_new[k] = cell_of(L.newp[a][0], L.newp[a][1], L.newp[a][2], A.box);
@@ -186,9 +188,10 @@ __device__ void delta_pass(const ChainRunArgs& A, const AzizDev& az, ChainLds& L
     }
     // (bead k, partner j) pairs over every lane: t = k N + j
     double to = 0.0, tn = 0.0;
-    const int qk = kChainThreads / N, rj = kChainThreads % N;
+    const int T = (int)blockDim.x;
+    const int qk = T / N, rj = T % N;
     int k = (int)threadIdx.x / N, j = (int)threadIdx.x % N;
-    for (int t = threadIdx.x; t < nb * N; t += kChainThreads) {
+    for (int t = threadIdx.x; t < nb * N; t += T) {
         if (j != me) {
             const int m = L.slice[k], a = L.at[k];
             const double* px = A.pos + (int64_t)m * N;
@@ -223,7 +226,7 @@ __device__ void delta_pass(const ChainRunArgs& A, const AzizDev& az, ChainLds& L
     __syncthreads();
     if (threadIdx.x == 0) {
         double so = 0.0, sn = 0.0;
-        for (int w = 0; w < kChainThreads / 64; ++w) {
+        for (int w = 0; w < (int)blockDim.x / 64; ++w) {
             so += L.red[w][0];
             sn += L.red[w][1];
         }
@@ -233,10 +236,124 @@ __device__ void delta_pass(const ChainRunArgs& A, const AzizDev& az, ChainLds& L
     __syncthreads();
 }
 
+// gradVnnSquared(bead1) (src/action.cpp:1282-1337, external gradV = 0) of the
+// bead (m, me) placed at p1: the interaction list (partners with r^2 < rc^2
+// in neighbouring cells, src/lookuptable.cpp:464-516) as a bit mask, then one
+// lane per bead2 with its sum over bead3; returns (on lane 0)
+// sum_2 [|F2|^2 + 2 F2.F3] + |F1|^2, F2 = gradV(r2 - r1), F1 = -sum F2.
+template <int MODE>
+__device__ double gnn_sum(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int m, int me, const double p1[3],
+                          int c1) {
+#pragma clang fp contract(off)
+    const double inv_dr = 1.0 / az.dr;
+    const bool use_cells = A.box.ngrid >= 4;
+    const int N = A.nalloc;
+    const double* px = A.pos + (int64_t)m * N;
+    const double* py = px + A.plane;
+    const double* pz = py + A.plane;
+    for (int w = threadIdx.x; w < (N + 31) / 32; w += (int)blockDim.x) L.inlist[w] = 0u;
+    __syncthreads();
+    for (int j = threadIdx.x; j < N; j += (int)blockDim.x) {
+        if (j == me) continue;
+        const double r2 = sep_r2<MODE>(px[j], py[j], pz[j], p1[0], p1[1], p1[2], A.box);
+        const bool in = (r2 < A.box.rc2) && (!use_cells || cells_adjacent(c1, A.cell[m * N + j], A.box.ngrid));
+        if (in) atomicOr(&L.inlist[j >> 5], 1u << (j & 31));
+    }
+    __syncthreads();
+    double term = 0.0, f1x = 0.0, f1y = 0.0, f1z = 0.0;
+    for (int b2 = threadIdx.x; b2 < N; b2 += (int)blockDim.x) {
+        if (!((L.inlist[b2 >> 5] >> (b2 & 31)) & 1u)) continue;
+        const double x2 = px[b2], y2 = py[b2], z2 = pz[b2];
+        double sx, sy, sz;
+        if (MODE == 0) {
+            sx = min_image_fast(x2 - p1[0], A.box.L, A.box.invL);
+            sy = min_image_fast(y2 - p1[1], A.box.L, A.box.invL);
+            sz = min_image_fast(z2 - p1[2], A.box.L, A.box.invL);
+        } else {
+            sx = min_image_exact(x2 - p1[0], A.box.L, A.box.invL);
+            sy = min_image_exact(y2 - p1[1], A.box.L, A.box.invL);
+            sz = min_image_exact(z2 - p1[2], A.box.L, A.box.invL);
+        }
+        const double g2 = grad_factor_from_r2<MODE>(norm2_exact(sx, sy, sz), az, inv_dr);
+        const double F2x = g2 * sx, F2y = g2 * sy, F2z = g2 * sz;
+        f1x -= F2x;
+        f1y -= F2y;
+        f1z -= F2z;
+        double F3x = 0.0, F3y = 0.0, F3z = 0.0;
+        for (int b3 = 0; b3 < N; ++b3) {
+            if (b3 == b2 || !((L.inlist[b3 >> 5] >> (b3 & 31)) & 1u)) continue;
+            double tx, ty, tz;
+            if (MODE == 0) {
+                tx = min_image_fast(x2 - px[b3], A.box.L, A.box.invL);
+                ty = min_image_fast(y2 - py[b3], A.box.L, A.box.invL);
+                tz = min_image_fast(z2 - pz[b3], A.box.L, A.box.invL);
+            } else {
+                tx = min_image_exact(x2 - px[b3], A.box.L, A.box.invL);
+                ty = min_image_exact(y2 - py[b3], A.box.L, A.box.invL);
+                tz = min_image_exact(z2 - pz[b3], A.box.L, A.box.invL);
+            }
+            const double g3 = grad_factor_from_r2<MODE>(norm2_exact(tx, ty, tz), az, inv_dr);
+            F3x += g3 * tx;
+            F3y += g3 * ty;
+            F3z += g3 * tz;
+        }
+        term += norm2_exact(F2x, F2y, F2z) + 2.0 * ((F2x * F3x + F2y * F3y) + F2z * F3z);
+    }
+    term = wave_sum(term);
+    f1x = wave_sum(f1x);
+    f1y = wave_sum(f1y);
+    f1z = wave_sum(f1z);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        L.red4[wave][0] = term;
+        L.red4[wave][1] = f1x;
+        L.red4[wave][2] = f1y;
+        L.red4[wave][3] = f1z;
+    }
+    __syncthreads();
+    double G = 0.0;
+    if (threadIdx.x == 0) {
+        double t = 0.0, fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int w = 0; w < (int)blockDim.x / 64; ++w) {
+            t += L.red4[w][0];
+            fx += L.red4[w][1];
+            fy += L.red4[w][2];
+            fz += L.red4[w][3];
+        }
+        G = t + norm2_exact(fx, fy, fz);
+    }
+    __syncthreads();
+    return G;
+}
+
+// The shift-1 bead actions' correction gradVFactor tau^3 lambda gradVnnSquared
+// (src/action.cpp:497-503) of every listed bead at its old and new position,
+// added to L.ao / L.an.
+template <int MODE>
+__device__ void correction_pass(const ChainRunArgs& A, const AzizDev& az, ChainLds& L) {
+    const double t3l = A.tau * A.tau * A.tau * A.lambda;
+    for (int k = 0; k < L.nb; ++k) {
+        const int m = L.slice[k];
+        const double gvf = (m & 1) ? A.gvf1 : A.gvf0;
+        if (!(gvf > kCorrEps)) continue;
+        const int a = L.at[k];
+        const bool use_cells = A.box.ngrid >= 4;
+        const int co = use_cells ? cell_of(L.oldp[a][0], L.oldp[a][1], L.oldp[a][2], A.box) : 0;
+        const int cn = use_cells ? cell_of(L.newp[a][0], L.newp[a][1], L.newp[a][2], A.box) : 0;
+        const double go = gnn_sum<MODE>(A, az, L, m, L.ptcl, L.oldp[a], co);
+        const double gn = gnn_sum<MODE>(A, az, L, m, L.ptcl, L.newp[a], cn);
+        if (threadIdx.x == 0) {
+            L.ao += gvf * t3l * go;
+            L.an += gvf * t3l * gn;
+        }
+    }
+    __syncthreads();
+}
+
 // Write list entries [k0, nb) to the path, with the cell-list update of
 // Path::updateBead; the beads sit on distinct slices, one lane each.
 __device__ void commit_list(const ChainRunArgs& A, ChainLds& L, int k0) {
-    for (int k = k0 + (int)threadIdx.x; k < L.nb; k += kChainThreads) {
+    for (int k = k0 + (int)threadIdx.x; k < L.nb; k += (int)blockDim.x) {
         const int m = L.slice[k], i = L.ptcl;
         const double* r = L.newp[L.at[k]];
         double* q = A.pos + (int64_t)m * A.nalloc + i;
@@ -267,7 +384,7 @@ __device__ double rho0(const ChainRunArgs& A, const double v[3], int Mlen) {
 }
 
 // StagingMove::attemptMove (src/move.cpp:1315-1408), lane 0 proposes.
-template <int MODE>
+template <int MODE, bool COR>
 __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* acc, double* info) {
 #pragma clang fp contract(off)
     const int M = A.num_slices;
@@ -294,7 +411,7 @@ __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, i
     __syncthreads();
     // MoveBase::sampleWindingSector (src/move.cpp:359-396): the sector weights
     // on the lanes, their running sum on lane 0 in the reference's order
-    for (int n = threadIdx.x; n < A.num_wind; n += kChainThreads) {
+    for (int n = threadIdx.x; n < A.num_wind; n += (int)blockDim.x) {
         double vw[3];
         for (int k = 0; k < 3; ++k) vw[k] = n == 0 ? s_vel[k] : s_vel[k] + A.wind[3 * n + k] * A.side;
         L.cumrho[n] = rho0(A, vw, s_len);
@@ -345,7 +462,7 @@ __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, i
         mt_refill(L.mt);
         if (threadIdx.x == 0) draw_polar(L.mt, 3 * cnt, L.gx, L.gr);
         __syncthreads();
-        for (int c = threadIdx.x; c < 3 * cnt; c += kChainThreads) L.gx[c] = L.gx[c] * sqrt(-2.0 * log(L.gr[c]) / L.gr[c]);
+        for (int c = threadIdx.x; c < 3 * cnt; c += (int)blockDim.x) L.gx[c] = L.gx[c] * sqrt(-2.0 * log(L.gr[c]) / L.gr[c]);
         __syncthreads();
         if (threadIdx.x == 0) {
             const int stageLength = s_len;
@@ -375,6 +492,7 @@ __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, i
         __syncthreads();
     }
     delta_pass<MODE>(A, az, L);
+    if (COR) correction_pass<MODE>(A, az, L);
     if (threadIdx.x == 0) {
         const double dlt = L.an - L.ao;
         info[1] = dlt;
@@ -386,7 +504,7 @@ __device__ void staging(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, i
 }
 
 // BisectionMove::attemptMove, local action (src/move.cpp:1484-1661).
-template <int MODE>
+template <int MODE, bool COR>
 __device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* acc, double* info) {
 #pragma clang fp contract(off)
     const int M = A.num_slices;
@@ -457,10 +575,10 @@ __device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L,
                 mt_refill(L.mt);
                 if (threadIdx.x == 0) draw_polar(L.mt, 3 * cnt, L.gx, L.gr);
                 __syncthreads();
-                for (int c = threadIdx.x; c < 3 * cnt; c += kChainThreads)
+                for (int c = threadIdx.x; c < 3 * cnt; c += (int)blockDim.x)
                     L.gx[c] = L.gx[c] * sqrt(-2.0 * log(L.gr[c]) / L.gr[c]);
                 __syncthreads();
-                for (int t = threadIdx.x; t < cnt; t += kChainThreads) {
+                for (int t = threadIdx.x; t < cnt; t += (int)blockDim.x) {
                     const int n = L.at[kb + t], o = n + 1;
                     double r[3];
                     for (int d = 0; d < 3; ++d) r[d] = pib(L.seg[o + shift][d] - L.seg[o - shift][d], A.side);
@@ -477,6 +595,7 @@ __device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L,
             }
         }
         delta_pass<MODE>(A, az, L);
+        if (COR && shift == 1) correction_pass<MODE>(A, az, L);
         if (threadIdx.x == 0) {
             const double dlt = L.an - L.ao;
             info[1] = dlt;
@@ -504,7 +623,7 @@ __device__ void bisection(const ChainRunArgs& A, const AzizDev& az, ChainLds& L,
 }
 
 // CenterOfMassMove::attemptMove for a closed worldline (src/move.cpp:1124-1259).
-template <int MODE>
+template <int MODE, bool COR>
 __device__ void com(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* acc, double* info) {
 #pragma clang fp contract(off)
     const int M = A.num_slices;
@@ -520,7 +639,7 @@ __device__ void com(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* 
         L.nb = M;
     }
     __syncthreads();
-    for (int m = threadIdx.x; m < M; m += kChainThreads) {
+    for (int m = threadIdx.x; m < M; m += (int)blockDim.x) {
         L.at[m] = m;
         L.slice[m] = m;
         L.co[m] = ((m & 1) ? A.vf1 : A.vf0) * A.tau;
@@ -529,6 +648,7 @@ __device__ void com(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* 
     }
     __syncthreads();
     delta_pass<MODE>(A, az, L);
+    if (COR) correction_pass<MODE>(A, az, L);
     if (threadIdx.x == 0) {
         const double dlt = L.an - L.ao;
         info[1] = dlt;
@@ -538,20 +658,22 @@ __device__ void com(const ChainRunArgs& A, const AzizDev& az, ChainLds& L, int* 
     __syncthreads();
     if (!L.accept) {
         // the reference shifts the worldline back (it does not restore saved positions)
-        for (int m = threadIdx.x; m < M; m += kChainThreads)
+        for (int m = threadIdx.x; m < M; m += (int)blockDim.x)
             for (int k = 0; k < 3; ++k) L.newp[m][k] = pib(L.newp[m][k] - sh[k], A.side);
         __syncthreads();
     }
     commit_list(A, L, 0);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kChainThreads) void k_chain_run(const ChainRunArgs* __restrict__ args,
-                                                             const AzizDev* __restrict__ azs) {
+// COR: the action carries the gradient correction; its three-body sums need
+// more registers, so that variant runs 512 lanes (two waves per SIMD).
+template <int MODE, bool COR>
+__global__ __launch_bounds__(COR ? kChainThreadsCor : kChainThreads) void k_chain_run(
+    const ChainRunArgs* __restrict__ args, const AzizDev* __restrict__ azs) {
     __shared__ ChainLds L;
     const ChainRunArgs A = args[blockIdx.x];
     const AzizDev az = azs[blockIdx.x];
-    for (int t = threadIdx.x; t < kMt; t += kChainThreads) L.mt.s[0][t] = A.mt[t];
+    for (int t = threadIdx.x; t < kMt; t += (int)blockDim.x) L.mt.s[0][t] = A.mt[t];
     if (threadIdx.x == 0) {
         L.mt.cur = 0;
         L.mt.next = (int)A.mt[kMt + 1];
@@ -569,14 +691,14 @@ __global__ __launch_bounds__(kChainThreads) void k_chain_run(const ChainRunArgs*
             L.accept = 0;
         }
         if (kind == 0)
-            staging<MODE>(A, az, L, acc, info);
+            staging<MODE, COR>(A, az, L, acc, info);
         else if (kind == 1)
-            bisection<MODE>(A, az, L, acc, info);
+            bisection<MODE, COR>(A, az, L, acc, info);
         else
-            com<MODE>(A, az, L, acc, info);
+            com<MODE, COR>(A, az, L, acc, info);
         __syncthreads();
     }
-    for (int t = threadIdx.x; t < kMt; t += kChainThreads) A.mt[t] = L.mt.s[L.mt.cur][t];
+    for (int t = threadIdx.x; t < kMt; t += (int)blockDim.x) A.mt[t] = L.mt.s[L.mt.cur][t];
     if (threadIdx.x == 0) {
         A.mt[kMt] = (uint32_t)(kMt - L.mt.next);
         A.mt[kMt + 1] = (uint32_t)L.mt.next;
@@ -586,14 +708,25 @@ __global__ __launch_bounds__(kChainThreads) void k_chain_run(const ChainRunArgs*
 
 }  // namespace
 
-hipError_t launch_chain_run(int mode, hipStream_t st, int n_chains, const ChainRunArgs* args, const AzizDev* azs) {
+hipError_t launch_chain_run(int mode, bool cor, hipStream_t st, int n_chains, const ChainRunArgs* args,
+                            const AzizDev* azs) {
     (void)hipGetLastError();
-    if (mode == 1)
-        hipLaunchKernelGGL(k_chain_run<1>, dim3(n_chains), dim3(kChainThreads), 0, st, args, azs);
-    else if (mode == 2)
-        hipLaunchKernelGGL(k_chain_run<2>, dim3(n_chains), dim3(kChainThreads), 0, st, args, azs);
-    else
-        hipLaunchKernelGGL(k_chain_run<0>, dim3(n_chains), dim3(kChainThreads), 0, st, args, azs);
+    const dim3 g(n_chains), b(cor ? kChainThreadsCor : kChainThreads);
+    if (cor) {
+        if (mode == 1)
+            hipLaunchKernelGGL((k_chain_run<1, true>), g, b, 0, st, args, azs);
+        else if (mode == 2)
+            hipLaunchKernelGGL((k_chain_run<2, true>), g, b, 0, st, args, azs);
+        else
+            hipLaunchKernelGGL((k_chain_run<0, true>), g, b, 0, st, args, azs);
+    } else {
+        if (mode == 1)
+            hipLaunchKernelGGL((k_chain_run<1, false>), g, b, 0, st, args, azs);
+        else if (mode == 2)
+            hipLaunchKernelGGL((k_chain_run<2, false>), g, b, 0, st, args, azs);
+        else
+            hipLaunchKernelGGL((k_chain_run<0, false>), g, b, 0, st, args, azs);
+    }
     return hipGetLastError();
 }
 

@@ -194,9 +194,11 @@ hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t pl
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,
                          int* blocks_per_slice);
 // Device-resident chain (pimc_dchain.hip): one workgroup runs n_moves moves.
-constexpr int kChainThreads = 768;   // 12 waves: three per SIMD
+constexpr int kChainThreads = 768;      // 12 waves: three per SIMD
+constexpr int kChainThreadsCor = 512;   // with the gradient correction: two per SIMD
 constexpr int kChainMaxBeads = 512;   // beads one move may list (M for CenterOfMass)
 constexpr int kChainMaxWind = 343;    // winding sectors, (2 max_wind + 1)^3 with max_wind <= 3
+constexpr int kChainMaxList = 8192;   // beads per slice the correction's list mask covers
 struct ChainRunArgs {
     double* pos;
     int64_t plane;
@@ -204,6 +206,7 @@ struct ChainRunArgs {
     int *cell, *label, *numlab, *hash;
     BoxDev box;
     double side, tau, lambda, vf0, vf1;
+    double gvf0, gvf1;   // gradVFactor of the GSF / Li-Broughton correction (0: none)
     double sqrtLambdaTau, sqrt2LambdaTau, com_delta;
     int Mbar, levels, var_updates, num_wind;
     const int* wind;     // [num_wind][3]
@@ -215,7 +218,8 @@ struct ChainRunArgs {
     int* near;           // decisions inside the guard band
 };
 // args / azs: device arrays, one entry per chain (workgroup).
-hipError_t launch_chain_run(int mode, hipStream_t st, int n_chains, const ChainRunArgs* args, const AzizDev* azs);
+hipError_t launch_chain_run(int mode, bool cor, hipStream_t st, int n_chains, const ChainRunArgs* args,
+                            const AzizDev* azs);
 hipError_t launch_virial(int mode, hipStream_t st, const VirialArgs& A, const AzizDev& az, int* blocks_per_slice);
 hipError_t launch_update_beads(hipStream_t st, const int* sp, const double* p3, int n, double* pos, int64_t plane,
                                int nalloc, int num_slices, int* cell, int* label, int* numlab, int* hash,

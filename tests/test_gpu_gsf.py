@@ -137,3 +137,34 @@ def test_bare_action_and_correction_parts(orc, gpu, mode, shift, name, action):
         total = ctx.action_list(sp, shift=1)
         assert abs(total - (bare.sum() + cor.sum())) <= 1e-12 * abs(total)
     assert ctx.potentialActionCorrection(sp, shift=shift) == pytest.approx(cor.sum(), rel=1e-14, abs=0)
+
+
+@pytest.mark.parametrize("mode,N,M,rc_frac", [(2, 16, 40, 1.0), (1, 24, 32, 1.0), (2, 64, 24, 0.25)])
+def test_gsf_device_resident_run(orc, gpu, mode, N, M, rc_frac):
+    """The device-resident chain under the GSF action: every shift-1 bead action
+    carries gradVFactor tau^3 lambda gradVnnSquared, evaluated on the device at
+    the old and new positions; decisions equal the oracle's replay, dU agrees to
+    1e-9 relative in the table mode and 1e-7 in the FP64 closed form (whose
+    per-pair values differ from the table's by ~1e-13, amplified where dU is a
+    small difference of large corrected bead actions; the FP32-transcendental
+    mode's bead actions are checked against the action scale in
+    test_bead_actions_and_delta), and the run continues the host stream."""
+    ref, ctx, _, _ = pair(orc, gpu, N, M, 0.01, rc_frac, mode, GSF)
+    seed, Mbar, levels, com_delta = 77, 8, 3, 0.5
+    moves = orc.Moves(ref, Mbar, levels, True, 1, com_delta)
+    rng = orc.MTRand(seed)
+    chain = gpu.Chain(ctx, seed, Mbar, levels, True, 1, com_delta)
+    kinds = [k % 3 for k in range(300)]
+    names = ("staging", "bisection", "com")
+    acc, info, near = chain.run_device(kinds)
+    tol = 1e-9 if mode == 2 else 1e-7
+    for k, kd in enumerate(kinds):
+        acc_r, info_r = getattr(moves, names[kd])(rng)
+        assert bool(acc_r) == bool(acc[k]), f"move {k} ({names[kd]}): oracle {acc_r} device {acc[k]}"
+        assert abs(info[k, 1] - info_r[1]) <= tol * (1.0 + abs(info_r[1])), (k, info[k], info_r)
+    for k in range(30):                               # the host continues the stream
+        acc_r, _ = getattr(moves, names[k % 3])(rng)
+        acc_g, _ = chain.move(names[k % 3])
+        assert bool(acc_r) == acc_g
+    print(f"GSF device run mode {mode} N={N}: {int(acc.sum())}/300 accepted, {near} near the threshold")
+    assert np.allclose(ctx.download_path(), ref.beads(), rtol=0, atol=1e-9)

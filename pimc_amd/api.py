@@ -69,6 +69,8 @@ SIGNATURES = {
     "pimc_total_action": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pimc_potential_slices": (C.c_int, [C.c_void_p, _dp, _ip]),
     "pimc_energy": (C.c_int, [C.c_void_p, _ip, _dp]),
+    "pimc_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "pimc_autotune": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "pimc_virial_terms": (C.c_int, [C.c_void_p, _ip, C.c_int, _dp]),
     "pimc_virial_energy": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_double, _dp]),
     "pimc_action_list": (C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp]),
@@ -201,6 +203,16 @@ class Context:
         """LocalAction VFactor, gradVFactor (GSF / Li-Broughton correction) and estimator period."""
         vf, gf = _f64(vfactor), _f64(gradvfactor)
         self._check(lib().pimc_set_action_factors(self.h, _p(vf, _dp), _p(gf, _dp), period), "set_action_factors")
+
+    def set_launch_shape(self, pairs_per_lane=0, grid_per_cu=0, stage_in_lds=-1):
+        self._check(lib().pimc_set_launch_shape(self.h, pairs_per_lane, grid_per_cu, stage_in_lds),
+                    "set_launch_shape")
+
+    def autotune(self, reps=5):
+        """Fastest launch shape of the all-pairs kernel: dict(pairs_per_lane, grid_per_cu, stage_in_lds, ms)."""
+        o = np.zeros(4)
+        self._check(lib().pimc_autotune(self.h, reps, _p(o, _dp)), "autotune")
+        return dict(pairs_per_lane=int(o[0]), grid_per_cu=int(o[1]), stage_in_lds=int(o[2]), ms=o[3])
 
     def set_v_mode(self, mode):
         self._check(lib().pimc_set_v_mode(self.h, mode), "set_v_mode")

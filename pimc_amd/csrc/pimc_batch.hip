@@ -385,7 +385,7 @@ int pimc_batch_info(const pimc_batch* b, int64_t* total_pairs, int* n_items, int
 int pimc_batch_autotune(pimc_batch* b, int reps, double* out4) {
     if (!b || reps < 1) return bfail(b, PIMC_E_ARG, "bad argument");
     const int ppls[] = {4, 8, 16, 32, 64, 128};
-    const int gmul[] = {1, 2, 4, 8};
+    const int gmul[] = {2, 4, 8, 16};
     const bool graph = b->use_graph;
     b->use_graph = true;
     std::vector<double> act(b->desc.size());

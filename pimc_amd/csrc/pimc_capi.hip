@@ -378,9 +378,11 @@ static void build_grid_nn(pimc_ctx* c) {
 // Pairs per lane of a tile: about two tiles per persistent workgroup (fine
 // enough for the pair-count balance of the ranges, few enough partials), within
 // [2, 1024].
+// About two tiles per workgroup, but never tiles under 16 pairs per lane: below
+// that a tile's fixed cost (descriptor, partial, barrier) outweighs its pairs.
 static int pairs_per_lane_for(int64_t total_pairs, int G) {
     const int64_t per_lane = total_pairs / ((int64_t)G * kPairThreads * 2);
-    return (int)std::max<int64_t>(2, std::min<int64_t>(1024, per_lane));
+    return (int)std::max<int64_t>(16, std::min<int64_t>(1024, per_lane));
 }
 
 static void free_dev(pimc_ctx* c, void* p) {
@@ -400,8 +402,9 @@ static int rebuild_work(pimc_ctx* c) {
         maxn = std::max(maxn, c->h_nbs[m]);
     }
     c->lds_beads = maxn <= kLdsBeads ? std::max(maxn, 1) : 0;
-    const int G0 = pair_grid_for(c->lds_beads, c->num_cus);
-    const int ppl = pairs_per_lane_for(total, G0);
+    c->pair_lds = c->tune_lds == 0 ? 0 : c->lds_beads;
+    const int G0 = c->tune_gmul > 0 ? c->tune_gmul * std::max(1, c->num_cus) : pair_grid_for(c->lds_beads, c->num_cus);
+    const int ppl = c->tune_ppl > 0 ? c->tune_ppl : pairs_per_lane_for(total, G0);
     std::vector<RowTile> tiles;
     for (int m = 0; m < M; ++m) {
         const int n = c->h_nbs[m];
@@ -739,7 +742,7 @@ int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* a
         const KernelTimer kt = c->kernel_timer(0);
         const unsigned long long seq = ++c->seq;
         HIP_OR_FAIL(c, launch_pair_action(c->desc.v_mode, c->stream, c->d_tiles, c->n_tiles, c->d_ranges, c->n_groups,
-                                          c->d_slices, M, c->d_walker, 1, c->d_pos, az, c->lds_beads, c->d_partials,
+                                          c->d_slices, M, c->d_walker, 1, c->d_pos, az, c->pair_lds, c->d_partials,
                                           c->d_vslice, c->d_map, &kt, true, c->d_done(), seq,
                                           cor ? c->d_g2part : nullptr, c->g2_blocks, c->d_g2slice));
         if (v_slices) {
@@ -959,6 +962,64 @@ int pimc_virial_energy(pimc_ctx* c, const int32_t* next_links, int window, doubl
     out19[17] = thermE;
     out19[18] = Pressure;
     return PIMC_OK;
+}
+
+int pimc_set_launch_shape(pimc_ctx* c, int pairs_per_lane, int grid_per_cu, int stage_in_lds) {
+    if (!c) return fail(c, PIMC_E_ARG, "null context");
+    if (pairs_per_lane < 0 || pairs_per_lane > 4096 || grid_per_cu < 0 || grid_per_cu > 64 || stage_in_lds < -1 ||
+        stage_in_lds > 1)
+        return fail(c, PIMC_E_ARG, "launch shape out of range");
+    c->tune_ppl = pairs_per_lane;
+    c->tune_gmul = grid_per_cu;
+    c->tune_lds = stage_in_lds;
+    return c->have_path ? rebuild_work(c) : PIMC_OK;
+}
+
+int pimc_autotune(pimc_ctx* c, int reps, double* out4) {
+    if (!c || reps < 1) return fail(c, PIMC_E_ARG, "bad argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    const int ppls[] = {0, 8, 16, 32, 64, 128};   // 0: the automatic choice
+    const int gmul[] = {2, 4, 8, 16};
+    const int ldss[] = {1, 0};
+    c->resolve_timing();
+    const bool timing = c->timing;
+    const double keep_ms = c->kernel_ms[0];
+    const long keep_n = c->kernel_count[0];
+    c->timing = true;
+    double best = 1e30, act = 0.0;
+    int best_p = 0, best_g = 0, best_l = -1, tried = 0, r;
+    for (int ppl : ppls)
+        for (int gm : gmul)
+            for (int l : ldss) {
+                if ((r = pimc_set_launch_shape(c, ppl, gm, l))) return r;
+                if ((r = pimc_total_action(c, 1, nullptr, &act, nullptr))) return r;   // warm
+                c->resolve_timing();
+                const double ms0 = c->kernel_ms[0];
+                const long n0 = c->kernel_count[0];
+                for (int k = 0; k < reps; ++k)
+                    if ((r = pimc_total_action(c, 1, nullptr, &act, nullptr))) return r;
+                c->resolve_timing();
+                const double avg = (c->kernel_ms[0] - ms0) / std::max<long>(1, c->kernel_count[0] - n0);
+                ++tried;
+                if (avg < best) {
+                    best = avg;
+                    best_p = ppl;
+                    best_g = gm;
+                    best_l = l;
+                }
+            }
+    c->timing = timing;
+    c->kernel_ms[0] = keep_ms;   // the tuning launches stay out of the kernel totals
+    c->kernel_count[0] = keep_n;
+    r = pimc_set_launch_shape(c, best_p, best_g, best_l);
+    if (out4) {
+        out4[0] = best_p;
+        out4[1] = best_g;
+        out4[2] = best_l;
+        out4[3] = best;
+    }
+    (void)tried;
+    return r;
 }
 
 int pimc_energy(pimc_ctx* c, const int32_t* next_links, double* out8) {

@@ -67,6 +67,9 @@ struct pimc_ctx {
     unsigned long long* d_done() { return reinterpret_cast<unsigned long long*>(d_map + kDoneSlot); }
     int wait_done(unsigned long long s);   // spin, then fall back to a stream sync
     int lds_beads = 0;            // slice positions staged in LDS (0: read from HBM)
+    int pair_lds = 0;             // the pair kernel's staging (lds_beads, or 0 when tuned off)
+    // launch shape of the pair kernel (pimc_set_launch_shape / pimc_autotune; 0 / -1: automatic)
+    int tune_ppl = 0, tune_gmul = 0, tune_lds = -1;
     // action factors (LocalAction VFactor, gradVFactor, estimator period); the
     // per-call vfactor argument overrides VFactor when given
     double vf_default[2] = {1.0, 1.0};

@@ -44,6 +44,23 @@ def test_total_action_n16_m125_seeds(orc, gpu, mode):
     assert worst <= TOL[mode]
 
 
+def test_launch_shapes_and_autotune(orc, gpu):
+    """Every launch shape of the all-pairs kernel (tile size, workgroups per CU,
+    LDS staging or cache reads) gives the oracle's action in the table mode, and
+    the autotuner keeps one of them."""
+    ref, ctx, _, _ = setup_pair(orc, gpu, 100, 24, 0.004, mode=2)
+    b = ref.total_action()
+    for shape in ((16, 2, 1), (16, 16, 0), (64, 4, 1), (1, 8, 0), (0, 0, -1)):
+        ctx.set_launch_shape(*shape)
+        a = ctx.potentialAction()
+        assert abs(a - b) <= TOL[2] * abs(b), (shape, a, b)
+    t = ctx.autotune(2)
+    assert t["grid_per_cu"] in (2, 4, 8, 16) and t["stage_in_lds"] in (0, 1) and t["ms"] > 0
+    assert abs(ctx.potentialAction() - b) <= TOL[2] * abs(b)
+    with pytest.raises(gpu.PimcError):
+        ctx.set_launch_shape(16, 100, 1)
+
+
 @pytest.mark.parametrize("mode", [0, 2])
 def test_total_action_and_slices_n256_m167(orc, gpu, mode):
     """Single walker N=256, M=167, tau=0.004."""

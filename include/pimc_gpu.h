@@ -275,9 +275,11 @@ typedef struct {
     uint32_t seed;
     int threads;        /* client threads submitting (chains split among them); 0 = 1 */
 } pimc_load_desc;
-/* stats10 = {completed, req/s, pair evals/s, p50_us, p99_us, mean_us, accept frac,
- *            batches, requests per batch, seconds}; latency counts from arrival. */
-int  pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* load, double* stats10);
+/* stats12 = {completed, req/s, pair evals/s, p50_us, p99_us, mean_us, accept frac,
+ *            batches, requests per batch, seconds, service p50_us, service p99_us};
+ * latency counts from arrival, service from submission to the worker retiring
+ * the result. */
+int  pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* load, double* stats12);
 
 /* ---- one Markov chain: the diagonal moves over a resident path ---------------- */
 /* StagingMove / BisectionMove / CenterOfMassMove::attemptMove (src/move.cpp:

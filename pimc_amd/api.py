@@ -556,10 +556,10 @@ class Server:
     def run_synthetic(self, n_chains, rate, duration_s, seg=(4, 64), p_com=0.1, com_sigma=0.1, lam=6.0596,
                       seed=139853, threads=1):
         d = _LoadDesc(n_chains, rate, duration_s, seg[0], seg[1], p_com, com_sigma, lam, seed, threads)
-        o = np.zeros(10)
+        o = np.zeros(12)
         self._check(lib().pimc_server_run_synthetic(self.h, C.byref(d), _p(o, _dp)), "run_synthetic")
         keys = ["completed", "req_per_s", "pair_evals_per_s", "p50_us", "p99_us", "mean_us", "accept_frac",
-                "batches", "req_per_batch", "seconds"]
+                "batches", "req_per_batch", "seconds", "service_p50_us", "service_p99_us"]
         return dict(zip(keys, o))
 
 

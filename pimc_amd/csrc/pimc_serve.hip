@@ -122,6 +122,13 @@ __global__ __launch_bounds__(kVnnThreads) void k_serve_vnn(const ServeGroupDev* 
 
 // One wave per request: per-bead sums (chunks in order), the actions and dU
 // (fixed shuffle tree), the Metropolis decision, the commit, the result.
+// Publishes a batch's sequence number once every kernel before it on the
+// stream has finished (stream order), so the worker reads a host cache line
+// instead of querying an event.
+__global__ void k_serve_publish(unsigned long long* done, unsigned long long seq) {
+    if (threadIdx.x == 0) publish_done(done, seq);
+}
+
 __global__ __launch_bounds__(64) void k_serve_finalize(const ServeReqDev* __restrict__ reqs,
                                                        const VnnBead* __restrict__ beads,
                                                        const ServeChainDev* __restrict__ chains,
@@ -177,6 +184,7 @@ __global__ void k_serve_cells(const double* __restrict__ pos, int64_t plane, int
 
 using namespace pimc;
 using Clock = std::chrono::steady_clock;
+constexpr size_t kZeroCopyBytes = 16384;   // batches read in place from mapped host memory up to this size
 
 namespace {
 
@@ -253,11 +261,16 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     bool busy = false;
-    char* h_desc = nullptr;           // pinned: reqs | beads | groups
+    char* h_desc = nullptr;           // pinned, mapped: reqs | beads | groups
+    char* m_desc = nullptr;           // the device's address of h_desc (read in place)
     char* d_desc = nullptr;
     double* d_partials = nullptr;
     double* h_out = nullptr;          // mapped pinned results [kMaxReqPerBatch][4]
     double* d_out = nullptr;
+    unsigned long long* h_done = nullptr;   // mapped completion word: the last finished batch
+    unsigned long long* d_done = nullptr;
+    unsigned long long seq = 0;             // the batch in flight
+    Clock::time_point t_fire{};
     std::vector<uint64_t> reqs;
     size_t desc_bytes = 0;
 };
@@ -293,6 +306,7 @@ struct pimc_server {
     int n_slots = 2;
     std::thread worker;
     std::atomic<bool> stop{false};
+    std::atomic<uint64_t> work_events{0};   // bumped on every submit / state change the worker must see
     std::string err;
     int64_t launched_batches = 0, launched_reqs = 0;
     int64_t pairs_done = 0;
@@ -500,6 +514,8 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
     }
     L->chains = d_chains;
     S.busy = true;
+    S.seq += 1;   // the sequence number this batch publishes when it completes
+    S.t_fire = Clock::now();
     ++launched_batches;
     launched_reqs += (int64_t)rq.size();
     return 1;
@@ -511,10 +527,18 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
 hipError_t pimc_server::fire_batch(const BatchLaunch& L) {
     Slot& S = slots[L.slot];
     (void)hipGetLastError();
-    hipError_t e = hipMemcpyAsync(S.d_desc, S.h_desc, L.bytes, hipMemcpyHostToDevice, S.stream);
-    const ServeReqDev* d_rq = (const ServeReqDev*)S.d_desc;
-    const VnnBead* d_bd = (const VnnBead*)(S.d_desc + L.o_bd);
-    const ServeGroupDev* d_gr = (const ServeGroupDev*)(S.d_desc + L.o_gr);
+    // A small batch's descriptors are read in place from mapped host memory: a
+    // copy would put a DMA start in front of the kernels.  Larger batches go
+    // over in one copy, since every workgroup re-reads its descriptors.
+    hipError_t e = hipSuccess;
+    const char* base = S.m_desc;
+    if (L.bytes > kZeroCopyBytes || !base) {
+        e = hipMemcpyAsync(S.d_desc, S.h_desc, L.bytes, hipMemcpyHostToDevice, S.stream);
+        base = S.d_desc;
+    }
+    const ServeReqDev* d_rq = (const ServeReqDev*)base;
+    const VnnBead* d_bd = (const VnnBead*)(base + L.o_bd);
+    const ServeGroupDev* d_gr = (const ServeGroupDev*)(base + L.o_gr);
     if (e == hipSuccess) {
         dim3 g((unsigned)L.n_groups), b(kVnnThreads);
         if (desc.v_mode == 0)
@@ -528,9 +552,9 @@ hipError_t pimc_server::fire_batch(const BatchLaunch& L) {
                                S.d_partials);
         hipLaunchKernelGGL(k_serve_finalize, dim3((unsigned)L.n_reqs), dim3(64), 0, S.stream, d_rq, d_bd, L.chains,
                            S.d_partials, S.d_out);
+        hipLaunchKernelGGL(k_serve_publish, dim3(1), dim3(64), 0, S.stream, S.d_done, S.seq);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipEventRecord(S.ev, S.stream);
     return e;
 }
 
@@ -561,12 +585,19 @@ void pimc_server::run() {
         int nd = 0;
         bool any_busy = false;
         for (int k = 0; k < n_slots; ++k) {
-            if (!slots[k].busy) continue;
+            Slot& S = slots[k];
+            if (!S.busy) continue;
             any_busy = true;
-            const hipError_t q = hipEventQuery(slots[k].ev);
-            if (q != hipErrorNotReady) {
+            if (__atomic_load_n(S.h_done, __ATOMIC_ACQUIRE) == S.seq) {
                 done_k[nd] = k;
-                done_e[nd++] = q;
+                done_e[nd++] = hipSuccess;
+            } else if (Clock::now() - S.t_fire > std::chrono::milliseconds(200)) {
+                // a batch this late may have failed: ask the stream
+                const hipError_t q = hipStreamQuery(S.stream);
+                if (q != hipErrorNotReady && __atomic_load_n(S.h_done, __ATOMIC_ACQUIRE) != S.seq) {
+                    done_k[nd] = k;
+                    done_e[nd++] = q == hipSuccess ? hipErrorUnknown : q;
+                }
             }
         }
         BatchLaunch L[kSlots];
@@ -588,10 +619,30 @@ void pimc_server::run() {
                 ++nl;
             }
         if (nl == 0 && nd == 0) {
-            if (!any_busy)
-                cv_work.wait_for(lk, std::chrono::milliseconds(1));
+            if (!any_busy) {
+                // idle: spin on the submit counter for up to 2 ms (a sleeping
+                // worker costs every request a thread wake-up), then block
+                const uint64_t seen = work_events.load(std::memory_order_acquire);
+                lk.unlock();
+                const auto t0 = Clock::now();
+                bool woke = false;
+                while (!stop.load(std::memory_order_relaxed) && Clock::now() - t0 < std::chrono::milliseconds(2)) {
+                    if (work_events.load(std::memory_order_acquire) != seen) {
+                        woke = true;
+                        break;
+                    }
+                    for (int p = 0; p < 64; ++p) __builtin_ia32_pause();
+                }
+                if (!woke) {
+                    lk.lock();
+                    if (work_events.load(std::memory_order_acquire) == seen && !stop.load())
+                        cv_work.wait_for(lk, std::chrono::milliseconds(1));
+                    lk.unlock();
+                }
+                continue;
+            }
             lk.unlock();
-            if (any_busy) std::this_thread::yield();
+            std::this_thread::yield();
             continue;
         }
         lk.unlock();
@@ -630,13 +681,20 @@ int pimc_server_create(const pimc_server_desc* d, pimc_server** out) {
         if (hipEventCreateWithFlags(&S.ev, hipEventDisableTiming) != hipSuccess) return PIMC_E_HIP;
         S.desc_bytes = sizeof(ServeReqDev) * kMaxReqPerBatch + sizeof(VnnBead) * kMaxBeadsPerBatch +
                        sizeof(ServeGroupDev) * kMaxGroupsPerBatch + 1024;
-        if (hipHostMalloc((void**)&S.h_desc, S.desc_bytes, hipHostMallocDefault) != hipSuccess) return PIMC_E_NOMEM;
+        if (hipHostMalloc((void**)&S.h_desc, S.desc_bytes, hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess)
+            return PIMC_E_NOMEM;
+        if (hipHostGetDevicePointer((void**)&S.m_desc, S.h_desc, 0) != hipSuccess) return PIMC_E_HIP;
         if (hipMalloc(&S.d_desc, S.desc_bytes) != hipSuccess) return PIMC_E_NOMEM;
         if (hipMalloc(&S.d_partials, sizeof(double) * 2 * kMaxGroupsPerBatch) != hipSuccess) return PIMC_E_NOMEM;
         if (hipHostMalloc((void**)&S.h_out, sizeof(double) * 4 * kMaxReqPerBatch,
                           hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
             return PIMC_E_NOMEM;
         if (hipHostGetDevicePointer((void**)&S.d_out, S.h_out, 0) != hipSuccess) return PIMC_E_HIP;
+        if (hipHostMalloc((void**)&S.h_done, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+            return PIMC_E_NOMEM;
+        *S.h_done = 0;
+        if (hipHostGetDevicePointer((void**)&S.d_done, S.h_done, 0) != hipSuccess) return PIMC_E_HIP;
     }
     pimc_server* raw = s.get();
     raw->worker = std::thread([raw] { raw->run(); });
@@ -656,6 +714,7 @@ void pimc_server_destroy(pimc_server* s) {
         if (S.d_desc) (void)hipFree(S.d_desc);
         if (S.d_partials) (void)hipFree(S.d_partials);
         if (S.h_out) (void)hipHostFree(S.h_out);
+        if (S.h_done) (void)hipHostFree(S.h_done);
         if (S.stream) (void)hipStreamDestroy(S.stream);
     }
     if (s->d_chains) (void)hipFree(s->d_chains);
@@ -711,6 +770,7 @@ int pimc_server_remove_chain(pimc_server* s, int chain) {
     }
     c.state = 2;
     s->admit_waiting();
+    s->work_events.fetch_add(1, std::memory_order_release);
     s->cv_work.notify_all();
     return PIMC_OK;
 }
@@ -783,6 +843,7 @@ static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, c
     c.queue.push_back(id);
     if (c.queue.size() == 1 && c.state == 1 && !c.busy) s->ready.push_back(id);
     *req_id = id;
+    s->work_events.fetch_add(1, std::memory_order_release);
     s->cv_work.notify_one();
     return PIMC_OK;
 }
@@ -851,6 +912,7 @@ struct GenChain {
 struct GenStats {
     int64_t completed = 0, accepted = 0, pairs = 0;
     std::vector<double> lat;
+    std::vector<double> svc;   // submit -> result retired by the worker
     int err = 0;
 };
 
@@ -955,6 +1017,7 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
             if ((st.err = pimc_server_wait(s, id, o))) return;
             const auto t_done = Clock::now();
             st.lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
+            st.svc.push_back(o[4]);
             ++st.completed;
             const pimc_desc& d = s->chains[c].d;
             st.pairs += 2LL * ((int64_t)g.sp.size() / 2) * (d.num_alloc - 1);
@@ -1038,10 +1101,11 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
     const double secs = std::chrono::duration<double>(Clock::now() - t0).count();
     int64_t b1 = 0, r1 = 0;
     pimc_server_stats(s, &b1, &r1, nullptr, nullptr);
-    std::vector<double> lat;
+    std::vector<double> lat, svc;
     int64_t completed = 0, accepted = 0, pairs = 0;
     for (auto& x : st) {
         lat.insert(lat.end(), x.lat.begin(), x.lat.end());
+        svc.insert(svc.end(), x.svc.begin(), x.svc.end());
         completed += x.completed;
         accepted += x.accepted;
         pairs += x.pairs;
@@ -1061,5 +1125,9 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
     stats[7] = (double)(b1 - b0);
     stats[8] = (b1 > b0) ? (double)(r1 - r0) / (b1 - b0) : 0.0;
     stats[9] = secs;
+    std::sort(svc.begin(), svc.end());
+    auto spct = [&](double q) { return svc.empty() ? 0.0 : svc[std::min(svc.size() - 1, (size_t)(q * (svc.size() - 1)))]; };
+    stats[10] = spct(0.50);
+    stats[11] = spct(0.99);
     return PIMC_OK;
 }

@@ -169,6 +169,15 @@ def _f64(a, shape=None):
     return a if shape is None else a.reshape(shape)
 
 
+def _ptr(a):
+    """Address of a C-contiguous array's data: ctypes.addressof over the buffer
+    (about 1 us), or the ndarray.ctypes route for read-only arrays."""
+    try:
+        return C.addressof(C.c_char.from_buffer(a))
+    except (TypeError, ValueError, BufferError):
+        return a.ctypes.data
+
+
 def _i32(a, shape=None):
     a = np.ascontiguousarray(a, dtype=np.int32)
     return a if shape is None else a.reshape(shape)
@@ -186,6 +195,7 @@ class Context:
         self.h = h
         self.M, self.N, self.side = num_slices, num_alloc, side
         self._out = np.zeros(4)         # reused result buffer of the per-move calls
+        self._out_ptr = self._out.ctypes.data
 
     def close(self):
         if getattr(self, "h", None):
@@ -252,9 +262,8 @@ class Context:
         """vfactor None: the context's VFactor (set_action_factors; {1, 1} by default)."""
         vf = None if vfactor is None else _f64(vfactor)
         vs = np.zeros((self.M, 2)) if with_slices else None
-        self._check(lib().pimc_total_action(self.h, shift, None if vf is None else vf.ctypes.data,
-                                            self._out.ctypes.data, None if vs is None else vs.ctypes.data),
-                    "total_action")
+        self._check(lib().pimc_total_action(self.h, shift, None if vf is None else _ptr(vf), self._out_ptr,
+                                            None if vs is None else _ptr(vs)), "total_action")
         a = float(self._out[0])
         return (a, vs) if with_slices else a
 
@@ -330,12 +339,10 @@ class Context:
         if npos.shape[0] != sp.shape[0] or (opos is not None and opos.shape[0] != sp.shape[0]):
             raise PimcError("delta_action: bead list and positions differ in length")
         vf = None if vfactor is None else _f64(vfactor)
-        o = self._out
-        self._check(lib().pimc_delta_action(self.h, sp.ctypes.data, None if opos is None else opos.ctypes.data,
-                                            npos.ctypes.data, sp.shape[0], shift,
-                                            None if vf is None else vf.ctypes.data, int(bool(commit)),
-                                            o.ctypes.data), "delta_action")
-        return o[:3].copy()
+        self._check(lib().pimc_delta_action(self.h, _ptr(sp), None if opos is None else _ptr(opos), _ptr(npos),
+                                            sp.shape[0], shift, None if vf is None else _ptr(vf),
+                                            int(bool(commit)), self._out_ptr), "delta_action")
+        return self._out[:3].copy()
 
     def ssf(self, q):
         q = _f64(q, (-1, 3))

@@ -491,10 +491,18 @@ static int run_gradv2(pimc_ctx* c, const AzizDev& az) {
         if ((r = ctx_dalloc(c, &c->d_g2part, (size_t)M * B))) return r;
         if (!c->d_g2slice && (r = ctx_dalloc(c, &c->d_g2slice, (size_t)M))) return r;
     }
+    const int C = gradv2_chunks(N, M, c->num_cus);
+    if (C > 1 && c->cap_g2f < (size_t)M * C * N * 3) {
+        int r;
+        free_dev(c, c->d_g2f);
+        c->d_g2f = nullptr;
+        if ((r = ctx_dalloc(c, &c->d_g2f, (size_t)M * C * N * 3))) return r;
+        c->cap_g2f = (size_t)M * C * N * 3;
+    }
     int blocks = 0;
     c->timed_begin(5);
     HIP_OR_FAIL(c, launch_gradv2(c->desc.v_mode, c->stream, c->d_pos, c->plane, N, M, c->d_nbs, c->box, az,
-                                 c->lds_beads, c->d_g2part, &blocks));
+                                 c->lds_beads, c->d_g2part, &blocks, C, c->d_g2f));
     c->timed_end();
     c->g2_blocks = blocks;
     return PIMC_OK;
@@ -826,6 +834,18 @@ static int run_virial(pimc_ctx* c, const int32_t* next_links, int window, std::v
     A.lds_beads = N <= kLdsBeads ? N : 0;
     A.box = c->box;
     A.part = c->d_virial;
+    A.chunks = gradv2_chunks(N, M, c->num_cus);
+    A.fpart = nullptr;
+    if (A.chunks > 1) {
+        const size_t fneed = (size_t)M * A.chunks * N * 9;
+        if (c->cap_vf < fneed) {
+            free_dev(c, c->d_vf);
+            c->d_vf = nullptr;
+            if ((r = ctx_dalloc(c, &c->d_vf, fneed))) return r;
+            c->cap_vf = fneed;
+        }
+        A.fpart = c->d_vf;
+    }
     int blocks = 0;
     c->timed_begin(7);
     HIP_OR_FAIL(c, launch_virial(c->desc.v_mode, c->stream, A, az, &blocks));

@@ -95,7 +95,11 @@ struct pimc_ctx {
     double* h_req_pos = nullptr;
     double* d_per_bead = nullptr;
     double* h_per_bead = nullptr;
+    double* d_g2f = nullptr;            // gradVSquared partial forces [M][chunks][N][3]
+    size_t cap_g2f = 0;
     double* d_virial = nullptr;         // virial per-workgroup sums [M][blocks][7]
+    double* d_vf = nullptr;             // virial chunk partials [M][chunks][N][9]
+    size_t cap_vf = 0;
     size_t cap_virial = 0;
     double* d_per_bead_cor = nullptr;   // per-bead corrections [cap_req][2]
     double* h_per_bead_cor = nullptr;

@@ -84,6 +84,8 @@ struct VirialArgs {
     int lds_beads;       // slices with at most this many beads are staged in LDS
     BoxDev box;
     double* part;        // [M][blocks][kVirialTerms]
+    int chunks;          // partner chunks (1: one pass per row block)
+    double* fpart;       // chunk partials [M][chunks][nalloc][9] = F, T (chunks > 1)
 };
 
 // A Vnn evaluation request for one bead: partners are the active beads of its slice.
@@ -192,7 +194,9 @@ hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInlin
 // gradVSquared(slice) partials [M][blocks_per_slice] for every slice.
 hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,
-                         int* blocks_per_slice);
+                         int* blocks_per_slice, int chunks = 1, double* fpart = nullptr);
+// Partner chunks of the gradVSquared sum that fill the CUs (1: one pass).
+int gradv2_chunks(int nalloc, int num_slices, int num_cus);
 // Device-resident chain (pimc_dchain.hip): one workgroup runs n_moves moves.
 constexpr int kChainThreads = 768;      // 12 waves: three per SIMD
 constexpr int kChainThreadsCor = 512;   // with the gradient correction: two per SIMD

@@ -376,6 +376,92 @@ __global__ __launch_bounds__(256) void k_gradv2(const double* __restrict__ pos, 
     if (threadIdx.x == 0) part[(int64_t)m * gridDim.x + b] = s;
 }
 
+// The same sum split over C partner chunks when the slices alone leave CUs idle
+// (N = 256: one workgroup per slice): block (b C + c, m) takes rows 256 b ...
+// of slice m against partners [c Nc, (c+1) Nc) and writes the rows' partial
+// forces; k_gradv2_sum adds the C partials of each row in chunk order.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gradv2_chunk(const double* __restrict__ pos, int64_t plane, int nalloc,
+                                                      const int* __restrict__ nbs, BoxDev box, AzizDev az,
+                                                      int lds_beads, int C, double* __restrict__ fpart) {
+    extern __shared__ double lds[];
+    double* lx = lds;
+    double* ly = lds + lds_beads;
+    double* lz = lds + 2 * lds_beads;
+    const int m = blockIdx.y, b = blockIdx.x / C, c = blockIdx.x - b * C;
+    const int n = nbs[m];
+    const double* px = pos + (int64_t)m * nalloc;
+    const double* py = px + plane;
+    const double* pz = py + plane;
+    const bool in_lds = n <= lds_beads;
+    if (in_lds)
+        for (int t = threadIdx.x; t < n; t += 256) {
+            lx[t] = px[t];
+            ly[t] = py[t];
+            lz[t] = pz[t];
+        }
+    __syncthreads();
+    const double inv_dr = 1.0 / az.dr;
+    const int i = b * 256 + threadIdx.x;
+    const int nc = (n + C - 1) / C;
+    const int j0 = c * nc, j1 = min(n, j0 + nc);
+    if (i >= n) return;
+    const double* qx = in_lds ? lx : px;
+    const double* qy = in_lds ? ly : py;
+    const double* qz = in_lds ? lz : pz;
+    const double xi = qx[i], yi = qy[i], zi = qz[i];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        if (j == i) continue;
+        double dx, dy, dz;
+        if (MODE == 0) {
+            dx = min_image_fast(xi - qx[j], box.L, box.invL);
+            dy = min_image_fast(yi - qy[j], box.L, box.invL);
+            dz = min_image_fast(zi - qz[j], box.L, box.invL);
+        } else {
+            dx = min_image_exact(xi - qx[j], box.L, box.invL);
+            dy = min_image_exact(yi - qy[j], box.L, box.invL);
+            dz = min_image_exact(zi - qz[j], box.L, box.invL);
+        }
+        const double g = grad_factor_from_r2<MODE>(norm2_exact(dx, dy, dz), az, inv_dr);
+        fx += g * dx;
+        fy += g * dy;
+        fz += g * dz;
+    }
+    double* o = fpart + (((int64_t)m * C + c) * nalloc + i) * 3;
+    o[0] = fx;
+    o[1] = fy;
+    o[2] = fz;
+}
+
+__global__ __launch_bounds__(256) void k_gradv2_sum(const int* __restrict__ nbs, int nalloc, int C,
+                                                    const double* __restrict__ fpart, double* __restrict__ part) {
+    __shared__ double red[4];
+    const int m = blockIdx.y, b = blockIdx.x;
+    const int i = b * 256 + threadIdx.x;
+    double f2 = 0.0;
+    if (i < nbs[m]) {
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int c = 0; c < C; ++c) {
+            const double* o = fpart + (((int64_t)m * C + c) * nalloc + i) * 3;
+            fx += o[0];
+            fy += o[1];
+            fz += o[2];
+        }
+        f2 = norm2_exact(fx, fy, fz);
+    }
+    const double s = block_sum<256>(f2, red);
+    if (threadIdx.x == 0) part[(int64_t)m * gridDim.x + b] = s;
+}
+
+int gradv2_chunks(int nalloc, int num_slices, int num_cus) {
+    const int B = (nalloc + 255) / 256;
+    const int64_t blocks = (int64_t)B * num_slices;
+    const int64_t want = 4LL * std::max(1, num_cus);
+    if (blocks >= want) return 1;
+    return (int)std::min<int64_t>(16, (want + blocks - 1) / blocks);
+}
+
 __device__ __forceinline__ double group16_sum(double v) {
     v += __shfl_xor(v, 8, 16);
     v += __shfl_xor(v, 4, 16);
@@ -869,12 +955,26 @@ hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInlin
 
 hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,
-                         int* blocks_per_slice) {
+                         int* blocks_per_slice, int chunks, double* fpart) {
     (void)hipGetLastError();
     const int B = (nalloc + 255) / 256;
     *blocks_per_slice = B;
-    dim3 g(B, num_slices), b(256);
     const size_t lds = sizeof(double) * 3 * (size_t)lds_beads;
+    if (chunks > 1 && fpart) {
+        dim3 gc(B * chunks, num_slices), b(256);
+        if (mode == 0)
+            hipLaunchKernelGGL(k_gradv2_chunk<0>, gc, b, lds, st, pos, plane, nalloc, nbs, box, az, lds_beads, chunks,
+                               fpart);
+        else if (mode == 1)
+            hipLaunchKernelGGL(k_gradv2_chunk<1>, gc, b, lds, st, pos, plane, nalloc, nbs, box, az, lds_beads, chunks,
+                               fpart);
+        else
+            hipLaunchKernelGGL(k_gradv2_chunk<2>, gc, b, lds, st, pos, plane, nalloc, nbs, box, az, lds_beads, chunks,
+                               fpart);
+        hipLaunchKernelGGL(k_gradv2_sum, dim3(B, num_slices), b, 0, st, nbs, nalloc, chunks, fpart, part);
+        return hipGetLastError();
+    }
+    dim3 g(B, num_slices), b(256);
     if (mode == 0)
         hipLaunchKernelGGL(k_gradv2<0>, g, b, lds, st, pos, plane, nalloc, nbs, box, az, lds_beads, part);
     else if (mode == 1)

@@ -63,11 +63,11 @@ __device__ __forceinline__ void step_link(const int* links, int M, int nalloc, i
 // The sum over partners of one bead: F (3) and, when need_t, the symmetric T
 // (xx, xy, xz, yy, yz, zz).
 template <int MODE, typename P>
-__device__ __forceinline__ void bead_pairs(P px, P py, P pz, int n, int i, const BoxDev& box, const AzizDev& a,
-                                           bool need_t, double F[3], double T[6]) {
+__device__ __forceinline__ void bead_pairs(P px, P py, P pz, int j0, int j1, int i, const BoxDev& box,
+                                           const AzizDev& a, bool need_t, double F[3], double T[6]) {
 #pragma clang fp contract(off)
     const double xi = px[i], yi = py[i], zi = pz[i];
-    for (int j = 0; j < n; ++j) {
+    for (int j = j0; j < j1; ++j) {
         if (j == i) continue;
         const double s0 = min_image_exact(xi - px[j], box.L, box.invL);
         const double s1 = min_image_exact(yi - py[j], box.L, box.invL);
@@ -100,6 +100,70 @@ __device__ __forceinline__ void bead_pairs(P px, P py, P pz, int n, int i, const
     }
 }
 
+// The per-bead terms from the bead's F and T: (T F).r, the window centroid
+// offset delta and the link displacements, into o[kVirialTerms].
+__device__ void bead_finish(const VirialArgs& A, int m, int i, const double F[3], const double T[6], double* o) {
+#pragma clang fp contract(off)
+    // T F (apply_matrix_vector_product, include/common.h:187-199)
+    double TF[3];
+    TF[0] = ((0.0 + T[0] * F[0]) + T[1] * F[1]) + T[2] * F[2];
+    TF[1] = ((0.0 + T[1] * F[0]) + T[3] * F[1]) + T[4] * F[2];
+    TF[2] = ((0.0 + T[2] * F[0]) + T[4] * F[1]) + T[5] * F[2];
+    const PathView P{A.pos, A.plane, A.nalloc};
+    double r1[3];
+    P.at(m, i, r1);
+    // window centroid (src/action.cpp:1621-1645): running separations along
+    // next / prev links from the bead, the bead itself counted for gamma = 0
+    double more[3] = {0.0, 0.0, 0.0}, less[3] = {0.0, 0.0, 0.0}, com[3] = {0.0, 0.0, 0.0};
+    int nm = m, ni = i, pm = m, pi = i;
+    double nold[3] = {r1[0], r1[1], r1[2]}, pold[3] = {r1[0], r1[1], r1[2]};
+    for (int gamma = 0; gamma < A.window; ++gamma) {
+        if (gamma > 0) {
+            step_link(A.next, A.num_slices, A.nalloc, nm, ni, 1);
+            step_link(A.prev, A.num_slices, A.nalloc, pm, pi, -1);
+        }
+        double rn[3], rp[3], sn[3], sp[3];
+        P.at(nm, ni, rn);
+        P.at(pm, pi, rp);
+        sep_of(rn, nold, A.box, sn);
+        sep_of(rp, pold, A.box, sp);
+        for (int q = 0; q < 3; ++q) {
+            more[q] += sn[q];
+            less[q] += sp[q];
+            com[q] += (r1[q] + more[q]) + (r1[q] + less[q]);
+            nold[q] = rn[q];
+            pold[q] = rp[q];
+        }
+    }
+    double delta[3];
+    for (int q = 0; q < 3; ++q) {
+        com[q] /= (2.0 * A.window);
+        delta[q] = min_image_exact(r1[q] - com[q], A.box.L, A.box.invL);
+    }
+    // link displacements (src/estimator.cpp:1125-1146)
+    int km = m, ki = i;
+    double vel1[3] = {0.0, 0.0, 0.0}, vel2[3] = {0.0, 0.0, 0.0}, kold[3] = {r1[0], r1[1], r1[2]};
+    for (int gamma = 1; gamma <= A.window; ++gamma) {
+        step_link(A.next, A.num_slices, A.nalloc, km, ki, 1);
+        double rk[3], sk[3];
+        P.at(km, ki, rk);
+        sep_of(rk, kold, A.box, sk);
+        for (int q = 0; q < 3; ++q) {
+            vel1[q] += sk[q];
+            kold[q] = rk[q];
+        }
+        if (gamma == 1)
+            for (int q = 0; q < 3; ++q) vel2[q] = sk[q];
+    }
+    o[0] = dot3(F, r1);
+    o[1] = dot3(TF, r1);
+    o[2] = dot3(F, delta);
+    o[3] = dot3(TF, delta);
+    o[4] = dot3(F, F);
+    o[5] = -dot3(vel1, vel2);
+    o[6] = -dot3(vel2, vel2);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_virial(VirialArgs A, AzizDev az) {
     extern __shared__ double lds[];
@@ -128,67 +192,79 @@ __global__ __launch_bounds__(256) void k_virial(VirialArgs A, AzizDev az) {
         const bool need_t = (m & 1) ? A.t_odd : A.t_even;
         double F[3] = {0.0, 0.0, 0.0}, T[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         if (in_lds)
-            bead_pairs<MODE>(lx, ly, lz, n, i, A.box, az, need_t, F, T);
+            bead_pairs<MODE>(lx, ly, lz, 0, n, i, A.box, az, need_t, F, T);
         else
-            bead_pairs<MODE>(px, py, pz, n, i, A.box, az, need_t, F, T);
-        // T F (apply_matrix_vector_product, include/common.h:187-199)
-        double TF[3];
-        TF[0] = ((0.0 + T[0] * F[0]) + T[1] * F[1]) + T[2] * F[2];
-        TF[1] = ((0.0 + T[1] * F[0]) + T[3] * F[1]) + T[4] * F[2];
-        TF[2] = ((0.0 + T[2] * F[0]) + T[4] * F[1]) + T[5] * F[2];
-        const PathView P{A.pos, A.plane, A.nalloc};
-        double r1[3];
-        P.at(m, i, r1);
-        // window centroid (src/action.cpp:1621-1645): running separations along
-        // next / prev links from the bead, the bead itself counted for gamma = 0
-        double more[3] = {0.0, 0.0, 0.0}, less[3] = {0.0, 0.0, 0.0}, com[3] = {0.0, 0.0, 0.0};
-        int nm = m, ni = i, pm = m, pi = i;
-        double nold[3] = {r1[0], r1[1], r1[2]}, pold[3] = {r1[0], r1[1], r1[2]};
-        for (int gamma = 0; gamma < A.window; ++gamma) {
-            if (gamma > 0) {
-                step_link(A.next, A.num_slices, A.nalloc, nm, ni, 1);
-                step_link(A.prev, A.num_slices, A.nalloc, pm, pi, -1);
-            }
-            double rn[3], rp[3], sn[3], sp[3];
-            P.at(nm, ni, rn);
-            P.at(pm, pi, rp);
-            sep_of(rn, nold, A.box, sn);
-            sep_of(rp, pold, A.box, sp);
-            for (int q = 0; q < 3; ++q) {
-                more[q] += sn[q];
-                less[q] += sp[q];
-                com[q] += (r1[q] + more[q]) + (r1[q] + less[q]);
-                nold[q] = rn[q];
-                pold[q] = rp[q];
-            }
+            bead_pairs<MODE>(px, py, pz, 0, n, i, A.box, az, need_t, F, T);
+        bead_finish(A, m, i, F, T, o);
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < kVirialTerms; ++q) {
+        const double v = wave_sum(o[q]);
+        if (lane == 0) red[wave][q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kVirialTerms) {
+        const int q = threadIdx.x;
+        A.part[((int64_t)m * gridDim.x + b) * kVirialTerms + q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+    }
+}
+
+// Partner chunks when the slices alone leave CUs idle: block (b C + c, m)
+// writes the partial F and T of rows 256 b ... against partners [c Nc, (c+1) Nc).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_virial_chunk(VirialArgs A, AzizDev az) {
+    extern __shared__ double lds[];
+    const int C = A.chunks;
+    const int m = blockIdx.y, b = blockIdx.x / C, c = blockIdx.x - b * C;
+    const int n = A.nbs[m];
+    const double* px = A.pos + (int64_t)m * A.nalloc;
+    const double* py = px + A.plane;
+    const double* pz = py + A.plane;
+    const bool in_lds = n <= A.lds_beads;
+    double* lx = lds;
+    double* ly = lds + A.lds_beads;
+    double* lz = lds + 2 * A.lds_beads;
+    if (in_lds)
+        for (int t = threadIdx.x; t < n; t += 256) {
+            lx[t] = px[t];
+            ly[t] = py[t];
+            lz[t] = pz[t];
         }
-        double delta[3];
-        for (int q = 0; q < 3; ++q) {
-            com[q] /= (2.0 * A.window);
-            delta[q] = min_image_exact(r1[q] - com[q], A.box.L, A.box.invL);
+    __syncthreads();
+    const int i = b * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int nc = (n + C - 1) / C;
+    const int j0 = c * nc, j1 = min(n, j0 + nc);
+    const bool need_t = (m & 1) ? A.t_odd : A.t_even;
+    double F[3] = {0.0, 0.0, 0.0}, T[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (in_lds)
+        bead_pairs<MODE>(lx, ly, lz, j0, j1, i, A.box, az, need_t, F, T);
+    else
+        bead_pairs<MODE>(px, py, pz, j0, j1, i, A.box, az, need_t, F, T);
+    double* o = A.fpart + (((int64_t)m * C + c) * A.nalloc + i) * 9;
+    for (int q = 0; q < 3; ++q) o[q] = F[q];
+    for (int q = 0; q < 6; ++q) o[3 + q] = T[q];
+}
+
+// The chunks of each row summed in chunk order, then the per-bead terms.
+__global__ __launch_bounds__(256) void k_virial_sum(VirialArgs A) {
+    __shared__ double red[4][kVirialTerms];
+    const int C = A.chunks;
+    const int m = blockIdx.y, b = blockIdx.x;
+    const int n = A.nbs[m];
+    const int i = b * 256 + threadIdx.x;
+    double o[kVirialTerms];
+#pragma unroll
+    for (int q = 0; q < kVirialTerms; ++q) o[q] = 0.0;
+    if (i < n) {
+        double F[3] = {0.0, 0.0, 0.0}, T[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < C; ++c) {
+            const double* p = A.fpart + (((int64_t)m * C + c) * A.nalloc + i) * 9;
+            for (int q = 0; q < 3; ++q) F[q] += p[q];
+            for (int q = 0; q < 6; ++q) T[q] += p[3 + q];
         }
-        // link displacements (src/estimator.cpp:1125-1146)
-        int km = m, ki = i;
-        double vel1[3] = {0.0, 0.0, 0.0}, vel2[3] = {0.0, 0.0, 0.0}, kold[3] = {r1[0], r1[1], r1[2]};
-        for (int gamma = 1; gamma <= A.window; ++gamma) {
-            step_link(A.next, A.num_slices, A.nalloc, km, ki, 1);
-            double rk[3], sk[3];
-            P.at(km, ki, rk);
-            sep_of(rk, kold, A.box, sk);
-            for (int q = 0; q < 3; ++q) {
-                vel1[q] += sk[q];
-                kold[q] = rk[q];
-            }
-            if (gamma == 1)
-                for (int q = 0; q < 3; ++q) vel2[q] = sk[q];
-        }
-        o[0] = dot3(F, r1);
-        o[1] = dot3(TF, r1);
-        o[2] = dot3(F, delta);
-        o[3] = dot3(TF, delta);
-        o[4] = dot3(F, F);
-        o[5] = -dot3(vel1, vel2);
-        o[6] = -dot3(vel2, vel2);
+        bead_finish(A, m, i, F, T, o);
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
@@ -211,6 +287,15 @@ hipError_t launch_virial(int mode, hipStream_t st, const VirialArgs& A, const Az
     *blocks_per_slice = B;
     dim3 g(B, A.num_slices), b(256);
     const size_t lds = sizeof(double) * 3 * (size_t)A.lds_beads;
+    if (A.chunks > 1 && A.fpart) {
+        dim3 gc(B * A.chunks, A.num_slices);
+        if (mode == 2)
+            hipLaunchKernelGGL(k_virial_chunk<2>, gc, b, lds, st, A, az);
+        else
+            hipLaunchKernelGGL(k_virial_chunk<1>, gc, b, lds, st, A, az);
+        hipLaunchKernelGGL(k_virial_sum, g, b, 0, st, A);
+        return hipGetLastError();
+    }
     if (mode == 2)
         hipLaunchKernelGGL(k_virial<2>, g, b, lds, st, A, az);
     else

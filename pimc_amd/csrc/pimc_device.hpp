@@ -115,13 +115,19 @@ __device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
     ix2 = ix2 * fma(-x2, ix2, 2.0);                  // one Newton step: FP64 accurate
     const double ix6 = ix2 * ix2 * ix2;
     const double disp = ix6 * fma(fma(a.eC10d, ix2, a.eC8d), ix2, a.eC6d);
-    // F(x) = exp(-(D/x - 1)^2) for x < D, with D/x - 1 = (D - x) x / x^2
-    const double t = (a.D - x) * x * ix2;
-    const double F = (k < a.k_d) ? split_exp2(-1.4426950408889634074 * t * t) : 1.0;
-    // Below x = 0.01 (k < k_hc) valueV is the bare repulsion; there t >= D/0.01 - 1
-    // > 120, so exp2(-t^2 log2 e) underflows to exactly 0 in FP32, F = 0 and
-    // fma(-disp, 0, rep) == rep: the hard-core branch needs no select.
-    return fma(-disp, F, rep);
+    // F = 1 at and above D: rep - disp (== fma(-disp, 1, rep)); a wave whose
+    // pairs all lie beyond D skips the damping block
+    double v = rep - disp;
+    if (k < a.k_d) {
+        // F(x) = exp(-(D/x - 1)^2) for x < D, with D/x - 1 = (D - x) x / x^2
+        const double t = (a.D - x) * x * ix2;
+        const double F = split_exp2(-1.4426950408889634074 * t * t);
+        // Below x = 0.01 (k < k_hc) valueV is the bare repulsion; there t >= D/0.01 - 1
+        // > 120, so exp2(-t^2 log2 e) underflows to exactly 0 in FP32, F = 0 and
+        // fma(-disp, 0, rep) == rep: the hard-core branch needs no select.
+        v = fma(-disp, F, rep);
+    }
+    return v;
 }
 
 // valueV at the bin radius, FP64 arithmetic.  Requires k_eps <= k < table_length.

@@ -16,6 +16,7 @@
 // fixed-order wave/workgroup trees and a fixed-order final pass, so results do
 // not depend on how many other slices or walkers share the launch.
 #include <algorithm>
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include "pimc_kernels.hpp"
@@ -25,60 +26,159 @@ namespace pimc {
 // ---------------------------------------------------------------------------
 // All i<j pairs of each slice, row tiles (RowTile in pimc_kernels.hpp).
 // ---------------------------------------------------------------------------
+// A loop-invariant value held in a vector register, so that an instruction
+// with two loop-invariant operands reads at most one scalar register (the
+// constant bus of gfx950 takes one) and needs no copy per pair.
+template <typename T>
+__device__ __forceinline__ T in_vgpr(T x) {
+    asm("" : "+v"(x));
+    return x;
+}
+
+// Mixed-mode bin of one pair, the row bead's part precomputed.  The minimum
+// image in box units: u = x_j/L + (1/2 - x_i/L) = (x_j - x_i)/L + 1/2 and the
+// separation is L (fract(u) - 1/2), here scaled to bins by S = L/dr, so the bin
+// is the integer part of |w| (an FP32 root refined by one Newton step in FP64).
+// u carries an FP64 rounding of ~1e-16, |w| an error of ~1e-8 bin at L = 45 A:
+// the bin differs from the exact one only when |r|/dr lies that close to an
+// integer (the image choice itself can differ only at |x_j - x_i| = L/2, where
+// both images are equally far).
+struct RowMixed {
+    double cx, cy, cz;   // 1/2 - x_i/L per axis
+    double invL, S, Sh;  // 1/L, L/dr, -S/2
+};
+
+__device__ __forceinline__ int bin_mixed(const RowMixed& q, double xj, double yj, double zj) {
+    const double wx = fma(__builtin_amdgcn_fract(fma(xj, q.invL, q.cx)), q.S, q.Sh);
+    const double wy = fma(__builtin_amdgcn_fract(fma(yj, q.invL, q.cy)), q.S, q.Sh);
+    const double wz = fma(__builtin_amdgcn_fract(fma(zj, q.invL, q.cz)), q.S, q.Sh);
+    const double r2 = fma(wz, wz, fma(wy, wy, wx * wx));
+    const float r2f = (float)r2;
+    const float rs = __builtin_amdgcn_rsqf(r2f);   // ~1/|w|
+    const float r0 = r2f * rs;                     // ~|w|
+    const double r0d = (double)r0;
+    const double res = fma(-r0d, r0d, r2);         // r2 - r0^2
+    return (int)fma((double)(0.5f * rs), res, r0d);
+}
+
 // One lane: row bead i (held in registers) against partners j = (i + d) mod n for
-// d = d, d + G, ... < d_end (0 <= i < n, 1 <= d <= n/2).
+// d = d, d + G, ... < d_end (0 <= i < n, 1 <= d <= n/2), T iterations, the
+// count of the tile's first lane group (uniform, so the loop runs on a scalar
+// counter; a lane group with one offset fewer masks its last iteration).
+// PS > 0: the slice is staged in LDS as x | y | z planes PS doubles apart, the
+// partner walked by its byte offset, so one address register serves the three
+// reads; PS == 0: px, py, pz in global memory.
 // MASK: the offsets reach d = n/2 of an even n, whose pairs count once (rows
 // i < n/2): the lane skips d == skip_d.
-template <int MODE, bool MASK, typename P>
-__device__ __forceinline__ double row_loop(P px, P py, P pz, int n, int i, int d, int d_end, int G, const BoxDev& box,
-                                           const AzizDev& a, double inv_dr, bool factors, const WormDev& worm,
-                                           int slice) {
-    const double xi = px[i], yi = py[i], zi = pz[i];
-    int j = i + d;
-    if (j >= n) j -= n;
-    const int step = G % n;   // j + step < 2n, so one wrap per step
+template <int MODE, bool MASK, int PS>
+__device__ __forceinline__ double row_loop(const double* px, const double* py, const double* pz, int n, int i,
+                                           int d, int d_end, int T, int G, const BoxDev& box, const AzizDev& a,
+                                           double inv_dr, bool factors, const WormDev& worm, int slice) {
+    const double xi = PS ? px[i] : px[i];
+    const double yi = PS ? px[PS + i] : py[i];
+    const double zi = PS ? px[2 * PS + i] : pz[i];
+    int j0 = i + d;
+    if (j0 >= n) j0 -= n;
+    constexpr unsigned kW = PS ? 8u : 1u;   // units of the partner cursor: bytes (LDS) or beads
+    const unsigned step = in_vgpr((unsigned)(G % n) * kW);
+    const unsigned span = in_vgpr((unsigned)n * kW);
+    unsigned jc = (unsigned)j0 * kW;
     const int skip_d = (MASK && i >= (n >> 1)) ? (n >> 1) : -1;
-    double acc = 0.0;
-    for (; d < d_end; d += G) {
-        const double r2 = sep_r2<MODE>(xi, yi, zi, px[j], py[j], pz[j], box);
-        double v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+    RowMixed q;
+    if (MODE == 0) {
+        q.invL = box.invL;
+        q.cx = fma(-xi, box.invL, 0.5);
+        q.cy = fma(-yi, box.invL, 0.5);
+        q.cz = fma(-zi, box.invL, 0.5);
+        q.S = box.L * inv_dr;
+        q.Sh = in_vgpr(-0.5 * q.S);
+    }
+    const int lenm1 = in_vgpr(a.table_length - 1);
+    AzizDev av = a;
+    if (MODE == 0) {
+        av.rep_c1 = in_vgpr(a.rep_c1);
+        av.eC8d = in_vgpr(a.eC8d);
+    }
+    const char* lb = reinterpret_cast<const char*>(px);
+    auto pair = [&](unsigned c) -> double {
+        double xj, yj, zj;
+        if (PS) {
+            xj = *reinterpret_cast<const double*>(lb + c);
+            yj = *reinterpret_cast<const double*>(lb + c + 8 * PS);
+            zj = *reinterpret_cast<const double*>(lb + c + 16 * PS);
+        } else {
+            xj = px[c];
+            yj = py[c];
+            zj = pz[c];
+        }
+        double v;
+        if (MODE == 0) {
+            const int k = bin_mixed(q, xj, yj, zj);
+            // k_eps <= k < tableLength, else V = 0 (include/potential.h:252-257): the
+            // clamped bin keeps the arithmetic finite, and only a wave holding such a
+            // pair pays for the select
+            const int kk = min(max(k, a.k_eps), lenm1);
+            v = aziz_vmix(kk, av);
+            if (__builtin_amdgcn_ballot_w64(k != kk)) {
+                asm volatile("");
+                v = (k != kk) ? 0.0 : v;
+            }
+        } else {
+            const double r2 = sep_r2<MODE>(xi, yi, zi, xj, yj, zj, box);
+            v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+        }
         if (factors) {
+            const int j = (int)(c / kW);
             const int lo = i < j ? i : j, hi = i < j ? j : i;
             v *= pair_factor(bead_state(worm, slice, lo), bead_state(worm, slice, hi));
         }
+        return v;
+    };
+    auto advance = [&]() {
+        const unsigned jn = jc + step;
+        jc = min(jn, jn - span);   // unsigned: jn - span wraps above jn while jn < span
+    };
+    double acc = 0.0;
+    int dd = d;
+    for (int t = 1; t < T; ++t) {
+        const double v = pair(jc);
         if (MASK)
-            acc += (d == skip_d) ? 0.0 : v;
+            acc += (dd == skip_d) ? 0.0 : v;
         else
             acc += v;
-        j += step;
-        if (j >= n) j -= n;
+        dd += G;
+        advance();
     }
+    // the last offset: present for the lane groups whose d reaches it
+    const double v = pair(jc);
+    const bool ok = (d + (T - 1) * G < d_end) && !(MASK && (d + (T - 1) * G) == skip_d);
+    acc += ok ? v : 0.0;
     return acc;
 }
 
-template <int MODE, typename P>
-__device__ __forceinline__ double row_tile(P px, P py, P pz, int n, int i, int d, int d_end, int G, bool mask,
-                                           const BoxDev& box, const AzizDev& a, double inv_dr, bool factors,
-                                           const WormDev& worm, int slice) {
-    return mask ? row_loop<MODE, true>(px, py, pz, n, i, d, d_end, G, box, a, inv_dr, factors, worm, slice)
-                : row_loop<MODE, false>(px, py, pz, n, i, d, d_end, G, box, a, inv_dr, factors, worm, slice);
+template <int MODE, int PS>
+__device__ __forceinline__ double row_tile(const double* px, const double* py, const double* pz, int n, int i, int d,
+                                           int d_end, int T, int G, bool mask, const BoxDev& box, const AzizDev& a,
+                                           double inv_dr, bool factors, const WormDev& worm, int slice) {
+    return mask ? row_loop<MODE, true, PS>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, factors, worm, slice)
+                : row_loop<MODE, false, PS>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, factors, worm, slice);
 }
 
 // Persistent: the workgroups of one XCD take consecutive tile ranges (so a
 // slice is staged into one XCD's L2), each range a run of tiles grouped by
 // slice; a slice is restaged in LDS only when it changes.  One FP64 partial per
-// tile, whatever the grid.
-template <int MODE>
+// tile, whatever the grid.  PS: LDS plane stride in doubles (slices of up to PS
+// beads are staged; 0: no staging).
+template <int MODE, int PS>
 __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __restrict__ tiles,
                                                               const int* __restrict__ ranges,
                                                               const WalkerDesc* __restrict__ walkers,
                                                               const double* __restrict__ pos, AzizDev az,
-                                                              int lds_beads, double* __restrict__ partials) {
-    extern __shared__ double lds[];   // x | y | z of one slice, lds_beads each
-    __shared__ double red[kPairThreads / 64];
-    double* lx = lds;
-    double* ly = lds + lds_beads;
-    double* lz = lds + 2 * lds_beads;
+                                                              double* __restrict__ partials) {
+    // x | y | z of one slice, PS each, then the block sum's wave partials: a
+    // static array, so its address is a constant folded into the reads' offsets
+    __shared__ double lds[3 * PS + kPairThreads / 64];
+    double* red = lds + 3 * PS;
     const int r = xcd_contiguous_item(blockIdx.x, gridDim.x);
     const int k_begin = ranges[r], k_end = ranges[r + 1];
     if (k_begin >= k_end) return;
@@ -96,13 +196,13 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
         const double* px = pos + tl.pos;
         const double* py = px + tl.plane;
         const double* pz = py + tl.plane;
-        const bool in_lds = n <= lds_beads;
+        const bool in_lds = PS > 0 && n <= PS;
         if (in_lds && staged != tl.slice) {
             __syncthreads();   // the previous slice is no longer read
             for (int t = threadIdx.x; t < n; t += kPairThreads) {
-                lx[t] = px[t];
-                ly[t] = py[t];
-                lz[t] = pz[t];
+                lds[t] = px[t];
+                lds[PS + t] = py[t];
+                lds[2 * PS + t] = pz[t];
             }
             staged = tl.slice;
             __syncthreads();
@@ -115,12 +215,12 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
         if (grp < G && i < tl.i_end) {
             const bool factors = worm_touches_slice(wk.worm, tl.m);
             const int d = tl.d0 + grp, d_end = tl.d0 + tl.dn;
+            const int T = (tl.dn + G - 1) / G;
             const bool mask = ((n & 1) == 0) && d_end > (n >> 1) && tl.i_end > (n >> 1);
-            // two copies of the loop so the LDS copy compiles to ds_read (no flat addressing)
-            acc = in_lds ? row_tile<MODE>(lx, ly, lz, n, i, d, d_end, G, mask, wk.box, a, inv_dr, factors, wk.worm,
-                                          tl.m)
-                         : row_tile<MODE>(px, py, pz, n, i, d, d_end, G, mask, wk.box, a, inv_dr, factors, wk.worm,
-                                          tl.m);
+            acc = in_lds ? row_tile<MODE, PS>(lds, nullptr, nullptr, n, i, d, d_end, T, G, mask, wk.box, a, inv_dr,
+                                              factors, wk.worm, tl.m)
+                         : row_tile<MODE, 0>(px, py, pz, n, i, d, d_end, T, G, mask, wk.box, a, inv_dr, factors,
+                                             wk.worm, tl.m);
         }
         const double s = block_sum<kPairThreads>(acc, red);
         if (threadIdx.x == 0) partials[k] = s;
@@ -854,17 +954,32 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (n_tiles <= 0 || G <= 0) return hipSuccess;
     dim3 g(G), b(kPairThreads);
-    const size_t lds = sizeof(double) * 3 * (size_t)lds_beads;
-    auto go = [&](auto kern) {
-        hipLaunchKernelGGL(kern, g, b, lds, st, tiles, ranges, walkers, pos, az, lds_beads, partials);
+    // LDS plane stride: the smallest of 256 / 512 / 1024 / 2048 beads that holds
+    // the largest staged slice (a compile-time stride lets the three plane
+    // reads share one address)
+    const int PS = lds_beads <= 0 ? 0 : lds_beads <= 256 ? 256 : lds_beads <= 512 ? 512 : lds_beads <= 1024 ? 1024 : 2048;
+    if (lds_beads > kLdsBeads) return hipErrorInvalidValue;
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, b, 0, st, tiles, ranges, walkers, pos, az, partials); };
+    auto by_ps = [&](auto mode_tag) {
+        constexpr int MODE = decltype(mode_tag)::value;
+        if (PS == 0)
+            go(k_pair_action<MODE, 0>);
+        else if (PS == 256)
+            go(k_pair_action<MODE, 256>);
+        else if (PS == 512)
+            go(k_pair_action<MODE, 512>);
+        else if (PS == 1024)
+            go(k_pair_action<MODE, 1024>);
+        else
+            go(k_pair_action<MODE, 2048>);
     };
     if (timer && timer->start) (void)hipEventRecord(timer->start, st);
     if (mode == 0)
-        go(k_pair_action<0>);
+        by_ps(std::integral_constant<int, 0>{});
     else if (mode == 1)
-        go(k_pair_action<1>);
+        by_ps(std::integral_constant<int, 1>{});
     else
-        go(k_pair_action<2>);
+        by_ps(std::integral_constant<int, 2>{});
     if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
     if (finalize)
         hipLaunchKernelGGL(k_pair_finalize, dim3(1), dim3(kFinThreads), 0, st, n_tiles, slices, n_slices, walkers,

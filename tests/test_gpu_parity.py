@@ -86,6 +86,36 @@ def test_total_action_bridges_and_odd_sizes(orc, gpu):
         assert abs(a - b) <= 1e-6 * max(abs(b), 1e-12), (N, M, a, b)
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_total_action_degenerate_pairs(orc, gpu, mode):
+    """Pairs outside the table: two coincident beads (bin 0, V = 0) and a pair
+    half a box apart on every axis (|r| = maxSep, bin tableLength, V = 0),
+    include/potential.h:252-257."""
+    N, M = 16, 4
+    ref, ctx, beads, L = setup_pair(orc, gpu, N, M, 0.004, mode=mode)
+    beads = beads.copy()
+    beads[0, 1] = beads[0, 0]
+    beads[1, 2] = [-0.25 * L] * 3
+    beads[1, 3] = [0.25 * L] * 3
+    ref.set_beads(beads)
+    ctx.upload_path(beads)
+    a, vs = ctx.potentialAction(with_slices=True)
+    b = ref.total_action()
+    assert abs(a - b) <= TOL[mode] * abs(b), (a, b)
+    for m in range(M):
+        v_ref = ref.V_slice(m)[1]
+        assert abs(vs[m, 1] - v_ref) <= TOL[mode] * max(abs(v_ref), 1.0), (m, vs[m, 1], v_ref)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("N", [300, 700])
+def test_total_action_lds_strides(orc, gpu, mode, N):
+    """Slices staged with the 512- and 1024-bead LDS plane strides."""
+    ref, ctx, _, _ = setup_pair(orc, gpu, N, 3, 0.004, mode=mode)
+    a, b = ctx.potentialAction(), ref.total_action()
+    assert abs(a - b) <= TOL[mode] * abs(b), (N, a, b)
+
+
 def _bisection_case(orc, gpu, N, M, seg, seed, mode=0, rc_frac=1.0):
     ref, ctx, beads, L = setup_pair(orc, gpu, N, M, 0.004, rc_frac=rc_frac, seed=seed, mode=mode)
     rng = np.random.default_rng(seed)

@@ -151,6 +151,11 @@ AzizDev make_aziz_dev(const AzizParams& p, double max_sep) {
     a.eC6d = p.eps * p.C6;
     a.eC8d = p.eps * p.C8;
     a.eC10d = p.eps * p.C10;
+    const double s2 = a.s * a.s, s6 = s2 * s2 * s2;
+    a.eC6k = a.eC6d / s6;
+    a.eC8k = a.eC8d / (s6 * s2);
+    a.eC10k = a.eC10d / (s6 * s2 * s2);
+    a.Ds = p.D / a.s;
     a.eps = p.eps;
     a.A = p.A;
     a.alpha = p.alpha;

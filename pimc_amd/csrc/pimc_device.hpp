@@ -34,6 +34,8 @@ struct AzizDev {
     //   eps*A*exp(-alpha x + beta x^2) = 2^(c0 + c1 k + c2 k^2)
     double rep_c0, rep_c1, rep_c2;
     double eC6d, eC8d, eC10d;   // epsilon * C6, C8, C10
+    // the dispersion as a polynomial in 1/k^2: epsilon C_n / s^n, and D / s
+    double eC6k, eC8k, eC10k, Ds;
     // FP64 closed form
     double eps, A, alpha, beta, D, C6, C8, C10;
     double eps_rm;         // epsilon / rm (dV/dr = (epsilon/rm) dV/dx)
@@ -106,28 +108,45 @@ __device__ __forceinline__ double split_exp2(double e) {
     return (double)fast_exp2(eh) * fma(el, 0.69314718055994530942, 1.0);
 }
 
-__device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
-    const double kd = (double)k;
-    const double rep = split_exp2(fma(fma(a.rep_c2, kd, a.rep_c1), kd, a.rep_c0));
-    const double x = kd * a.s;
-    const double x2 = x * x;
-    double ix2 = (double)__builtin_amdgcn_rcpf((float)x2);
-    ix2 = ix2 * fma(-x2, ix2, 2.0);                  // one Newton step: FP64 accurate
-    const double ix6 = ix2 * ix2 * ix2;
-    const double disp = ix6 * fma(fma(a.eC10d, ix2, a.eC8d), ix2, a.eC6d);
+// The branch-free part (repulsion and dispersion) and the damping step apart,
+// so that a loop can run the first part of two pairs back to back.
+struct VmixHead {
+    double rep, disp, kd, iq;
+};
+
+__device__ __forceinline__ VmixHead aziz_vmix_head(int k, const AzizDev& a) {
+    VmixHead h;
+    h.kd = (double)k;
+    h.rep = split_exp2(fma(fma(a.rep_c2, h.kd, a.rep_c1), h.kd, a.rep_c0));
+    // 1/x^2 = s^-2 / k^2: the dispersion is a polynomial in 1/k^2 with the powers
+    // of s folded into its coefficients.  k^2 is exact in FP64; 1/k^2 is the
+    // hardware FP64 reciprocal estimate (relative error ~2^-23) refined by one
+    // Newton step (~2^-46).
+    const double q = h.kd * h.kd;
+    const double r0 = __builtin_amdgcn_rcp(q);
+    h.iq = fma(r0, fma(-q, r0, 1.0), r0);
+    h.disp = (h.iq * h.iq * h.iq) * fma(fma(a.eC10k, h.iq, a.eC8k), h.iq, a.eC6k);
+    return h;
+}
+
+__device__ __forceinline__ double aziz_vmix_damp(int k, const VmixHead& h, const AzizDev& a) {
     // F = 1 at and above D: rep - disp (== fma(-disp, 1, rep)); a wave whose
     // pairs all lie beyond D skips the damping block
-    double v = rep - disp;
+    double v = h.rep - h.disp;
     if (k < a.k_d) {
-        // F(x) = exp(-(D/x - 1)^2) for x < D, with D/x - 1 = (D - x) x / x^2
-        const double t = (a.D - x) * x * ix2;
+        // F(x) = exp(-(D/x - 1)^2) for x < D, with D/x - 1 = (D/s) k / k^2 - 1
+        const double t = fma(a.Ds * h.kd, h.iq, -1.0);
         const double F = split_exp2(-1.4426950408889634074 * t * t);
         // Below x = 0.01 (k < k_hc) valueV is the bare repulsion; there t >= D/0.01 - 1
         // > 120, so exp2(-t^2 log2 e) underflows to exactly 0 in FP32, F = 0 and
         // fma(-disp, 0, rep) == rep: the hard-core branch needs no select.
-        v = fma(-disp, F, rep);
+        v = fma(-h.disp, F, h.rep);
     }
     return v;
+}
+
+__device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
+    return aziz_vmix_damp(k, aziz_vmix_head(k, a), a);
 }
 
 // valueV at the bin radius, FP64 arithmetic.  Requires k_eps <= k < table_length.

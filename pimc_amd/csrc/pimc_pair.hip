@@ -53,32 +53,45 @@ __device__ __forceinline__ int bin_mixed(const RowMixed& q, double xj, double yj
     const double wy = fma(__builtin_amdgcn_fract(fma(yj, q.invL, q.cy)), q.S, q.Sh);
     const double wz = fma(__builtin_amdgcn_fract(fma(zj, q.invL, q.cz)), q.S, q.Sh);
     const double r2 = fma(wz, wz, fma(wy, wy, wx * wx));
-    const float r2f = (float)r2;
-    const float rs = __builtin_amdgcn_rsqf(r2f);   // ~1/|w|
-    const float r0 = r2f * rs;                     // ~|w|
-    const double r0d = (double)r0;
-    const double res = fma(-r0d, r0d, r2);         // r2 - r0^2
-    return (int)fma((double)(0.5f * rs), res, r0d);
+    // |w|: the hardware FP64 reciprocal-root estimate (relative error ~2^-23)
+    // and one Newton step, |w| = r0 + (r2 - r0^2) y / 2 (~2^-46)
+    const double y = __builtin_amdgcn_rsq(r2);
+    const double r0 = r2 * y;
+    const double res = fma(-r0, r0, r2);
+    return (int)fma(res * y, 0.5, r0);
 }
 
 // One lane: row bead i (held in registers) against partners j = (i + d) mod n for
 // d = d, d + G, ... < d_end (0 <= i < n, 1 <= d <= n/2), T iterations, the
 // count of the tile's first lane group (uniform, so the loop runs on a scalar
 // counter; a lane group with one offset fewer masks its last iteration).
-// PS > 0: the slice is staged in LDS as x | y | z planes PS doubles apart, the
+// PS > 0: the slice is staged in LDS as x | y | z planes PSL doubles apart, the
 // partner walked by its byte offset, so one address register serves the three
-// reads; PS == 0: px, py, pz in global memory.
+// reads; PS == 0: px, py, pz in global memory.  DUP: each LDS plane holds the
+// slice followed by its first n/2 + 1 beads again, so j = i + d < 3n/2 needs no
+// wrap; otherwise the cursor wraps with an unsigned min.
 // MASK: the offsets reach d = n/2 of an even n, whose pairs count once (rows
 // i < n/2): the lane skips d == skip_d.
+template <int PS>
+constexpr bool pair_lds_dup() {
+    return PS > 0 && PS <= 512;
+}
+template <int PS>
+constexpr int pair_lds_plane() {
+    return pair_lds_dup<PS>() ? 2 * PS : PS;
+}
+
 template <int MODE, bool MASK, int PS>
 __device__ __forceinline__ double row_loop(const double* px, const double* py, const double* pz, int n, int i,
                                            int d, int d_end, int T, int G, const BoxDev& box, const AzizDev& a,
                                            double inv_dr, bool factors, const WormDev& worm, int slice) {
+    constexpr bool DUP = pair_lds_dup<PS>();
+    constexpr int PSL = pair_lds_plane<PS>();
     const double xi = PS ? px[i] : px[i];
-    const double yi = PS ? px[PS + i] : py[i];
-    const double zi = PS ? px[2 * PS + i] : pz[i];
+    const double yi = PS ? px[PSL + i] : py[i];
+    const double zi = PS ? px[2 * PSL + i] : pz[i];
     int j0 = i + d;
-    if (j0 >= n) j0 -= n;
+    if (!DUP && j0 >= n) j0 -= n;
     constexpr unsigned kW = PS ? 8u : 1u;   // units of the partner cursor: bytes (LDS) or beads
     const unsigned step = in_vgpr((unsigned)(G % n) * kW);
     const unsigned span = in_vgpr((unsigned)n * kW);
@@ -97,61 +110,86 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     AzizDev av = a;
     if (MODE == 0) {
         av.rep_c1 = in_vgpr(a.rep_c1);
-        av.eC8d = in_vgpr(a.eC8d);
+        av.eC8k = in_vgpr(a.eC8k);
     }
     const char* lb = reinterpret_cast<const char*>(px);
-    auto pair = [&](unsigned c) -> double {
+    // A pair in two parts: head (reads, bin, the branch-free terms) and tail (the
+    // damping step, the out-of-table select, worm factors).  Running the heads of
+    // two pairs back to back, fenced so the scheduler interleaves them, measured
+    // slower at N = 2048 x 250 (1.009 ms against 0.981) and N = 256 x 167.
+    struct Head {
+        VmixHead h;
+        double v;
+        int k, kk;
+        unsigned c;
+    };
+    auto head = [&](unsigned c) -> Head {
+        Head p;
+        p.c = c;
         double xj, yj, zj;
         if (PS) {
             xj = *reinterpret_cast<const double*>(lb + c);
-            yj = *reinterpret_cast<const double*>(lb + c + 8 * PS);
-            zj = *reinterpret_cast<const double*>(lb + c + 16 * PS);
+            yj = *reinterpret_cast<const double*>(lb + c + 8 * PSL);
+            zj = *reinterpret_cast<const double*>(lb + c + 16 * PSL);
         } else {
             xj = px[c];
             yj = py[c];
             zj = pz[c];
         }
-        double v;
         if (MODE == 0) {
-            const int k = bin_mixed(q, xj, yj, zj);
+            p.k = bin_mixed(q, xj, yj, zj);
             // k_eps <= k < tableLength, else V = 0 (include/potential.h:252-257): the
             // clamped bin keeps the arithmetic finite, and only a wave holding such a
             // pair pays for the select
-            const int kk = min(max(k, a.k_eps), lenm1);
-            v = aziz_vmix(kk, av);
-            if (__builtin_amdgcn_ballot_w64(k != kk)) {
-                asm volatile("");
-                v = (k != kk) ? 0.0 : v;
-            }
+            p.kk = min(max(p.k, a.k_eps), lenm1);
+            p.h = aziz_vmix_head(p.kk, av);
         } else {
             const double r2 = sep_r2<MODE>(xi, yi, zi, xj, yj, zj, box);
-            v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+            p.v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+        }
+        return p;
+    };
+    auto tail = [&](const Head& p) -> double {
+        double v;
+        if (MODE == 0) {
+            v = aziz_vmix_damp(p.kk, p.h, av);
+            if (__builtin_amdgcn_ballot_w64(p.k != p.kk)) {
+                asm volatile("");
+                v = (p.k != p.kk) ? 0.0 : v;
+            }
+        } else {
+            v = p.v;
         }
         if (factors) {
-            const int j = (int)(c / kW);
+            int j = (int)(p.c / kW);
+            if (DUP && j >= n) j -= n;
             const int lo = i < j ? i : j, hi = i < j ? j : i;
             v *= pair_factor(bead_state(worm, slice, lo), bead_state(worm, slice, hi));
         }
         return v;
     };
+    auto pair = [&](unsigned c) -> double { return tail(head(c)); };
     auto advance = [&]() {
         const unsigned jn = jc + step;
-        jc = min(jn, jn - span);   // unsigned: jn - span wraps above jn while jn < span
+        jc = DUP ? jn : min(jn, jn - span);   // unsigned: jn - span wraps above jn while jn < span
     };
     double acc = 0.0;
     int dd = d;
-    for (int t = 1; t < T; ++t) {
-        const double v = pair(jc);
+    auto add = [&](double v) {
         if (MASK)
             acc += (dd == skip_d) ? 0.0 : v;
         else
             acc += v;
         dd += G;
+    };
+    for (int t = 1; t < T; ++t) {
+        add(pair(jc));
         advance();
     }
-    // the last offset: present for the lane groups whose d reaches it
-    const double v = pair(jc);
+    // the last offset: present for the lane groups whose d reaches it (a lane
+    // past d_end reads bead 0 and drops it)
     const bool ok = (d + (T - 1) * G < d_end) && !(MASK && (d + (T - 1) * G) == skip_d);
+    const double v = pair(ok ? jc : 0u);
     acc += ok ? v : 0.0;
     return acc;
 }
@@ -175,10 +213,11 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
                                                               const WalkerDesc* __restrict__ walkers,
                                                               const double* __restrict__ pos, AzizDev az,
                                                               double* __restrict__ partials) {
-    // x | y | z of one slice, PS each, then the block sum's wave partials: a
+    // x | y | z of one slice, PSL each, then the block sum's wave partials: a
     // static array, so its address is a constant folded into the reads' offsets
-    __shared__ double lds[3 * PS + kPairThreads / 64];
-    double* red = lds + 3 * PS;
+    constexpr int PSL = pair_lds_plane<PS>();
+    __shared__ double lds[3 * PSL + kPairThreads / 64];
+    double* red = lds + 3 * PSL;
     const int r = xcd_contiguous_item(blockIdx.x, gridDim.x);
     const int k_begin = ranges[r], k_end = ranges[r + 1];
     if (k_begin >= k_end) return;
@@ -200,9 +239,15 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
         if (in_lds && staged != tl.slice) {
             __syncthreads();   // the previous slice is no longer read
             for (int t = threadIdx.x; t < n; t += kPairThreads) {
-                lds[t] = px[t];
-                lds[PS + t] = py[t];
-                lds[2 * PS + t] = pz[t];
+                const double x = px[t], y = py[t], z = pz[t];
+                lds[t] = x;
+                lds[PSL + t] = y;
+                lds[2 * PSL + t] = z;
+                if (pair_lds_dup<PS>() && t <= (n >> 1)) {
+                    lds[n + t] = x;
+                    lds[PSL + n + t] = y;
+                    lds[2 * PSL + n + t] = z;
+                }
             }
             staged = tl.slice;
             __syncthreads();

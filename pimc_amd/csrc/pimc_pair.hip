@@ -861,6 +861,43 @@ __global__ void k_cell_labels(const int* __restrict__ cell, int* __restrict__ la
     }
 }
 
+// The same labels with one wave per slice: the slice's beads in order, 64 at a
+// time; within a group of 64, the lanes of each cell (one ballot per distinct
+// cell) take consecutive labels after the cell's running count, kept in LDS.
+// Label order, hash and counts equal the serial insertion order above.
+constexpr int kCellLabelsLds = 8192;   // cells counted in LDS (beyond: the serial kernel)
+__global__ __launch_bounds__(64) void k_cell_labels_wave(const int* __restrict__ cell, int* __restrict__ label,
+                                                         int* __restrict__ numlab, int* __restrict__ hash,
+                                                         int nalloc, int num_slices, int cells) {
+    __shared__ int cnt[kCellLabelsLds];
+    const int m = blockIdx.x, lane = threadIdx.x;
+    for (int c = lane; c < cells; c += 64) cnt[c] = 0;
+    __syncthreads();
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));   // lanes < this one
+    for (int base = 0; base < nalloc; base += 64) {
+        const int i = base + lane;
+        const int c = (i < nalloc) ? cell[m * nalloc + i] : -1;
+        uint64_t todo = __builtin_amdgcn_ballot_w64(c >= 0);
+        int lab = -1;
+        while (todo) {
+            const int leader = __builtin_ctzll(todo);
+            const int cl = __builtin_amdgcn_readlane(c, leader);
+            const uint64_t grp = __builtin_amdgcn_ballot_w64(c == cl);
+            const int start = cnt[cl];
+            if (c == cl) lab = start + __builtin_popcountll(grp & below);
+            __syncthreads();   // every lane has read cnt[cl] before lane `leader` advances it
+            if (lane == leader) cnt[cl] = start + __builtin_popcountll(grp);
+            __syncthreads();
+            todo &= ~grp;
+        }
+        if (i < nalloc) {
+            label[m * nalloc + i] = lab;
+            if (c >= 0) hash[((int64_t)c * num_slices + m) * nalloc + lab] = i;
+        }
+    }
+    for (int c = lane; c < cells; c += 64) numlab[c * num_slices + m] = cnt[c];
+}
+
 // LookupTable::updateInteractionList (src/lookuptable.cpp:464-516), one bead,
 // walking the 27 de-duplicated neighbour cells in order.
 __global__ void k_interaction_list(int m, int me, const double* __restrict__ pos, int64_t plane, int nalloc,
@@ -1160,8 +1197,12 @@ hipError_t launch_cell_build(hipStream_t st, const double* pos, int64_t plane, i
     (void)hipGetLastError();  // drop any stale error of an earlier call
     const int total = nalloc * num_slices;
     hipLaunchKernelGGL(k_cell_assign, dim3((total + 255) / 256), dim3(256), 0, st, pos, plane, total, cell, on, box);
-    hipLaunchKernelGGL(k_cell_labels, dim3((num_slices + 63) / 64), dim3(64), 0, st, cell, label, numlab, hash,
-                       nalloc, num_slices, cells);
+    if (cells <= kCellLabelsLds)
+        hipLaunchKernelGGL(k_cell_labels_wave, dim3(num_slices), dim3(64), 0, st, cell, label, numlab, hash, nalloc,
+                           num_slices, cells);
+    else
+        hipLaunchKernelGGL(k_cell_labels, dim3((num_slices + 63) / 64), dim3(64), 0, st, cell, label, numlab, hash,
+                           nalloc, num_slices, cells);
     return hipGetLastError();
 }
 

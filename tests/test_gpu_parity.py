@@ -195,6 +195,13 @@ def test_cell_lists_bit_exact(orc, gpu, frac):
         assert np.array_equal(lg, lr) and np.array_equal(sg, sr)
 
 
+@pytest.mark.parametrize("N,frac", [(300, 0.3), (700, 0.5)])
+def test_cell_lists_many_beads(orc, gpu, N, frac):
+    """Labels over several 64-bead groups per slice, in the serial insertion order."""
+    ref, ctx, _, _ = setup_pair(orc, gpu, N, 3, 0.004, rc_frac=frac)
+    _compare_cells(ctx.cell_lists(), ref.grid_dump())
+
+
 @pytest.mark.parametrize("frac", [1.0, 0.5, 0.22])
 def test_action_list_vnn_cutoff(orc, gpu, frac):
     ref, ctx, beads, L = setup_pair(orc, gpu, 128, 8, 0.004, rc_frac=frac, mode=2)

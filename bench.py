@@ -91,9 +91,10 @@ def main():
     for k in range(args.warmup):
         step(k, lat)
     lat = {s: [] for s in segs}
-    # the pair kernels' own durations are stamped by their dispatches (HIP
-    # events from a pre-created pool) over the timed region itself
-    ctx.set_timing(True)
+    # the all-pairs kernel's own duration is stamped by its dispatches (HIP
+    # events from a pre-created pool) over the timed region itself; the Delta V
+    # launches carry no event records there
+    ctx.set_timing(True, kinds=["pair_action"])
     pimc_amd.device_synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -101,6 +102,12 @@ def main():
     pimc_amd.device_synchronize()
     elapsed = time.perf_counter() - t0
     pa_ms, pa_n = ctx.kernel_time("pair_action")
+    ctx.set_timing(False)
+    # the Delta V kernel's duration, from a second pass over the same requests
+    ctx.set_timing(True, kinds=["delta_action"])
+    for k in range(args.steps):
+        for s, (sp, new) in zip(segs, reqs[args.warmup + k]):
+            ctx.delta_action(sp, new)
     dv_ms, dv_n = ctx.kernel_time("delta_action")
     ctx.set_timing(False)
 

@@ -398,6 +398,11 @@ int pimc_gen_worldlines(uint32_t seed, double side, int M, int N, double lambda,
  * k_vnn), without their reductions; "gradv2" and "gradvnn" the gradient
  * correction's slice and three-body kernels. */
 int pimc_set_timing(pimc_ctx* ctx, int enable);
+/* Timing for a subset of the kernels above: bit k of `kinds` brackets the k-th
+ * name in the list (0: "pair_action", 1: "delta_action", ...); 0 turns timing
+ * off.  A caller that needs one kernel's duration leaves the others' launches
+ * free of event records. */
+int pimc_set_timing_kinds(pimc_ctx* ctx, unsigned kinds);
 int pimc_kernel_time(pimc_ctx* ctx, const char* name, double* total_ms, long* count);
 int pimc_device_synchronize(void);
 /* A plain device-to-device streaming copy of `bytes` run `iters` times (no

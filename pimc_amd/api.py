@@ -18,6 +18,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpimc_gpu.so")
 
 V_FP32, V_FP64, V_TABLE = 0, 1, 2
+# kernel names of pimc_kernel_time, in the bit order of pimc_set_timing_kinds
+KERNEL_NAMES = ("pair_action", "delta_action", "rho", "ssf", "isf", "gradv2", "gradvnn", "virial", "chain_run")
 VIRIAL_COLUMNS = ("K_op", "K_cv", "V_op", "V_cv", "E", "E_mu", "K_op/N", "K_cv/N", "V_op/N", "V_cv/N", "E/N",
                   "EEcv*Beta^2", "Ecv*Beta", "dEdB", "CvCov1", "CvCov2", "CvCov3", "E_th", "P")
 
@@ -91,6 +93,7 @@ SIGNATURES = {
     "pimc_aziz_tailV": (C.c_double, [C.c_void_p]),
     "pimc_aziz_table_length": (C.c_int, [C.c_void_p]),
     "pimc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "pimc_set_timing_kinds": (C.c_int, [C.c_void_p, C.c_uint]),
     "pimc_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_long)]),
     "pimc_device_synchronize": (C.c_int, []),
     "pimc_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _dp]),
@@ -381,8 +384,15 @@ class Context:
                     "interaction_list")
         return lst[:cnt.value].copy(), sep[:cnt.value].copy()
 
-    def set_timing(self, on=True):
-        self._check(lib().pimc_set_timing(self.h, int(on)), "set_timing")
+    def set_timing(self, on=True, kinds=None):
+        """Event timing of the hot kernels; kinds: names to time (None: all)."""
+        if kinds is None or not on:
+            self._check(lib().pimc_set_timing(self.h, int(on)), "set_timing")
+            return
+        mask = 0
+        for k in kinds:
+            mask |= 1 << KERNEL_NAMES.index(k)
+        self._check(lib().pimc_set_timing_kinds(self.h, mask), "set_timing_kinds")
 
     def kernel_time(self, name):
         ms = C.c_double()

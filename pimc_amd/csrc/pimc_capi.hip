@@ -247,7 +247,7 @@ static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf"
 constexpr int kTimingSlots = 2048;
 
 int pimc_ctx::timed_slot(int k) {
-    if (!timing) return -1;
+    if (!timing || !((timing_kinds >> k) & 1u)) return -1;
     if (ev_pool.empty()) {
         ev_pool.assign(2 * kTimingSlots, nullptr);
         for (auto& e : ev_pool)
@@ -1008,9 +1008,11 @@ int pimc_autotune(pimc_ctx* c, int reps, double* out4) {
     const int ldss[] = {1, 0};
     c->resolve_timing();
     const bool timing = c->timing;
+    const unsigned kinds = c->timing_kinds;
     const double keep_ms = c->kernel_ms[0];
     const long keep_n = c->kernel_count[0];
     c->timing = true;
+    c->timing_kinds = ~0u;
     double best = 1e30, act = 0.0;
     int best_p = 0, best_g = 0, best_l = -1, tried = 0, r;
     for (int ppl : ppls)
@@ -1034,6 +1036,7 @@ int pimc_autotune(pimc_ctx* c, int reps, double* out4) {
                 }
             }
     c->timing = timing;
+    c->timing_kinds = kinds;
     c->kernel_ms[0] = keep_ms;   // the tuning launches stay out of the kernel totals
     c->kernel_count[0] = keep_n;
     r = pimc_set_launch_shape(c, best_p, best_g, best_l);
@@ -1412,6 +1415,15 @@ int pimc_set_timing(pimc_ctx* c, int enable) {
     if (!c) return fail(nullptr, PIMC_E_ARG, "null ctx");
     c->resolve_timing();
     c->timing = enable != 0;
+    c->timing_kinds = ~0u;
+    return PIMC_OK;
+}
+
+int pimc_set_timing_kinds(pimc_ctx* c, unsigned kinds) {
+    if (!c) return fail(nullptr, PIMC_E_ARG, "null ctx");
+    c->resolve_timing();
+    c->timing = kinds != 0;
+    c->timing_kinds = kinds;
     return PIMC_OK;
 }
 

@@ -126,6 +126,7 @@ struct pimc_ctx {
     // the pool is exhausted or the totals are read, so timed calls add only the
     // event records
     bool timing = false;
+    unsigned timing_kinds = ~0u;        // kinds bracketed while timing (bit k: kKernelNames[k])
     struct Pending {
         int kind;
         int ev;    // index into ev_pool

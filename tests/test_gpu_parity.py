@@ -61,6 +61,22 @@ def test_launch_shapes_and_autotune(orc, gpu):
         ctx.set_launch_shape(16, 100, 1)
 
 
+def test_timing_kinds(orc, gpu):
+    """pimc_set_timing_kinds brackets only the named kernels."""
+    ref, ctx, beads, L = setup_pair(orc, gpu, 32, 8, 0.004)
+    sp = np.array([[1, 3], [2, 3]], dtype=np.int32)
+    new = beads[sp[:, 0], sp[:, 1]] + 0.1
+    ctx.set_timing(True, kinds=["pair_action"])
+    ctx.potentialAction()
+    ctx.delta_action(sp, new)
+    assert ctx.kernel_time("pair_action")[1] == 1 and ctx.kernel_time("delta_action")[1] == 0
+    ctx.set_timing(True, kinds=["delta_action"])
+    ctx.potentialAction()
+    ctx.delta_action(sp, new)
+    assert ctx.kernel_time("pair_action")[1] == 1 and ctx.kernel_time("delta_action")[1] == 1
+    ctx.set_timing(False)
+
+
 @pytest.mark.parametrize("mode", [0, 2])
 def test_total_action_and_slices_n256_m167(orc, gpu, mode):
     """Single walker N=256, M=167, tau=0.004."""

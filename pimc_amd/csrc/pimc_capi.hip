@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -684,6 +685,10 @@ static int ensure_req(pimc_ctx* c, int n) {
     if ((r = ctx_halloc(c, &c->h_per_bead_cor, 2 * (size_t)cap))) return r;
     const int chunks = (c->desc.num_alloc + kVnnChunk - 1) / kVnnChunk;
     if ((r = ctx_dalloc(c, &c->d_vnn_partials, 2 * (size_t)cap * chunks))) return r;
+    if (!c->d_vnn_ticket) {
+        if ((r = ctx_dalloc(c, &c->d_vnn_ticket, 1))) return r;
+        HIP_OR_FAIL(c, hipMemsetAsync(c->d_vnn_ticket, 0, sizeof(unsigned), c->stream));
+    }
     c->cap_req = cap;
     return PIMC_OK;
 }
@@ -1174,6 +1179,7 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
     }
     A.tau = c->desc.tau;
     A.partials = c->d_vnn_partials;
+    A.ticket = c->d_vnn_ticket;   // one kernel unless the gradient correction adds its three-body pass
     A.per_bead = c->d_per_bead;
     A.out = c->d_map + 8;
     A.done = c->d_done();

@@ -104,6 +104,7 @@ struct pimc_ctx {
     double* d_per_bead_cor = nullptr;   // per-bead corrections [cap_req][2]
     double* h_per_bead_cor = nullptr;
     double* d_vnn_partials = nullptr;
+    unsigned* d_vnn_ticket = nullptr;   // arrival counter of the one-kernel Delta V (zero between calls)
     // structure factors
     int cap_q = 0;
     double* d_q = nullptr;

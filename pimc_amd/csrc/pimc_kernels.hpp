@@ -129,6 +129,9 @@ struct VnnArgs {
     double gvf0, gvf1, lambda;
     int cor_any_shift;     // evaluate the correction at any shift (potentialActionCorrection)
     double* per_bead_cor;  // [n_beads][2] corrections at the old / new position, or null
+    // arrival counter of the one-kernel form (zero between calls), or null: the
+    // last workgroup of k_vnn to finish runs the finalisation itself
+    unsigned* ticket;
 };
 
 constexpr int kGnnGroup = 16;   // bead2 per three-body workgroup (16 lanes each)

@@ -415,7 +415,13 @@ __device__ __forceinline__ void vnn_partner(const VnnArgs& A, const VnnSubject& 
     }
 }
 
-template <int MODE>
+__device__ __forceinline__ void vnn_finalize_body(const VnnArgs& A, const VnnInline& req);
+
+// FUSED: the partials go out with an agent-scope release through the arrival
+// counter, and the workgroup that arrives last (every other one has read its
+// positions by then) acquires them and runs the finalisation and the commit,
+// so a Delta V is one kernel.  No workgroup waits for another.
+template <int MODE, bool FUSED>
 __global__ __launch_bounds__(kVnnThreads) void k_vnn(VnnArgs A, VnnInline req, AzizDev az) {
     static_assert(kVnnThreads == 256, "four waves per Vnn workgroup");
     __shared__ double red[kVnnThreads / 64][2];
@@ -436,10 +442,25 @@ __global__ __launch_bounds__(kVnnThreads) void k_vnn(VnnArgs A, VnnInline req, A
         red[wave][1] = acc_n;
     }
     __syncthreads();
-    if (threadIdx.x < 2) {
-        const int r = threadIdx.x;
-        A.partials[((int64_t)b * A.chunks + c) * 2 + r] = ((red[0][r] + red[1][r]) + red[2][r]) + red[3][r];
+    if (!FUSED) {
+        if (threadIdx.x < 2) {
+            const int r = threadIdx.x;
+            A.partials[((int64_t)b * A.chunks + c) * 2 + r] = ((red[0][r] + red[1][r]) + red[2][r]) + red[3][r];
+        }
+        return;
     }
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < 2; ++r)
+            A.partials[((int64_t)b * A.chunks + c) * 2 + r] = ((red[0][r] + red[1][r]) + red[2][r]) + red[3][r];
+        const unsigned total = (unsigned)(gridDim.x * gridDim.y);
+        const unsigned t = __hip_atomic_fetch_add(A.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (t + 1 == total);
+    }
+    __syncthreads();
+    if (!last) return;
+    if (threadIdx.x == 0) __hip_atomic_store(A.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    vnn_finalize_body(A, req);
 }
 
 // The committed bead updates of a request, one lane per bead (the beads sit on
@@ -746,7 +767,7 @@ __global__ __launch_bounds__(256) void k_gradvnn(VnnArgs A, VnnInline req, AzizD
 // Per-bead Vnn (chunks in order), the two actions and dU (a fixed shuffle tree
 // over beads), then the committed bead updates, one lane per bead (the beads sit
 // on distinct slices).
-__global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) {
+__device__ __forceinline__ void vnn_finalize_body(const VnnArgs& A, const VnnInline& req) {
     __shared__ double red[3][4];
     double ao = 0.0, an = 0.0, du = 0.0;
     const double tau_s = A.shift * A.tau;
@@ -805,6 +826,8 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
         if (threadIdx.x == 0) publish_done(A.done, A.seq);
     }
 }
+
+__global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) { vnn_finalize_body(A, req); }
 
 // ---------------------------------------------------------------------------
 // Sequential bead updates (any slices, list order) -- Path::updateBead.
@@ -1125,13 +1148,24 @@ hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInlin
     if (A.n_beads <= 0) return hipSuccess;
     dim3 g(A.chunks, A.n_beads), b(kVnnThreads);
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, b, 0, st, A, req, az); };
+    const bool fused = A.ticket && !gnn_part;
     if (timer && timer->start) (void)hipEventRecord(timer->start, st);
+    if (fused) {
+        if (mode == 0)
+            go(k_vnn<0, true>);
+        else if (mode == 1)
+            go(k_vnn<1, true>);
+        else
+            go(k_vnn<2, true>);
+        if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
+        return hipGetLastError();
+    }
     if (mode == 0)
-        go(k_vnn<0>);
+        go(k_vnn<0, false>);
     else if (mode == 1)
-        go(k_vnn<1>);
+        go(k_vnn<1, false>);
     else
-        go(k_vnn<2>);
+        go(k_vnn<2, false>);
     if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
     if (gnn_part) {
         VnnArgs B = A;

@@ -355,7 +355,7 @@ class Context:
 
     def isf(self, q):
         q = _f64(q, (-1, 3))
-        o = np.zeros((q.shape[0], self.M))
+        o = np.empty((q.shape[0], self.M))   # every row is written
         self._check(lib().pimc_isf(self.h, _p(q, _dp), q.shape[0], _p(o, _dp)), "isf")
         return o
 

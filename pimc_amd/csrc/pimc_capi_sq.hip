@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "pimc_host.hpp"
@@ -190,10 +191,16 @@ int pimc_isf(pimc_ctx* c, const double* q, int nq, double* fqt) {
         c->timed_begin(4);
         SHIP(c, launch_isf(c->stream, c->d_rho, n, M, inv_n, c->d_fqt, 0));
         c->timed_end();
-        std::vector<double> host((size_t)n * M);
-        SHIP(c, hipMemcpyAsync(host.data(), c->d_fqt, sizeof(double) * n * M, hipMemcpyDeviceToHost, c->stream));
+        if (n == nq) {
+            // every q in this pass, in order: straight into the caller's array
+            SHIP(c, hipMemcpyAsync(fqt, c->d_fqt, sizeof(double) * n * M, hipMemcpyDeviceToHost, c->stream));
+            SHIP(c, hipStreamSynchronize(c->stream));
+            continue;
+        }
+        std::unique_ptr<double[]> host(new double[(size_t)n * M]);
+        SHIP(c, hipMemcpyAsync(host.get(), c->d_fqt, sizeof(double) * n * M, hipMemcpyDeviceToHost, c->stream));
         SHIP(c, hipStreamSynchronize(c->stream));
-        for (int k = 0; k < n; ++k) std::memcpy(fqt + (size_t)idx[k] * M, host.data() + (size_t)k * M, sizeof(double) * M);
+        for (int k = 0; k < n; ++k) std::memcpy(fqt + (size_t)idx[k] * M, host.get() + (size_t)k * M, sizeof(double) * M);
     }
     return PIMC_OK;
 }

@@ -244,16 +244,60 @@ __device__ __forceinline__ double aziz_dvdr_mix(int k, const AzizDev& a) {
     return (T1 + att) * a.inv_rm;
 }
 
+// valued2Vdr2 at the bin radius with the FP32 transcendental units, the
+// counterpart of aziz_d2vdr2_64 (the same hard-core epsilon/rm factor).
+__device__ __forceinline__ double aziz_d2vdr2_mix(int k, const AzizDev& a) {
+    const double kd = (double)k;
+    const double x = kd * a.s;
+    const double rep = split_exp2(fma(fma(a.rep_c2, kd, a.rep_c1), kd, a.rep_c0));   // epsilon folded in
+    const double ab = (a.alpha - 2.0 * a.beta * x) * (a.alpha - 2.0 * a.beta * x);
+    const double T1 = rep * (2.0 * a.beta + ab);
+    if (k < a.k_hc) return T1 * a.inv_rm;
+    double ix = (double)__builtin_amdgcn_rcpf((float)x);
+    ix = ix * fma(-x, ix, 2.0);
+    ix = ix * fma(-x, ix, 2.0);                  // two Newton steps: FP64 accurate
+    const double ix2 = ix * ix;
+    const double ix6 = ix2 * ix2 * ix2;
+    double F = 1.0, dF = 0.0, d2F = 0.0;
+    if (k < a.k_d) {
+        const double D = a.D;
+        const double t = fma(D, ix, -1.0);
+        const double e = split_exp2(-1.4426950408889634074 * t * t);
+        F = e;
+        dF = 2.0 * D * ix2 * t * e;
+        d2F = 2.0 * D * ix2 * ix * (2.0 * D * D * D * ix2 * ix - 4.0 * D * D * ix2 - D * ix + 2.0) * e;
+    }
+    // epsilon * [ -(42 C6 x^-8 + 72 C8 x^-10 + 110 C10 x^-12) F
+    //             + 2 (6 C6 x^-7 + 8 C8 x^-9 + 10 C10 x^-11) dF - (C6 x^-6 + C8 x^-8 + C10 x^-10) d2F ]
+    const double p2 = ix6 * ix2 * fma(fma(110.0 * a.eC10d, ix2, 72.0 * a.eC8d), ix2, 42.0 * a.eC6d);
+    const double p1 = ix6 * ix * fma(fma(10.0 * a.eC10d, ix2, 8.0 * a.eC8d), ix2, 6.0 * a.eC6d);
+    const double p0 = ix6 * fma(fma(a.eC10d, ix2, a.eC8d), ix2, a.eC6d);
+    const double att = -p2 * F + 2.0 * p1 * dF - p0 * d2F;
+    return (T1 + att) * (a.inv_rm * a.inv_rm);
+}
+
 // The gradient factor g of gradV(sep) = g * sep, g = lookupdVdr[k] / |sep|
 // (include/potential.h:996-1002 with direct(), :249-260); 0 outside the table.
+// Mixed-mode bin with both |r| and 1/|r|: the hardware FP64 reciprocal-root
+// estimate (relative error ~2^-23) refined by one Newton step (~2^-46), then
+// |r| = r2 / |r|; the bin differs from the exact one only when |r|/dr lies
+// within ~1e-13 relative of an integer.
+__device__ __forceinline__ int bin_rsq(double r2, double inv_dr, double* r_out, double* ir_out) {
+    const double y0 = __builtin_amdgcn_rsq(r2);
+    const double h = fma(-(r2 * y0), y0, 1.0);
+    const double y = fma(0.5 * y0, h, y0);
+    const double r = r2 * y;
+    *r_out = r;
+    *ir_out = y;
+    return (int)(r * inv_dr);
+}
+
 template <int MODE>
 __device__ __forceinline__ double grad_factor_from_r2(double r2, const AzizDev& a, double inv_dr) {
     int k;
-    double rn;
+    double rn, ir = 0.0;
     if (MODE == 0) {
-        float rs;
-        k = bin_fast(r2, inv_dr, &rs);
-        rn = __builtin_sqrt(r2);
+        k = bin_rsq(r2, inv_dr, &rn, &ir);
     } else {
         k = bin_exact(r2, a.dr, &rn);
     }
@@ -266,6 +310,7 @@ __device__ __forceinline__ double grad_factor_from_r2(double r2, const AzizDev& 
         d = aziz_dvdr64(kk, a);
     else
         d = valid ? a.dtable[k] : 0.0;
+    if (MODE == 0) return valid ? d * ir : 0.0;
     return valid ? d / rn : 0.0;
 }
 

@@ -67,33 +67,48 @@ __device__ __forceinline__ void bead_pairs(P px, P py, P pz, int j0, int j1, int
                                            const AzizDev& a, bool need_t, double F[3], double T[6]) {
 #pragma clang fp contract(off)
     const double xi = px[i], yi = py[i], zi = pz[i];
+    const double inv_dr = 1.0 / a.dr;
     for (int j = j0; j < j1; ++j) {
         if (j == i) continue;
-        const double s0 = min_image_exact(xi - px[j], box.L, box.invL);
-        const double s1 = min_image_exact(yi - py[j], box.L, box.invL);
-        const double s2 = min_image_exact(zi - pz[j], box.L, box.invL);
-        double rn;
-        const int k = bin_exact(norm2_exact(s0, s1, s2), a.dr, &rn);
+        double s0, s1, s2, rn, ir;
+        int k;
+        if (MODE == 0) {
+            s0 = min_image_fast(xi - px[j], box.L, box.invL);
+            s1 = min_image_fast(yi - py[j], box.L, box.invL);
+            s2 = min_image_fast(zi - pz[j], box.L, box.invL);
+            k = bin_rsq(norm2_exact(s0, s1, s2), inv_dr, &rn, &ir);
+        } else {
+            s0 = min_image_exact(xi - px[j], box.L, box.invL);
+            s1 = min_image_exact(yi - py[j], box.L, box.invL);
+            s2 = min_image_exact(zi - pz[j], box.L, box.invL);
+            k = bin_exact(norm2_exact(s0, s1, s2), a.dr, &rn);
+            // one division per pair: the reference's quotients by |r|, |r|^2 and
+            // |r|^3 become products with 1/|r| (a rounding apart)
+            ir = 1.0 / rn;
+        }
         // outside (0, tableLength) gradV and grad2V are 0: the pair adds nothing
         if ((unsigned)(k - a.k_eps) >= (unsigned)(a.table_length - a.k_eps)) continue;
-        const double d = (MODE == 2) ? a.dtable[k] : aziz_dvdr64(k, a);
-        const double g = d / rn;
+        const double d = (MODE == 2) ? a.dtable[k] : (MODE == 1 ? aziz_dvdr64(k, a) : aziz_dvdr_mix(k, a));
+        // |gradV| = |dV/dr|
+        const double g = d * ir;
         const double g0 = g * s0, g1 = g * s1, g2 = g * s2;
         F[0] += g0;
         F[1] += g1;
         F[2] += g2;
         if (need_t) {
-            const double d2 = (MODE == 2) ? a.d2table[k] : aziz_d2vdr2_64(k, a);
-            const double dV = __builtin_sqrt(norm2_exact(g0, g1, g2));
-            const double r2 = rn * rn, r3 = rn * rn * rn, diag = dV / rn;
+            const double d2 = (MODE == 2) ? a.d2table[k] : (MODE == 1 ? aziz_d2vdr2_64(k, a) : aziz_d2vdr2_mix(k, a));
+            const double dV = __builtin_fabs(d);
+            const double ir2 = ir * ir;
+            const double diag = dV * ir;
+            // r r^T (d2V/r^2 - |gradV|/r^3) + I |gradV|/r
+            const double c = d2 * ir2 - dV * ir2 * ir;
             const double s[3] = {s0, s1, s2};
             int q = 0;
 #pragma unroll
             for (int u = 0; u < 3; ++u)
 #pragma unroll
                 for (int v = u; v < 3; ++v, ++q) {
-                    const double ss = s[u] * s[v];
-                    T[q] += ss * d2 / r2 - ss * dV / r3;
+                    T[q] += s[u] * s[v] * c;
                     if (u == v) T[q] += diag;
                 }
         }
@@ -291,15 +306,19 @@ hipError_t launch_virial(int mode, hipStream_t st, const VirialArgs& A, const Az
         dim3 gc(B * A.chunks, A.num_slices);
         if (mode == 2)
             hipLaunchKernelGGL(k_virial_chunk<2>, gc, b, lds, st, A, az);
-        else
+        else if (mode == 1)
             hipLaunchKernelGGL(k_virial_chunk<1>, gc, b, lds, st, A, az);
+        else
+            hipLaunchKernelGGL(k_virial_chunk<0>, gc, b, lds, st, A, az);
         hipLaunchKernelGGL(k_virial_sum, g, b, 0, st, A);
         return hipGetLastError();
     }
     if (mode == 2)
         hipLaunchKernelGGL(k_virial<2>, g, b, lds, st, A, az);
-    else
+    else if (mode == 1)
         hipLaunchKernelGGL(k_virial<1>, g, b, lds, st, A, az);
+    else
+        hipLaunchKernelGGL(k_virial<0>, g, b, lds, st, A, az);
     return hipGetLastError();
 }
 

@@ -4,8 +4,9 @@ oracle on identical seeded worldlines, with periodic links and with a
 permutation across the imaginary-time boundary.
 
 Tolerances, relative to the largest magnitude of the compared column: 1e-9 in
-the closed-form modes (dV/dr and d2V/dr2 at x = k dr / rm against the tables'
-accumulated radii), 1e-11 in the table mode (per-pair values from the same
+the FP64 closed-form mode (dV/dr and d2V/dr2 at x = k dr / rm against the tables'
+accumulated radii), 1e-6 in the FP32-transcendental mode (exp through v_exp_f32
+with a split exponent), 1e-11 in the table mode (per-pair values from the same
 tables, summation order differs).  The estimator's potential columns carry V(slice),
 whose FP32-transcendental mode is good to 1e-6 (TOL_E)."""
 import numpy as np
@@ -14,7 +15,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 DENSITY, LAMBDA = 0.02198, 6.0596
-TOL = {0: 1e-9, 1: 1e-9, 2: 1e-11}
+TOL = {0: 1e-6, 1: 1e-9, 2: 1e-11}
 TOL_E = {0: 1e-6, 1: 1e-9, 2: 1e-11}
 GSF = dict(vfactor=(2.0 / 3.0, 4.0 / 3.0), gradvfactor=(0.0, 2.0 / 9.0), period=2)
 LB = dict(vfactor=(1.0, 1.0), gradvfactor=(1.0 / 12.0, 1.0 / 12.0), period=2)

@@ -80,20 +80,6 @@ __device__ __forceinline__ int bin_exact(double r2, double dr, double* r_out) {
     return (int)(r / dr);
 }
 
-// FP32-mode bin: an FP32 root refined by one Newton step in FP64, then a
-// multiply by 1/dr.  It differs from the exact bin only when |r|/dr lies within
-// a few FP64 ulps of an integer.
-__device__ __forceinline__ int bin_fast(double r2, double inv_dr, float* rs_out) {
-    float r2f = (float)r2;
-    float rs = __builtin_amdgcn_rsqf(r2f);          // ~1/|r|
-    float r0 = r2f * rs;                            // ~|r|
-    double r0d = (double)r0;
-    double res = fma(-r0d, r0d, r2);                // r2 - r0^2, exact enough
-    double r = fma((double)(0.5f * rs), res, r0d);  // r0 + res/(2 r0)
-    *rs_out = rs;
-    return (int)(r * inv_dr);
-}
-
 __device__ __forceinline__ float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 
 // valueV at the bin radius with the chip's FP32 transcendental units (exp2, rcp)
@@ -320,8 +306,8 @@ template <int MODE>
 __device__ __forceinline__ double pair_value_from_r2(double r2, const AzizDev& a, double inv_dr) {
     int k;
     if (MODE == 0) {
-        float rs;
-        k = bin_fast(r2, inv_dr, &rs);
+        double r, ir;
+        k = bin_rsq(r2, inv_dr, &r, &ir);
     } else {
         double r;
         k = bin_exact(r2, a.dr, &r);

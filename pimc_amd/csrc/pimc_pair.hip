@@ -87,7 +87,7 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
                                            double inv_dr, bool factors, const WormDev& worm, int slice) {
     constexpr bool DUP = pair_lds_dup<PS>();
     constexpr int PSL = pair_lds_plane<PS>();
-    const double xi = PS ? px[i] : px[i];
+    const double xi = px[i];
     const double yi = PS ? px[PSL + i] : py[i];
     const double zi = PS ? px[2 * PSL + i] : pz[i];
     int j0 = i + d;

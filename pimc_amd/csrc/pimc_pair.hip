@@ -81,10 +81,10 @@ constexpr int pair_lds_plane() {
     return pair_lds_dup<PS>() ? 2 * PS : PS;
 }
 
-template <int MODE, bool MASK, int PS>
+template <int MODE, bool MASK, int PS, bool FACTORS>
 __device__ __forceinline__ double row_loop(const double* px, const double* py, const double* pz, int n, int i,
                                            int d, int d_end, int T, int G, const BoxDev& box, const AzizDev& a,
-                                           double inv_dr, bool factors, const WormDev& worm, int slice) {
+                                           double inv_dr, const WormDev& worm, int slice) {
     constexpr bool DUP = pair_lds_dup<PS>();
     constexpr int PSL = pair_lds_plane<PS>();
     const double xi = px[i];
@@ -160,7 +160,7 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
         } else {
             v = p.v;
         }
-        if (factors) {
+        if (FACTORS) {
             int j = (int)(p.c / kW);
             if (DUP && j >= n) j -= n;
             const int lo = i < j ? i : j, hi = i < j ? j : i;
@@ -194,12 +194,17 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     return acc;
 }
 
+// The loop variants: MASK (the d = n/2 offset inside the tile) and FACTORS (a
+// worm on the slice), both fixed per tile, so neither costs a branch per pair.
 template <int MODE, int PS>
 __device__ __forceinline__ double row_tile(const double* px, const double* py, const double* pz, int n, int i, int d,
                                            int d_end, int T, int G, bool mask, const BoxDev& box, const AzizDev& a,
                                            double inv_dr, bool factors, const WormDev& worm, int slice) {
-    return mask ? row_loop<MODE, true, PS>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, factors, worm, slice)
-                : row_loop<MODE, false, PS>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, factors, worm, slice);
+    if (factors)
+        return mask ? row_loop<MODE, true, PS, true>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice)
+                    : row_loop<MODE, false, PS, true>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice);
+    return mask ? row_loop<MODE, true, PS, false>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice)
+                : row_loop<MODE, false, PS, false>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice);
 }
 
 // Persistent: the workgroups of one XCD take consecutive tile ranges (so a

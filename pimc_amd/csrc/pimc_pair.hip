@@ -1133,16 +1133,25 @@ int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lan
     return (int64_t)n * (n - 1) / 2;
 }
 
+// A tile costs its pairs plus a fixed part (descriptor, slice staging, block
+// sum): 128 walkers of N = 16 (120 pairs per tile) ran 30 k tiles in 73 us,
+// against 6 us for their pairs at the N = 256 rate, about 1200 pair
+// evaluations per tile.  Ranges balanced by this cost keep a workgroup of many
+// small slices from finishing last in a batch of uneven walkers; over 1024 /
+// 2048 / 4096 the 512-walker batch took 2.00 / 1.85 / 2.00 ms (3.03 ms when
+// balanced by pairs alone).
+constexpr int64_t kTileCost = 2048;
+
 void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int>& ranges) {
     const int T = (int)tiles.size();
     ranges.assign(G + 1, T);
     ranges[0] = 0;
     int64_t tot = 0;
-    for (const auto& t : tiles) tot += row_tile_pairs(t);
+    for (const auto& t : tiles) tot += row_tile_pairs(t) + kTileCost;
     int64_t acc = 0;
     int g = 1;
     for (int k = 0; k < T && g < G; ++k) {
-        acc += row_tile_pairs(tiles[k]);
+        acc += row_tile_pairs(tiles[k]) + kTileCost;
         while (g < G && acc * G >= tot * g) ranges[g++] = k + 1;
     }
 }

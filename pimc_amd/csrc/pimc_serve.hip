@@ -804,15 +804,20 @@ static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, c
                          int shift, const double vfactor[2], int commit_mode, double u, int tag, uint64_t* req_id) {
     if (!s || !slice_ptcl || !new_pos || n < 1 || !req_id || shift < 1 || commit_mode < 0 || commit_mode > 2)
         return PIMC_E_ARG;
-    std::lock_guard<std::mutex> g(s->mu);
-    if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
-    if (chain < 0 || chain >= (int)s->chains.size() || s->chains[chain].state == 2)
-        return sfail(s, PIMC_E_ARG, "unknown chain");
-    Chain& c = s->chains[chain];
     if (n > kMaxBeadsPerBatch / 4) return sfail(s, PIMC_E_ARG, "too many beads in one request");
-    std::vector<char> seen(c.d.num_slices, 0);
+    // the chain's extents under the lock; the request is built outside it, so
+    // the server lock is held only to queue it
+    int num_slices = 0, num_alloc = 0;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
+        if (chain < 0 || chain >= (int)s->chains.size() || s->chains[chain].state == 2)
+            return sfail(s, PIMC_E_ARG, "unknown chain");
+        num_slices = s->chains[chain].d.num_slices;
+        num_alloc = s->chains[chain].d.num_alloc;
+    }
+    std::vector<char> seen(num_slices, 0);
     Request R;
-    R.id = s->next_id++;
     R.chain = chain;
     R.shift = shift;
     R.vf[0] = vfactor ? vfactor[0] : 1.0;
@@ -822,9 +827,14 @@ static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, c
     R.beads.resize(n);
     for (int k = 0; k < n; ++k) {
         const int m = slice_ptcl[2 * k], i = slice_ptcl[2 * k + 1];
-        if (m < 0 || m >= c.d.num_slices || i < 0 || i >= c.d.num_alloc)
+        if (m < 0 || m >= num_slices || i < 0 || i >= num_alloc) {
+            std::lock_guard<std::mutex> g(s->mu);
             return sfail(s, PIMC_E_ARG, "bead locator out of range");
-        if (seen[m]) return sfail(s, PIMC_E_ARG, "request beads must sit on distinct slices");
+        }
+        if (seen[m]) {
+            std::lock_guard<std::mutex> g(s->mu);
+            return sfail(s, PIMC_E_ARG, "request beads must sit on distinct slices");
+        }
         seen[m] = 1;
         VnnBead& b = R.beads[k];
         b.walker = chain;
@@ -836,8 +846,12 @@ static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, c
             b.new_pos[t] = new_pos[3 * k + t];
         }
     }
-    R.t_submit = Clock::now();
     R.tag = tag;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->chains[chain].state == 2) return sfail(s, PIMC_E_ARG, "unknown chain");
+    Chain& c = s->chains[chain];
+    R.id = s->next_id++;
+    R.t_submit = Clock::now();
     const uint64_t id = R.id;
     s->reqs.emplace(id, std::move(R));
     c.queue.push_back(id);
@@ -845,6 +859,26 @@ static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, c
     *req_id = id;
     s->work_events.fetch_add(1, std::memory_order_release);
     s->cv_work.notify_one();
+    return PIMC_OK;
+}
+
+// The results of the ids in `ids` (all completed) in one critical section;
+// the requests' storage is released after the lock.
+static int server_take(pimc_server* s, const std::vector<uint64_t>& ids, std::vector<double>& out5) {
+    std::vector<Request> gone;
+    gone.reserve(ids.size());
+    out5.resize(5 * ids.size());
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (size_t k = 0; k < ids.size(); ++k) {
+            auto it = s->reqs.find(ids[k]);
+            if (it == s->reqs.end() || !it->second.done) return sfail(s, PIMC_E_STATE, "result not ready");
+            for (int t = 0; t < 4; ++t) out5[5 * k + t] = it->second.out[t];
+            out5[5 * k + 4] = std::chrono::duration<double, std::micro>(it->second.t_done - it->second.t_submit).count();
+            gone.push_back(std::move(it->second));
+            s->reqs.erase(it);
+        }
+    }
     return PIMC_OK;
 }
 
@@ -867,7 +901,9 @@ int pimc_server_wait(pimc_server* s, uint64_t req_id, double* out5) {
         for (int t = 0; t < 4; ++t) out5[t] = it->second.out[t];
         out5[4] = std::chrono::duration<double, std::micro>(it->second.t_done - it->second.t_submit).count();
     }
+    Request gone = std::move(it->second);   // its storage is released after the lock
     s->reqs.erase(it);
+    lk.unlock();
     return PIMC_OK;
 }
 
@@ -986,6 +1022,7 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
         for (int c : mine)
             if ((st.err = submit(c, Clock::now()))) return;
     std::vector<uint64_t> finished;
+    std::vector<double> outs;
     while (true) {
         const auto now = Clock::now();
         const bool open = now < t_end;
@@ -1007,14 +1044,15 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
                 return;
             }
         }
-        for (uint64_t id : finished) {
+        if (!finished.empty() && (st.err = server_take(s, finished, outs))) return;
+        for (size_t f = 0; f < finished.size(); ++f) {
+            const uint64_t id = finished[f];
             auto it = by_id.find(id);
             if (it == by_id.end()) continue;
             const int c = it->second;
             by_id.erase(it);
             GenChain& g = gc[c];
-            double o[5];
-            if ((st.err = pimc_server_wait(s, id, o))) return;
+            const double* o = outs.data() + 5 * f;
             const auto t_done = Clock::now();
             st.lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
             st.svc.push_back(o[4]);

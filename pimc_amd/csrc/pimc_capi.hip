@@ -1207,6 +1207,8 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
         if (cor_out)
             for (int k = 0; k < n; ++k) cor_out[k] = cor ? c->h_per_bead_cor[2 * k] : 0.0;
     } else if (c->wait_done(A.seq) != 0) {
+        // the arrival counter may have been left part-way: clear it for the next call
+        (void)hipMemsetAsync(c->d_vnn_ticket, 0, sizeof(unsigned), c->stream);
         return fail(c, PIMC_E_HIP, "delta action: completion not published");
     }
     if (out3)

@@ -117,48 +117,40 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     // damping step, the out-of-table select, worm factors).  Running the heads of
     // two pairs back to back, fenced so the scheduler interleaves them, measured
     // slower at N = 2048 x 250 (1.009 ms against 0.981) and N = 256 x 167.
-    struct Head {
-        VmixHead h;
-        double v;
-        int k, kk;
+    struct Pos {
+        double x, y, z;
         unsigned c;
     };
-    auto head = [&](unsigned c) -> Head {
-        Head p;
+    auto load = [&](unsigned c) -> Pos {
+        Pos p;
         p.c = c;
-        double xj, yj, zj;
         if (PS) {
-            xj = *reinterpret_cast<const double*>(lb + c);
-            yj = *reinterpret_cast<const double*>(lb + c + 8 * PSL);
-            zj = *reinterpret_cast<const double*>(lb + c + 16 * PSL);
+            p.x = *reinterpret_cast<const double*>(lb + c);
+            p.y = *reinterpret_cast<const double*>(lb + c + 8 * PSL);
+            p.z = *reinterpret_cast<const double*>(lb + c + 16 * PSL);
         } else {
-            xj = px[c];
-            yj = py[c];
-            zj = pz[c];
-        }
-        if (MODE == 0) {
-            p.k = bin_mixed(q, xj, yj, zj);
-            // k_eps <= k < tableLength, else V = 0 (include/potential.h:252-257): the
-            // clamped bin keeps the arithmetic finite, and only a wave holding such a
-            // pair pays for the select
-            p.kk = min(max(p.k, a.k_eps), lenm1);
-            p.h = aziz_vmix_head(p.kk, av);
-        } else {
-            const double r2 = sep_r2<MODE>(xi, yi, zi, xj, yj, zj, box);
-            p.v = pair_value_from_r2<MODE>(r2, a, inv_dr);
+            p.x = px[c];
+            p.y = py[c];
+            p.z = pz[c];
         }
         return p;
     };
-    auto tail = [&](const Head& p) -> double {
+    auto eval = [&](const Pos& p) -> double {
         double v;
         if (MODE == 0) {
-            v = aziz_vmix_damp(p.kk, p.h, av);
-            if (__builtin_amdgcn_ballot_w64(p.k != p.kk)) {
+            const int k = bin_mixed(q, p.x, p.y, p.z);
+            // k_eps <= k < tableLength, else V = 0 (include/potential.h:252-257): the
+            // clamped bin keeps the arithmetic finite, and only a wave holding such a
+            // pair pays for the select
+            const int kk = min(max(k, a.k_eps), lenm1);
+            v = aziz_vmix(kk, av);
+            if (__builtin_amdgcn_ballot_w64(k != kk)) {
                 asm volatile("");
-                v = (p.k != p.kk) ? 0.0 : v;
+                v = (k != kk) ? 0.0 : v;
             }
         } else {
-            v = p.v;
+            const double r2 = sep_r2<MODE>(xi, yi, zi, p.x, p.y, p.z, box);
+            v = pair_value_from_r2<MODE>(r2, a, inv_dr);
         }
         if (FACTORS) {
             int j = (int)(p.c / kW);
@@ -168,7 +160,6 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
         }
         return v;
     };
-    auto pair = [&](unsigned c) -> double { return tail(head(c)); };
     auto advance = [&]() {
         const unsigned jn = jc + step;
         jc = DUP ? jn : min(jn, jn - span);   // unsigned: jn - span wraps above jn while jn < span
@@ -182,14 +173,25 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
             acc += v;
         dd += G;
     };
-    for (int t = 1; t < T; ++t) {
-        add(pair(jc));
-        advance();
-    }
-    // the last offset: present for the lane groups whose d reaches it (a lane
+    // the last offset is present for the lane groups whose d reaches it (a lane
     // past d_end reads bead 0 and drops it)
     const bool ok = (d + (T - 1) * G < d_end) && !(MASK && (d + (T - 1) * G) == skip_d);
-    const double v = pair(ok ? jc : 0u);
+    if (T == 1) {
+        const double v = eval(load(ok ? jc : 0u));
+        return ok ? v : 0.0;
+    }
+    // each partner is read one iteration ahead of its evaluation
+    Pos p = load(jc);
+    for (int t = 1; t < T - 1; ++t) {
+        advance();
+        const Pos pn = load(jc);
+        add(eval(p));
+        p = pn;
+    }
+    advance();
+    const Pos pl = load(ok ? jc : 0u);
+    add(eval(p));
+    const double v = eval(pl);
     acc += ok ? v : 0.0;
     return acc;
 }

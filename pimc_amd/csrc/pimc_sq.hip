@@ -11,9 +11,10 @@
 // (O(n_q M^2)) instead of O(n_q M^2 N^2) trigonometric pair terms.
 //
 // Lattice q = (2 pi / L) n: exp(i q.r) = prod_d exp(2 pi i n_d s_d), s_d = x_d / L.
-// Each workgroup stages per-bead axis phasors exp(2 pi i n s_d) for |n| <= nmax in
-// LDS (phase reduced in FP64, FP32 sincospi), then each lane accumulates a run of
-// q-vectors as two complex FP32 products per (q, bead) with FP64 per-chunk sums.
+// Each workgroup stages per-bead axis phasors exp(2 pi i n s_d) for -nmax <= n <=
+// nmax in LDS (phase reduced in FP64, FP32 sincospi; the negative n are the
+// conjugates), then each lane accumulates a run of q-vectors as two complex
+// FP32 products per (q, bead), 8 FMA-type instructions, with FP64 per-chunk sums.
 #include <hip/hip_runtime.h>
 
 #include "pimc_kernels.hpp"
@@ -28,13 +29,14 @@ constexpr int kReseed = 4;         // exact sincospi every kReseed powers, produ
 // rho_q(m) for lattice q.  Lane t of a workgroup takes q = base + t + 256 k
 // (k < QPT), so a wave walks consecutive q: mostly one (n_x, n_y) and consecutive
 // n_z, which makes the x/y phasor reads LDS broadcasts and the z reads
-// consecutive words.  Tables are bead-major: ph[axis][bead][n].
+// consecutive words.  Tables are bead-major: ph[axis][bead][n + nmax].
 template <int QPT>
 __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
     extern __shared__ float2 ph[];
     const int m = blockIdx.y;
     const int n_beads = A.nbs[m];
-    const int w0 = A.nmax[0] + 1, w1 = A.nmax[1] + 1, w2 = A.nmax[2] + 1;
+    const int h0 = A.nmax[0], h1 = A.nmax[1], h2 = A.nmax[2];
+    const int w0 = 2 * h0 + 1, w1 = 2 * h1 + 1, w2 = 2 * h2 + 1;
     float2* phx = ph;
     float2* phy = phx + w0 * kRhoBeadChunk;
     float2* phz = phy + w1 * kRhoBeadChunk;
@@ -43,8 +45,7 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
     const double* pz = py + A.plane;
 
     const int qbase = blockIdx.x * kRhoThreads * QPT + threadIdx.x;
-    int ax[QPT], ay[QPT], az[QPT];
-    float sx[QPT], sy[QPT], sz[QPT];
+    int ix[QPT], iy[QPT], iz[QPT];
 #pragma unroll
     for (int k = 0; k < QPT; ++k) {
         const int qi = qbase + k * kRhoThreads;
@@ -54,12 +55,9 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
             ny = A.qn[3 * qi + 1];
             nz = A.qn[3 * qi + 2];
         }
-        ax[k] = nx < 0 ? -nx : nx;
-        ay[k] = ny < 0 ? -ny : ny;
-        az[k] = nz < 0 ? -nz : nz;
-        sx[k] = nx < 0 ? -1.f : 1.f;
-        sy[k] = ny < 0 ? -1.f : 1.f;
-        sz[k] = nz < 0 ? -1.f : 1.f;
+        ix[k] = nx + h0;
+        iy[k] = ny + h1;
+        iz[k] = nz + h2;
     }
     double are[QPT], aim[QPT];
 #pragma unroll
@@ -68,13 +66,14 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
     for (int b0 = 0; b0 < n_beads; b0 += kRhoBeadChunk) {
         const int nb = min(kRhoBeadChunk, n_beads - b0);
         __syncthreads();
-        // one lane per (axis, bead) row: exp(2 pi i n s) for n = 0..nmax
+        // one lane per (axis, bead) row: exp(2 pi i n s) for n = 0..nmax, and the
+        // conjugates at -n
         for (int r = threadIdx.x; r < 3 * kRhoBeadChunk; r += kRhoThreads) {
             const int axis = r / kRhoBeadChunk, bb = r % kRhoBeadChunk;
-            const int w = axis == 0 ? w0 : (axis == 1 ? w1 : w2);
-            float2* row = (axis == 0 ? phx : (axis == 1 ? phy : phz)) + bb * w;
+            const int h = axis == 0 ? h0 : (axis == 1 ? h1 : h2);
+            float2* row = (axis == 0 ? phx : (axis == 1 ? phy : phz)) + bb * (2 * h + 1) + h;
             if (bb >= nb) {
-                for (int n = 0; n < w; ++n) row[n] = make_float2(0.f, 0.f);
+                for (int n = -h; n <= h; ++n) row[n] = make_float2(0.f, 0.f);
                 continue;
             }
             const double x = (axis == 0 ? px : (axis == 1 ? py : pz))[b0 + bb];
@@ -85,7 +84,7 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
                 sincospif((float)(2.0 * red), &s1, &c1);
             }
             float cr = 1.f, ci = 0.f;
-            for (int n = 0; n < w; ++n) {
+            for (int n = 0; n <= h; ++n) {
                 if (n % kReseed == 0) {
                     const double t = (double)n * sfrac;
                     const double red = t - rint(t);
@@ -97,6 +96,7 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
                     ci = ni;
                 }
                 row[n] = make_float2(cr, ci);
+                row[-n] = make_float2(cr, -ci);
             }
         }
         __syncthreads();
@@ -106,14 +106,15 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
         for (int bb = 0; bb < nb; ++bb) {
 #pragma unroll
             for (int k = 0; k < QPT; ++k) {
-                const float2 ex = phx[bb * w0 + ax[k]];
-                const float2 ey = phy[bb * w1 + ay[k]];
-                const float2 ez = phz[bb * w2 + az[k]];
-                const float exi = sx[k] * ex.y, eyi = sy[k] * ey.y, ezi = sz[k] * ez.y;
-                const float xr = fmaf(ex.x, ey.x, -exi * eyi);
-                const float xi = fmaf(ex.x, eyi, exi * ey.x);
-                sre[k] += fmaf(xr, ez.x, -xi * ezi);
-                sim[k] += fmaf(xr, ezi, xi * ez.x);
+                const float2 ex = phx[bb * w0 + ix[k]];
+                const float2 ey = phy[bb * w1 + iy[k]];
+                const float2 ez = phz[bb * w2 + iz[k]];
+                const float xr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+                const float xi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                sre[k] = fmaf(xr, ez.x, sre[k]);
+                sre[k] = fmaf(-xi, ez.y, sre[k]);
+                sim[k] = fmaf(xr, ez.y, sim[k]);
+                sim[k] = fmaf(xi, ez.x, sim[k]);
             }
         }
 #pragma unroll
@@ -267,7 +268,7 @@ hipError_t launch_rho(bool lattice, hipStream_t st, const RhoArgs& A) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (A.nq <= 0) return hipSuccess;
     if (lattice) {
-        const size_t lds = sizeof(float2) * (A.nmax[0] + A.nmax[1] + A.nmax[2] + 3) * kRhoBeadChunk;
+        const size_t lds = sizeof(float2) * (2 * (A.nmax[0] + A.nmax[1] + A.nmax[2]) + 3) * kRhoBeadChunk;
         // the most q per lane that still leaves >= 2048 workgroups
         auto blocks = [&](int qpt) { return (int64_t)((A.nq + kRhoThreads * qpt - 1) / (kRhoThreads * qpt)) * A.num_slices; };
         int qpt = 8;

@@ -45,6 +45,9 @@ def main():
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (HBM counter scaling)")
+    ap.add_argument("--pair-only", type=int, nargs=2, metavar=("N", "M"), default=None,
+                    help="profiling driver: only the full potential action of N beads x M slices, 5 calls "
+                         "(mixed mode, rc = L, or L/2 from N = 2048), no bench line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU sharding is a design note "
@@ -54,6 +57,16 @@ def main():
     import pimc_amd
     from pimc_amd import worldlines
     from pimc_amd.workloads import DENSITY, LAMBDA, SEED0, box_side, lattice_q
+    if args.pair_only:
+        N, M = args.pair_only
+        L = box_side(N, DENSITY)
+        b = worldlines.generate(SEED0, L, M, N, LAMBDA, 0.004, 1)
+        ctx = pimc_amd.Context(L, M, N, 0.004, LAMBDA, L / 2 if N >= 2048 else L, v_mode=pimc_amd.V_FP32)
+        ctx.upload_path(b)
+        for _ in range(5):
+            a = ctx.potentialAction()
+        print(json.dumps({"pair_only": [N, M], "action": a}), flush=True)
+        return
 
     N, T, tau, M = 256, 1.5, 0.004, 167      # M as the workload states (the reference would force 166)
     L = box_side(N, DENSITY)

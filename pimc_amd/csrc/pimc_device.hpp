@@ -94,45 +94,30 @@ __device__ __forceinline__ double split_exp2(double e) {
     return (double)fast_exp2(eh) * fma(el, 0.69314718055994530942, 1.0);
 }
 
-// The branch-free part (repulsion and dispersion) and the damping step apart,
-// so that a loop can run the first part of two pairs back to back.
-struct VmixHead {
-    double rep, disp, kd, iq;
-};
-
-__device__ __forceinline__ VmixHead aziz_vmix_head(int k, const AzizDev& a) {
-    VmixHead h;
-    h.kd = (double)k;
-    h.rep = split_exp2(fma(fma(a.rep_c2, h.kd, a.rep_c1), h.kd, a.rep_c0));
+__device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
+    const double kd = (double)k;
+    const double rep = split_exp2(fma(fma(a.rep_c2, kd, a.rep_c1), kd, a.rep_c0));
     // 1/x^2 = s^-2 / k^2: the dispersion is a polynomial in 1/k^2 with the powers
     // of s folded into its coefficients.  k^2 is exact in FP64; 1/k^2 is the
     // hardware FP64 reciprocal estimate (relative error ~2^-23) refined by one
     // Newton step (~2^-46).
-    const double q = h.kd * h.kd;
+    const double q = kd * kd;
     const double r0 = __builtin_amdgcn_rcp(q);
-    h.iq = fma(r0, fma(-q, r0, 1.0), r0);
-    h.disp = (h.iq * h.iq * h.iq) * fma(fma(a.eC10k, h.iq, a.eC8k), h.iq, a.eC6k);
-    return h;
-}
-
-__device__ __forceinline__ double aziz_vmix_damp(int k, const VmixHead& h, const AzizDev& a) {
+    const double iq = fma(r0, fma(-q, r0, 1.0), r0);
+    const double disp = (iq * iq * iq) * fma(fma(a.eC10k, iq, a.eC8k), iq, a.eC6k);
     // F = 1 at and above D: rep - disp (== fma(-disp, 1, rep)); a wave whose
     // pairs all lie beyond D skips the damping block
-    double v = h.rep - h.disp;
+    double v = rep - disp;
     if (k < a.k_d) {
         // F(x) = exp(-(D/x - 1)^2) for x < D, with D/x - 1 = (D/s) k / k^2 - 1
-        const double t = fma(a.Ds * h.kd, h.iq, -1.0);
+        const double t = fma(a.Ds * kd, iq, -1.0);
         const double F = split_exp2(-1.4426950408889634074 * t * t);
         // Below x = 0.01 (k < k_hc) valueV is the bare repulsion; there t >= D/0.01 - 1
         // > 120, so exp2(-t^2 log2 e) underflows to exactly 0 in FP32, F = 0 and
         // fma(-disp, 0, rep) == rep: the hard-core branch needs no select.
-        v = fma(-h.disp, F, h.rep);
+        v = fma(-disp, F, rep);
     }
     return v;
-}
-
-__device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
-    return aziz_vmix_damp(k, aziz_vmix_head(k, a), a);
 }
 
 // valueV at the bin radius, FP64 arithmetic.  Requires k_eps <= k < table_length.

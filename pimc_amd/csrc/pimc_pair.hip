@@ -113,10 +113,9 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
         av.eC8k = in_vgpr(a.eC8k);
     }
     const char* lb = reinterpret_cast<const char*>(px);
-    // A pair in two parts: head (reads, bin, the branch-free terms) and tail (the
-    // damping step, the out-of-table select, worm factors).  Running the heads of
-    // two pairs back to back, fenced so the scheduler interleaves them, measured
-    // slower at N = 2048 x 250 (1.009 ms against 0.981) and N = 256 x 167.
+    // A partner's position (load) and its pair term (eval).  Evaluating two pairs'
+    // branch-free parts back to back, fenced so the scheduler interleaves them,
+    // measured slower at N = 2048 x 250 (1.009 ms against 0.981) and N = 256 x 167.
     struct Pos {
         double x, y, z;
         unsigned c;

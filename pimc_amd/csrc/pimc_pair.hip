@@ -74,11 +74,13 @@ __device__ __forceinline__ int bin_mixed(const RowMixed& q, double xj, double yj
 // i < n/2): the lane skips d == skip_d.
 template <int PS>
 constexpr bool pair_lds_dup() {
-    return PS > 0 && PS <= 512;
+    // at 2048 beads the extra 24 KB per workgroup costs more occupancy than the
+    // wrap it saves (N = 2048 x 250: 1.013 ms against 0.952)
+    return PS > 0 && PS <= 1024;
 }
 template <int PS>
 constexpr int pair_lds_plane() {
-    return pair_lds_dup<PS>() ? 2 * PS : PS;
+    return pair_lds_dup<PS>() ? PS + PS / 2 + 64 : PS;   // room for n + n/2 + 1 beads, 64-bead aligned
 }
 
 template <int MODE, bool MASK, int PS, bool FACTORS>

@@ -94,9 +94,12 @@ __device__ __forceinline__ double split_exp2(double e) {
     return (double)fast_exp2(eh) * fma(el, 0.69314718055994530942, 1.0);
 }
 
+// BETA0: the caller knows beta = 0 (rep_c2 = 0; the 1979 potential), so the
+// exponent's quadratic term (an exact no-op then) is left out.
+template <bool BETA0 = false>
 __device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
     const double kd = (double)k;
-    const double rep = split_exp2(fma(fma(a.rep_c2, kd, a.rep_c1), kd, a.rep_c0));
+    const double rep = split_exp2(BETA0 ? fma(a.rep_c1, kd, a.rep_c0) : fma(fma(a.rep_c2, kd, a.rep_c1), kd, a.rep_c0));
     // 1/x^2 = s^-2 / k^2: the dispersion is a polynomial in 1/k^2 with the powers
     // of s folded into its coefficients.  k^2 is exact in FP64; 1/k^2 is the
     // hardware FP64 reciprocal estimate (relative error ~2^-23) refined by one

@@ -83,7 +83,7 @@ constexpr int pair_lds_plane() {
     return pair_lds_dup<PS>() ? PS + PS / 2 + 64 : PS;   // room for n + n/2 + 1 beads, 64-bead aligned
 }
 
-template <int MODE, bool MASK, int PS, bool FACTORS>
+template <int MODE, bool MASK, int PS, bool FACTORS, bool BETA0, bool NOWRAP>
 __device__ __forceinline__ double row_loop(const double* px, const double* py, const double* pz, int n, int i,
                                            int d, int d_end, int T, int G, const BoxDev& box, const AzizDev& a,
                                            double inv_dr, const WormDev& worm, int slice) {
@@ -144,7 +144,7 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
             // clamped bin keeps the arithmetic finite, and only a wave holding such a
             // pair pays for the select
             const int kk = min(max(k, a.k_eps), lenm1);
-            v = aziz_vmix(kk, av);
+            v = aziz_vmix<BETA0>(kk, av);
             if (__builtin_amdgcn_ballot_w64(k != kk)) {
                 asm volatile("");
                 v = (k != kk) ? 0.0 : v;
@@ -163,7 +163,7 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     };
     auto advance = [&]() {
         const unsigned jn = jc + step;
-        jc = DUP ? jn : min(jn, jn - span);   // unsigned: jn - span wraps above jn while jn < span
+        jc = (DUP || NOWRAP) ? jn : min(jn, jn - span);   // unsigned: jn - span wraps above jn while jn < span
     };
     double acc = 0.0;
     int dd = d;
@@ -197,17 +197,24 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     return acc;
 }
 
-// The loop variants: MASK (the d = n/2 offset inside the tile) and FACTORS (a
-// worm on the slice), both fixed per tile, so neither costs a branch per pair.
+// The loop variants, each fixed per tile so none costs a branch per pair: MASK
+// (the d = n/2 offset inside the tile), FACTORS (a worm on the slice), BETA0
+// (beta = 0, mixed mode) and NOWRAP (no row + offset of the tile reaches n, so
+// the partner cursor needs no wrap even without the duplicated staging).
 template <int MODE, int PS>
 __device__ __forceinline__ double row_tile(const double* px, const double* py, const double* pz, int n, int i, int d,
                                            int d_end, int T, int G, bool mask, const BoxDev& box, const AzizDev& a,
-                                           double inv_dr, bool factors, const WormDev& worm, int slice) {
-    if (factors)
-        return mask ? row_loop<MODE, true, PS, true>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice)
-                    : row_loop<MODE, false, PS, true>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice);
-    return mask ? row_loop<MODE, true, PS, false>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice)
-                : row_loop<MODE, false, PS, false>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice);
+                                           double inv_dr, bool factors, bool nowrap, const WormDev& worm,
+                                           int slice) {
+#define PIMC_ROW(M_, F_, B_, W_) \
+    row_loop<MODE, M_, PS, F_, B_, W_>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice)
+    if (factors) return mask ? PIMC_ROW(true, true, false, false) : PIMC_ROW(false, true, false, false);
+    if (mask) return PIMC_ROW(true, false, false, false);
+    const bool beta0 = MODE == 0 && a.rep_c2 == 0.0;
+    if (!pair_lds_dup<PS>() && nowrap)
+        return beta0 ? PIMC_ROW(false, false, true, true) : PIMC_ROW(false, false, false, true);
+    return beta0 ? PIMC_ROW(false, false, true, false) : PIMC_ROW(false, false, false, false);
+#undef PIMC_ROW
 }
 
 // Persistent: the workgroups of one XCD take consecutive tile ranges (so a
@@ -270,10 +277,11 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
             const int d = tl.d0 + grp, d_end = tl.d0 + tl.dn;
             const int T = (tl.dn + G - 1) / G;
             const bool mask = ((n & 1) == 0) && d_end > (n >> 1) && tl.i_end > (n >> 1);
+            const bool nowrap = (tl.i_end - 1) + (d_end - 1) < n;
             acc = in_lds ? row_tile<MODE, PS>(lds, nullptr, nullptr, n, i, d, d_end, T, G, mask, wk.box, a, inv_dr,
-                                              factors, wk.worm, tl.m)
+                                              factors, nowrap, wk.worm, tl.m)
                          : row_tile<MODE, 0>(px, py, pz, n, i, d, d_end, T, G, mask, wk.box, a, inv_dr, factors,
-                                             wk.worm, tl.m);
+                                             nowrap, wk.worm, tl.m);
         }
         const double s = block_sum<kPairThreads>(acc, red);
         if (threadIdx.x == 0) partials[k] = s;

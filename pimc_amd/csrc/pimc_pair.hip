@@ -70,8 +70,10 @@ __device__ __forceinline__ int bin_mixed(const RowMixed& q, double xj, double yj
 // reads; PS == 0: px, py, pz in global memory.  DUP: each LDS plane holds the
 // slice followed by its first n/2 + 1 beads again, so j = i + d < 3n/2 needs no
 // wrap; otherwise the cursor wraps with an unsigned min.
-// MASK: the offsets reach d = n/2 of an even n, whose pairs count once (rows
-// i < n/2): the lane skips d == skip_d.
+// mask: the offsets reach d = n/2 of an even n, whose pairs count once (rows
+// i < n/2).  The group holding offset n/2 = d_end - 1 has it as its largest
+// offset, and its offset count, ceil((dn - g)/G) for g = (dn - 1) mod G, is T:
+// so d = n/2 only ever falls in the last iteration, where the lane drops it.
 template <int PS>
 constexpr bool pair_lds_dup() {
     // at 2048 beads the extra 24 KB per workgroup costs more occupancy than the
@@ -83,10 +85,10 @@ constexpr int pair_lds_plane() {
     return pair_lds_dup<PS>() ? PS + PS / 2 + 64 : PS;   // room for n + n/2 + 1 beads, 64-bead aligned
 }
 
-template <int MODE, bool MASK, int PS, bool FACTORS, bool BETA0, bool NOWRAP>
+template <int MODE, int PS, bool FACTORS, bool BETA0, bool NOWRAP>
 __device__ __forceinline__ double row_loop(const double* px, const double* py, const double* pz, int n, int i,
-                                           int d, int d_end, int T, int G, const BoxDev& box, const AzizDev& a,
-                                           double inv_dr, const WormDev& worm, int slice) {
+                                           int d, int d_end, int T, int G, bool mask, const BoxDev& box,
+                                           const AzizDev& a, double inv_dr, const WormDev& worm, int slice) {
     constexpr bool DUP = pair_lds_dup<PS>();
     constexpr int PSL = pair_lds_plane<PS>();
     const double xi = px[i];
@@ -98,7 +100,7 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     const unsigned step = in_vgpr((unsigned)(G % n) * kW);
     const unsigned span = in_vgpr((unsigned)n * kW);
     unsigned jc = (unsigned)j0 * kW;
-    const int skip_d = (MASK && i >= (n >> 1)) ? (n >> 1) : -1;
+    const int skip_d = (mask && i >= (n >> 1)) ? (n >> 1) : -1;
     RowMixed q;
     if (MODE == 0) {
         q.invL = box.invL;
@@ -166,17 +168,10 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
         jc = (DUP || NOWRAP) ? jn : min(jn, jn - span);   // unsigned: jn - span wraps above jn while jn < span
     };
     double acc = 0.0;
-    int dd = d;
-    auto add = [&](double v) {
-        if (MASK)
-            acc += (dd == skip_d) ? 0.0 : v;
-        else
-            acc += v;
-        dd += G;
-    };
+    auto add = [&](double v) { acc += v; };
     // the last offset is present for the lane groups whose d reaches it (a lane
     // past d_end reads bead 0 and drops it)
-    const bool ok = (d + (T - 1) * G < d_end) && !(MASK && (d + (T - 1) * G) == skip_d);
+    const bool ok = (d + (T - 1) * G < d_end) && (d + (T - 1) * G) != skip_d;
     if (T == 1) {
         const double v = eval(load(ok ? jc : 0u));
         return ok ? v : 0.0;
@@ -197,23 +192,21 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     return acc;
 }
 
-// The loop variants, each fixed per tile so none costs a branch per pair: MASK
-// (the d = n/2 offset inside the tile), FACTORS (a worm on the slice), BETA0
-// (beta = 0, mixed mode) and NOWRAP (no row + offset of the tile reaches n, so
-// the partner cursor needs no wrap even without the duplicated staging).
+// The loop variants, each fixed per tile so none costs a branch per pair:
+// FACTORS (a worm on the slice), BETA0 (beta = 0, mixed mode) and NOWRAP (no
+// row + offset of the tile reaches n, so the partner cursor needs no wrap even
+// without the duplicated staging).
 template <int MODE, int PS>
 __device__ __forceinline__ double row_tile(const double* px, const double* py, const double* pz, int n, int i, int d,
                                            int d_end, int T, int G, bool mask, const BoxDev& box, const AzizDev& a,
                                            double inv_dr, bool factors, bool nowrap, const WormDev& worm,
                                            int slice) {
-#define PIMC_ROW(M_, F_, B_, W_) \
-    row_loop<MODE, M_, PS, F_, B_, W_>(px, py, pz, n, i, d, d_end, T, G, box, a, inv_dr, worm, slice)
-    if (factors) return mask ? PIMC_ROW(true, true, false, false) : PIMC_ROW(false, true, false, false);
-    if (mask) return PIMC_ROW(true, false, false, false);
+#define PIMC_ROW(F_, B_, W_) \
+    row_loop<MODE, PS, F_, B_, W_>(px, py, pz, n, i, d, d_end, T, G, mask, box, a, inv_dr, worm, slice)
+    if (factors) return PIMC_ROW(true, false, false);
     const bool beta0 = MODE == 0 && a.rep_c2 == 0.0;
-    if (!pair_lds_dup<PS>() && nowrap)
-        return beta0 ? PIMC_ROW(false, false, true, true) : PIMC_ROW(false, false, false, true);
-    return beta0 ? PIMC_ROW(false, false, true, false) : PIMC_ROW(false, false, false, false);
+    if (!pair_lds_dup<PS>() && nowrap) return beta0 ? PIMC_ROW(false, true, true) : PIMC_ROW(false, false, true);
+    return beta0 ? PIMC_ROW(false, true, false) : PIMC_ROW(false, false, false);
 #undef PIMC_ROW
 }
 

@@ -45,8 +45,8 @@ constexpr double kCorrEps = 1.0e-7;
 // workgroup's 256 lanes hold one row each (n >= 256) or form 256/n groups of n
 // rows (n < 256), group g taking offsets d0 + g, d0 + g + 256/n, ...; a lane's
 // row bead stays in registers while its partner index steps through the slice.
-// For n >= 256 the d = n/2 pairs form tiles of their own (rows < n/2); for
-// n < 256 they stay in the slice's tile and the kernel masks rows >= n/2.
+// The d = n/2 pairs of an even n stay in the tile holding that offset (always
+// the last iteration of its lane group), where rows >= n/2 drop them.
 // A tile carries its slice's fields, so a workgroup reaches the positions and
 // the walker in one dependent load after the tile's.
 struct RowTile {

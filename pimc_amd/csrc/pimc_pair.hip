@@ -1127,12 +1127,13 @@ int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lan
         for (int d0 = 1; d0 <= dmax; d0 += dn) tile(0, n, d0, std::min(dn, dmax - d0 + 1));
         return (int64_t)n * (n - 1) / 2;
     }
-    const int full = (n - 1) / 2;                      // offsets taken by every row
+    // row blocks of 256 against offsets 1..n/2; the d = n/2 pairs of an even n
+    // count once: rows >= n/2 drop that offset, in the last iteration of the
+    // block's last tile
+    const int dmax = n / 2;
     for (int i0 = 0; i0 < n; i0 += R) {
         const int i_end = std::min(i0 + R, n);
-        for (int d0 = 1; d0 <= full; d0 += dn) tile(i0, i_end, d0, std::min(dn, full - d0 + 1));
-        if ((n & 1) == 0 && i0 < n / 2)                 // the d = n/2 pairs, once each
-            tile(i0, std::min(i_end, n / 2), n / 2, 1);
+        for (int d0 = 1; d0 <= dmax; d0 += dn) tile(i0, i_end, d0, std::min(dn, dmax - d0 + 1));
     }
     return (int64_t)n * (n - 1) / 2;
 }

@@ -124,6 +124,22 @@ def test_total_action_degenerate_pairs(orc, gpu, mode):
 
 
 @pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("year", [1987, 1995])
+def test_total_action_other_aziz_years(orc, gpu, mode, year):
+    """The 1987 and 1995 parameter sets (beta != 0 in 1987: the repulsion's
+    quadratic exponent term) through the all-pairs kernel."""
+    N, M = 64, 12
+    L = orc.box_side(N, DENSITY)
+    beads = orc.gen_worldlines(SEED0, L, M, N, LAMBDA, 0.004, 1)
+    ref = orc.State(L, M, N, 0.004, LAMBDA, L, year=year)
+    ref.set_beads(beads)
+    ctx = gpu.Context(L, M, N, 0.004, LAMBDA, L, year=year, v_mode=mode)
+    ctx.upload_path(beads)
+    a, b = ctx.potentialAction(), ref.total_action()
+    assert abs(a - b) <= TOL[mode] * abs(b), (year, mode, a, b)
+
+
+@pytest.mark.parametrize("mode", [0, 2])
 @pytest.mark.parametrize("N", [300, 700])
 def test_total_action_lds_strides(orc, gpu, mode, N):
     """Slices staged with the 512- and 1024-bead LDS plane strides."""

@@ -609,6 +609,65 @@ __global__ __launch_bounds__(256) void k_gradv2_chunk(const double* __restrict__
     o[2] = fz;
 }
 
+// Mixed mode, slices of at most 256 beads: each unordered pair once.  Chunk c
+// takes offsets d in [c Nd, (c+1) Nd) of 1..n/2; lane i adds g s_ij to its own
+// force and -g s_ij (gradV(r_j - r_i)) to partner j's entry in its wave's LDS
+// array (within one wave the partners i + d are distinct, and every wave has
+// its own array, so no two lanes touch one entry at once); the row's partial
+// force is its own sum plus the four waves' entries in wave order.  Half the
+// gradV evaluations of k_gradv2_chunk for the same partial layout.
+__global__ __launch_bounds__(256) void k_gradv2_sym(const double* __restrict__ pos, int64_t plane, int nalloc,
+                                                    const int* __restrict__ nbs, BoxDev box, AzizDev az, int C,
+                                                    double* __restrict__ fpart) {
+    __shared__ double lp[3][256];
+    __shared__ double fj[4][3][256];
+    const int m = blockIdx.y, c = blockIdx.x;
+    const int n = nbs[m];
+    const int i = threadIdx.x, wave = threadIdx.x >> 6;
+    const double* px = pos + (int64_t)m * nalloc;
+    if (i < n) {
+        lp[0][i] = px[i];
+        lp[1][i] = px[plane + i];
+        lp[2][i] = px[2 * plane + i];
+    }
+#pragma unroll
+    for (int w = 0; w < 4; ++w)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) fj[w][a][i] = 0.0;
+    __syncthreads();
+    const double inv_dr = 1.0 / az.dr;
+    const int nd = n / 2;
+    const int per = (nd + C - 1) / C;
+    const int d0 = 1 + c * per, d1 = min(nd + 1, d0 + per);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (i < n) {
+        const double xi = lp[0][i], yi = lp[1][i], zi = lp[2][i];
+        for (int d = d0; d < d1; ++d) {
+            if ((n & 1) == 0 && d == nd && i >= nd) continue;   // the d = n/2 pairs once each
+            int j = i + d;
+            if (j >= n) j -= n;
+            const double dx = min_image_fast(xi - lp[0][j], box.L, box.invL);
+            const double dy = min_image_fast(yi - lp[1][j], box.L, box.invL);
+            const double dz = min_image_fast(zi - lp[2][j], box.L, box.invL);
+            const double g = grad_factor_from_r2<0>(norm2_exact(dx, dy, dz), az, inv_dr);
+            const double gx = g * dx, gy = g * dy, gz = g * dz;
+            fx += gx;
+            fy += gy;
+            fz += gz;
+            fj[wave][0][j] -= gx;
+            fj[wave][1][j] -= gy;
+            fj[wave][2][j] -= gz;
+        }
+    }
+    __syncthreads();
+    if (i < n) {
+        double* o = fpart + (((int64_t)m * C + c) * nalloc + i) * 3;
+        o[0] = (((fx + fj[0][0][i]) + fj[1][0][i]) + fj[2][0][i]) + fj[3][0][i];
+        o[1] = (((fy + fj[0][1][i]) + fj[1][1][i]) + fj[2][1][i]) + fj[3][1][i];
+        o[2] = (((fz + fj[0][2][i]) + fj[1][2][i]) + fj[2][2][i]) + fj[3][2][i];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_gradv2_sum(const int* __restrict__ nbs, int nalloc, int C,
                                                     const double* __restrict__ fpart, double* __restrict__ part) {
     __shared__ double red[4];
@@ -1210,6 +1269,12 @@ hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t pl
     const int B = (nalloc + 255) / 256;
     *blocks_per_slice = B;
     const size_t lds = sizeof(double) * 3 * (size_t)lds_beads;
+    if (chunks > 1 && fpart && mode == 0 && nalloc <= 256) {
+        hipLaunchKernelGGL(k_gradv2_sym, dim3(chunks, num_slices), dim3(256), 0, st, pos, plane, nalloc, nbs, box, az,
+                           chunks, fpart);
+        hipLaunchKernelGGL(k_gradv2_sum, dim3(B, num_slices), dim3(256), 0, st, nbs, nalloc, chunks, fpart, part);
+        return hipGetLastError();
+    }
     if (chunks > 1 && fpart) {
         dim3 gc(B * chunks, num_slices), b(256);
         if (mode == 0)

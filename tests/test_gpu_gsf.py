@@ -47,6 +47,15 @@ def test_total_action_and_energy(orc, gpu, mode, name, action):
         assert abs(eg[k] - er[k]) <= TOL[mode] * max(abs(er[k]), 1.0), (k, eg[k], er[k])
 
 
+@pytest.mark.parametrize("N,M", [(33, 20), (200, 30), (256, 167)])
+def test_gradv_squared_sizes(orc, gpu, N, M):
+    """gradVSquared through each unordered pair once (mixed mode, slices of at
+    most 256 beads), odd and even sizes, against the oracle's GSF action."""
+    ref, ctx, _, _ = pair(orc, gpu, N, M, 0.004, 1.0, 0, GSF)
+    a, b = ctx.potentialAction(), ref.total_action()
+    assert abs(a - b) <= TOL[0] * abs(b), (N, M, a, b)
+
+
 @pytest.mark.parametrize("mode", [0, 2])
 @pytest.mark.parametrize("N,M,rc_frac", [(24, 32, 1.0), (256, 24, 0.25)])
 def test_bead_actions_and_delta(orc, gpu, mode, N, M, rc_frac):

@@ -51,3 +51,16 @@ def test_package_exports_the_mirror_classes():
                  "device_synchronize", "copy_probe", "V_FP32", "V_FP64", "V_TABLE"):
         assert hasattr(pimc_amd, name), name
     assert set(pimc_amd.__all__) <= set(dir(pimc_amd))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message before any
+    device work."""
+    import json
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=60, cwd=ROOT)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "one GPU" in line["error"]

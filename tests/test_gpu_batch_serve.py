@@ -117,16 +117,23 @@ def test_server_delta_v_and_mix(orc):
     assert abs(alone["dU"] - (newa - old)) <= 1e-6 * scale
     # the same request in a mix of many others
     others = []
+    pairs = 2 * (2 * len(sp) * (w0["N"] - 1))   # Vnn at the old and new position, alone and mixed
     for rep in range(30):
         k = 1 + rep % 2
         w, b = chains[k]
         s2, n2 = _segment(rng, w["M"], w["N"], w["side"], b, int(rng.integers(4, 12)))
         others.append(srv.submit(ids[k], s2, n2))
+        pairs += 2 * len(s2) * (w["N"] - 1)
     mixed = srv.submit(ids[0], sp, new)
     res = [srv.wait(r) for r in others]
     m = srv.wait(mixed)
     assert np.array_equal(m["raw"][:4], alone["raw"][:4])
     assert all(np.isfinite(r["dU"]) for r in res)
+    # the server's counters: every request launched once, its pairs counted once
+    st = srv.stats()
+    assert st["requests"] == 32 and 1 <= st["batches"] <= 32
+    assert st["pair_evals"] == pairs
+    assert 0 < st["free_bytes"] < (1 << 28)
 
 
 def test_server_commit_metropolis_and_budget(orc):

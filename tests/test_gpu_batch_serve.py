@@ -48,6 +48,11 @@ def test_batch_against_oracle_and_mix(orc):
     for k, (w, b) in enumerate(ws):
         ref = oracle_action(orc, w, b)
         assert abs(a[k] - ref) <= 1e-6 * abs(ref), (k, a[k], ref)
+    # the work list: every slice's n(n-1)/2 pairs once, at least one tile per slice
+    inf = bt.info()
+    assert inf["total_pairs"] == sum(w["M"] * w["N"] * (w["N"] - 1) // 2 for w, _ in ws)
+    assert inf["n_items"] >= sum(w["M"] for w, _ in ws)
+    assert 1 <= inf["n_groups"] <= inf["n_items"] and inf["last_ms"] >= 0.0
     # alone vs in the mix, same tiles: bit-identical
     for k in (0, 5, 11):
         solo = pimc_amd.Batch([ws[k][0]])

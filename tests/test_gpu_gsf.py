@@ -146,6 +146,7 @@ def test_bare_action_and_correction_parts(orc, gpu, mode, shift, name, action):
         total = ctx.action_list(sp, shift=1)
         assert abs(total - (bare.sum() + cor.sum())) <= 1e-12 * abs(total)
     assert ctx.potentialActionCorrection(sp, shift=shift) == pytest.approx(cor.sum(), rel=1e-14, abs=0)
+    assert np.array_equal(ctx.barePotentialAction(sp, shift=shift), bare)
 
 
 @pytest.mark.parametrize("mode,N,M,rc_frac", [(2, 16, 40, 1.0), (1, 24, 32, 1.0), (2, 64, 24, 0.25)])

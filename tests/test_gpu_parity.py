@@ -206,6 +206,39 @@ def _compare_cells(g, r):
             assert np.array_equal(g["hash"][c, m, :n], r["hash"][c, m, :n])
 
 
+def test_empty_and_invalid_requests(orc, gpu):
+    """Empty bead and q lists give zeros and empty results; out-of-range
+    locators and two moved beads on one slice raise before any launch; the
+    context answers as before afterwards."""
+    ref, ctx, beads, L = setup_pair(orc, gpu, 32, 20, 0.004)
+    a0 = ctx.potentialAction()
+    empty_sp = np.zeros((0, 2), dtype=np.int32)
+    assert np.array_equal(ctx.delta_action(empty_sp, np.zeros((0, 3))), np.zeros(3))
+    assert ctx.action_list(empty_sp) == 0.0
+    assert ctx.ssf(np.zeros((0, 3))).shape == (0,)
+    assert ctx.isf(np.zeros((0, 3))).shape == (0, 20)
+    with pytest.raises(gpu.PimcError):
+        ctx.delta_action([[20, 0]], [[0.0, 0.0, 0.0]])              # slice out of range
+    with pytest.raises(gpu.PimcError):
+        ctx.delta_action([[0, 32]], [[0.0, 0.0, 0.0]])              # particle out of range
+    with pytest.raises(gpu.PimcError):
+        ctx.delta_action([[3, 1], [3, 2]], beads[[3, 3], [1, 2]])   # two moved beads on one slice
+    with pytest.raises(gpu.PimcError):
+        ctx.action_list([[-1, 0]])
+    assert ctx.potentialAction() == a0
+    sp = np.array([[5, 7], [6, 7]], dtype=np.int32)
+    new = beads[sp[:, 0], sp[:, 1]] + 0.1
+    new -= L * np.floor(new / L + 0.5)
+    out = ctx.delta_action(sp, new)
+    old_ref = ref.action_list(sp)
+    for (m, i), q in zip(sp, new):
+        ref.update_bead(int(m), int(i), q)
+    new_ref = ref.action_list(sp)
+    scale = abs(old_ref) + abs(new_ref)
+    assert abs(out[0] - old_ref) <= 1e-6 * scale
+    assert abs(out[2] - (new_ref - old_ref)) <= 1e-6 * scale
+
+
 @pytest.mark.parametrize("frac", [1.0, 0.5, 0.3, 0.22])
 def test_cell_lists_bit_exact(orc, gpu, frac):
     ref, ctx, beads, L = setup_pair(orc, gpu, 96, 6, 0.004, rc_frac=frac)

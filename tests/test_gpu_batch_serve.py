@@ -190,3 +190,32 @@ def test_server_synthetic_load(orc):
     for c in (0, 7, 31):
         b = srv.download(c)
         assert np.all(np.isfinite(b))
+
+
+def test_batch_tiny_walkers(orc):
+    """Walkers of 1, 2 and 3 particles (no pair, one pair, odd n) mixed with a
+    larger one, and a lone one-particle walker: the batch matches the oracle
+    (0 where there is no pair), and so does the single-walker context."""
+    import pimc_amd
+    ws = [walker(orc, N, M, 1.5, SEED0 + k) for k, (N, M) in enumerate([(1, 8), (2, 12), (3, 10), (16, 8)])]
+    bt = pimc_amd.Batch([w for w, _ in ws])
+    for k, (_, b) in enumerate(ws):
+        bt.upload(k, b)
+    a = bt.potentialAction()
+    assert a[0] == 0.0
+    for k, (w, b) in enumerate(ws[1:], start=1):
+        ref = oracle_action(orc, w, b)
+        assert abs(a[k] - ref) <= 1e-6 * abs(ref), (k, a[k], ref)
+    assert bt.info()["total_pairs"] == 12 * 1 + 10 * 3 + 8 * 120
+    lone = pimc_amd.Batch([ws[0][0]])
+    lone.upload(0, ws[0][1])
+    assert lone.potentialAction()[0] == 0.0
+    w, b = ws[0]
+    ctx = pimc_amd.Context(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"])
+    ctx.upload_path(b)
+    assert ctx.potentialAction() == 0.0
+    w, b = ws[1]
+    ctx = pimc_amd.Context(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"])
+    ctx.upload_path(b)
+    ref = oracle_action(orc, w, b)
+    assert abs(ctx.potentialAction() - ref) <= 1e-6 * abs(ref)

@@ -336,9 +336,17 @@ int pimc_batch_total_action(pimc_batch* b, int shift, const double vfactor[2], d
     if ((r = set_weights(b, shift, vfactor))) return r;
     AzizDev az;
     if ((r = batch_aziz(b, &az))) return r;
-    hipEvent_t e0, e1;
-    BHIP(b, hipEventCreate(&e0));
-    BHIP(b, hipEventCreate(&e1));
+    // the two timing events are destroyed on every return path
+    struct Events {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
+    BHIP(b, hipEventCreate(&ev.e0));
+    BHIP(b, hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     BHIP(b, hipEventRecord(e0, b->stream));
     if (b->use_graph) {
         if (!b->exec) {
@@ -365,8 +373,6 @@ int pimc_batch_total_action(pimc_batch* b, int shift, const double vfactor[2], d
                                b->stream));
     BHIP(b, hipStreamSynchronize(b->stream));
     (void)hipEventElapsedTime(&b->last_ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return PIMC_OK;
 }
 

@@ -368,18 +368,10 @@ def chain_leg(pimc_amd, beads, L, M, N, tau, cpu, moves=3000):
 def batched_leg(pimc_amd, plain_value):
     """512 walkers, N in {16..256}, M in {64..512}, T in [1,3] K, Gaussian-bridge
     worldlines rounded to FP32; autotuned tiles; hipGraph replays."""
-    from pimc_amd.workloads import LAMBDA, SEED0, box_side
-    rng = np.random.default_rng(SEED0)
-    ws, beads = [], []
-    for k in range(512):
-        N = int(rng.choice([16, 32, 64, 128, 256]))
-        M = int(rng.choice([64, 128, 256, 512]))
-        T = float(rng.uniform(1.0, 3.0))
-        L = box_side(N)
-        tau = 1.0 / (M * T)
-        b = pimc_amd.api.gen_worldlines(SEED0 + k, L, M, N, LAMBDA, tau, 1).astype(np.float32).astype(np.float64)
-        ws.append(dict(side=L, M=M, N=N, tau=tau, lam=LAMBDA, rc=L))
-        beads.append(b)
+    from pimc_amd.workloads import LAMBDA, batched_walkers
+    ws = batched_walkers()
+    beads = [pimc_amd.api.gen_worldlines(w["seed"], w["side"], w["M"], w["N"], LAMBDA, w["tau"], w["kind"])
+             .astype(np.float32).astype(np.float64) for w in ws]
     bt = pimc_amd.Batch(ws)
     for k, b in enumerate(beads):
         bt.upload(k, b)
@@ -408,17 +400,11 @@ def served_leg(pimc_amd, plain_dv_rate, seconds):
     """1024 resident chains, N in [32,512], M in [64,512]; Poisson arrivals of
     Staging-style segment moves (4..64 slices) and 10% CenterOfMass moves
     (sigma 0.1 A); device-side Metropolis and commit."""
-    from pimc_amd.workloads import LAMBDA, SEED0, box_side
-    rng = np.random.default_rng(SEED0 + 1)
+    from pimc_amd.workloads import LAMBDA, box_side, served_chains
     srv = pimc_amd.Server(12 << 30, streams=4, max_side=box_side(512) + 1.0)
-    for k in range(1024):
-        N = int(rng.integers(32, 513))
-        M = int(rng.integers(64, 513))
-        T = float(rng.uniform(1.0, 3.0))
-        L = box_side(N)
-        tau = 1.0 / (M * T)
-        b = pimc_amd.api.gen_worldlines(SEED0 + 10000 + k, L, M, N, LAMBDA, tau, 1)
-        srv.add_chain(L, M, N, tau, LAMBDA, L, b)
+    for w in served_chains():
+        b = pimc_amd.api.gen_worldlines(w["seed"], w["side"], w["M"], w["N"], LAMBDA, w["tau"], w["kind"])
+        srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b)
     threads = 4   # client threads submitting (the chains split among them)
     out = {"workload": "1024 chains, N in [32,512], M in [64,512], segment moves of 4..64 slices + 10% CoM, "
                        "Poisson arrivals", "client_threads": threads, "rates": {}}

@@ -36,3 +36,36 @@ def lattice_q(side, qmag_max=None, nmax=None):
     q = n * 2.0 * math.pi / side
     mag = np.sqrt(((0.0 + q[:, 0] * q[:, 0]) + q[:, 1] * q[:, 1]) + q[:, 2] * q[:, 2])
     return q[mag <= lim]
+
+
+def batched_walkers(n=512, seed=SEED0):
+    """The batched workload's walkers: N uniform over {16, 32, 64, 128, 256}, M
+    over {64, 128, 256, 512}, T uniform in [1, 3] K, tau = 1/(M T), rc = L.
+    Returns dicts (side, M, N, tau, lam, rc, seed, kind); walker k's worldlines
+    are the Gaussian bridge (kind 1) of MTRand(seed + k), rounded to FP32."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        N = int(rng.choice([16, 32, 64, 128, 256]))
+        M = int(rng.choice([64, 128, 256, 512]))
+        T = float(rng.uniform(1.0, 3.0))
+        L = box_side(N)
+        out.append(dict(side=L, M=M, N=N, tau=1.0 / (M * T), lam=LAMBDA, rc=L, seed=seed + k, kind=1))
+    return out
+
+
+def served_chains(n=1024, seed=SEED0 + 1):
+    """The serving workload's chains: N uniform in [32, 512], M in [64, 512], T
+    uniform in [1, 3] K, tau = 1/(M T), rc = L; chain k's worldlines are the
+    Gaussian bridge (kind 1) of MTRand(SEED0 + 10000 + k)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        N = int(rng.integers(32, 513))
+        M = int(rng.integers(64, 513))
+        T = float(rng.uniform(1.0, 3.0))
+        L = box_side(N)
+        out.append(dict(side=L, M=M, N=N, tau=1.0 / (M * T), lam=LAMBDA, rc=L, seed=SEED0 + 10000 + k, kind=1))
+    return out

@@ -161,3 +161,53 @@ def test_energy_estimator(orc):
     K = 1.5 * N / tau - (vel ** 2).sum() / (4 * LAMBDA * tau * tau * M)
     assert e["K"] == pytest.approx(K, rel=1e-11)
     assert e["E/N"] == pytest.approx((e["K"] + e["V"]) / N, rel=1e-14)
+
+
+def _numpy_sep_hist(bm, L, n=None):
+    """updateSepHist over the i<j pairs of one slice (src/action.cpp:216-224):
+    50 bins of dSep = sqrt(3) L / 100 (:192); a bin index of 50 or more is dropped."""
+    n = bm.shape[0] if n is None else n
+    i, j = np.triu_indices(n, 1)
+    s = put_in_bc(bm[j] - bm[i], L)
+    r = np.sqrt(((0.0 + s[:, 0] * s[:, 0]) + s[:, 1] * s[:, 1]) + s[:, 2] * s[:, 2])
+    k = np.floor(r / (0.5 * math.sqrt(3.0) * L / 50.0)).astype(np.int64)
+    return np.bincount(k[k < 50], minlength=50)
+
+
+def test_sep_hist_numpy(orc):
+    """The oracle's separation histogram (the side effect of V(slice)) against a
+    numpy histogram of the same slice's minimum-image pair distances, for several
+    sizes and both worldline kinds; after several V(slice) calls it holds the last
+    slice only (the histogram is cleared per call, :902-947)."""
+    for N, M, seed, kind in ((16, 5, 139853, 0), (64, 7, 139854, 1), (33, 3, 139855, 0), (200, 2, 139856, 1)):
+        st, b, L = make_state(orc, N, M, 0.004, seed=seed, kind=kind)
+        for m in range(M):
+            _, h = st.V_slice(m, hist=True)
+            assert np.array_equal(h, _numpy_sep_hist(b[m], L)), (N, m)
+        _, h2 = st.V_slice(M - 1, hist=True)
+        _, h0 = st.V_slice(0, hist=True)
+        assert np.array_equal(h0, _numpy_sep_hist(b[0], L))
+        if M > 1:
+            assert not np.array_equal(h0, h2)
+
+
+def test_sep_hist_edges(orc):
+    """Bin edges of the histogram: a coincident pair counts in bin 0, a pair half
+    a box apart on every axis (|r| = sqrt(3) L / 2, bin 50) is dropped, and pairs
+    just under half a box apart on one axis count in bin int(L / 2 / dSep) = 28."""
+    N, M = 4, 1
+    L = orc.box_side(N, DENSITY)
+    dsep = 0.5 * math.sqrt(3.0) * L / 50.0
+    b = np.zeros((M, N, 3))
+    b[0, 0] = [-0.25 * L] * 3
+    b[0, 1] = [0.25 * L] * 3            # |r| = sqrt(3) L / 2 from bead 0
+    b[0, 2] = [-0.25 * L] * 3           # coincident with bead 0, also the corner from bead 1
+    b[0, 3] = [-0.25 * L + 0.5 * L * (1 - 1e-9)] + [-0.25 * L] * 2   # |r| just under L/2 from 0 and 2
+    st = orc.State(L, M, N, 0.004, LAMBDA, L)
+    st.set_beads(b)
+    _, h = st.V_slice(0, hist=True)
+    expect = _numpy_sep_hist(b[0], L)
+    assert np.array_equal(h, expect)
+    assert h[0] == 1                    # the coincident pair
+    assert h.sum() == 6 - 2             # both corner pairs dropped
+    assert h[int(0.5 * L * (1 - 1e-9) / dsep)] == 2 == h[28]

@@ -84,3 +84,24 @@ def test_isf_identities(orc):
     assert np.allclose(got[:2, 0], st.ssf(q[:2]), rtol=1e-12)
     for t in range(1, M):
         assert got[0, t] == pytest.approx(got[0, M - t], rel=1e-12)
+
+
+def test_lattice_rho_and_isf_helpers(orc):
+    """The FP64 phasor helpers the full-size GPU tests compare against: the
+    factored lattice density modes equal the direct sum exp(i q.r) (negative n_x
+    included), and the FFT correlation equals the defining lag sum, the oracle's
+    S(q) and F(q,tau) included."""
+    from tests._oracle_util import isf_from_rho, lattice_rho
+    M, N = 7, 9
+    st, b, L = _state(orc, N, M)
+    nvec = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [2, -3, 1], [-2, 3, -1], [0, 4, -4], [-4, 0, 2], [3, 3, 3]])
+    q = nvec * 2.0 * math.pi / L
+    rho = lattice_rho(b, nvec, L)
+    assert np.allclose(rho, phasor_rho(b, q), rtol=1e-12, atol=1e-11)
+    f = isf_from_rho(rho, N)
+    direct = np.zeros((len(q), M))
+    for t in range(M):
+        direct[:, t] = (rho * np.conj(np.roll(rho, -t, axis=1))).real.sum(axis=1) / N
+    assert np.allclose(f, direct, rtol=1e-12, atol=1e-10)
+    assert np.allclose(f, st.isf(q), rtol=1e-10, atol=1e-9)
+    assert np.allclose(f[:, 0], st.ssf(q), rtol=1e-10, atol=1e-9)

@@ -180,6 +180,16 @@ int pimc_delta_action(pimc_ctx* ctx, const int32_t* slice_ptcl, const double* ol
  * q [nq][3].  Lattice q (q L / 2pi integer) use the phasor factorisation
  * sum_m |rho_q(m)|^2; other q use the minimum-image pair sum. */
 int pimc_ssf(pimc_ctx* ctx, const double* q, int nq, double* sq);
+/* pimc_ssf without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate (hipMalloc, hipFree or host allocations) inside
+ * their loop -- harnesses that forbid it, tight sampling loops whose q lists
+ * change size.  Same arguments and results as pimc_ssf, bit for bit: the q list
+ * is walked in runs of one kind (lattice / general) that fit one 32 MiB
+ * device array per GPU, held by the library until the process ends, instead of
+ * the context's buffers grown to the largest list seen.  Any nq is accepted
+ * (M <= 512 as for pimc_ssf).  Calls that overlap on one GPU (several threads,
+ * several contexts' streams) share that array and race. */
+int pimc_ssf_noalloc(pimc_ctx* ctx, const double* q, int nq, double* sq);
 /* IntermediateScatteringFunctionEstimator::accumulate (src/estimator.cpp:3923-3961):
  * fqt [nq][M], fqt[i][t] = isf(q_i, t)/N for all M lags, raw positions. */
 int pimc_isf(pimc_ctx* ctx, const double* q, int nq, double* fqt);

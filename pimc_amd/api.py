@@ -81,6 +81,7 @@ SIGNATURES = {
     "pimc_delta_action": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                     C.c_int, C.c_void_p]),
     "pimc_ssf": (C.c_int, [C.c_void_p, _dp, C.c_int, _dp]),
+    "pimc_ssf_noalloc": (C.c_int, [C.c_void_p, _dp, C.c_int, _dp]),
     "pimc_isf": (C.c_int, [C.c_void_p, _dp, C.c_int, _dp]),
     "pimc_cell_dims": (C.c_int, [C.c_void_p, _ip, _ip]),
     "pimc_cell_lists": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip]),
@@ -351,6 +352,13 @@ class Context:
         q = _f64(q, (-1, 3))
         o = np.zeros(q.shape[0])
         self._check(lib().pimc_ssf(self.h, _p(q, _dp), q.shape[0], _p(o, _dp)), "ssf")
+        return o
+
+    def ssf_noalloc(self, q):
+        """S(q) through pimc_ssf_noalloc: no device or host allocation in the library."""
+        q = _f64(q, (-1, 3))
+        o = np.zeros(q.shape[0])
+        self._check(lib().pimc_ssf_noalloc(self.h, _p(q, _dp), q.shape[0], _p(o, _dp)), "ssf_noalloc")
         return o
 
     def isf(self, q):

@@ -537,8 +537,9 @@ int pimc_ctx_create(const pimc_desc* d, pimc_ctx** out) {
     c->params = p;
     {
         int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+        HIP_OR_FAIL(nullptr, hipGetDevice(&dev));
+        c->device = dev;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
             c->num_cus = cus;
     }
     const int M = d->num_slices, N = d->num_alloc;

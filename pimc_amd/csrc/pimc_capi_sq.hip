@@ -26,20 +26,6 @@ int sfail(pimc_ctx* c, int code, const std::string& m) {
         if (_e != hipSuccess) return sfail((c), PIMC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-// q = (2 pi / L) n with integer n, |n_d| <= rho_max_nmax()
-bool lattice_q(const double* q, double L, int* n3) {
-    const double f = L / (2.0 * M_PI);
-    for (int d = 0; d < 3; ++d) {
-        const double x = q[d] * f;
-        const double r = std::nearbyint(x);
-        if (std::fabs(r) > rho_max_nmax()) return false;
-        const double back = r * 2.0 * M_PI / L;
-        if (std::fabs(back - q[d]) > 1e-9 * std::max(1.0, std::fabs(q[d]))) return false;
-        n3[d] = (int)r;
-    }
-    return true;
-}
-
 template <typename T>
 int grow(pimc_ctx* c, T** p, size_t* cap, size_t need) {
     if (need <= *cap && *p) return PIMC_OK;
@@ -101,6 +87,23 @@ int compute_rho(pimc_ctx* c, bool lattice, const std::vector<int>& qn, const std
 }
 
 }  // namespace
+
+namespace pimc {
+
+bool lattice_q(const double* q, double L, int* n3) {
+    const double f = L / (2.0 * M_PI);
+    for (int d = 0; d < 3; ++d) {
+        const double x = q[d] * f;
+        const double r = std::nearbyint(x);
+        if (std::fabs(r) > rho_max_nmax()) return false;
+        const double back = r * 2.0 * M_PI / L;
+        if (std::fabs(back - q[d]) > 1e-9 * std::max(1.0, std::fabs(q[d]))) return false;
+        n3[d] = (int)r;
+    }
+    return true;
+}
+
+}  // namespace pimc
 
 extern "C" {
 

@@ -20,6 +20,10 @@ AzizDev make_aziz_dev(const AzizParams& p, double max_sep);
 void build_table(const AzizParams& p, int length, int what, std::vector<double>& out);
 double aziz_tail(const AzizParams& p, double rc);
 
+// q = (2 pi / L) n with integer n, |n_d| <= rho_max_nmax(): the lattice
+// wavevectors whose S(q) and F(q,tau) use the density-mode kernels.
+bool lattice_q(const double* q, double L, int* n3);
+
 struct TableCache;
 TableCache& table_cache();
 const double* cached_table(int year, int what, int length, std::string* err);
@@ -56,6 +60,7 @@ struct pimc_ctx {
     int n_tiles = 0, cap_tiles = 0;
     int n_groups = 0, cap_groups = 0;
     int num_cus = 256;
+    int device = 0;               // the HIP device current when the context was created
     double* d_vslice = nullptr;
     double* d_out = nullptr;
     double* h_map = nullptr;      // 64 doubles of mapped, coherent pinned host memory

@@ -114,10 +114,22 @@ int pimc_autotune(pimc_ctx* ctx, int reps, double* out4);
  * is the separation histogram the reference leaves behind after evaluating the
  * last slice, M-1 (src/action.cpp:216-224). */
 int pimc_potential_slices(pimc_ctx* ctx, double* v_slices, int32_t* sep_hist);
-/* EnergyEstimator::accumulate potential and kinetic terms for the primitive
- * action (src/estimator.cpp:940-1029): out8 = {K, V, V_ext, V_int, E, K/N, V/N, E/N}.
+/* EnergyEstimator::accumulate (src/estimator.cpp:940-1029) with the context's
+ * action factors: the nine columns in the reference's order (:922)
+ *   out9 = {K, V, V_ext, V_int, E, E_mu, K/N, V/N, E/N},  E_mu = E - mu N.
  * next_links [M][N_alloc][2] may be NULL (periodic closed worldlines). */
-int pimc_energy(pimc_ctx* ctx, const int32_t* next_links, double* out8);
+int pimc_energy(pimc_ctx* ctx, const int32_t* next_links, double mu, double* out9);
+/* LocalAction::gradVSquared(slice) (src/action.cpp:1188-1223) for every slice:
+ * g2 [M] = sum over active beads i of |sum_{j != i} gradV(r_i - r_j)|^2, the
+ * values the gradient-corrected action, derivPotentialActionTau (:751-767) and
+ * derivPotentialActionLambda (:796-802) weight. */
+int pimc_gradv_squared(pimc_ctx* ctx, double* g2);
+/* A counter of the context's state (no reference counterpart): every call that
+ * changes positions, the bead mask, the worm, the pair-term mode or the action
+ * factors increases it (upload, update, a committed Delta V, chain moves), so a
+ * caller can cache per-slice values -- V(slice), gradVSquared(slice), the virial
+ * terms -- and refresh them only when it moves. */
+int pimc_path_version(const pimc_ctx* ctx, uint64_t* version);
 /* The virial pair terms of LocalAction for every slice (src/action.cpp:1446-1786,
  * :776-787; ActionBase virtuals include/action.h:75-76, :183-194), with the
  * context's action factors (pimc_set_action_factors) at shift 1:

@@ -36,7 +36,7 @@ class Action {
     // The constants LocalAction reads from constants() and the box from Prism.
     Action(double side, int numTimeSlices, int numAlloc, double tau, double lambda, double rc,
            std::array<double, 2> VFactor = {1.0, 1.0}, int year = 1979, int v_mode = PIMC_V_MIXED)
-        : VFactor_(VFactor), M_(numTimeSlices), N_(numAlloc) {
+        : VFactor_(VFactor), M_(numTimeSlices), N_(numAlloc), tau_(tau), lambda_(lambda) {
         pimc_desc d{side, numTimeSlices, numAlloc, tau, lambda, rc, year, v_mode};
         check(pimc_ctx_create(&d, &ctx_), nullptr, "pimc_ctx_create");
     }
@@ -108,25 +108,51 @@ class Action {
               ctx_, "delta_action");
         return out;
     }
+    // LocalAction's factors (src/setup.cpp:1232-1256): VFactor, gradVFactor of the
+    // GSF / Li-Broughton correction, estimator period.
+    void setActionFactors(std::array<double, 2> VFactor, std::array<double, 2> gradVFactor, int period) {
+        check(pimc_set_action_factors(ctx_, VFactor.data(), gradVFactor.data(), period), ctx_, "set_action_factors");
+        VFactor_ = VFactor;
+        gradVFactor_ = gradVFactor;
+    }
     // LocalAction::potential(int) == V(int) = {Vext, Vint} (src/action.cpp:902-947)
     std::array<double, 2> potential(int slice) {
         refreshSlices();
         return {vs_[2 * slice], vs_[2 * slice + 1]};
     }
-    // LocalAction::derivPotentialActionTau(int) (src/action.cpp:751-767), primitive
+    // LocalAction::derivPotentialActionTau(int) (src/action.cpp:751-767):
+    // VFactor sum(V(slice)) + 3 gradVFactor tau^2 lambda gradVSquared(slice)
     double derivPotentialActionTau(int slice) {
-        auto v = potential(slice);
-        return VFactor_[slice % 2] * (v[0] + v[1]);
+        refreshSlices();
+        const int eo = slice % 2;
+        double dU = VFactor_[eo] * (vs_[2 * slice] + vs_[2 * slice + 1]);
+        if (gradVFactor_[eo] > kEps) dU += 3.0 * gradVFactor_[eo] * tau_ * tau_ * lambda_ * g2_[slice];
+        return dU;
     }
-    // Call before reading several slices of one configuration (one launch for all).
+    // LocalAction::derivPotentialActionLambda(int) (src/action.cpp:796-802)
+    double derivPotentialActionLambda(int slice) {
+        refreshSlices();
+        const int eo = slice % 2;
+        return gradVFactor_[eo] > kEps ? gradVFactor_[eo] * tau_ * tau_ * tau_ * g2_[slice] : 0.0;
+    }
+    // V(slice) (and gradVSquared(slice) with a finite correction) for every slice
+    // in one pass, kept until the context's state changes: an estimator's
+    // per-slice calls (src/estimator.cpp:985-987) cost one launch per sample.
     void refreshSlices() {
+        uint64_t v = 0;
+        check(pimc_path_version(ctx_, &v), ctx_, "path_version");
+        if (v == stamp_ && !vs_.empty()) return;
         vs_.resize(2 * (size_t)M_);
         check(pimc_potential_slices(ctx_, vs_.data(), nullptr), ctx_, "potential_slices");
+        g2_.assign(M_, 0.0);
+        if (gradVFactor_[0] > kEps || gradVFactor_[1] > kEps)
+            check(pimc_gradv_squared(ctx_, g2_.data()), ctx_, "gradv_squared");
+        stamp_ = v;
     }
-    // EnergyEstimator::accumulate terms {K, V, V_ext, V_int, E, K/N, V/N, E/N}
-    std::array<double, 8> energy(const std::vector<int32_t>& nextLinks = {}) {
-        std::array<double, 8> o{};
-        check(pimc_energy(ctx_, nextLinks.empty() ? nullptr : nextLinks.data(), o.data()), ctx_, "energy");
+    // EnergyEstimator::accumulate columns {K, V, V_ext, V_int, E, E_mu, K/N, V/N, E/N}
+    std::array<double, 9> energy(const std::vector<int32_t>& nextLinks = {}, double mu = 0.0) {
+        std::array<double, 9> o{};
+        check(pimc_energy(ctx_, nextLinks.empty() ? nullptr : nextLinks.data(), mu, o.data()), ctx_, "energy");
         return o;
     }
     // StaticStructureFactorEstimator::accumulate contribution sf/N
@@ -146,11 +172,15 @@ class Action {
     pimc_ctx* raw() { return ctx_; }
 
    private:
+    static constexpr double kEps = 1.0e-7;   // a gradVFactor counts above EPS (src/action.cpp:760)
     pimc_ctx* ctx_ = nullptr;
     std::array<double, 2> VFactor_;
+    std::array<double, 2> gradVFactor_{0.0, 0.0};
     int shift_ = 1;
     int M_, N_;
-    std::vector<double> vs_;
+    double tau_, lambda_;
+    std::vector<double> vs_, g2_;
+    uint64_t stamp_ = ~uint64_t(0);
 };
 
 // AzizPotential as a batched PotentialBase (include/potential.h:86, :985-1018).

@@ -1481,7 +1481,7 @@ double orc_gradVnn2(void* p, int slice, int ptcl) {
     return st->gradVnnSquared(b);
 }
 
-void orc_energy(void* p, double* out) {
+void orc_energy(void* p, double mu, double* out) {
     // EnergyEstimator::accumulate (estimator.cpp:940-1029), PIMC, primitive/local action
     State* st = static_cast<State*>(p);
     int N = st->trueNumParticles();
@@ -1538,9 +1538,10 @@ void orc_energy(void* p, double* out) {
     out[2] = totVop[0];
     out[3] = totVop[1];
     out[4] = totK + totV;
-    out[5] = N > 0 ? totK / (1.0 * N) : 0.0;
-    out[6] = N > 0 ? totV / (1.0 * N) : 0.0;
-    out[7] = N > 0 ? (totK + totV) / (1.0 * N) : 0.0;
+    out[5] = totK + totV - mu * N;   // E_mu (estimator.cpp:1011)
+    out[6] = N > 0 ? totK / (1.0 * N) : 0.0;
+    out[7] = N > 0 ? totV / (1.0 * N) : 0.0;
+    out[8] = N > 0 ? (totK + totV) / (1.0 * N) : 0.0;
 }
 
 void orc_ssf(void* p, const double* q3, int nq, double* out) {

@@ -93,7 +93,7 @@ void   orc_virial_terms(void* st, int W, double* out);
 void   orc_virial_energy(void* st, int W, double mu, double* out19);
 /* EnergyEstimator::accumulate (estimator.cpp:940-1029), primitive PIMC:
  * out = {K, V, V_ext, V_int, E, K/N, V/N, E/N} for one sample. */
-void   orc_energy(void* st, double* out8);
+void   orc_energy(void* st, double mu, double* out9);   /* K V V_ext V_int E E_mu K/N V/N E/N */
 /* StaticStructureFactorEstimator::accumulate contribution sf/N (estimator.cpp:3705-3737) */
 void   orc_ssf(void* st, const double* q3, int nq, double* out);
 /* IntermediateScatteringFunctionEstimator::accumulate contribution isf/N, [nq][M] (:3923-3961) */

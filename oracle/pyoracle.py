@@ -30,6 +30,7 @@ def left_pack(beads, nxt, prv, worm):
     return b, n, p, w
 
 
+ENERGY_COLUMNS = ("K", "V", "V_ext", "V_int", "E", "E_mu", "K/N", "V/N", "E/N")
 VIRIAL_COLUMNS = ("K_op", "K_cv", "V_op", "V_cv", "E", "E_mu", "K_op/N", "K_cv/N", "V_op/N", "V_cv/N", "E/N",
                   "EEcv*Beta^2", "Ecv*Beta", "dEdB", "CvCov1", "CvCov2", "CvCov3", "E_th", "P")
 
@@ -89,7 +90,7 @@ def lib():
             "orc_gradVnn2": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
             "orc_bare_action_list": (None, [C.c_void_p, ip, C.c_int, dp]),
             "orc_action_correction": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
-            "orc_energy": (None, [C.c_void_p, dp]),
+            "orc_energy": (None, [C.c_void_p, C.c_double, dp]),
             "orc_virial_terms": (None, [C.c_void_p, C.c_int, dp]),
             "orc_left_pack": (None, [C.c_int, C.c_int, dp, ip, ip, C.POINTER(C.c_uint32)]),
             "orc_virial_energy": (None, [C.c_void_p, C.c_int, C.c_double, dp]),
@@ -307,10 +308,10 @@ class State:
         p = np.ascontiguousarray(pos, dtype=np.float64)
         lib().orc_update_bead(self.h, slice_, ptcl, _p(p, dp))
 
-    def energy(self):
-        o = np.zeros(8)
-        lib().orc_energy(self.h, _p(o, dp))
-        return dict(zip(["K", "V", "V_ext", "V_int", "E", "K/N", "V/N", "E/N"], o))
+    def energy(self, mu=0.0):
+        o = np.zeros(9)
+        lib().orc_energy(self.h, mu, _p(o, dp))
+        return dict(zip(ENERGY_COLUMNS, o))
 
     def virial_terms(self, window=5):
         """[M][6] = rDOTgradUterm1, rDOTgradUterm2, deltadotgradUterm1,

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libpimc_gpu.so")
 V_FP32, V_FP64, V_TABLE = 0, 1, 2
 # kernel names of pimc_kernel_time, in the bit order of pimc_set_timing_kinds
 KERNEL_NAMES = ("pair_action", "delta_action", "rho", "ssf", "isf", "gradv2", "gradvnn", "virial", "chain_run")
+ENERGY_COLUMNS = ("K", "V", "V_ext", "V_int", "E", "E_mu", "K/N", "V/N", "E/N")
 VIRIAL_COLUMNS = ("K_op", "K_cv", "V_op", "V_cv", "E", "E_mu", "K_op/N", "K_cv/N", "V_op/N", "V_cv/N", "E/N",
                   "EEcv*Beta^2", "Ecv*Beta", "dEdB", "CvCov1", "CvCov2", "CvCov3", "E_th", "P")
 
@@ -70,7 +71,9 @@ SIGNATURES = {
     # the two per-move calls take raw addresses (c_void_p): no ctypes pointer objects per call
     "pimc_total_action": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "pimc_potential_slices": (C.c_int, [C.c_void_p, _dp, _ip]),
-    "pimc_energy": (C.c_int, [C.c_void_p, _ip, _dp]),
+    "pimc_energy": (C.c_int, [C.c_void_p, _ip, C.c_double, _dp]),
+    "pimc_gradv_squared": (C.c_int, [C.c_void_p, _dp]),
+    "pimc_path_version": (C.c_int, [C.c_void_p, _u64p]),
     "pimc_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "pimc_autotune": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "pimc_virial_terms": (C.c_int, [C.c_void_p, _ip, C.c_int, _dp]),
@@ -233,8 +236,8 @@ class Context:
 
     def upload_path(self, beads, bead_on=None, num_beads=None):
         b = _f64(beads, (self.M, self.N, 3))
-        on = None if bead_on is None else np.ascontiguousarray(bead_on, dtype=np.uint8)
-        nb = None if num_beads is None else _i32(num_beads)
+        on = None if bead_on is None else np.ascontiguousarray(bead_on, dtype=np.uint8).reshape(self.M, self.N)
+        nb = None if num_beads is None else _i32(num_beads, (self.M,))
         self._check(lib().pimc_upload_path(self.h, _p(b, _dp), _p(on, _bp), _p(nb, _ip)), "upload_path")
 
     def load_state(self, state):
@@ -249,7 +252,7 @@ class Context:
         return p
 
     def set_worm(self, head=(-1, -1), tail=(-1, -1), special1=(-1, -1), special2=(-1, -1)):
-        arrs = [_i32(x) for x in (head, tail, special1, special2)]
+        arrs = [_i32(x, (2,)) for x in (head, tail, special1, special2)]
         self._check(lib().pimc_set_worm(self.h, *[_p(a, _ip) for a in arrs]), "set_worm")
 
     def download_path(self):
@@ -277,25 +280,37 @@ class Context:
         self._check(lib().pimc_potential_slices(self.h, _p(vs, _dp), _p(h, _ip)), "potential_slices")
         return (vs, h) if sep_hist else vs
 
-    def energy(self, next_links=None):
-        o = np.zeros(8)
-        nx = None if next_links is None else _i32(next_links)
-        self._check(lib().pimc_energy(self.h, _p(nx, _ip), _p(o, _dp)), "energy")
-        return dict(zip(["K", "V", "V_ext", "V_int", "E", "K/N", "V/N", "E/N"], o))
+    def energy(self, next_links=None, mu=0.0):
+        """EnergyEstimator::accumulate, the reference's nine columns (src/estimator.cpp:922-1016)."""
+        o = np.zeros(9)
+        nx = None if next_links is None else _i32(next_links, (self.M, self.N, 2))
+        self._check(lib().pimc_energy(self.h, _p(nx, _ip), mu, _p(o, _dp)), "energy")
+        return dict(zip(ENERGY_COLUMNS, o))
+
+    def gradVSquared(self):
+        """LocalAction::gradVSquared(slice) for every slice [M] (src/action.cpp:1188-1223)."""
+        o = np.zeros(self.M)
+        self._check(lib().pimc_gradv_squared(self.h, _p(o, _dp)), "gradv_squared")
+        return o
+
+    def path_version(self):
+        v = C.c_uint64()
+        self._check(lib().pimc_path_version(self.h, C.byref(v)), "path_version")
+        return v.value
 
     def virial_terms(self, window=5, next_links=None):
         """[M][6]: rDOTgradUterm1, rDOTgradUterm2, deltadotgradUterm1,
         deltadotgradUterm2, virialKinCorrection, secondderivPotentialActionTau
         (src/action.cpp:1446-1786, :776-787)."""
         o = np.zeros((self.M, 6))
-        nx = None if next_links is None else _i32(next_links)
+        nx = None if next_links is None else _i32(next_links, (self.M, self.N, 2))
         self._check(lib().pimc_virial_terms(self.h, _p(nx, _ip), window, _p(o, _dp)), "virial_terms")
         return o
 
     def virial_energy(self, window=5, mu=0.0, next_links=None):
         """VirialEnergyEstimator::accumulate, one call (src/estimator.cpp:1086-1251)."""
         o = np.zeros(19)
-        nx = None if next_links is None else _i32(next_links)
+        nx = None if next_links is None else _i32(next_links, (self.M, self.N, 2))
         self._check(lib().pimc_virial_energy(self.h, _p(nx, _ip), window, mu, _p(o, _dp)), "virial_energy")
         return dict(zip(VIRIAL_COLUMNS, o))
 
@@ -329,7 +344,7 @@ class Context:
 
     def action_range(self, start, end, next_links=None, shift=1, vfactor=None):
         s, e = _i32(start), _i32(end)
-        nx = None if next_links is None else _i32(next_links)
+        nx = None if next_links is None else _i32(next_links, (self.M, self.N, 2))
         vf = None if vfactor is None else _f64(vfactor)
         a = np.zeros(1)
         self._check(lib().pimc_action_range(self.h, _p(s, _ip), _p(e, _ip), _p(nx, _ip), shift, _p(vf, _dp),

@@ -603,6 +603,7 @@ int pimc_set_v_mode(pimc_ctx* c, int v_mode) {
     if (!c) return fail(nullptr, PIMC_E_ARG, "null ctx");
     if (v_mode < PIMC_V_FP32 || v_mode > PIMC_V_TABLE) return fail(c, PIMC_E_ARG, "invalid v_mode");
     c->desc.v_mode = v_mode;
+    ++c->path_version;
     return PIMC_OK;
 }
 
@@ -630,6 +631,7 @@ int pimc_upload_path(pimc_ctx* c, const double* beads, const uint8_t* bead_on, c
                                      c->d_numlab, c->d_hash, c->cells, c->box));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     c->have_path = true;
+    ++c->path_version;
     return rebuild_work(c);
 }
 
@@ -644,6 +646,7 @@ int pimc_set_worm(pimc_ctx* c, const int32_t head[2], const int32_t tail[2], con
     cp(c->worm.tail, tail);
     cp(c->worm.sp1, special1);
     cp(c->worm.sp2, special2);
+    ++c->path_version;
     return PIMC_OK;
 }
 
@@ -717,7 +720,37 @@ int pimc_update_beads(pimc_ctx* c, const int32_t* sp, const double* pos, int n) 
     HIP_OR_FAIL(c, launch_update_beads(c->stream, c->d_req_sp, c->d_req_pos, n, c->d_pos, c->plane,
                                        c->desc.num_alloc, c->desc.num_slices, c->d_cell, c->d_label, c->d_numlab,
                                        c->d_hash, c->box, distinct));
+    ++c->path_version;
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    return PIMC_OK;
+}
+
+int pimc_path_version(const pimc_ctx* c, uint64_t* version) {
+    if (!c || !version) return fail(nullptr, PIMC_E_ARG, "null argument");
+    *version = c->path_version;
+    return PIMC_OK;
+}
+
+int pimc_gradv_squared(pimc_ctx* c, double* g2) {
+    if (!c || !g2) return fail(c, PIMC_E_ARG, "null argument");
+    if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
+    const int M = c->desc.num_slices;
+    int r;
+    AzizDev az = ctx_aziz(c, &r);
+    if (r) return r;
+    if ((r = run_gradv2(c, az))) return r;
+    // the per-block partials of every slice, summed in block order as the
+    // pair kernel's finalisation sums them into the corrected action
+    const int B = c->g2_blocks;
+    std::vector<double> part((size_t)M * B);
+    HIP_OR_FAIL(c, hipMemcpyAsync(part.data(), c->d_g2part, sizeof(double) * part.size(), hipMemcpyDeviceToHost,
+                                  c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    for (int m = 0; m < M; ++m) {
+        double t = 0.0;
+        for (int b = 0; b < B; ++b) t += part[(size_t)m * B + b];
+        g2[m] = t;
+    }
     return PIMC_OK;
 }
 
@@ -1056,8 +1089,8 @@ int pimc_autotune(pimc_ctx* c, int reps, double* out4) {
     return r;
 }
 
-int pimc_energy(pimc_ctx* c, const int32_t* next_links, double* out8) {
-    if (!c || !out8) return fail(c, PIMC_E_ARG, "null argument");
+int pimc_energy(pimc_ctx* c, const int32_t* next_links, double mu, double* out9) {
+    if (!c || !out9) return fail(c, PIMC_E_ARG, "null argument");
     if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
     const int M = c->desc.num_slices, N = c->desc.num_alloc;
     std::vector<double> vs(2 * (size_t)M);
@@ -1123,14 +1156,16 @@ int pimc_energy(pimc_ctx* c, const int32_t* next_links, double* out8) {
     totK += (classical + t1);
     const double totV = t2 - t1 + tailV;
     vop1 += tailV;
-    out8[0] = totK;
-    out8[1] = totV;
-    out8[2] = vop0;
-    out8[3] = vop1;
-    out8[4] = totK + totV;
-    out8[5] = Np > 0 ? totK / Np : 0.0;
-    out8[6] = Np > 0 ? totV / Np : 0.0;
-    out8[7] = Np > 0 ? (totK + totV) / Np : 0.0;
+    // the estimator's columns K V V_ext V_int E E_mu K/N V/N E/N (src/estimator.cpp:922)
+    out9[0] = totK;
+    out9[1] = totV;
+    out9[2] = vop0;
+    out9[3] = vop1;
+    out9[4] = totK + totV;
+    out9[5] = totK + totV - mu * Np;
+    out9[6] = Np > 0 ? totK / Np : 0.0;
+    out9[7] = Np > 0 ? totV / Np : 0.0;
+    out9[8] = Np > 0 ? (totK + totV) / Np : 0.0;
     return PIMC_OK;
 }
 
@@ -1290,11 +1325,16 @@ int pimc_action_range(pimc_ctx* c, const int32_t start[2], const int32_t end[2],
             *ni = i;
         }
     };
+    if (start[0] < 0 || start[0] >= M || start[1] < 0 || start[1] >= N || end[0] < 0 || end[0] >= M || end[1] < 0 ||
+        end[1] >= N)
+        return fail(c, PIMC_E_ARG, "range start or end out of range");
     int sm, si;
     nxt(end[0], end[1], &sm, &si);
     std::vector<int32_t> list;
     int m = start[0], i = start[1];
-    const int limit = M * N + 1;
+    // a walk from start reaches end's successor within M*N steps or never
+    const long limit = (long)M * N;
+    long steps = 0;
     do {
         if (m < 0 || m >= M || i < 0 || i >= N) return fail(c, PIMC_E_ARG, "range walks off the path");
         // LocalAction skips inactive beads (src/action.cpp:532-541)
@@ -1306,7 +1346,7 @@ int pimc_action_range(pimc_ctx* c, const int32_t start[2], const int32_t end[2],
         nxt(m, i, &a, &b);
         m = a;
         i = b;
-        if ((int)list.size() > 2 * limit) return fail(c, PIMC_E_ARG, "range does not terminate");
+        if (++steps > limit) return fail(c, PIMC_E_ARG, "range does not reach end's successor");
     } while (!(m == sm && i == si));
     return pimc_action_list(c, list.data(), (int)list.size() / 2, shift, vfactor, action, nullptr);
 }
@@ -1341,6 +1381,7 @@ int pimc_delta_action(pimc_ctx* c, const int32_t* sp, const double* old_pos, con
             q.new_pos[d] = new_pos[3 * k + d];
         }
     }
+    if (commit) ++c->path_version;
     return run_vnn(c, n, shift, vfactor, commit != 0, out3, nullptr);
 }
 
@@ -1417,6 +1458,7 @@ int pimc_set_action_factors(pimc_ctx* c, const double vfactor[2], const double g
     c->gvf[1] = gradvfactor[1];
     c->period = period;
     c->walker_valid = false;
+    ++c->path_version;
     return PIMC_OK;
 }
 

@@ -470,6 +470,7 @@ static int run_chains_on_device(pimc_chain* const* chains, int n, int n_moves, c
     if (d_args) (void)hipFree(d_args);
     if (d_azs) (void)hipFree(d_azs);
     if (e != hipSuccess) return lead->fail(PIMC_E_HIP, std::string("chain kernel: ") + hipGetErrorString(e));
+    for (int c = 0; c < n; ++c) ++chains[c]->ctx->path_version;   // the launch moved their beads
     for (int c = 0; c < n; ++c) {
         int r = chains[c]->device_results(n_moves, accepted ? accepted + (size_t)c * n_moves : nullptr,
                                           info ? info + (size_t)c * 3 * n_moves : nullptr, near ? near + c : nullptr);

@@ -41,6 +41,7 @@ struct pimc_ctx {
     int cells = 1;
     hipStream_t stream = nullptr;
     bool have_path = false;
+    uint64_t path_version = 0;    // bumped by every call that changes positions, mask, worm or action factors
     std::string err;
 
     // device state

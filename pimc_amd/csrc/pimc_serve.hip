@@ -293,7 +293,9 @@ struct pimc_server {
     Arena arena;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::vector<Chain> chains;
+    // a deque: add_chain's push_back keeps references to the other chains valid
+    // (remove_chain and download hold one across a wait that drops the lock)
+    std::deque<Chain> chains;
     std::deque<int> waiting_chains;
     std::vector<ServeChainDev> chain_table;
     ServeChainDev* d_chains = nullptr;
@@ -895,14 +897,16 @@ int pimc_server_wait(pimc_server* s, uint64_t req_id, double* out5) {
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(req_id);
     if (it == s->reqs.end()) return sfail(s, PIMC_E_ARG, "unknown request");
-    s->cv_done.wait(lk, [&] { return it->second.done || s->kernel_error; });
-    if (!it->second.done) return sfail(s, s->kernel_error, s->kernel_error_msg);
+    // an element pointer survives rehashes by concurrent submits; an iterator does not
+    Request* R = &it->second;
+    s->cv_done.wait(lk, [&] { return R->done || s->kernel_error; });
+    if (!R->done) return sfail(s, s->kernel_error, s->kernel_error_msg);
     if (out5) {
-        for (int t = 0; t < 4; ++t) out5[t] = it->second.out[t];
-        out5[4] = std::chrono::duration<double, std::micro>(it->second.t_done - it->second.t_submit).count();
+        for (int t = 0; t < 4; ++t) out5[t] = R->out[t];
+        out5[4] = std::chrono::duration<double, std::micro>(R->t_done - R->t_submit).count();
     }
-    Request gone = std::move(it->second);   // its storage is released after the lock
-    s->reqs.erase(it);
+    Request gone = std::move(*R);   // its storage is released after the lock
+    s->reqs.erase(req_id);
     lk.unlock();
     return PIMC_OK;
 }
@@ -937,6 +941,7 @@ int pimc_server_stats(pimc_server* s, int64_t* batches, int64_t* requests, int64
 namespace {
 
 struct GenChain {
+    pimc_desc d{};                     // the chain's description, copied under the server lock
     std::vector<double> mirror;        // [M][N][3], kept equal to the device copy
     std::deque<Clock::time_point> arrivals;
     uint64_t inflight = 0;
@@ -970,7 +975,7 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
     std::deque<int> runnable;   // own chains with an arrival waiting and nothing in flight
     auto propose = [&](int c) {
         GenChain& g = gc[c];
-        const pimc_desc& d = s->chains[c].d;
+        const pimc_desc& d = g.d;
         const int M = d.num_slices, N = d.num_alloc;
         g.sp.clear();
         g.prop.clear();
@@ -1057,7 +1062,7 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
             st.lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
             st.svc.push_back(o[4]);
             ++st.completed;
-            const pimc_desc& d = s->chains[c].d;
+            const pimc_desc& d = g.d;
             st.pairs += 2LL * ((int64_t)g.sp.size() / 2) * (d.num_alloc - 1);
             if (o[3] > 0.5) {
                 ++st.accepted;
@@ -1120,8 +1125,12 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
             s->done_lists.clear();
         }
     } track_off{s};
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (int c = 0; c < nc; ++c) gc[c].d = s->chains[c].d;
+    }
     for (int c = 0; c < nc; ++c) {
-        const pimc_desc& d = s->chains[c].d;
+        const pimc_desc& d = gc[c].d;
         gc[c].mirror.resize(3 * (size_t)d.num_slices * d.num_alloc);
         int r = pimc_server_download(s, c, gc[c].mirror.data());
         if (r) return r;

@@ -32,14 +32,34 @@ int main(int argc, char** argv) {
         act.setShift(2);
         const double shifted = act.potentialAction();
         act.setShift(1);
-        const auto e = act.energy();
+        const auto e = act.energy({}, 0.5);
+        // the cache follows the path: V(3) after bead (3, 5) moves
+        act.updateBead(pimc_gpu::beadLocator{3, 5}, np[1]);
+        const double v3_after = act.potential(3)[1];
+        act.updateBead(pimc_gpu::beadLocator{3, 5}, pimc_gpu::dVec{beads[3 * ((size_t)3 * N + 5)],
+                                                                  beads[3 * ((size_t)3 * N + 5) + 1],
+                                                                  beads[3 * ((size_t)3 * N + 5) + 2]});
+        // GSF factors: the thermodynamic V of EnergyEstimator::accumulate from the
+        // adapter's per-slice tau and lambda derivatives (src/estimator.cpp:985-1003)
+        act.setActionFactors({2.0 / 3.0, 4.0 / 3.0}, {0.0, 2.0 / 9.0}, 2);
+        double t1 = 0.0, t2 = 0.0;
+        for (int m = 0; m < M; ++m) {
+            t1 += act.derivPotentialActionLambda(m);
+            t2 += act.derivPotentialActionTau(m);
+        }
+        t1 *= lambda / (tau * M);
+        t2 /= 1.0 * M;
+        const auto eg = act.energy();
         pimc_gpu::Potential pot(1979, std::sqrt(3.0) * L / 2.0, L);
         const double v = pot.V(pimc_gpu::dVec{2.9673, 0.0, 0.0});
         std::printf("{\"N\": %d, \"M\": %d, \"L\": %.17g, \"total\": %.17g, \"v3\": [%.17g, %.17g], \"d3\": %.17g, "
                     "\"list\": %.17g, \"range\": %.17g, \"dv\": [%.17g, %.17g, %.17g], \"shifted\": %.17g, "
-                    "\"E\": [%.17g, %.17g, %.17g, %.17g], \"V_rm\": %.17g, \"tailV\": %.17g}\n",
+                    "\"E\": [%.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g], \"V_rm\": %.17g, "
+                    "\"tailV\": %.17g, \"v3_after\": %.17g, \"gsf_t1\": %.17g, \"gsf_t2\": %.17g, "
+                    "\"gsf_E\": [%.17g, %.17g, %.17g, %.17g, %.17g]}\n",
                     N, M, L, total, v3[0], v3[1], d3, list, range, dv[0], dv[1], dv[2], shifted, e[0], e[1], e[2],
-                    e[3], v, pot.tailV);
+                    e[3], e[4], e[5], e[6], e[7], e[8], v, pot.tailV, v3_after, t1, t2, eg[0], eg[1], eg[2], eg[3],
+                    eg[4]);
     } catch (const std::exception& ex) {
         std::fprintf(stderr, "adapter error: %s\n", ex.what());
         return 1;

@@ -219,3 +219,81 @@ def test_batch_tiny_walkers(orc):
     ctx.upload_path(b)
     ref = oracle_action(orc, w, b)
     assert abs(ctx.potentialAction() - ref) <= 1e-6 * abs(ref)
+
+
+def test_server_remove_while_adding(orc):
+    """remove_chain waits (lock released) until its chain's queued request has run;
+    chains added meanwhile by another thread grow the server's chain container,
+    and the waiting call still releases the right chain's memory and admits the
+    next waiting chain."""
+    import threading
+    import pimc_amd
+    w, b = walker(orc, 64, 16, 1.5, SEED0, kind=0)
+    need = 16 * 64 * (24 + 4 + 1) + 3 * 256
+    srv = pimc_amd.Server(int(need * 1.5), streams=2, max_side=40.0)
+    args = (w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b)
+    c0 = srv.add_chain(*args)
+    c1 = srv.add_chain(*args)
+    assert srv.chain_state(c0) == 1 and srv.chain_state(c1) == 0
+    rng = np.random.default_rng(9)
+    sp, new = _segment(rng, 16, 64, w["side"], b, 6)
+    r1 = srv.submit(c1, sp, new, commit_mode=2)     # runs once c1 is admitted
+    result = []
+
+    def remove():
+        try:
+            srv.remove_chain(c1)
+            result.append("ok")
+        except Exception as ex:       # noqa: BLE001 -- reported by the main thread
+            result.append(repr(ex))
+
+    t = threading.Thread(target=remove)
+    t.start()
+    import time
+    time.sleep(0.05)
+    assert t.is_alive() and not result
+    extra = [srv.add_chain(*args) for _ in range(300)]
+    assert all(srv.chain_state(e) == 0 for e in extra)
+    srv.remove_chain(c0)                            # frees room: c1 in, its request runs, c1 out
+    t.join(timeout=30)
+    assert not t.is_alive() and result == ["ok"]
+    assert srv.wait(r1)["accepted"]
+    assert srv.chain_state(c1) == 2 and srv.chain_state(c0) == 2
+    assert srv.chain_state(extra[0]) == 1
+    assert all(srv.chain_state(e) == 0 for e in extra[1:])
+    got = srv.download(extra[0])
+    assert np.array_equal(got, b)
+    srv.close()
+
+
+def test_action_range_inactive_and_bad_end(orc):
+    """pimc_action_range over next links: a walk through inactive beads that never
+    reaches end's successor, and an end outside the path, are refused with an error
+    (no endless walk on the host)."""
+    import pimc_amd
+    N, M = 8, 6
+    L = orc.box_side(N, DENSITY)
+    b = orc.gen_worldlines(SEED0, L, M, N, LAMBDA, 0.004, 0)
+    on = np.ones((M, N), dtype=np.uint8)
+    on[:, N - 1] = 0                                 # the last column inactive (left-packed slices)
+    nb = np.full(M, N - 1, dtype=np.int32)
+    ctx = pimc_amd.Context(L, M, N, 0.004, LAMBDA, L)
+    ctx.upload_path(b, on, nb)
+    nxt = np.zeros((M, N, 2), dtype=np.int32)
+    for m in range(M):
+        for i in range(N):
+            nxt[m, i] = ((m + 1) % M, i)
+    # start on the inactive worldline, end on another one: the walk cycles forever
+    with pytest.raises(pimc_amd.PimcError, match="does not reach"):
+        ctx.action_range((0, N - 1), (3, 2), next_links=nxt)
+    with pytest.raises(pimc_amd.PimcError, match="out of range"):
+        ctx.action_range((0, 1), (M, 1), next_links=nxt)
+    with pytest.raises(pimc_amd.PimcError, match="out of range"):
+        ctx.action_range((0, 1), (2, -1), next_links=nxt)
+    # a proper range still works
+    prv = np.array([[[(m - 1) % M, i] for i in range(N)] for m in range(M)], dtype=np.int32)
+    st = orc.State(L, M, N, 0.004, LAMBDA, L)
+    st.set_beads(b)
+    st.set_topology(on, nb, nxt, prv)
+    assert ctx.action_range((1, 2), (4, 2), next_links=nxt) == pytest.approx(st.action_range((1, 2), (4, 2)),
+                                                                             rel=1e-6)

@@ -7,8 +7,9 @@ src/move.cpp on the same worldlines.
   dU agree to 1e-9 relative (FP64 closed form and table modes: the device dU
   differs from the host's only by summation order, ~1e-15 relative, so a
   decision flip would need u within that of exp(-dU)).
-* Independent seeds, FP32 pair terms: the mean total energy per particle of
-  the two runs agrees within 5 combined standard errors (10 blocks each)."""
+* Independent seeds, FP32 pair terms on the device: the mean energies per
+  particle (thermodynamic E and V, centroid-virial E) of a long oracle chain and
+  of 64 device-resident chains agree within 3 combined standard errors."""
 import numpy as np
 import pytest
 
@@ -58,32 +59,61 @@ def test_markov_run_matches_oracle(orc, gpu, mode, rc_frac, N, M):
     assert abs(a_g - a_r) <= 1e-9 * abs(a_r)
 
 
-def blocked(x, nb=10):
+def blocked(x, nb=20):
     x = np.asarray(x)[: len(x) // nb * nb].reshape(nb, -1).mean(axis=1)
     return x.mean(), x.std(ddof=1) / np.sqrt(nb)
 
 
 def test_markov_energy_independent_seeds(orc, gpu):
-    N, M, tau = 16, 40, 0.01
-    ref, ctx, L = make_pair(orc, gpu, N, M, tau, 1.0, kind=1, mode=0)
-    Mbar, levels, com_delta = 8, 3, 0.5
+    """Independent runs agree within statistical error bars.  N=16, M=20,
+    tau=0.02 (T = 2.5 K), Staging / Bisection / CenterOfMass in rotation.
+    Oracle: one chain of 250 000 sweeps after 2 000 of burn-in, sampled every 4
+    sweeps.  Device: 64 chains with their own worldlines and seeds run on the
+    device (one launch per 4 sweeps of every chain), FP32 pair terms, each
+    sampled after every launch.  Compared: the thermodynamic E/N and V/N of
+    EnergyEstimator and the centroid-virial E/N of VirialEnergyEstimator
+    (window 5), each within 3 combined standard errors (20 blocks per side);
+    the oracle's standard error of E/N is about 0.1 K."""
+    N, M, tau = 16, 20, 0.02
+    Mbar, levels, com_delta = 6, 2, 0.5
+    kinds = ("staging", "bisection", "com")
+    ref, _, L = make_pair(orc, gpu, N, M, tau, 1.0, kind=1, mode=0)
     moves = orc.Moves(ref, Mbar, levels, True, 1, com_delta)
     rng = orc.MTRand(777)
-    chain = gpu.Chain(ctx, 31337, Mbar, levels, True, 1, com_delta)
-    kinds = ("staging", "bisection", "com")
-    e_r, e_g = [], []
-    n_sweeps, per_sample, burn = 6000, 6, 1000
-    for k in range(n_sweeps):
+    r_e, r_v, r_vir = [], [], []
+    for k in range(252000):
         for kind in kinds:
             getattr(moves, kind)(rng)
-            chain.move(kind)
-        if k >= burn and k % per_sample == 0:
-            e_r.append(ref.energy()["E/N"])
-            e_g.append(ctx.energy()["E/N"])
-    (mr, sr), (mg, sg) = blocked(e_r), blocked(e_g)
-    print(f"E/N oracle {mr:.4f} +- {sr:.4f}  device {mg:.4f} +- {sg:.4f}  ({len(e_r)} samples)")
-    assert sr > 0 and sg > 0
-    assert abs(mr - mg) <= 5.0 * np.hypot(sr, sg)
+        if k >= 2000 and k % 4 == 0:
+            e = ref.energy()
+            r_e.append(e["E/N"])
+            r_v.append(e["V/N"])
+            if k % 40 == 0:
+                r_vir.append(ref.virial_energy(5)["E/N"])
+    n_ch = 64
+    ctxs, chains = [], []
+    for c in range(n_ch):
+        ctx = gpu.Context(L, M, N, tau, LAMBDA, L, v_mode=0)
+        ctx.upload_path(orc.gen_worldlines(SEED0 + 500 + c, L, M, N, LAMBDA, tau, 1))
+        ctxs.append(ctx)
+        chains.append(gpu.Chain(ctx, 20000 + c, Mbar, levels, True, 1, com_delta))
+    gpu.run_chains_device(chains, [k % 3 for k in range(6000)])          # burn-in
+    g_e, g_v, g_vir = [], [], []
+    rounds = 1000
+    for r in range(rounds):
+        gpu.run_chains_device(chains, [k % 3 for k in range(12)])
+        es = [ctx.energy() for ctx in ctxs]
+        g_e.append(np.mean([e["E/N"] for e in es]))
+        g_v.append(np.mean([e["V/N"] for e in es]))
+        if r % 10 == 0:
+            g_vir.append(np.mean([ctx.virial_energy(5)["E/N"] for ctx in ctxs]))
+    for name, a, b in (("E/N", r_e, g_e), ("V/N", r_v, g_v), ("E_vir/N", r_vir, g_vir)):
+        (mr, sr), (mg, sg) = blocked(a), blocked(b)
+        print(f"{name}: oracle {mr:.4f} +- {sr:.4f} ({len(a)} samples)  device {mg:.4f} +- {sg:.4f} "
+              f"({len(b)} rounds x {n_ch} chains)", flush=True)
+        assert sr > 0 and sg > 0
+        assert abs(mr - mg) <= 3.0 * np.hypot(sr, sg), name
+    assert blocked(r_e)[1] <= 0.15
 
 
 @pytest.mark.parametrize("mode,rc_frac,N,M", [(2, 1.0, 16, 40), (2, 0.25, 128, 32), (1, 0.5, 64, 48)])

@@ -161,6 +161,10 @@ def test_energy_estimator(orc):
     K = 1.5 * N / tau - (vel ** 2).sum() / (4 * LAMBDA * tau * tau * M)
     assert e["K"] == pytest.approx(K, rel=1e-11)
     assert e["E/N"] == pytest.approx((e["K"] + e["V"]) / N, rel=1e-14)
+    # E_mu = E - mu N (src/estimator.cpp:1011); mu = 0 leaves E
+    e2 = st.energy(mu=-7.2)
+    assert e2["E_mu"] == pytest.approx(e2["E"] + 7.2 * N, rel=1e-14)
+    assert e["E_mu"] == e["E"]
 
 
 def _numpy_sep_hist(bm, L, n=None):

@@ -254,12 +254,13 @@ int  pimc_batch_info(const pimc_batch* b, int64_t* total_pairs, int* n_items, in
  * the fastest; out4 = {pairs_per_lane, grid, ms, candidates}. */
 int  pimc_batch_autotune(pimc_batch* b, int reps, double* out4);
 
-/* ---- serving form: Delta V requests of many chains, continuous batching ------- */
-/* Each request is the Delta V of one move over beads on distinct slices of one
- * chain (the Staging / CenterOfMass contract, src/move.cpp:1124-1408), decided on
- * the device by rand() < exp(-(A_new - A_old)) with the caller's uniform deviate
- * and committed when accepted.  No reference counterpart: the reference runs one
- * chain per process. */
+/* ---- serving form: Delta V and S(q) requests of many chains, continuous batching */
+/* A Delta V request is the move over beads on distinct slices of one chain (the
+ * Staging / CenterOfMass contract, src/move.cpp:1124-1408), decided on the device
+ * by rand() < exp(-(A_new - A_old)) with the caller's uniform deviate and
+ * committed when accepted; an S(q) request is the static structure factor of the
+ * chain (pimc_server_submit_ssf).  No reference counterpart: the reference runs
+ * one chain per process. */
 typedef struct pimc_server pimc_server;
 typedef struct {
     size_t hbm_budget;  /* bytes of chain state kept resident (arena size) */
@@ -282,8 +283,20 @@ int  pimc_server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, co
                         int n, int shift, const double vfactor[2], int commit_mode, double u,
                         uint64_t* req_id);
 int  pimc_server_poll(pimc_server* s, uint64_t req_id, int* done);
-/* out5 = {A_old, A_new, dU, accepted, latency_us}; releases the request. */
+/* out5 = {A_old, A_new, dU, accepted, latency_us}; releases the request.
+ * (A Delta V request only; S(q) requests are read with pimc_server_wait_ssf.) */
 int  pimc_server_wait(pimc_server* s, uint64_t req_id, double* out5);
+/* StaticStructureFactorEstimator::accumulate (src/estimator.cpp:3705-3737) on a
+ * resident chain: sq[i] = sf(q_i)/N of the chain's worldlines at the time the
+ * request reaches the device.  It is queued behind the chain's earlier requests
+ * (so it sees every move submitted before it) and takes a batch slot of its own
+ * when one is free.  q [nq][3] must be lattice vectors 2 pi n / L of the chain's
+ * box with |n_d| <= 40 (PIMC_E_UNSUPPORTED otherwise); 1 <= nq <= 65536.  The
+ * values equal pimc_ssf's on the same worldlines bit for bit. */
+int  pimc_server_submit_ssf(pimc_server* s, int chain, const double* q, int nq, uint64_t* req_id);
+/* Waits for an S(q) request: sq [nq] receives S(q); latency_us (may be NULL) the
+ * time from submission to completion.  Releases the request. */
+int  pimc_server_wait_ssf(pimc_server* s, uint64_t req_id, double* sq, double* latency_us);
 int  pimc_server_stats(pimc_server* s, int64_t* batches, int64_t* requests, int64_t* pair_evals,
                        size_t* free_bytes);
 typedef struct {
@@ -296,12 +309,16 @@ typedef struct {
     double lambda;      /* segment proposal width sqrt(lambda tau) */
     uint32_t seed;
     int threads;        /* client threads submitting (chains split among them); 0 = 1 */
+    double p_ssf;       /* fraction of arrivals that are S(q) requests (0: none) */
+    int ssf_nmax;       /* their q: every 2 pi n / L with |n| <= ssf_nmax (4: 257 q) */
 } pimc_load_desc;
-/* stats12 = {completed, req/s, pair evals/s, p50_us, p99_us, mean_us, accept frac,
- *            batches, requests per batch, seconds, service p50_us, service p99_us};
+/* stats16 = {completed, req/s, pair evals/s, p50_us, p99_us, mean_us, accept frac,
+ *            batches, requests per batch, seconds, service p50_us, service p99_us,
+ *            S(q) completed, S(q) p50_us, S(q) p99_us, (q, bead) phasors/s};
+ * the first twelve count the Delta V requests, the last four the S(q) ones;
  * latency counts from arrival, service from submission to the worker retiring
  * the result. */
-int  pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* load, double* stats12);
+int  pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* load, double* stats16);
 
 /* ---- one Markov chain: the diagonal moves over a resident path ---------------- */
 /* StagingMove / BisectionMove / CenterOfMassMove::attemptMove (src/move.cpp:

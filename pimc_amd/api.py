@@ -47,7 +47,7 @@ class _ServerDesc(C.Structure):
 class _LoadDesc(C.Structure):
     _fields_ = [("n_chains", C.c_int), ("rate", C.c_double), ("duration_s", C.c_double), ("seg_min", C.c_int),
                 ("seg_max", C.c_int), ("p_com", C.c_double), ("com_sigma", C.c_double), ("lam", C.c_double),
-                ("seed", C.c_uint32), ("threads", C.c_int)]
+                ("seed", C.c_uint32), ("threads", C.c_int), ("p_ssf", C.c_double), ("ssf_nmax", C.c_int)]
 
 
 class _ChainDesc(C.Structure):
@@ -122,6 +122,8 @@ SIGNATURES = {
     "pimc_server_submit": (C.c_int, [_vp, C.c_int, _ip, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_double, _u64p]),
     "pimc_server_poll": (C.c_int, [_vp, C.c_uint64, C.POINTER(C.c_int)]),
     "pimc_server_wait": (C.c_int, [_vp, C.c_uint64, _dp]),
+    "pimc_server_submit_ssf": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _u64p]),
+    "pimc_server_wait_ssf": (C.c_int, [_vp, C.c_uint64, _dp, _dp]),
     "pimc_server_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p, C.POINTER(C.c_size_t)]),
     "pimc_server_run_synthetic": (C.c_int, [_vp, C.POINTER(_LoadDesc), _dp]),
     "pimc_set_action_factors": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
@@ -533,6 +535,7 @@ class Server:
         self.h = h
         self.year, self.v_mode = year, v_mode
         self.shapes = {}
+        self._nq = {}
 
     def close(self):
         if getattr(self, "h", None):
@@ -587,6 +590,22 @@ class Server:
         self._check(lib().pimc_server_wait(self.h, rid, _p(o, _dp)), "wait")
         return dict(A_old=o[0], A_new=o[1], dU=o[2], accepted=bool(o[3] > 0.5), latency_us=o[4], raw=o)
 
+    def submit_ssf(self, chain, q):
+        """S(q) of a resident chain (lattice q of its box), queued behind its earlier requests."""
+        q = _f64(q, (-1, 3))
+        rid = C.c_uint64()
+        self._check(lib().pimc_server_submit_ssf(self.h, chain, _p(q, _dp), q.shape[0], C.byref(rid)),
+                    "submit_ssf")
+        self._nq[rid.value] = q.shape[0]
+        return rid.value
+
+    def wait_ssf(self, rid):
+        """(S(q) [nq], latency_us) of an S(q) request."""
+        o = np.zeros(self._nq.pop(rid))
+        lat = np.zeros(1)
+        self._check(lib().pimc_server_wait_ssf(self.h, rid, _p(o, _dp), _p(lat, _dp)), "wait_ssf")
+        return o, float(lat[0])
+
     def stats(self):
         b, r, p = C.c_int64(), C.c_int64(), C.c_int64()
         f = C.c_size_t()
@@ -594,12 +613,14 @@ class Server:
         return dict(batches=b.value, requests=r.value, pair_evals=p.value, free_bytes=f.value)
 
     def run_synthetic(self, n_chains, rate, duration_s, seg=(4, 64), p_com=0.1, com_sigma=0.1, lam=6.0596,
-                      seed=139853, threads=1):
-        d = _LoadDesc(n_chains, rate, duration_s, seg[0], seg[1], p_com, com_sigma, lam, seed, threads)
-        o = np.zeros(12)
+                      seed=139853, threads=1, p_ssf=0.0, ssf_nmax=4):
+        d = _LoadDesc(n_chains, rate, duration_s, seg[0], seg[1], p_com, com_sigma, lam, seed, threads, p_ssf,
+                      ssf_nmax)
+        o = np.zeros(16)
         self._check(lib().pimc_server_run_synthetic(self.h, C.byref(d), _p(o, _dp)), "run_synthetic")
         keys = ["completed", "req_per_s", "pair_evals_per_s", "p50_us", "p99_us", "mean_us", "accept_frac",
-                "batches", "req_per_batch", "seconds", "service_p50_us", "service_p99_us"]
+                "batches", "req_per_batch", "seconds", "service_p50_us", "service_p99_us",
+                "ssf_completed", "ssf_p50_us", "ssf_p99_us", "ssf_phasors_per_s"]
         return dict(zip(keys, o))
 
 

@@ -1,4 +1,4 @@
-// pimc_serve.hip -- the serving form: Delta V (and S(q)) requests from many
+// pimc_serve.hip -- the serving form: Delta V and S(q) requests from many
 // concurrent Markov chains with continuous batching.
 //
 // * Chains are admitted into a pre-allocated HBM arena (the budget) by first fit;
@@ -13,6 +13,11 @@
 // * Each request is reduced by its own fixed-order tree and decided and
 //   committed on the device, so its result does not depend on the batch it
 //   travelled in.
+// * An S(q) request (StaticStructureFactorEstimator::accumulate on a resident
+//   chain) takes a batch slot of its own when one is free: the density-mode
+//   kernels run over its q list in runs that fit the slot's rho buffer and write
+//   S(q) into the slot's mapped result block.  It is queued with the chain's
+//   moves, so it sees every move submitted before it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +45,7 @@ struct ServeChainDev {
     int64_t plane;
     const uint8_t* on;  // [M][N]
     int* cell;          // [M][N] packed cell numbers
+    int* nbs;           // [M] beads per slice (N: every bead is on)
     int nalloc, num_slices;
     int table_length;
     BoxDev box;
@@ -190,6 +196,8 @@ namespace {
 
 constexpr int kSlots = 4;            // batches in flight
 constexpr int kMaxReqPerBatch = 4096;
+constexpr int kMaxSsfQ = 1 << 16;                      // q-vectors of one S(q) request
+constexpr size_t kSsfRhoBytes = size_t(32) << 20;      // rho_q(m) buffer of a slot
 constexpr int kMaxBeadsPerBatch = 1 << 17;
 constexpr int kMaxGroupsPerBatch = 1 << 18;
 
@@ -245,12 +253,15 @@ struct Chain {
 
 struct Request {
     uint64_t id = 0;
+    int kind = 0;                     // 0: Delta V, 1: S(q)
     int chain = -1;
     int shift = 1;
     double vf[2] = {1.0, 1.0};
     int commit_mode = 0;
     double u = 0.0;
     std::vector<VnnBead> beads;
+    std::vector<int> qn;              // S(q): lattice components [nq][3]
+    std::vector<double> sq;           // S(q): result [nq]
     Clock::time_point t_submit, t_done;
     bool done = false;
     int tag = -1;                     // load-generator thread that submitted it (-1: none)
@@ -273,6 +284,10 @@ struct Slot {
     Clock::time_point t_fire{};
     std::vector<uint64_t> reqs;
     size_t desc_bytes = 0;
+    int kind = 0;                     // batch in flight: 0 Delta V requests, 1 one S(q) request
+    double* d_rho = nullptr;          // S(q): rho_q(m) of one run [M][run][2]
+    double* h_sq = nullptr;           // S(q): mapped results [kMaxSsfQ]
+    double* m_sq = nullptr;           // the device's address of h_sq
 };
 
 // One packed batch, handed from pack_batch (under the lock) to fire_batch (without).
@@ -282,6 +297,11 @@ struct BatchLaunch {
     int n_groups = 0, n_reqs = 0;
     AzizDev az{};
     const ServeChainDev* chains = nullptr;
+    // S(q) batch: the chain, its q count and inverse particle count (qn staged in h_desc)
+    int kind = 0;
+    ServeChainDev chain{};
+    int nq = 0;
+    double inv_n = 0.0, invL = 0.0;
 };
 
 }  // namespace
@@ -322,18 +342,25 @@ struct pimc_server {
     std::vector<VnnBead> pack_bd;
     std::vector<ServeGroupDev> pack_gr;
 
+    int64_t launched_ssf = 0;
+
     void run();
     int pack_batch(int slot, BatchLaunch* L);
     hipError_t fire_batch(const BatchLaunch& L);
+    hipError_t fire_ssf(const BatchLaunch& L);
     void complete(Slot& s);
     void admit_waiting();
 };
 
 namespace {
 
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Arena block of a chain: positions, cell numbers, bead mask, beads per slice.
 size_t chain_bytes(const pimc_desc& d) {
     const size_t MN = (size_t)d.num_slices * d.num_alloc;
-    return MN * 3 * sizeof(double) + MN * sizeof(int) + MN + 3 * 256;
+    return align256(MN * 3 * sizeof(double)) + align256(MN * sizeof(int)) + align256(MN) +
+           align256(sizeof(int) * (size_t)d.num_slices);
 }
 
 int sfail(pimc_server* s, int code, const std::string& m) {
@@ -352,8 +379,9 @@ int place_chain(pimc_server* s, int id, const double* beads) {
     c.off = off;
     char* base = s->arena.base + off;
     double* pos = (double*)base;
-    int* cell = (int*)(base + ((MN * 3 * sizeof(double) + 255) & ~size_t(255)));
-    uint8_t* on = (uint8_t*)((char*)cell + ((MN * sizeof(int) + 255) & ~size_t(255)));
+    int* cell = (int*)(base + align256(MN * 3 * sizeof(double)));
+    uint8_t* on = (uint8_t*)((char*)cell + align256(MN * sizeof(int)));
+    int* nbs = (int*)((char*)on + align256(MN));
     std::vector<double> soa(3 * MN);
     for (size_t t = 0; t < MN; ++t) {
         soa[t] = beads[3 * t];
@@ -364,11 +392,14 @@ int place_chain(pimc_server* s, int id, const double* beads) {
     if (hipMemcpyAsync(pos, soa.data(), sizeof(double) * 3 * MN, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
     std::vector<uint8_t> ones(MN, 1);
     if (hipMemcpyAsync(on, ones.data(), MN, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
+    std::vector<int> counts(M, N);
+    if (hipMemcpyAsync(nbs, counts.data(), sizeof(int) * M, hipMemcpyHostToDevice, st) != hipSuccess) return -1;
     ServeChainDev& cd = c.dev;
     cd.pos = pos;
     cd.plane = (int64_t)MN;
     cd.on = on;
     cd.cell = cell;
+    cd.nbs = nbs;
     cd.nalloc = N;
     cd.num_slices = M;
     BoxDev& b = cd.box;
@@ -466,9 +497,38 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
     bd.clear();
     gr.clear();
     S.reqs.clear();
+    if (reqs[ready.front()].kind == 1) {
+        // an S(q) request takes the slot alone
+        const uint64_t id = ready.front();
+        ready.pop_front();
+        Request& R = reqs[id];
+        Chain& C = chains[R.chain];
+        const int nq = (int)R.qn.size() / 3;
+        std::memcpy(S.h_desc, R.qn.data(), sizeof(int) * R.qn.size());
+        L->kind = 1;
+        L->slot = si;
+        L->chain = C.dev;
+        L->nq = nq;
+        L->inv_n = 1.0 / C.d.num_alloc;
+        L->invL = 1.0 / C.d.side;
+        L->bytes = sizeof(int) * R.qn.size();
+        S.reqs.push_back(id);
+        S.kind = 1;
+        C.busy = true;
+        S.busy = true;
+        S.seq += 1;
+        S.t_fire = Clock::now();
+        ++launched_batches;
+        ++launched_reqs;
+        ++launched_ssf;
+        return 1;
+    }
+    S.kind = 0;
+    L->kind = 0;
     while (!ready.empty() && (int)rq.size() < kMaxReqPerBatch) {
         const uint64_t id = ready.front();
         Request& R = reqs[id];
+        if (R.kind == 1) break;       // S(q) goes in the next free slot
         Chain& C = chains[R.chain];
         const int chunks = (C.d.num_alloc + kVnnChunk - 1) / kVnnChunk;
         const int nb = (int)R.beads.size();
@@ -529,6 +589,7 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
 hipError_t pimc_server::fire_batch(const BatchLaunch& L) {
     Slot& S = slots[L.slot];
     (void)hipGetLastError();
+    if (L.kind == 1) return fire_ssf(L);
     // A small batch's descriptors are read in place from mapped host memory: a
     // copy would put a DMA start in front of the kernels.  Larger batches go
     // over in one copy, since every workgroup re-reads its descriptors.
@@ -560,11 +621,55 @@ hipError_t pimc_server::fire_batch(const BatchLaunch& L) {
     return e;
 }
 
+// An S(q) request: its q list goes to the slot's device block, then per run of
+// q that fits the slot's rho buffer the density modes (k_rho_lattice) and
+// S(q) = sum_m |rho_q(m)|^2 / N (k_ssf_from_rho) straight into the slot's mapped
+// result block -- the kernels and the per-q order of pimc_ssf.
+hipError_t pimc_server::fire_ssf(const BatchLaunch& L) {
+    Slot& S = slots[L.slot];
+    const ServeChainDev& C = L.chain;
+    const int M = C.num_slices;
+    hipError_t e = hipMemcpyAsync(S.d_desc, S.h_desc, L.bytes, hipMemcpyHostToDevice, S.stream);
+    const int* d_qn = reinterpret_cast<const int*>(S.d_desc);
+    const int* h_qn = reinterpret_cast<const int*>(S.h_desc);
+    const int run = (int)std::max<size_t>(1, kSsfRhoBytes / (sizeof(double) * 2 * (size_t)M));
+    for (int k0 = 0; e == hipSuccess && k0 < L.nq; k0 += run) {
+        const int n = std::min(run, L.nq - k0);
+        RhoArgs A{};
+        A.pos = C.pos;
+        A.plane = C.plane;
+        A.nalloc = C.nalloc;
+        A.num_slices = M;
+        A.nbs = C.nbs;
+        A.invL = L.invL;
+        A.nq = n;
+        A.qn = d_qn + 3 * (size_t)k0;
+        A.rho = S.d_rho;
+        for (int d = 0; d < 3; ++d) A.nmax[d] = 0;
+        for (int k = k0; k < k0 + n; ++k)
+            for (int d = 0; d < 3; ++d) A.nmax[d] = std::max(A.nmax[d], std::abs(h_qn[3 * k + d]));
+        e = launch_rho(true, S.stream, A);
+        if (e == hipSuccess) e = launch_isf(S.stream, S.d_rho, n, M, L.inv_n, S.m_sq + k0, 1);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_serve_publish, dim3(1), dim3(64), 0, S.stream, S.d_done, S.seq);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
 void pimc_server::complete(Slot& S) {
     const auto now = Clock::now();
     for (size_t k = 0; k < S.reqs.size(); ++k) {
         Request& R = reqs[S.reqs[k]];
-        for (int t = 0; t < 4; ++t) R.out[t] = S.h_out[4 * k + t];
+        if (R.kind == 1) {
+            const int nq = (int)R.qn.size() / 3;
+            R.sq.assign(S.h_sq, S.h_sq + nq);
+            R.out[0] = nq;
+            R.out[1] = R.out[2] = R.out[3] = 0.0;
+        } else {
+            for (int t = 0; t < 4; ++t) R.out[t] = S.h_out[4 * k + t];
+        }
         R.done = true;
         R.t_done = now;
         if (track_done && R.tag >= 0 && R.tag < (int)done_lists.size()) done_lists[R.tag].push_back(R.id);
@@ -697,6 +802,11 @@ int pimc_server_create(const pimc_server_desc* d, pimc_server** out) {
             return PIMC_E_NOMEM;
         *S.h_done = 0;
         if (hipHostGetDevicePointer((void**)&S.d_done, S.h_done, 0) != hipSuccess) return PIMC_E_HIP;
+        if (hipMalloc(&S.d_rho, kSsfRhoBytes) != hipSuccess) return PIMC_E_NOMEM;
+        if (hipHostMalloc((void**)&S.h_sq, sizeof(double) * kMaxSsfQ, hipHostMallocMapped | hipHostMallocCoherent) !=
+            hipSuccess)
+            return PIMC_E_NOMEM;
+        if (hipHostGetDevicePointer((void**)&S.m_sq, S.h_sq, 0) != hipSuccess) return PIMC_E_HIP;
     }
     pimc_server* raw = s.get();
     raw->worker = std::thread([raw] { raw->run(); });
@@ -717,6 +827,8 @@ void pimc_server_destroy(pimc_server* s) {
         if (S.d_partials) (void)hipFree(S.d_partials);
         if (S.h_out) (void)hipHostFree(S.h_out);
         if (S.h_done) (void)hipHostFree(S.h_done);
+        if (S.d_rho) (void)hipFree(S.d_rho);
+        if (S.h_sq) (void)hipHostFree(S.h_sq);
         if (S.stream) (void)hipStreamDestroy(S.stream);
     }
     if (s->d_chains) (void)hipFree(s->d_chains);
@@ -802,6 +914,26 @@ int pimc_server_download(pimc_server* s, int chain, double* beads) {
 
 }  // extern "C"
 
+// Queues a built request behind its chain's earlier ones (the chain must not
+// have been removed); the worker sees it at once when the chain is idle.
+static int server_enqueue(pimc_server* s, Request&& R, uint64_t* req_id) {
+    std::lock_guard<std::mutex> g(s->mu);
+    const int chain = R.chain;
+    if (chain < 0 || chain >= (int)s->chains.size() || s->chains[chain].state == 2)
+        return sfail(s, PIMC_E_ARG, "unknown chain");
+    Chain& c = s->chains[chain];
+    R.id = s->next_id++;
+    R.t_submit = Clock::now();
+    const uint64_t id = R.id;
+    s->reqs.emplace(id, std::move(R));
+    c.queue.push_back(id);
+    if (c.queue.size() == 1 && c.state == 1 && !c.busy) s->ready.push_back(id);
+    *req_id = id;
+    s->work_events.fetch_add(1, std::memory_order_release);
+    s->cv_work.notify_one();
+    return PIMC_OK;
+}
+
 static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos, int n,
                          int shift, const double vfactor[2], int commit_mode, double u, int tag, uint64_t* req_id) {
     if (!s || !slice_ptcl || !new_pos || n < 1 || !req_id || shift < 1 || commit_mode < 0 || commit_mode > 2)
@@ -849,19 +981,22 @@ static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, c
         }
     }
     R.tag = tag;
-    std::lock_guard<std::mutex> g(s->mu);
-    if (s->chains[chain].state == 2) return sfail(s, PIMC_E_ARG, "unknown chain");
-    Chain& c = s->chains[chain];
-    R.id = s->next_id++;
-    R.t_submit = Clock::now();
-    const uint64_t id = R.id;
-    s->reqs.emplace(id, std::move(R));
-    c.queue.push_back(id);
-    if (c.queue.size() == 1 && c.state == 1 && !c.busy) s->ready.push_back(id);
-    *req_id = id;
-    s->work_events.fetch_add(1, std::memory_order_release);
-    s->cv_work.notify_one();
-    return PIMC_OK;
+    return server_enqueue(s, std::move(R), req_id);
+}
+
+// S(q) of a chain's current worldlines for the lattice vectors qn [nq][3]
+// (q = 2 pi n / L), queued behind the chain's earlier requests.
+static int server_submit_ssf(pimc_server* s, int chain, std::vector<int>&& qn, int tag, uint64_t* req_id) {
+    const int nq = (int)qn.size() / 3;
+    if (nq < 1 || nq > kMaxSsfQ) return sfail(s, PIMC_E_ARG, "S(q) requests take 1..65536 q-vectors");
+    for (int v : qn)
+        if (std::abs(v) > rho_max_nmax()) return sfail(s, PIMC_E_UNSUPPORTED, "lattice component beyond 40");
+    Request R;
+    R.kind = 1;
+    R.chain = chain;
+    R.qn = std::move(qn);
+    R.tag = tag;
+    return server_enqueue(s, std::move(R), req_id);
 }
 
 // The results of the ids in `ids` (all completed) in one critical section;
@@ -891,12 +1026,48 @@ int pimc_server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, con
     return server_submit(s, chain, slice_ptcl, new_pos, n, shift, vfactor, commit_mode, u, -1, req_id);
 }
 
+int pimc_server_submit_ssf(pimc_server* s, int chain, const double* q, int nq, uint64_t* req_id) {
+    if (!s || !q || !req_id || nq < 1) return PIMC_E_ARG;
+    double side = 0.0;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
+        if (chain < 0 || chain >= (int)s->chains.size() || s->chains[chain].state == 2)
+            return sfail(s, PIMC_E_ARG, "unknown chain");
+        side = s->chains[chain].d.side;
+    }
+    if (nq > kMaxSsfQ) return sfail(s, PIMC_E_ARG, "S(q) requests take 1..65536 q-vectors");
+    std::vector<int> qn(3 * (size_t)nq);
+    for (int k = 0; k < nq; ++k)
+        if (!lattice_q(q + 3 * k, side, &qn[3 * (size_t)k]))
+            return sfail(s, PIMC_E_UNSUPPORTED, "S(q) requests take lattice q = 2 pi n / L, |n_d| <= 40");
+    return server_submit_ssf(s, chain, std::move(qn), -1, req_id);
+}
+
+int pimc_server_wait_ssf(pimc_server* s, uint64_t req_id, double* sq, double* latency_us) {
+    if (!s || !sq) return PIMC_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req_id);
+    if (it == s->reqs.end()) return sfail(s, PIMC_E_ARG, "unknown request");
+    Request* R = &it->second;
+    if (R->kind != 1) return sfail(s, PIMC_E_ARG, "not an S(q) request: use pimc_server_wait");
+    s->cv_done.wait(lk, [&] { return R->done || s->kernel_error; });
+    if (!R->done) return sfail(s, s->kernel_error, s->kernel_error_msg);
+    std::copy(R->sq.begin(), R->sq.end(), sq);
+    if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(R->t_done - R->t_submit).count();
+    Request gone = std::move(*R);
+    s->reqs.erase(req_id);
+    lk.unlock();
+    return PIMC_OK;
+}
+
 // out5 = {A_old, A_new, dU, accepted, latency_us}; releases the request.
 int pimc_server_wait(pimc_server* s, uint64_t req_id, double* out5) {
     if (!s) return PIMC_E_ARG;
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->reqs.find(req_id);
     if (it == s->reqs.end()) return sfail(s, PIMC_E_ARG, "unknown request");
+    if (it->second.kind != 0) return sfail(s, PIMC_E_ARG, "an S(q) request: use pimc_server_wait_ssf");
     // an element pointer survives rehashes by concurrent submits; an iterator does not
     Request* R = &it->second;
     s->cv_done.wait(lk, [&] { return R->done || s->kernel_error; });
@@ -943,6 +1114,7 @@ namespace {
 struct GenChain {
     pimc_desc d{};                     // the chain's description, copied under the server lock
     std::vector<double> mirror;        // [M][N][3], kept equal to the device copy
+    bool inflight_ssf = false;         // the request in flight is an S(q)
     std::deque<Clock::time_point> arrivals;
     uint64_t inflight = 0;
     Clock::time_point t_arrival;
@@ -954,11 +1126,14 @@ struct GenStats {
     int64_t completed = 0, accepted = 0, pairs = 0;
     std::vector<double> lat;
     std::vector<double> svc;   // submit -> result retired by the worker
+    int64_t ssf_completed = 0;
+    double ssf_phasors = 0.0;  // (q, bead) density-mode terms of the completed S(q) requests
+    std::vector<double> ssf_lat;
     int err = 0;
 };
 
 void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vector<GenChain>& gc,
-                Clock::time_point t0, Clock::time_point t_end, GenStats& st) {
+                const std::vector<int>& ssf_qn, Clock::time_point t0, Clock::time_point t_end, GenStats& st) {
     std::mt19937_64 rng(L->seed + 7919ull * (uint64_t)t);
     std::uniform_real_distribution<double> U(0.0, 1.0);
     std::normal_distribution<double> G(0.0, 1.0);
@@ -1012,10 +1187,16 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
     };
     auto submit = [&](int c, Clock::time_point t_arr) -> int {
         GenChain& g = gc[c];
-        propose(c);
         uint64_t id = 0;
-        const double vf[2] = {1.0, 1.0};
-        int r = server_submit(s, c, g.sp.data(), g.prop.data(), (int)g.sp.size() / 2, 1, vf, 1, U(rng), t, &id);
+        int r;
+        g.inflight_ssf = !ssf_qn.empty() && U(rng) < L->p_ssf;
+        if (g.inflight_ssf) {
+            r = server_submit_ssf(s, c, std::vector<int>(ssf_qn), t, &id);
+        } else {
+            propose(c);
+            const double vf[2] = {1.0, 1.0};
+            r = server_submit(s, c, g.sp.data(), g.prop.data(), (int)g.sp.size() / 2, 1, vf, 1, U(rng), t, &id);
+        }
         if (r) return r;
         g.inflight = id;
         g.t_arrival = t_arr;
@@ -1059,12 +1240,18 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
             GenChain& g = gc[c];
             const double* o = outs.data() + 5 * f;
             const auto t_done = Clock::now();
-            st.lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
-            st.svc.push_back(o[4]);
-            ++st.completed;
             const pimc_desc& d = g.d;
-            st.pairs += 2LL * ((int64_t)g.sp.size() / 2) * (d.num_alloc - 1);
-            if (o[3] > 0.5) {
+            if (g.inflight_ssf) {
+                st.ssf_lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
+                ++st.ssf_completed;
+                st.ssf_phasors += (double)(ssf_qn.size() / 3) * d.num_slices * d.num_alloc;
+            } else {
+                st.lat.push_back(std::chrono::duration<double, std::micro>(t_done - g.t_arrival).count());
+                st.svc.push_back(o[4]);
+                ++st.completed;
+                st.pairs += 2LL * ((int64_t)g.sp.size() / 2) * (d.num_alloc - 1);
+            }
+            if (!g.inflight_ssf && o[3] > 0.5) {
                 ++st.accepted;
                 for (size_t k = 0; k < g.sp.size() / 2; ++k) {
                     const size_t bi = (size_t)g.sp[2 * k] * d.num_alloc + g.sp[2 * k + 1];
@@ -1106,7 +1293,8 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
 
 extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L, double* stats) {
     if (!s || !L || !stats || L->n_chains < 1 || L->duration_s <= 0 || L->seg_min < 2 || L->seg_max < L->seg_min ||
-        L->threads < 0 || L->threads > 64)
+        L->threads < 0 || L->threads > 64 || !(L->p_ssf >= 0.0 && L->p_ssf <= 1.0) ||
+        (L->p_ssf > 0.0 && (L->ssf_nmax < 0 || L->ssf_nmax > 20)))
         return PIMC_E_ARG;
     const int nc = L->n_chains;
     const int T = std::max(1, std::min(L->threads, nc));
@@ -1139,9 +1327,22 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
     pimc_server_stats(s, &b0, &r0, nullptr, nullptr);
     const auto t0 = Clock::now();
     const auto t_end = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(L->duration_s));
+    // the S(q) requests' lattice vectors: every n with |n| <= ssf_nmax, in the
+    // reference's max_int order (lexicographic from -n, src/estimator.cpp:487-541)
+    std::vector<int> ssf_qn;
+    if (L->p_ssf > 0.0)
+        for (int a = -L->ssf_nmax; a <= L->ssf_nmax; ++a)
+            for (int b = -L->ssf_nmax; b <= L->ssf_nmax; ++b)
+                for (int c = -L->ssf_nmax; c <= L->ssf_nmax; ++c)
+                    if (a * a + b * b + c * c <= L->ssf_nmax * L->ssf_nmax) {
+                        ssf_qn.push_back(a);
+                        ssf_qn.push_back(b);
+                        ssf_qn.push_back(c);
+                    }
     std::vector<GenStats> st(T);
     std::vector<std::thread> th;
-    for (int t = 0; t < T; ++t) th.emplace_back(gen_thread, s, L, t, T, std::ref(gc), t0, t_end, std::ref(st[t]));
+    for (int t = 0; t < T; ++t)
+        th.emplace_back(gen_thread, s, L, t, T, std::ref(gc), std::cref(ssf_qn), t0, t_end, std::ref(st[t]));
     for (auto& x : th) x.join();
     for (auto& x : st)
         if (x.err) return x.err;
@@ -1176,5 +1377,19 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
     auto spct = [&](double q) { return svc.empty() ? 0.0 : svc[std::min(svc.size() - 1, (size_t)(q * (svc.size() - 1)))]; };
     stats[10] = spct(0.50);
     stats[11] = spct(0.99);
+    std::vector<double> sl;
+    int64_t ssf_done = 0;
+    double phasors = 0.0;
+    for (auto& x : st) {
+        sl.insert(sl.end(), x.ssf_lat.begin(), x.ssf_lat.end());
+        ssf_done += x.ssf_completed;
+        phasors += x.ssf_phasors;
+    }
+    std::sort(sl.begin(), sl.end());
+    auto qpct = [&](double q) { return sl.empty() ? 0.0 : sl[std::min(sl.size() - 1, (size_t)(q * (sl.size() - 1)))]; };
+    stats[12] = (double)ssf_done;
+    stats[13] = qpct(0.50);
+    stats[14] = qpct(0.99);
+    stats[15] = phasors / secs;
     return PIMC_OK;
 }

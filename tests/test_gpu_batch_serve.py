@@ -186,6 +186,10 @@ def test_server_synthetic_load(orc):
     assert st3["completed"] > 100 and st3["p99_us"] >= st3["p50_us"] > 0
     st4 = srv.run_synthetic(32, rate=0.0, duration_s=0.3, threads=4)
     assert st4["completed"] > 100 and 0.0 < st4["accept_frac"] < 1.0
+    # with 5 % S(q) requests (257 q each) mixed in
+    st5 = srv.run_synthetic(32, rate=20000.0, duration_s=0.5, threads=4, p_ssf=0.05, ssf_nmax=4)
+    assert st5["completed"] > 100 and st5["ssf_completed"] > 5
+    assert st5["ssf_p99_us"] >= st5["ssf_p50_us"] > 0 and st5["ssf_phasors_per_s"] > 0
     # the chains come back whole after concurrent clients
     for c in (0, 7, 31):
         b = srv.download(c)
@@ -297,3 +301,69 @@ def test_action_range_inactive_and_bad_end(orc):
     st.set_topology(on, nb, nxt, prv)
     assert ctx.action_range((1, 2), (4, 2), next_links=nxt) == pytest.approx(st.action_range((1, 2), (4, 2)),
                                                                              rel=1e-6)
+
+
+def test_server_ssf_requests(orc):
+    """S(q) requests on resident chains (src/estimator.cpp:3705-3737): equal to
+    pimc_ssf on a context holding the same worldlines (bit for bit) and to the
+    oracle (1e-5 of max(S, M)); the same alone and inside a mix of Delta V and
+    S(q) requests on other chains; queued behind the chain's earlier moves, so a
+    committed move is seen; non-lattice q refused."""
+    import pimc_amd
+    chains = [walker(orc, N, M, 1.5, SEED0 + 40 + k, kind=0) for k, (N, M) in enumerate([(64, 32), (128, 16), (32, 40)])]
+    srv = pimc_amd.Server(1 << 28, streams=3, max_side=40.0)
+    ids = [srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b) for w, b in chains]
+    w0, b0 = chains[0]
+    L0 = w0["side"]
+    q = orc.qvectors(L0, "max_int", "4 4 4")
+    q = q[np.einsum("qd,qd->q", q, q) <= (4 * 2 * np.pi / L0) ** 2 * (1 + 1e-12)]
+    assert q.shape[0] == 257
+    ctx = pimc_amd.Context(L0, w0["M"], w0["N"], w0["tau"], LAMBDA, w0["rc"])
+    ctx.upload_path(b0)
+    ref_ctx = ctx.ssf(q)
+    alone, lat = srv.wait_ssf(srv.submit_ssf(ids[0], q))
+    assert lat > 0
+    assert np.array_equal(alone, ref_ctx)
+    st = orc.State(L0, w0["M"], w0["N"], w0["tau"], LAMBDA, w0["rc"])
+    st.set_beads(b0)
+    pick = [0, 1, 128, 200, 256]
+    ref = st.ssf(q[pick])
+    assert np.all(np.abs(alone[pick] - ref) <= 1e-5 * np.maximum(ref, w0["M"]))
+    # in a mix of Delta V and S(q) requests on the other chains
+    rng = np.random.default_rng(12)
+    others, others_ssf = [], []
+    for rep in range(40):
+        k = 1 + rep % 2
+        w, b = chains[k]
+        s2, n2 = _segment(rng, w["M"], w["N"], w["side"], b, int(rng.integers(4, 12)))
+        others.append(srv.submit(ids[k], s2, n2))
+        if rep % 5 == 0:
+            qk = orc.qvectors(w["side"], "max_int", "3 3 3")
+            others_ssf.append(srv.submit_ssf(ids[k], qk))
+        if rep == 20:
+            mixed = srv.submit_ssf(ids[0], q)
+    got, _ = srv.wait_ssf(mixed)
+    assert np.array_equal(got, alone)
+    assert all(np.isfinite(srv.wait(r)["dU"]) for r in others)
+    assert all(np.all(np.isfinite(srv.wait_ssf(r)[0])) for r in others_ssf)
+    # a committed move, then S(q) on the same chain: the S(q) sees the move
+    sp, new = _segment(rng, w0["M"], w0["N"], L0, b0, 9)
+    r_move = srv.submit(ids[0], sp, new, commit_mode=2)
+    r_sq = srv.submit_ssf(ids[0], q)
+    assert srv.wait(r_move)["accepted"]
+    after, _ = srv.wait_ssf(r_sq)
+    ctx.update_beads(sp, new)
+    assert np.array_equal(after, ctx.ssf(q))
+    assert not np.array_equal(after, alone)
+    for (m, i), p in zip(sp, new):
+        st.update_bead(int(m), int(i), p)
+    ref2 = st.ssf(q[pick])
+    assert np.all(np.abs(after[pick] - ref2) <= 1e-5 * np.maximum(ref2, w0["M"]))
+    # errors
+    with pytest.raises(pimc_amd.PimcError, match="lattice"):
+        srv.submit_ssf(ids[0], [[0.3, 0.1, 0.2]])
+    rid = srv.submit_ssf(ids[1], q[:3] * L0 / chains[1][0]["side"])
+    with pytest.raises(pimc_amd.PimcError, match="wait_ssf"):
+        srv.wait(rid)
+    assert srv.wait_ssf(rid)[0].shape == (3,)
+    srv.close()

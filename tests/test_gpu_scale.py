@@ -11,7 +11,8 @@ Serving: the 1024 chains of the serving workload (N in [32, 512], M in [64,
 512]) resident in one arena.  Segment (Staging/Bisection) and CenterOfMass Delta
 V requests (src/move.cpp:1124-1259, :1315-1408) on chains spanning the size
 range against the oracle at |dU - dU_ref| <= 1e-6 (|A_old| + |A_new|), then the
-same requests inside a mix of requests on every chain: bit-identical results."""
+same requests inside a mix of requests on every chain: bit-identical results;
+an S(q) request longer than one slot's density-mode buffer equal to pimc_ssf."""
 import collections
 
 import numpy as np
@@ -141,4 +142,16 @@ def test_server_1024_chains(orc):
         assert np.array_equal(m["raw"][:4], a["raw"][:4])
     st = srv.stats()
     assert st["requests"] == 2 * len(reqs) + len(mix)
+    # S(q) on the chain with the most slices, with more q than one slot's
+    # rho buffer holds at that M: the same as pimc_ssf on a context
+    from pimc_amd.workloads import lattice_q
+    k = max(picks, key=lambda c: ws[c]["M"])
+    w = ws[k]
+    q = lattice_q(w["side"], nmax=10)                    # the 9261 q of the cube |n_d| <= 10
+    assert q.shape[0] * w["M"] * 16 > 32 << 20
+    sq, lat = srv.wait_ssf(srv.submit_ssf(k, q))
+    ctx = pimc_amd.Context(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"])
+    ctx.upload_path(beads[k])
+    assert np.array_equal(sq, ctx.ssf(q))
+    print(f"S(q) request on chain {k} (N={w['N']}, M={w['M']}): {q.shape[0]} q in {lat:.0f} us", flush=True)
     srv.close()

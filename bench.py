@@ -40,8 +40,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--isf-reps", type=int, default=5)
-    ap.add_argument("--legs", default="plain,gsf,virial,chain,batched,served",
-                    help="comma list of plain, gsf, virial, chain, batched, served (plain always runs)")
+    ap.add_argument("--legs", default="plain,large,gsf,virial,chain,batched,served",
+                    help="comma list of plain, large, gsf, virial, chain, batched, served (plain always runs)")
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--probe-copy", type=int, default=0,
                     help="also run a plain copy kernel of this many bytes (HBM counter scaling)")
@@ -192,8 +192,13 @@ def main():
         line["virial"] = virial_leg(pimc_amd, beads, L, M, N, tau, not args.no_cpu_baseline)
     if "chain" in legs:
         line["chain"] = chain_leg(pimc_amd, beads, L, M, N, tau, not args.no_cpu_baseline)
+    if "large" in legs:
+        line["large"] = large_leg(pimc_amd, not args.no_cpu_baseline)
+    plain_refs = {"step": value, "pair_kernel": pairs_full / pa_avg_s}
+    if cxx:
+        plain_refs["cxx_total_action"] = pairs_full / (cxx["total_action_p50_us"] * 1e-6)
     if "batched" in legs:
-        line["batched"] = batched_leg(pimc_amd, value)
+        line["batched"] = batched_leg(pimc_amd, plain_refs)
     if "served" in legs:
         line["served"] = served_leg(pimc_amd, plain_dv_rate, args.served_seconds)
     if args.probe_copy:
@@ -365,7 +370,90 @@ def chain_leg(pimc_amd, beads, L, M, N, tau, cpu, moves=3000):
     return out
 
 
-def batched_leg(pimc_amd, plain_value):
+def large_leg(pimc_amd, cpu, reps=10, sq_reps=3):
+    """The large system: N=2048 at 0.02198 A^-3, T=1.0 K, tau=0.004 (M=250), rc =
+    L/2 (2x2x2 LookupTable cells), Gaussian-bridge worldlines.  The full action
+    (524 032 000 pair evaluations), S(q) and F(q,tau) on the 100 401 lattice q
+    with |q| <= 4 A^-1 (all 250 lags), each with the roofline of its dominant
+    kernel from the kernels' own event timing."""
+    from pimc_amd.workloads import DENSITY, LAMBDA, SEED0, box_side, lattice_q
+    N, M, tau = 2048, 250, 0.004
+    L = box_side(N, DENSITY)
+    b = pimc_amd.gen_worldlines(SEED0, L, M, N, LAMBDA, tau, 1)
+    ctx = pimc_amd.Context(L, M, N, tau, LAMBDA, L / 2, v_mode=pimc_amd.V_FP32)
+    ctx.upload_path(b)
+    q = lattice_q(L, qmag_max=4.0)
+    nq = q.shape[0]
+    pairs = M * N * (N - 1) // 2
+    for _ in range(2):
+        ctx.potentialAction()
+    ctx.set_timing(True, kinds=["pair_action"])
+    pimc_amd.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        a = ctx.potentialAction()
+    wall = (time.perf_counter() - t0) / reps
+    pa_ms, pa_n = ctx.kernel_time("pair_action")
+    ctx.set_timing(False)
+    pa_s = pa_ms / max(pa_n, 1) * 1e-3
+    ach = pairs * F_PAIR / pa_s / 1e12
+    ctx.ssf(q)
+    ctx.set_timing(True, kinds=["rho", "ssf"])
+    t0 = time.perf_counter()
+    for _ in range(sq_reps):
+        ctx.ssf(q)
+    ssf_wall = (time.perf_counter() - t0) / sq_reps
+    rho_ms, rho_n = ctx.kernel_time("rho")
+    ctx.set_timing(False)
+    ctx.set_timing(True, kinds=["rho", "isf"])
+    t0 = time.perf_counter()
+    for _ in range(sq_reps):
+        ctx.isf(q)
+    isf_wall = (time.perf_counter() - t0) / sq_reps
+    rho2_ms, rho2_n = ctx.kernel_time("rho")
+    lag_ms, lag_n = ctx.kernel_time("isf")
+    ctx.set_timing(False)
+    ctx.close()
+    rho_s = (rho_ms + rho2_ms) / max(rho_n + rho2_n, 1) * 1e-3
+    lag_s = lag_ms / max(lag_n, 1) * 1e-3
+    rho_flop = nq * M * N * F_QB + nq * M * F_QMT
+    out = {"workload": "He-4 N=2048, rho=0.02198 A^-3, T=1.0 K, tau=0.004, M=250, rc=L/2, Aziz 1979, "
+                       "Gaussian-bridge worldlines; full action + S(q) + F(q,tau) on 100401 lattice q (|q|<=4 A^-1)",
+           "action": a, "pairs_per_launch": pairs, "total_action_wall_ms": wall * 1e3,
+           "pair_evals_per_s": pairs / pa_s,
+           "roofline_pair": {"bound": "valu", "kernel": "k_pair_action<mixed, 2048>", "achieved": ach,
+                             "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": ach / PEAK_FP32_TFLOPS,
+                             "avg_launch_us": pa_s * 1e6, "flop_per_pair": F_PAIR},
+           "n_q": int(nq), "ssf_wall_ms": ssf_wall * 1e3, "isf_wall_ms": isf_wall * 1e3,
+           "roofline_rho": {"bound": "valu", "kernel": "k_rho_lattice", "achieved": rho_flop / rho_s / 1e12,
+                            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                            "frac": rho_flop / rho_s / 1e12 / PEAK_FP32_TFLOPS, "avg_launch_ms": rho_s * 1e3,
+                            "flop_per_q_bead": F_QB, "q_bead_phasors_per_s": nq * M * N / rho_s},
+           "lag_kernel": {"kernel": "k_isf_from_rho", "avg_launch_ms": lag_s * 1e3,
+                          "tflops_algorithmic": nq * M * M * F_QMT / lag_s / 1e12}}
+    if cpu:
+        # the oracle on a bounded sample of the same worldlines: the pair sums of
+        # 4 slices, and the direct S(q) pair sum of one q over 4 slices
+        from oracle import pyoracle as orc
+        Ms = 4
+        st = orc.State(L, Ms, N, tau, LAMBDA, L / 2)
+        st.set_beads(np.ascontiguousarray(b[:Ms]))
+        t0 = time.perf_counter()
+        st.total_action()
+        t_a = time.perf_counter() - t0
+        qk = q[[int(np.argmax(np.abs(q[:, 2])))]]
+        t0 = time.perf_counter()
+        st.ssf(qk)
+        t_s = time.perf_counter() - t0
+        out["cpu_port"] = {"sample": f"{Ms} slices of N=2048: the total action's pair sums, and the direct S(q) "
+                                     f"pair sum of one q", "cores": 1,
+                           "pair_evals_per_s": Ms * N * (N - 1) // 2 / t_a,
+                           "ssf_pair_terms_per_s": Ms * N * (N - 1) // 2 / t_s,
+                           "full_action_s_scaled": t_a * M / Ms}
+    return out
+
+
+def batched_leg(pimc_amd, plain):
     """512 walkers, N in {16..256}, M in {64..512}, T in [1,3] K, Gaussian-bridge
     worldlines rounded to FP32; autotuned tiles; hipGraph replays."""
     from pimc_amd.workloads import LAMBDA, batched_walkers
@@ -393,7 +481,11 @@ def batched_leg(pimc_amd, plain_value):
             "pairs_per_launch": info["total_pairs"], "device_ms": dev_ms, "wall_ms": wall * 1e3,
             "pair_evals_per_s": rate, "tflops_algorithmic": rate * F_PAIR / 1e12,
             "frac_fp32_peak": rate * F_PAIR / 1e12 / PEAK_FP32_TFLOPS,
-            "tiles": tune, "vs_plain": rate / plain_value}
+            "tiles": tune,
+            "vs_plain": {k: rate / v for k, v in plain.items()},
+            "vs_plain_note": "step: the plain one-walker step from Python (full action + 3 dV); pair_kernel: the "
+                             "plain all-pairs kernel alone (events); cxx_total_action: pimc_total_action p50 wall "
+                             "from a C++ caller"}
 
 
 def served_leg(pimc_amd, plain_dv_rate, seconds):
@@ -406,20 +498,26 @@ def served_leg(pimc_amd, plain_dv_rate, seconds):
         b = pimc_amd.api.gen_worldlines(w["seed"], w["side"], w["M"], w["N"], LAMBDA, w["tau"], w["kind"])
         srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b)
     threads = 4   # client threads submitting (the chains split among them)
-    out = {"workload": "1024 chains, N in [32,512], M in [64,512], segment moves of 4..64 slices + 10% CoM, "
-                       "Poisson arrivals", "client_threads": threads, "rates": {}}
+    p_ssf = 0.01  # occasional S(q) requests: 257 lattice q (|n| <= 4) of the chain's box
+    out = {"workload": "1024 chains, N in [32,512], M in [64,512], segment moves of 4..64 slices + 10% CoM + "
+                       "1% S(q) requests (257 q), Poisson arrivals", "client_threads": threads, "rates": {}}
     for rate in (2e3, 2e4, 2e5):
-        st = srv.run_synthetic(1024, rate=rate, duration_s=seconds, threads=threads)
-        out["rates"][f"{int(rate)}"] = {k: float(v) for k, v in st.items()}
-    # closed loop (every chain always has a request in flight): throughput, with
-    # more client threads, since the submitting side is the limit there
+        st = srv.run_synthetic(1024, rate=rate, duration_s=seconds, threads=threads, p_ssf=p_ssf, ssf_nmax=4)
+        d = {k: float(v) for k, v in st.items()}
+        d["vs_plain_delta_v_cxx"] = d["pair_evals_per_s"] / plain_dv_rate
+        out["rates"][f"{int(rate)}"] = d
+    # closed loop (every chain always has a request in flight): the server's
+    # throughput limit, with more client threads since submitting is the limit there
     closed_threads = 8
     st = srv.run_synthetic(1024, rate=0.0, duration_s=seconds, threads=closed_threads)
     out["closed_loop"] = {k: float(v) for k, v in st.items()}
     out["closed_loop"]["client_threads"] = closed_threads
-    best = max([out["closed_loop"]] + list(out["rates"].values()), key=lambda d: d["pair_evals_per_s"])
-    out["pair_evals_per_s"] = best["pair_evals_per_s"]
-    out["vs_plain_delta_v"] = best["pair_evals_per_s"] / plain_dv_rate
+    out["closed_loop"]["vs_plain_delta_v_cxx"] = out["closed_loop"]["pair_evals_per_s"] / plain_dv_rate
+    top = out["rates"]["200000"]
+    out["pair_evals_per_s"] = top["pair_evals_per_s"]
+    out["vs_plain_delta_v"] = top["pair_evals_per_s"] / plain_dv_rate
+    out["vs_plain_note"] = ("served dV pair evaluations/s at the 200k req/s open-loop rate over the plain drop-in's "
+                            "dV pair evaluations/s (pimc_delta_action from a C++ caller, segments 8/16/32)")
     srv.close()
     return out
 
@@ -451,7 +549,19 @@ def cpu_baseline(beads, L, M, N, tau, lam, pairs_full, pairs_dv):
         lat.append((time.perf_counter() - t1) * 1e6)
         dv_pairs += pairs_dv[s]
     dv_t = time.perf_counter() - t0
+    # every host core this process may use (at most 16, a GPU's share of its box):
+    # the slices' pair sums over that many threads, the same sum as the serial one
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    st.total_action(threads=cores)
+    reps_mt, t_mt = 0, 0.0
+    t0 = time.perf_counter()
+    while t_mt < 5.0 and reps_mt < 50:
+        st.total_action(threads=cores)
+        reps_mt += 1
+        t_mt = time.perf_counter() - t0
     return {"value": (pairs + dv_pairs) / (t + dv_t), "unit": "bead-pair evaluations/s", "cores": 1,
+            "all_cores": {"cores": cores, "full_action_pairs_per_s": pairs_full * reps_mt / t_mt,
+                          "sample": f"{reps_mt} full potential actions, slices over {cores} threads"},
             "kind": "port",
             "sample": f"{reps} full potential actions of the N=256 M=167 configuration ({t:.1f} s) + "
                       f"{len(lat)} Bisection dV (old+new Vnn lists, {dv_t:.1f} s), single thread",

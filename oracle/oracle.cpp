@@ -15,6 +15,7 @@
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -486,23 +487,48 @@ struct State {
     }
     std::array<double, 2> Vslice(int slice) {
         // action.cpp:902-947 (the free external potential is identically 0)
+        sepHist.fill(0);
+        return VslicePairs(slice, true);
+    }
+    // The pair sum of V(slice); the separation histogram is filled when hist is
+    // set (the const form, hist = false, may run on several threads at once).
+    std::array<double, 2> VslicePairs(int slice, bool hist) {
         double totVint = 0.0;
         double totVext = 0.0;
         Bead b1{slice, 0}, b2{slice, 0};
         int n = numBeadsAtSlice[slice];
-        sepHist.fill(0);
         std::vector<double> extFactors;
         for (b1[1] = 0; b1[1] < n; b1[1]++) {
             BeadState s1 = getState(b1);
             extFactors.push_back(factor(s1));
             for (b2[1] = b1[1] + 1; b2[1] < n; b2[1]++) {
                 Vec sep = getSeparation(b2, b1);
-                updateSepHist(sep);
+                if (hist) updateSepHist(sep);
                 totVint += factor(s1, b2) * aziz.V(sep);
             }
         }
         for (double f : extFactors) totVext += f * 0.0;
         return {totVext, totVint};
+    }
+    // totalAction with the slices' pair sums spread over `threads` host threads
+    // (slice m on thread m mod threads) and added in slice order as totalAction
+    // adds them: the same value.  Primitive factors only (no gradient correction).
+    double totalActionThreads(int threads) {
+        std::vector<std::array<double, 2>> v(M);
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t)
+            pool.emplace_back([this, t, threads, &v] {
+                for (int slice = t; slice < M; slice += threads) v[slice] = VslicePairs(slice, false);
+            });
+        for (auto& th : pool) th.join();
+        double totU = 0.0;
+        for (int slice = 0; slice < M; slice++) {
+            double s = 0.0;
+            s += v[slice][0];
+            s += v[slice][1];
+            totU += VFactor[slice % 2] * tauShift() * s;
+        }
+        return totU;
     }
     double Vnn(const Bead& b1, double totVext) {
         // action.cpp:1044-1060
@@ -1282,6 +1308,11 @@ void orc_state_set_action(void* p, int shift, double vf0, double vf1) {
     st->VFactor[1] = vf1;
 }
 double orc_total_action(void* p) { return static_cast<State*>(p)->totalAction(); }
+double orc_total_action_threads(void* p, int threads) {
+    State* st = static_cast<State*>(p);
+    if (threads < 1 || st->gradVFactor[0] > kEPS || st->gradVFactor[1] > kEPS) return std::nan("");
+    return st->totalActionThreads(threads);
+}
 void orc_V_slice(void* p, int slice, double* out2, int32_t* hist) {
     State* st = static_cast<State*>(p);
     auto v = st->Vslice(slice);

@@ -69,6 +69,7 @@ void  orc_state_set_action(void* st, int shift, double vf0, double vf1); /* setS
 void* orc_state_aziz(void* st);
 
 double orc_total_action(void* st);                      /* LocalAction::potentialAction() action.cpp:456-472 */
+double orc_total_action_threads(void* st, int threads);  /* the same sum, slices over host threads */
 void   orc_V_slice(void* st, int slice, double* out2, int32_t* sep_hist50); /* V(int) :902-947 */
 double orc_Vnn_bead(void* st, int slice, int ptcl);     /* Vnn(bead) :1026-1060 */
 double orc_Vnn_slice(void* st, int slice);              /* Vnn(int)  :1069-1123 */

@@ -78,6 +78,7 @@ def lib():
             "orc_state_set_action": (None, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
             "orc_state_aziz": (C.c_void_p, [C.c_void_p]),
             "orc_total_action": (C.c_double, [C.c_void_p]),
+            "orc_total_action_threads": (C.c_double, [C.c_void_p, C.c_int]),
             "orc_V_slice": (None, [C.c_void_p, C.c_int, dp, ip]),
             "orc_Vnn_bead": (C.c_double, [C.c_void_p, C.c_int, C.c_int]),
             "orc_Vnn_slice": (C.c_double, [C.c_void_p, C.c_int]),
@@ -277,8 +278,12 @@ class State:
     def action_correction(self, slice_, ptcl):
         return lib().orc_action_correction(self.h, slice_, ptcl)
 
-    def total_action(self):
-        return lib().orc_total_action(self.h)
+    def total_action(self, threads=None):
+        """LocalAction::potentialAction(); threads: the slices' pair sums over that
+        many host threads (the same value, primitive factors only)."""
+        if threads is None:
+            return lib().orc_total_action(self.h)
+        return lib().orc_total_action_threads(self.h, threads)
 
     def V_slice(self, slice_, hist=False):
         o = np.zeros(2)

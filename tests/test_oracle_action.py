@@ -215,3 +215,14 @@ def test_sep_hist_edges(orc):
     assert h[0] == 1                    # the coincident pair
     assert h.sum() == 6 - 2             # both corner pairs dropped
     assert h[int(0.5 * L * (1 - 1e-9) / dsep)] == 2 == h[28]
+
+
+def test_total_action_threads(orc):
+    """The threaded total action (the all-cores CPU baseline) is the serial one
+    bit for bit: the same per-slice sums added in slice order."""
+    st, b, L = make_state(orc, 24, 9, 0.004, kind=1)
+    a = st.total_action()
+    for t in (1, 2, 4, 16):
+        assert st.total_action(threads=t) == a
+    st.set_action(2, (2.0 / 3.0, 4.0 / 3.0))
+    assert st.total_action(threads=3) == st.total_action()

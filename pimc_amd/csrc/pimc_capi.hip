@@ -308,6 +308,22 @@ int pimc_ctx::wait_done(unsigned long long s) {
     return __atomic_load_n(f, __ATOMIC_ACQUIRE) == s ? 0 : -1;
 }
 
+int pimc_ctx::wait_direct(int n, unsigned long long s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int k = 0;
+    for (unsigned it = 1;; ++it) {
+        while (k < n && __atomic_load_n(&h_vdir[k].seq, __ATOMIC_ACQUIRE) == s) ++k;
+        if (k == n) return 0;
+        if ((it & 4095u) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) break;
+        __builtin_ia32_pause();
+    }
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return (int)e;
+    for (; k < n; ++k)
+        if (__atomic_load_n(&h_vdir[k].seq, __ATOMIC_ACQUIRE) != s) return -1;
+    return 0;
+}
+
 // ===========================================================================
 // context management
 // ===========================================================================
@@ -583,6 +599,13 @@ int pimc_ctx_create(const pimc_desc* d, pimc_ctx** out) {
         c->host_allocs.push_back((void*)c->h_map);
         std::memset(c->h_map, 0, sizeof(double) * 64);
         e = hipHostGetDevicePointer((void**)&c->d_map, c->h_map, 0);
+        if (e != hipSuccess) return fail(nullptr, PIMC_E_HIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+        e = hipHostMalloc((void**)&c->h_vdir, sizeof(VnnDirectOut) * kVnnInline,
+                          hipHostMallocMapped | hipHostMallocCoherent);
+        if (e != hipSuccess) return fail(nullptr, PIMC_E_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        c->host_allocs.push_back((void*)c->h_vdir);
+        std::memset(c->h_vdir, 0, sizeof(VnnDirectOut) * kVnnInline);
+        e = hipHostGetDevicePointer((void**)&c->d_vdir, c->h_vdir, 0);
         if (e != hipSuccess) return fail(nullptr, PIMC_E_HIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
     }
     HIP_OR_FAIL(c.get(), hipMemsetAsync(c->d_hash, 0xff, sizeof(int) * (size_t)cells * M * N, c->stream));
@@ -1221,6 +1244,28 @@ static int run_vnn(pimc_ctx* c, int n, int shift, const double vf[2], bool commi
     A.done = c->d_done();
     A.seq = ++c->seq;
     VnnInline req;
+    // short requests without the correction or per-bead outputs: one workgroup
+    // per bead publishes its two sums; the weighted bead terms are added here in
+    // bead order
+    if (inline_req && A.chunks == 1 && !cor && !per_bead && !cor_out && out3) {
+        std::memcpy(req.beads, c->h_req_beads, sizeof(VnnBead) * n);
+        const KernelTimer kt = c->kernel_timer(1);
+        HIP_OR_FAIL(c, launch_vnn_direct(c->desc.v_mode, c->stream, A, req, az, c->d_vdir, &kt));
+        if (c->wait_direct(n, A.seq) != 0) return fail(c, PIMC_E_HIP, "delta action: results not published");
+        const double tau_s = shift * c->desc.tau;
+        double ao = 0.0, an = 0.0, du = 0.0;
+        for (int k = 0; k < n; ++k) {
+            const double co = ((req.beads[k].slice & 1) ? A.vfactor1 : A.vfactor0) * tau_s;
+            const double vo = c->h_vdir[k].v_old, vn = c->h_vdir[k].v_new;
+            ao += co * vo;
+            an += co * vn;
+            du += co * (vn - vo);
+        }
+        out3[0] = ao;
+        out3[1] = an;
+        out3[2] = du;
+        return PIMC_OK;
+    }
     if (inline_req) {
         std::memcpy(req.beads, c->h_req_beads, sizeof(VnnBead) * n);
     } else {

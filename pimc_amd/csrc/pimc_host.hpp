@@ -72,6 +72,10 @@ struct pimc_ctx {
     unsigned long long seq = 0;
     unsigned long long* d_done() { return reinterpret_cast<unsigned long long*>(d_map + kDoneSlot); }
     int wait_done(unsigned long long s);   // spin, then fall back to a stream sync
+    // the direct Delta V form's per-bead results (mapped, coherent pinned block)
+    pimc::VnnDirectOut* h_vdir = nullptr;
+    pimc::VnnDirectOut* d_vdir = nullptr;
+    int wait_direct(int n, unsigned long long s);   // every slot of n carries s
     int lds_beads = 0;            // slice positions staged in LDS (0: read from HBM)
     int pair_lds = 0;             // the pair kernel's staging (lds_beads, or 0 when tuned off)
     // launch shape of the pair kernel (pimc_set_launch_shape / pimc_autotune; 0 / -1: automatic)

@@ -142,6 +142,15 @@ struct VnnInline {
     VnnBead beads[kVnnInline];
 };
 
+// One moved bead's result of the direct Delta V form, written by its workgroup
+// into the caller's mapped block: Vnn at the old and the new position, then the
+// call's sequence number once both are visible.
+struct VnnDirectOut {
+    double v_old, v_new;
+    unsigned long long seq;
+    unsigned long long pad;
+};
+
 struct RhoArgs {
     const double* pos;     // SoA planes [3][M][nalloc]
     int64_t plane;
@@ -194,6 +203,10 @@ int pair_grid_for(int lds_beads, int num_cus);
 hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
                       const KernelTimer* timer = nullptr, double* gnn_part = nullptr, int lds_beads = 0,
                       const KernelTimer* gnn_timer = nullptr);
+// The direct form (k_vnn_direct): at most kVnnInline beads travelling inline, at
+// most kVnnChunk beads per slice; out [n_beads] is host-mapped.
+hipError_t launch_vnn_direct(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
+                             VnnDirectOut* out, const KernelTimer* timer = nullptr);
 // gradVSquared(slice) partials [M][blocks_per_slice] for every slice.
 hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,

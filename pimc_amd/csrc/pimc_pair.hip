@@ -897,6 +897,62 @@ __device__ __forceinline__ void vnn_finalize_body(const VnnArgs& A, const VnnInl
 
 __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) { vnn_finalize_body(A, req); }
 
+// Delta V of a short request with no hand-off between workgroups: workgroup b
+// sums bead b's Vnn over all its partners (the slice holds at most kVnnChunk
+// beads) at the old position on its first 256 lanes and at the new position on
+// the other 256, so each lane's chain is one pair term per partner; it commits
+// bead b when asked (every other workgroup reads another slice: the beads sit on
+// distinct slices) and publishes {Vnn_old, Vnn_new} with the call's sequence
+// number straight into the caller's mapped block, slot b.  The caller adds the
+// beads' weighted terms in bead order once every slot carries the sequence
+// number.  A lane's partners, and so its sum, are those of the two-position
+// kernels' lanes, and the waves are added in the same order.
+constexpr int kVnnDirectThreads = 2 * kVnnThreads;
+template <int MODE>
+__global__ __launch_bounds__(kVnnDirectThreads) void k_vnn_direct(VnnArgs A, VnnInline req, AzizDev az,
+                                                                  VnnDirectOut* out) {
+    __shared__ double red[kVnnDirectThreads / 64];
+    const int b = blockIdx.x;
+    const VnnBead q = req.beads[b];
+    const VnnSubject v = vnn_subject(A, q);
+    const double inv_dr = 1.0 / az.dr;
+    const bool at_new = threadIdx.x >= kVnnThreads;
+    const int j0 = threadIdx.x - (at_new ? kVnnThreads : 0);
+    const double sx = at_new ? v.nx : v.ox, sy = at_new ? v.ny : v.oy, sz = at_new ? v.nz : v.oz;
+    const int cs = at_new ? v.c_new : v.c_old;
+    double acc = 0.0;
+    for (int j = j0; j < A.nalloc; j += kVnnThreads) {
+        const bool on = (A.on[v.m * A.nalloc + j] != 0) && (j != v.me) && v.me_on;
+        const double xj = v.px[j], yj = v.py[j], zj = v.pz[j];
+        double f = 1.0;
+        if (v.factors) f = pair_factor(v.s1, bead_state(A.worm, v.m, j));
+        int cj = 0;
+        if (v.use_cells) cj = A.cell[v.m * A.nalloc + j];
+        const double r2 = sep_r2<MODE>(sx, sy, sz, xj, yj, zj, A.box);
+        const bool ok = on && (r2 < A.box.rc2) && (!v.use_cells || cells_adjacent(cs, cj, A.box.ngrid));
+        const double val = pair_value_from_r2<MODE>(r2, az, inv_dr);
+        acc += ok ? f * val : 0.0;
+    }
+    acc = wave_sum(acc);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();   // also: every lane's partner reads are done before the commit
+    if (threadIdx.x == 0) {
+        VnnDirectOut* o = out + b;
+        o->v_old = ((red[0] + red[1]) + red[2]) + red[3];
+        o->v_new = ((red[4] + red[5]) + red[6]) + red[7];
+        if (q.flags & 4) {
+            const int cn = cell_of(q.new_pos[0], q.new_pos[1], q.new_pos[2], A.box);
+            cell_update_bead(q.slice, q.ptcl, cn, A.nalloc, A.num_slices, A.cell, A.label, A.numlab, A.hash);
+            double* qx = A.pos + (int64_t)q.slice * A.nalloc + q.ptcl;
+            qx[0] = q.new_pos[0];
+            qx[A.plane] = q.new_pos[1];
+            qx[2 * A.plane] = q.new_pos[2];
+        }
+        publish_done(&o->seq, A.seq);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Sequential bead updates (any slices, list order) -- Path::updateBead.
 // ---------------------------------------------------------------------------
@@ -1218,6 +1274,23 @@ void balance_row_tiles(const std::vector<RowTile>& tiles, int G, std::vector<int
         acc += row_tile_pairs(tiles[k]) + kTileCost;
         while (g < G && acc * G >= tot * g) ranges[g++] = k + 1;
     }
+}
+
+hipError_t launch_vnn_direct(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
+                             VnnDirectOut* out, const KernelTimer* timer) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    if (A.n_beads <= 0) return hipSuccess;
+    if (A.n_beads > kVnnInline || A.nalloc > kVnnChunk || A.beads) return hipErrorInvalidValue;
+    dim3 g(A.n_beads), b(kVnnDirectThreads);
+    if (timer && timer->start) (void)hipEventRecord(timer->start, st);
+    if (mode == 0)
+        hipLaunchKernelGGL(k_vnn_direct<0>, g, b, 0, st, A, req, az, out);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_vnn_direct<1>, g, b, 0, st, A, req, az, out);
+    else
+        hipLaunchKernelGGL(k_vnn_direct<2>, g, b, 0, st, A, req, az, out);
+    if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
+    return hipGetLastError();
 }
 
 hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,

@@ -71,6 +71,14 @@ struct ServeGroupDev {
     int chunk;
 };
 
+// One request's result in the slot's mapped block: A_old, A_new, dU, accepted,
+// then the batch's sequence number once the four are visible.
+struct ServeOut {
+    double v[4];
+    unsigned long long seq;
+    unsigned long long pad;
+};
+
 template <int MODE>
 __global__ __launch_bounds__(kVnnThreads) void k_serve_vnn(const ServeGroupDev* __restrict__ groups,
                                                            const ServeReqDev* __restrict__ reqs,
@@ -126,20 +134,22 @@ __global__ __launch_bounds__(kVnnThreads) void k_serve_vnn(const ServeGroupDev* 
     }
 }
 
-// One wave per request: per-bead sums (chunks in order), the actions and dU
-// (fixed shuffle tree), the Metropolis decision, the commit, the result.
-// Publishes a batch's sequence number once every kernel before it on the
+// Publishes an S(q) batch's sequence number once every kernel before it on the
 // stream has finished (stream order), so the worker reads a host cache line
 // instead of querying an event.
 __global__ void k_serve_publish(unsigned long long* done, unsigned long long seq) {
     if (threadIdx.x == 0) publish_done(done, seq);
 }
 
+// One wave per request: per-bead sums (chunks in order), the actions and dU
+// (fixed shuffle tree), the Metropolis decision, the commit, then the result
+// and the batch's sequence number into the request's record of the slot's
+// mapped block (the worker retires a batch once every record carries it).
 __global__ __launch_bounds__(64) void k_serve_finalize(const ServeReqDev* __restrict__ reqs,
                                                        const VnnBead* __restrict__ beads,
                                                        const ServeChainDev* __restrict__ chains,
                                                        const double* __restrict__ partials,
-                                                       double* __restrict__ out) {
+                                                       ServeOut* __restrict__ out, unsigned long long seq) {
     const ServeReqDev R = reqs[blockIdx.x];
     const ServeChainDev C = chains[R.chain];
     const int lane = threadIdx.x;
@@ -172,11 +182,12 @@ __global__ __launch_bounds__(64) void k_serve_finalize(const ServeReqDev* __rest
         }
     }
     if (lane == 0) {
-        double* o = out + 4 * (int64_t)R.out_slot;
-        o[0] = ao;
-        o[1] = an;
-        o[2] = du;
-        o[3] = accept ? 1.0 : 0.0;
+        ServeOut* o = out + R.out_slot;
+        o->v[0] = ao;
+        o->v[1] = an;
+        o->v[2] = du;
+        o->v[3] = accept ? 1.0 : 0.0;
+        publish_done(&o->seq, seq);
     }
 }
 
@@ -276,8 +287,9 @@ struct Slot {
     char* m_desc = nullptr;           // the device's address of h_desc (read in place)
     char* d_desc = nullptr;
     double* d_partials = nullptr;
-    double* h_out = nullptr;          // mapped pinned results [kMaxReqPerBatch][4]
-    double* d_out = nullptr;
+    ServeOut* h_out = nullptr;        // mapped pinned results [kMaxReqPerBatch]
+    ServeOut* d_out = nullptr;
+    int n_reqs = 0;                   // requests of the Delta V batch in flight
     unsigned long long* h_done = nullptr;   // mapped completion word: the last finished batch
     unsigned long long* d_done = nullptr;
     unsigned long long seq = 0;             // the batch in flight
@@ -337,6 +349,9 @@ struct pimc_server {
     std::string kernel_error_msg;
     bool track_done = false;
     std::vector<std::vector<uint64_t>> done_lists;   // completed request ids per load-generator thread
+    // completions per load-generator thread, read without the lock
+    std::unique_ptr<std::atomic<uint64_t>[]> done_counts{new std::atomic<uint64_t>[64]()};
+    uint64_t packed_events = 0;       // work_events as of the last packing pass
 
     std::vector<ServeReqDev> pack_rq;
     std::vector<VnnBead> pack_bd;
@@ -565,6 +580,7 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
     L->bytes = L->o_gr + b_gr;
     L->n_groups = (int)gr.size();
     L->n_reqs = (int)rq.size();
+    S.n_reqs = L->n_reqs;
     L->slot = si;
     std::memcpy(S.h_desc, rq.data(), b_rq);
     std::memcpy(S.h_desc + L->o_bd, bd.data(), b_bd);
@@ -614,8 +630,7 @@ hipError_t pimc_server::fire_batch(const BatchLaunch& L) {
             hipLaunchKernelGGL(k_serve_vnn<2>, g, b, 0, S.stream, d_gr, d_rq, d_bd, L.chains, L.az, kVnnChunk,
                                S.d_partials);
         hipLaunchKernelGGL(k_serve_finalize, dim3((unsigned)L.n_reqs), dim3(64), 0, S.stream, d_rq, d_bd, L.chains,
-                           S.d_partials, S.d_out);
-        hipLaunchKernelGGL(k_serve_publish, dim3(1), dim3(64), 0, S.stream, S.d_done, S.seq);
+                           S.d_partials, S.d_out, S.seq);
         e = hipGetLastError();
     }
     return e;
@@ -668,11 +683,14 @@ void pimc_server::complete(Slot& S) {
             R.out[0] = nq;
             R.out[1] = R.out[2] = R.out[3] = 0.0;
         } else {
-            for (int t = 0; t < 4; ++t) R.out[t] = S.h_out[4 * k + t];
+            for (int t = 0; t < 4; ++t) R.out[t] = S.h_out[k].v[t];
         }
         R.done = true;
         R.t_done = now;
-        if (track_done && R.tag >= 0 && R.tag < (int)done_lists.size()) done_lists[R.tag].push_back(R.id);
+        if (track_done && R.tag >= 0 && R.tag < (int)done_lists.size()) {
+            done_lists[R.tag].push_back(R.id);
+            done_counts[R.tag].fetch_add(1, std::memory_order_release);
+        }
         Chain& C = chains[R.chain];
         C.busy = false;
         if (!C.queue.empty() && C.queue.front() == R.id) C.queue.pop_front();
@@ -686,30 +704,69 @@ void pimc_server::complete(Slot& S) {
 // lock released; takes the lock only to retire results and to pack requests.
 void pimc_server::run() {
     std::unique_lock<std::mutex> lk(mu, std::defer_lock);
+    // a batch has finished when its records (Delta V) or its completion word
+    // (S(q)) carry its sequence number
+    auto finished = [](const Slot& S) {
+        if (S.kind == 1) return __atomic_load_n(S.h_done, __ATOMIC_ACQUIRE) == S.seq;
+        for (int k = S.n_reqs - 1; k >= 0; --k)
+            if (__atomic_load_n(&S.h_out[k].seq, __ATOMIC_ACQUIRE) != S.seq) return false;
+        return true;
+    };
     while (!stop.load()) {
         int done_k[kSlots];
         hipError_t done_e[kSlots];
         int nd = 0;
-        bool any_busy = false;
+        bool any_busy = false, any_free = false;
         for (int k = 0; k < n_slots; ++k) {
             Slot& S = slots[k];
-            if (!S.busy) continue;
+            if (!S.busy) {
+                any_free = true;
+                continue;
+            }
             any_busy = true;
-            if (__atomic_load_n(S.h_done, __ATOMIC_ACQUIRE) == S.seq) {
+            if (finished(S)) {
                 done_k[nd] = k;
                 done_e[nd++] = hipSuccess;
             } else if (Clock::now() - S.t_fire > std::chrono::milliseconds(200)) {
                 // a batch this late may have failed: ask the stream
                 const hipError_t q = hipStreamQuery(S.stream);
-                if (q != hipErrorNotReady && __atomic_load_n(S.h_done, __ATOMIC_ACQUIRE) != S.seq) {
+                if (q != hipErrorNotReady && !finished(S)) {
                     done_k[nd] = k;
                     done_e[nd++] = q == hipSuccess ? hipErrorUnknown : q;
                 }
             }
         }
+        // nothing finished and nothing submitted since the last packing pass (or
+        // no slot free for it): no reason to take the lock
+        const uint64_t ev = work_events.load(std::memory_order_acquire);
+        if (nd == 0 && (ev == packed_events || !any_free)) {
+            if (any_busy) {
+                __builtin_ia32_pause();
+                continue;
+            }
+            // idle: spin on the submit counter for up to 2 ms (a sleeping
+            // worker costs every request a thread wake-up), then block
+            const auto t0 = Clock::now();
+            bool woke = false;
+            while (!stop.load(std::memory_order_relaxed) && Clock::now() - t0 < std::chrono::milliseconds(2)) {
+                if (work_events.load(std::memory_order_acquire) != packed_events) {
+                    woke = true;
+                    break;
+                }
+                for (int p = 0; p < 64; ++p) __builtin_ia32_pause();
+            }
+            if (!woke) {
+                lk.lock();
+                if (work_events.load(std::memory_order_acquire) == packed_events && !stop.load())
+                    cv_work.wait_for(lk, std::chrono::milliseconds(1));
+                lk.unlock();
+            }
+            continue;
+        }
         BatchLaunch L[kSlots];
         int nl = 0;
         lk.lock();
+        packed_events = work_events.load(std::memory_order_acquire);
         for (int t = 0; t < nd; ++t) {
             if (done_e[t] != hipSuccess) {
                 kernel_error = PIMC_E_HIP;
@@ -725,33 +782,6 @@ void pimc_server::run() {
                 if (r <= 0) break;
                 ++nl;
             }
-        if (nl == 0 && nd == 0) {
-            if (!any_busy) {
-                // idle: spin on the submit counter for up to 2 ms (a sleeping
-                // worker costs every request a thread wake-up), then block
-                const uint64_t seen = work_events.load(std::memory_order_acquire);
-                lk.unlock();
-                const auto t0 = Clock::now();
-                bool woke = false;
-                while (!stop.load(std::memory_order_relaxed) && Clock::now() - t0 < std::chrono::milliseconds(2)) {
-                    if (work_events.load(std::memory_order_acquire) != seen) {
-                        woke = true;
-                        break;
-                    }
-                    for (int p = 0; p < 64; ++p) __builtin_ia32_pause();
-                }
-                if (!woke) {
-                    lk.lock();
-                    if (work_events.load(std::memory_order_acquire) == seen && !stop.load())
-                        cv_work.wait_for(lk, std::chrono::milliseconds(1));
-                    lk.unlock();
-                }
-                continue;
-            }
-            lk.unlock();
-            std::this_thread::yield();
-            continue;
-        }
         lk.unlock();
         for (int t = 0; t < nl; ++t) {
             const hipError_t e = fire_batch(L[t]);
@@ -794,9 +824,10 @@ int pimc_server_create(const pimc_server_desc* d, pimc_server** out) {
         if (hipHostGetDevicePointer((void**)&S.m_desc, S.h_desc, 0) != hipSuccess) return PIMC_E_HIP;
         if (hipMalloc(&S.d_desc, S.desc_bytes) != hipSuccess) return PIMC_E_NOMEM;
         if (hipMalloc(&S.d_partials, sizeof(double) * 2 * kMaxGroupsPerBatch) != hipSuccess) return PIMC_E_NOMEM;
-        if (hipHostMalloc((void**)&S.h_out, sizeof(double) * 4 * kMaxReqPerBatch,
+        if (hipHostMalloc((void**)&S.h_out, sizeof(ServeOut) * kMaxReqPerBatch,
                           hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
             return PIMC_E_NOMEM;
+        std::memset(S.h_out, 0, sizeof(ServeOut) * kMaxReqPerBatch);   // no record carries a batch number yet
         if (hipHostGetDevicePointer((void**)&S.d_out, S.h_out, 0) != hipSuccess) return PIMC_E_HIP;
         if (hipHostMalloc((void**)&S.h_done, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
             return PIMC_E_NOMEM;
@@ -1209,6 +1240,8 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
             if ((st.err = submit(c, Clock::now()))) return;
     std::vector<uint64_t> finished;
     std::vector<double> outs;
+    uint64_t seen_done = 0;
+    unsigned polls = 0;
     while (true) {
         const auto now = Clock::now();
         const bool open = now < t_end;
@@ -1222,7 +1255,11 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
         }
         // completions of this thread's requests, in the order the server finished them
         finished.clear();
-        {
+        // the completion list is taken under the lock only when the worker has
+        // added to it (or every 1024th pass, to see a kernel error)
+        const uint64_t dc = s->done_counts[t].load(std::memory_order_acquire);
+        if (dc != seen_done || (++polls & 1023u) == 0) {
+            seen_done = dc;
             std::lock_guard<std::mutex> g(s->mu);
             finished.swap(s->done_lists[t]);
             if (s->kernel_error) {
@@ -1303,6 +1340,7 @@ extern "C" int pimc_server_run_synthetic(pimc_server* s, const pimc_load_desc* L
         std::lock_guard<std::mutex> g(s->mu);
         if (nc > (int)s->chains.size()) return sfail(s, PIMC_E_ARG, "more load chains than server chains");
         s->done_lists.assign(T, {});
+        for (int t = 0; t < T; ++t) s->done_counts[t].store(0, std::memory_order_relaxed);
         s->track_done = true;
     }
     struct TrackOff {

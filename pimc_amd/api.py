@@ -205,6 +205,7 @@ class Context:
         self.M, self.N, self.side = num_slices, num_alloc, side
         self._out = np.zeros(4)         # reused result buffer of the per-move calls
         self._out_ptr = self._out.ctypes.data
+        self._fn_dv = lib().pimc_delta_action
 
     def close(self):
         if getattr(self, "h", None):
@@ -364,6 +365,24 @@ class Context:
                                             sp.shape[0], shift, None if vf is None else _ptr(vf),
                                             int(bool(commit)), self._out_ptr), "delta_action")
         return self._out[:3].copy()
+
+    def prepare_delta(self, slice_ptcl, new_pos):
+        """A Delta V request checked and converted once, for repeated
+        delta_action_prepared calls (the arrays are kept alive with it)."""
+        sp = _i32(slice_ptcl, (-1, 2))
+        npos = _f64(new_pos, (-1, 3))
+        if npos.shape[0] != sp.shape[0]:
+            raise PimcError("delta_action: bead list and positions differ in length")
+        return (sp, npos, _ptr(sp), _ptr(npos), sp.shape[0])
+
+    def delta_action_prepared(self, req, commit=False):
+        """delta_action for a prepare_delta request at shift 1 with the context's
+        factors: {A_old, A_new, dU} in the context's reused result buffer (valid
+        until the next call)."""
+        rc = self._fn_dv(self.h, req[2], None, req[3], req[4], 1, None, int(bool(commit)), self._out_ptr)
+        if rc != 0:
+            self._check(rc, "delta_action")
+        return self._out
 
     def ssf(self, q):
         q = _f64(q, (-1, 3))

@@ -507,6 +507,8 @@ static void fill_walker(pimc_ctx* c, WalkerDesc* w, int shift, const double vf[2
 // gradVSquared(slice) for every slice into c->d_g2part (partials per slice).
 static int run_gradv2(pimc_ctx* c, const AzizDev& az) {
     const int M = c->desc.num_slices, N = c->desc.num_alloc;
+    if (c->desc.v_mode == PIMC_V_TABLE && !az.dtable)
+        return fail(c, PIMC_E_STATE, "gradVSquared in the table mode needs the dV/dr table");
     const int B = (N + 255) / 256;
     if (!c->d_g2part || c->g2_blocks < B) {
         int r;
@@ -761,6 +763,12 @@ int pimc_gradv_squared(pimc_ctx* c, double* g2) {
     int r;
     AzizDev az = ctx_aziz(c, &r);
     if (r) return r;
+    if (c->desc.v_mode == PIMC_V_TABLE && !az.dtable) {
+        // gradV gathers lookupdVdr whether or not the context's action carries the correction
+        std::string err;
+        az.dtable = table_cache().get(c->desc.aziz_year, 1, az.table_length, &err);
+        if (!az.dtable) return fail(c, PIMC_E_NOMEM, err);
+    }
     if ((r = run_gradv2(c, az))) return r;
     // the per-block partials of every slice, summed in block order as the
     // pair kernel's finalisation sums them into the corrected action

@@ -178,3 +178,41 @@ def test_gsf_device_resident_run(orc, gpu, mode, N, M, rc_frac):
         assert bool(acc_r) == acc_g
     print(f"GSF device run mode {mode} N={N}: {int(acc.sum())}/300 accepted, {near} near the threshold")
     assert np.allclose(ctx.download_path(), ref.beads(), rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_gradv_squared_per_slice(orc, gpu, mode):
+    """pimc_gradv_squared: LocalAction::gradVSquared(slice) of every slice
+    (src/action.cpp:1188-1223) against the oracle, with and without the
+    correction set on the context (the values do not depend on the factors)."""
+    ref, ctx, _, _ = pair(orc, gpu, 48, 12, 0.004, 1.0, mode, GSF, kind=0)
+    g = ctx.gradVSquared()
+    r = np.array([ref.gradV2_slice(m) for m in range(12)])
+    assert np.all(np.abs(g - r) <= TOL[mode] * np.abs(r)), np.max(np.abs(g - r) / np.abs(r))
+    ctx.set_action_factors((1.0, 1.0), (0.0, 0.0), 1)
+    assert np.array_equal(ctx.gradVSquared(), g)
+
+
+def test_path_version(orc, gpu):
+    """pimc_path_version moves with every call that changes what per-slice values
+    depend on, and only with those."""
+    ref, ctx, beads, L = pair(orc, gpu, 16, 8, 0.004, 1.0, 2, GSF)
+    v = ctx.path_version()
+    ctx.potentialAction()
+    ctx.energy()
+    ctx.gradVSquared()
+    ctx.delta_action([[2, 3]], beads[2, 3][None] + 0.1)          # evaluated, not committed
+    ctx.ssf([[0.0, 0.0, 2 * np.pi / L]])
+    ctx.download_path()
+    assert ctx.path_version() == v
+    steps = [lambda: ctx.update_beads([[1, 1]], beads[1, 1][None] + 0.05),
+             lambda: ctx.delta_action([[2, 3]], beads[2, 3][None] + 0.1, commit=True),
+             lambda: ctx.upload_path(beads),
+             lambda: ctx.set_worm(head=(1, 2)),
+             lambda: ctx.set_v_mode(0),
+             lambda: ctx.set_action_factors((1.0, 1.0), (0.0, 0.0), 1)]
+    for k, step in enumerate(steps):
+        step()
+        w = ctx.path_version()
+        assert w > v, k
+        v = w

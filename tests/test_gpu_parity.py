@@ -388,3 +388,19 @@ def test_isf_n256_m167_properties(orc, gpu):
     pick = [0, 1, 2688, 5376]
     sr = ref.ssf(q[pick])
     assert np.all(np.abs(s[pick] - sr) <= 1e-5 * np.maximum(np.abs(sr), 256 * 167 * 1e-3))
+
+
+def test_delta_action_prepared(orc, gpu):
+    """A request converted once (prepare_delta) gives delta_action's result bit
+    for bit, also when committing."""
+    ref, ctx, beads, L = setup_pair(orc, gpu, 64, 24, 0.004)
+    rng = np.random.default_rng(31)
+    sp = np.array([[m, 5] for m in range(3, 11)], dtype=np.int32)
+    new = beads[sp[:, 0], sp[:, 1]] + rng.normal(scale=0.2, size=(len(sp), 3))
+    a = ctx.delta_action(sp, new)
+    prep = ctx.prepare_delta(sp, new)
+    b = ctx.delta_action_prepared(prep)[:3].copy()
+    assert np.array_equal(a, b)
+    c = ctx.delta_action_prepared(prep, commit=True)[:3].copy()
+    assert np.array_equal(c, a)
+    assert np.array_equal(ctx.download_path()[sp[:, 0], sp[:, 1]], new)

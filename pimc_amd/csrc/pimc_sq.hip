@@ -13,9 +13,12 @@
 // Lattice q = (2 pi / L) n: exp(i q.r) = prod_d exp(2 pi i n_d s_d), s_d = x_d / L.
 // Each workgroup stages per-bead axis phasors exp(2 pi i n s_d) for -nmax <= n <=
 // nmax in LDS (phase reduced in FP64, FP32 sincospi; the negative n are the
-// conjugates), then each lane accumulates a run of q-vectors as two complex
-// FP32 products per (q, bead), 8 FMA-type instructions, with FP64 per-chunk sums.
+// conjugates), then each lane accumulates a run of q-vectors with FP32 complex
+// products and FP64 per-chunk sums; the X Y product of a row of q sharing
+// (n_x, n_y) is formed once per bead (k_rho_lattice).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "pimc_kernels.hpp"
 
@@ -30,8 +33,20 @@ constexpr int kReseed = 4;         // exact sincospi every kReseed powers, produ
 // (k < QPT), so a wave walks consecutive q: mostly one (n_x, n_y) and consecutive
 // n_z, which makes the x/y phasor reads LDS broadcasts and the z reads
 // consecutive words.  Tables are bead-major: ph[axis][bead][n + nmax].
+//
+// Rows: consecutive q of the list that share (n_x, n_y) -- in the reference's
+// order a q set is runs of n_z -- share the product X(n_x) Y(n_y) of each bead.
+// A workgroup first numbers the rows its q touch (a block scan over the q
+// range); when they fit the launch's row capacity (rows_cap, at most
+// kRhoRowsMax, sized with the q per workgroup), each bead chunk forms every row's product
+// once into LDS and a lane's term per (q, bead) is one complex product with
+// Z(n_z), 4 FMA instead of 8.  The products are formed by the same FP32
+// expressions as in the per-lane form, and the FP32 partial of every 32 beads
+// is added into the FP64 sum either way, so both forms give the same bits.
+constexpr int kRhoRowsMax = 128;   // rows of one workgroup's q range that share products
+
 template <int QPT>
-__global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
+__global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A, int rows_cap) {
     extern __shared__ float2 ph[];
     const int m = blockIdx.y;
     const int n_beads = A.nbs[m];
@@ -40,12 +55,17 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
     float2* phx = ph;
     float2* phy = phx + w0 * kRhoBeadChunk;
     float2* phz = phy + w1 * kRhoBeadChunk;
+    float2* xy = phz + w2 * kRhoBeadChunk;                                // [rows_cap][kRhoBeadChunk]
+    int2* rows = reinterpret_cast<int2*>(xy + rows_cap * kRhoBeadChunk);  // [rows_cap] (n_x, n_y)
+    int* scan = reinterpret_cast<int*>(xy);                                // [kRhoThreads * QPT], before xy is used
+    __shared__ int wave_tot[kRhoThreads / 64];
     const double* px = A.pos + (int64_t)m * A.nalloc;
     const double* py = px + A.plane;
     const double* pz = py + A.plane;
 
-    const int qbase = blockIdx.x * kRhoThreads * QPT + threadIdx.x;
-    int ix[QPT], iy[QPT], iz[QPT];
+    const int q0 = blockIdx.x * kRhoThreads * QPT;
+    const int qbase = q0 + threadIdx.x;
+    int ix[QPT], iy[QPT], iz[QPT], ro[QPT];
 #pragma unroll
     for (int k = 0; k < QPT; ++k) {
         const int qi = qbase + k * kRhoThreads;
@@ -58,6 +78,54 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
         ix[k] = nx + h0;
         iy[k] = ny + h1;
         iz[k] = nz + h2;
+        ro[k] = 0;
+    }
+    bool use_rows = false;
+    int n_rows = 0;
+    if (rows_cap > 0) {
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) {
+            const int qi = qbase + k * kRhoThreads;
+            const int start = qi < A.nq &&
+                              (qi == q0 || ix[k] - h0 != A.qn[3 * qi - 3] || iy[k] - h1 != A.qn[3 * qi - 2]);
+            scan[threadIdx.x + k * kRhoThreads] = start;
+        }
+        __syncthreads();
+        // inclusive prefix of the row starts over the workgroup's q range: lane t
+        // owns entries [t QPT, (t + 1) QPT), then a wave scan and the waves in order
+        int own[QPT];
+        int run = 0;
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) {
+            run += scan[threadIdx.x * QPT + k];
+            own[k] = run;
+        }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        int incl = run;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        if (lane == 63) wave_tot[wave] = incl;
+        __syncthreads();
+        int before = incl - run;
+        for (int w = 0; w < wave; ++w) before += wave_tot[w];
+        n_rows = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+        __syncthreads();   // every lane has read its entries
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) scan[threadIdx.x * QPT + k] = before + own[k];
+        __syncthreads();
+        use_rows = n_rows <= rows_cap;
+#pragma unroll
+        for (int k = 0; k < QPT; ++k) {
+            const int l = threadIdx.x + k * kRhoThreads;
+            const int r = scan[l] - 1;   // the row of q_l (starts counted up to and with it)
+            const bool first = l == 0 || scan[l - 1] != scan[l];
+            ro[k] = (use_rows && r >= 0 && r < rows_cap) ? r : 0;
+            if (use_rows && first && qbase + k * kRhoThreads < A.nq && r >= 0 && r < rows_cap)
+                rows[r] = make_int2(ix[k] - h0, iy[k] - h1);
+        }
     }
     double are[QPT], aim[QPT];
 #pragma unroll
@@ -103,18 +171,43 @@ __global__ __launch_bounds__(kRhoThreads) void k_rho_lattice(RhoArgs A) {
         float sre[QPT], sim[QPT];
 #pragma unroll
         for (int k = 0; k < QPT; ++k) sre[k] = sim[k] = 0.f;
-        for (int bb = 0; bb < nb; ++bb) {
-#pragma unroll
-            for (int k = 0; k < QPT; ++k) {
-                const float2 ex = phx[bb * w0 + ix[k]];
-                const float2 ey = phy[bb * w1 + iy[k]];
-                const float2 ez = phz[bb * w2 + iz[k]];
+        if (use_rows) {
+            // every row's X Y product of the chunk's beads, once
+            for (int t = threadIdx.x; t < n_rows * kRhoBeadChunk; t += kRhoThreads) {
+                const int r = t / kRhoBeadChunk, bb = t % kRhoBeadChunk;
+                const int2 rn = rows[r];
+                const float2 ex = phx[bb * w0 + rn.x + h0];
+                const float2 ey = phy[bb * w1 + rn.y + h1];
                 const float xr = fmaf(ex.x, ey.x, -ex.y * ey.y);
                 const float xi = fmaf(ex.x, ey.y, ex.y * ey.x);
-                sre[k] = fmaf(xr, ez.x, sre[k]);
-                sre[k] = fmaf(-xi, ez.y, sre[k]);
-                sim[k] = fmaf(xr, ez.y, sim[k]);
-                sim[k] = fmaf(xi, ez.x, sim[k]);
+                xy[t] = make_float2(xr, xi);
+            }
+            __syncthreads();
+            for (int bb = 0; bb < nb; ++bb) {
+#pragma unroll
+                for (int k = 0; k < QPT; ++k) {
+                    const float2 p = xy[ro[k] * kRhoBeadChunk + bb];
+                    const float2 ez = phz[bb * w2 + iz[k]];
+                    sre[k] = fmaf(p.x, ez.x, sre[k]);
+                    sre[k] = fmaf(-p.y, ez.y, sre[k]);
+                    sim[k] = fmaf(p.x, ez.y, sim[k]);
+                    sim[k] = fmaf(p.y, ez.x, sim[k]);
+                }
+            }
+        } else {
+            for (int bb = 0; bb < nb; ++bb) {
+#pragma unroll
+                for (int k = 0; k < QPT; ++k) {
+                    const float2 ex = phx[bb * w0 + ix[k]];
+                    const float2 ey = phy[bb * w1 + iy[k]];
+                    const float2 ez = phz[bb * w2 + iz[k]];
+                    const float xr = fmaf(ex.x, ey.x, -ex.y * ey.y);
+                    const float xi = fmaf(ex.x, ey.y, ex.y * ey.x);
+                    sre[k] = fmaf(xr, ez.x, sre[k]);
+                    sre[k] = fmaf(-xi, ez.y, sre[k]);
+                    sim[k] = fmaf(xr, ez.y, sim[k]);
+                    sim[k] = fmaf(xi, ez.x, sim[k]);
+                }
             }
         }
 #pragma unroll
@@ -268,16 +361,35 @@ hipError_t launch_rho(bool lattice, hipStream_t st, const RhoArgs& A) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (A.nq <= 0) return hipSuccess;
     if (lattice) {
-        const size_t lds = sizeof(float2) * (2 * (A.nmax[0] + A.nmax[1] + A.nmax[2]) + 3) * kRhoBeadChunk;
+        const size_t tables = sizeof(float2) * (2 * (A.nmax[0] + A.nmax[1] + A.nmax[2]) + 3) * kRhoBeadChunk;
         // the most q per lane that still leaves >= 2048 workgroups
         auto blocks = [&](int qpt) { return (int64_t)((A.nq + kRhoThreads * qpt - 1) / (kRhoThreads * qpt)) * A.num_slices; };
         int qpt = 8;
         while (qpt > 1 && blocks(qpt) < 2048) qpt >>= 1;
         dim3 g((A.nq + kRhoThreads * qpt - 1) / (kRhoThreads * qpt), A.num_slices), b(kRhoThreads);
-        if (qpt == 8) hipLaunchKernelGGL(k_rho_lattice<8>, g, b, lds, st, A);
-        else if (qpt == 4) hipLaunchKernelGGL(k_rho_lattice<4>, g, b, lds, st, A);
-        else if (qpt == 2) hipLaunchKernelGGL(k_rho_lattice<2>, g, b, lds, st, A);
-        else hipLaunchKernelGGL(k_rho_lattice<1>, g, b, lds, st, A);
+        // row products when a workgroup takes 1024 or more q (up to 128 rows; a
+        // q set in the reference's order has runs of n_z longer than 8, so this
+        // is rarely exceeded, and a workgroup with more rows takes the per-lane
+        // products); with fewer q per workgroup their set-up outweighs them.  The
+        // products' space first holds the row scan (256 QPT ints).
+        const int rows_cap = qpt >= 4 ? kRhoRowsMax : 0;
+        const size_t lds = tables + sizeof(float2) * rows_cap * kRhoBeadChunk + sizeof(int2) * rows_cap;
+        static bool attr_set = false;
+        if (!attr_set) {
+            // up to 63 + 32 + 1 KB at |n_d| <= 40, above the default 64 KB of dynamic LDS
+            // (the kernel's static LDS comes on top, within the 160 KB of a CU)
+            const void* fns[4] = {(const void*)k_rho_lattice<1>, (const void*)k_rho_lattice<2>,
+                                  (const void*)k_rho_lattice<4>, (const void*)k_rho_lattice<8>};
+            for (const void* f : fns) {
+                const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
+                if (e != hipSuccess) return e;
+            }
+            attr_set = true;
+        }
+        if (qpt == 8) hipLaunchKernelGGL(k_rho_lattice<8>, g, b, lds, st, A, rows_cap);
+        else if (qpt == 4) hipLaunchKernelGGL(k_rho_lattice<4>, g, b, lds, st, A, rows_cap);
+        else if (qpt == 2) hipLaunchKernelGGL(k_rho_lattice<2>, g, b, lds, st, A, rows_cap);
+        else hipLaunchKernelGGL(k_rho_lattice<1>, g, b, lds, st, A, rows_cap);
     } else {
         dim3 g((A.nq + kRhoThreads - 1) / kRhoThreads, A.num_slices), b(kRhoThreads);
         hipLaunchKernelGGL(k_rho_general, g, b, 0, st, A);

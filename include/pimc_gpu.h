@@ -101,14 +101,15 @@ int pimc_total_action(pimc_ctx* ctx, int shift, const double vfactor[2], double*
 /* Launch shape of the all-pairs kernel (no reference counterpart): about
  * pairs_per_lane pair evaluations per lane and tile (0: about two tiles per
  * workgroup), grid_per_cu persistent workgroups per CU (0: 4), stage_in_lds 1 /
- * 0 / -1 (stage each slice in LDS / read partners through the caches / automatic).
- * The tiles set the summation order of the per-tile partials, so a context's
- * results are reproducible for a fixed shape and differ across shapes by
- * rounding only. */
+ * 0 / 2 / -1 (stage each slice in LDS in bead order / read partners through the
+ * caches / stage slices of 512 beads and more in spatial order, mixed mode /
+ * automatic: staged, in spatial order above 1024 beads).  The tiles and the
+ * staging order set the summation order, so a context's results are
+ * reproducible for a fixed shape and differ across shapes by rounding only. */
 int pimc_set_launch_shape(pimc_ctx* ctx, int pairs_per_lane, int grid_per_cu, int stage_in_lds);
 /* Times pimc_total_action's pair kernel (reps launches each, kernel events) over
- * 48 shapes and keeps the fastest: out4 = {pairs_per_lane, grid_per_cu,
- * stage_in_lds, kernel ms} (may be NULL). */
+ * 48 shapes (72 where the spatial staging applies) and keeps the fastest: out4 =
+ * {pairs_per_lane, grid_per_cu, stage_in_lds, kernel ms} (may be NULL). */
 int pimc_autotune(pimc_ctx* ctx, int reps, double* out4);
 /* LocalAction::V(int) for every slice: v_slices [M][2]; sep_hist [50] (may be NULL)
  * is the separation histogram the reference leaves behind after evaluating the

@@ -145,6 +145,24 @@ AzizDev make_aziz_dev(const AzizParams& p, double max_sep) {
         if (f0 && f1 && f2) break;
         r += a.dr;
     }
+    // k_far: the first bin from which on eps A exp(-alpha x + beta x^2) is below
+    // 2^-53 of the dispersion.  The ratio falls with x beyond x = 10/alpha < D,
+    // so a bisection over [k_d, table_length] finds it (table_length: none).
+    auto far = [&](int k) {
+        const long double x = (long double)k * a.s, x2 = 1.0L / (x * x), x6 = x2 * x2 * x2;
+        const long double rep = (long double)p.A * std::exp(-(long double)p.alpha * x + (long double)p.beta * x * x);
+        const long double disp = x6 * ((long double)p.C6 + x2 * ((long double)p.C8 + x2 * (long double)p.C10));
+        return rep < std::ldexp(disp, -53);
+    };
+    int lo = a.k_d, hi = a.table_length;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (far(mid))
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    a.k_far = lo;
     const double log2e = 1.4426950408889634074;
     a.rep_c0 = std::log2(p.eps * p.A);
     a.rep_c1 = -p.alpha * a.s * log2e;
@@ -425,6 +443,13 @@ static int rebuild_work(pimc_ctx* c) {
     }
     c->lds_beads = maxn <= kLdsBeads ? std::max(maxn, 1) : 0;
     c->pair_lds = c->tune_lds == 0 ? 0 : c->lds_beads;
+    // spatial order and the window form (mixed mode): forced (2), or automatic
+    // for slices of more than 1024 beads, where most pairs lie beyond the
+    // repulsion's reach (N = 2048 x 250: pair kernel 801 -> 676 us; N = 1024: 232
+    // -> 214 us, about what the ordering pass costs after a path change)
+    c->pair_spatial = c->desc.v_mode == PIMC_V_MIXED && c->pair_lds >= 512 &&
+                      (c->tune_lds == 2 || (c->tune_lds == -1 && c->pair_lds > 1024));
+    c->perm_version = ~0ull;
     const int G0 = c->tune_gmul > 0 ? c->tune_gmul * std::max(1, c->num_cus) : pair_grid_for(c->lds_beads, c->num_cus);
     const int ppl = c->tune_ppl > 0 ? c->tune_ppl : pairs_per_lane_for(total, G0);
     std::vector<RowTile> tiles;
@@ -436,7 +461,7 @@ static int rebuild_work(pimc_ctx* c) {
         sl[m].first_item = (int)tiles.size();
         sl[m].pos = (int64_t)m * c->desc.num_alloc;
         sl[m].plane = c->plane;
-        append_row_tiles(m, sl[m], ppl, tiles);
+        append_row_tiles(m, sl[m], ppl, tiles, c->pair_spatial);
     }
     c->n_tiles = (int)tiles.size();
     c->n_groups = std::max(1, std::min(G0, c->n_tiles));
@@ -452,6 +477,7 @@ static int rebuild_work(pimc_ctx* c) {
         if ((r = ctx_dalloc(c, &c->d_partials, c->n_tiles))) return r;
         c->cap_tiles = c->n_tiles;
     }
+    if (c->pair_spatial && !c->d_perm && (r = ctx_dalloc(c, &c->d_perm, (size_t)M * c->desc.num_alloc))) return r;
     if (c->n_groups + 1 > c->cap_groups) {
         free_dev(c, c->d_ranges);
         c->d_ranges = nullptr;
@@ -629,7 +655,7 @@ int pimc_set_v_mode(pimc_ctx* c, int v_mode) {
     if (v_mode < PIMC_V_FP32 || v_mode > PIMC_V_TABLE) return fail(c, PIMC_E_ARG, "invalid v_mode");
     c->desc.v_mode = v_mode;
     ++c->path_version;
-    return PIMC_OK;
+    return c->have_path ? rebuild_work(c) : PIMC_OK;   // the tiles' shape follows the mode
 }
 
 int pimc_upload_path(pimc_ctx* c, const double* beads, const uint8_t* bead_on, const int32_t* num_beads) {
@@ -822,12 +848,19 @@ int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* a
     if (c->n_tiles > 0) {
         const bool cor = c->correction();
         if (cor && (r = run_gradv2(c, az))) return r;
+        const bool spatial = c->pair_spatial && c->desc.v_mode == PIMC_V_MIXED;
+        if (spatial && c->perm_version != c->path_version) {
+            HIP_OR_FAIL(c, launch_slice_order(c->stream, c->d_slices, M, c->d_pos, c->box.invL, c->d_perm,
+                                              c->desc.num_alloc));
+            c->perm_version = c->path_version;
+        }
         const KernelTimer kt = c->kernel_timer(0);
         const unsigned long long seq = ++c->seq;
         HIP_OR_FAIL(c, launch_pair_action(c->desc.v_mode, c->stream, c->d_tiles, c->n_tiles, c->d_ranges, c->n_groups,
                                           c->d_slices, M, c->d_walker, 1, c->d_pos, az, c->pair_lds, c->d_partials,
                                           c->d_vslice, c->d_map, &kt, true, c->d_done(), seq,
-                                          cor ? c->d_g2part : nullptr, c->g2_blocks, c->d_g2slice));
+                                          cor ? c->d_g2part : nullptr, c->g2_blocks, c->d_g2slice,
+                                          spatial ? c->d_perm : nullptr, c->desc.num_alloc));
         if (v_slices) {
             HIP_OR_FAIL(c, hipMemcpyAsync(c->h_stage, c->d_vslice, sizeof(double) * M, hipMemcpyDeviceToHost,
                                           c->stream));
@@ -1062,7 +1095,7 @@ int pimc_virial_energy(pimc_ctx* c, const int32_t* next_links, int window, doubl
 int pimc_set_launch_shape(pimc_ctx* c, int pairs_per_lane, int grid_per_cu, int stage_in_lds) {
     if (!c) return fail(c, PIMC_E_ARG, "null context");
     if (pairs_per_lane < 0 || pairs_per_lane > 4096 || grid_per_cu < 0 || grid_per_cu > 64 || stage_in_lds < -1 ||
-        stage_in_lds > 1)
+        stage_in_lds > 2)
         return fail(c, PIMC_E_ARG, "launch shape out of range");
     c->tune_ppl = pairs_per_lane;
     c->tune_gmul = grid_per_cu;
@@ -1075,7 +1108,9 @@ int pimc_autotune(pimc_ctx* c, int reps, double* out4) {
     if (!c->have_path) return fail(c, PIMC_E_STATE, "no path uploaded");
     const int ppls[] = {0, 8, 16, 32, 64, 128};   // 0: the automatic choice
     const int gmul[] = {2, 4, 8, 16};
-    const int ldss[] = {1, 0};
+    // staging: index order, none, and spatial order where it applies
+    std::vector<int> ldss = {1, 0};
+    if (c->lds_beads >= 512 && c->desc.v_mode == PIMC_V_MIXED) ldss.push_back(2);
     c->resolve_timing();
     const bool timing = c->timing;
     const unsigned kinds = c->timing_kinds;

@@ -28,6 +28,7 @@ struct AzizDev {
     int k_eps;          // first bin with x_k >= EPS (below: V = 0)
     int k_hc;           // first bin with x_k >= 0.01 (below: hard core eps*A*exp(..))
     int k_d;            // first bin with x_k >= D (at and above: F = 1)
+    int k_far;          // first bin from which on the repulsion is below 2^-53 of the dispersion
     double dr;          // 1e-6 rm
     double s;           // dr / rm, so x_k = k s
     // FP32 closed form, epsilon folded in, base-2 exponent of the repulsion:
@@ -121,6 +122,21 @@ __device__ __forceinline__ double aziz_vmix(int k, const AzizDev& a) {
         v = fma(-disp, F, rep);
     }
     return v;
+}
+
+// valueV of a far bin, k_far <= k < table_length: the repulsion there is below
+// 2^-53 of the dispersion, under the rounding of the FP64 difference, so V is
+// minus the dispersion.  1/k^2 comes from y, the FP64 reciprocal-root estimate
+// of |w| for a separation of |w| bins (k <= |w| < k + 1, y within ~2^-23): y^2
+// is within 2/k + 2^-22 of 1/k^2, under 1e-6 for k >= k_far (x >= 3.1 for the
+// 1979, 1987 and 1995 parameters), and one Newton step takes that to ~1e-12 with
+// no FP64 reciprocal.
+__device__ __forceinline__ double aziz_vfar(int k, double y, const AzizDev& a) {
+    const double kd = (double)k;
+    const double q = kd * kd;
+    const double r0 = y * y;
+    const double iq = fma(r0, fma(-q, r0, 1.0), r0);
+    return -((iq * iq * iq) * fma(fma(a.eC10k, iq, a.eC8k), iq, a.eC6k));
 }
 
 // valueV at the bin radius, FP64 arithmetic.  Requires k_eps <= k < table_length.

@@ -80,6 +80,9 @@ struct pimc_ctx {
     int pair_lds = 0;             // the pair kernel's staging (lds_beads, or 0 when tuned off)
     // launch shape of the pair kernel (pimc_set_launch_shape / pimc_autotune; 0 / -1: automatic)
     int tune_ppl = 0, tune_gmul = 0, tune_lds = -1;
+    bool pair_spatial = false;   // pair kernel stages slices in spatial order (mixed mode)
+    int* d_perm = nullptr;        // [M][N] spatial order of each slice (pair_spatial)
+    uint64_t perm_version = ~0ull;   // path_version d_perm was computed for (~0: none)
     // action factors (LocalAction VFactor, gradVFactor, estimator period); the
     // per-call vfactor argument overrides VFactor when given
     double vf_default[2] = {1.0, 1.0};

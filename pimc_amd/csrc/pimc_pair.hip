@@ -48,7 +48,7 @@ struct RowMixed {
     double invL, S, Sh;  // 1/L, L/dr, -S/2
 };
 
-__device__ __forceinline__ int bin_mixed(const RowMixed& q, double xj, double yj, double zj) {
+__device__ __forceinline__ int bin_mixed(const RowMixed& q, double xj, double yj, double zj, double* y_out = nullptr) {
     const double wx = fma(__builtin_amdgcn_fract(fma(xj, q.invL, q.cx)), q.S, q.Sh);
     const double wy = fma(__builtin_amdgcn_fract(fma(yj, q.invL, q.cy)), q.S, q.Sh);
     const double wz = fma(__builtin_amdgcn_fract(fma(zj, q.invL, q.cz)), q.S, q.Sh);
@@ -56,6 +56,7 @@ __device__ __forceinline__ int bin_mixed(const RowMixed& q, double xj, double yj
     // |w|: the hardware FP64 reciprocal-root estimate (relative error ~2^-23)
     // and one Newton step, |w| = r0 + (r2 - r0^2) y / 2 (~2^-46)
     const double y = __builtin_amdgcn_rsq(r2);
+    if (y_out) *y_out = y;
     const double r0 = r2 * y;
     const double res = fma(-r0, r0, r2);
     return (int)fma(res * y, 0.5, r0);
@@ -85,7 +86,32 @@ constexpr int pair_lds_plane() {
     return pair_lds_dup<PS>() ? PS + PS / 2 + 64 : PS;   // room for n + n/2 + 1 beads, 64-bead aligned
 }
 
-template <int MODE, int PS, bool FACTORS, bool BETA0, bool NOWRAP>
+// One mixed-mode pair term against the row bead of q.  FAR: a wave whose pairs
+// all lie in far bins (k_far <= k < table_length) takes aziz_vfar, no exp2,
+// damping test or clamp; any other wave the full term.  The far value differs
+// from the full one by at most an FP64 rounding and a ~1e-12 relative 1/k^2, so
+// which path a pair takes depends on its wave's other pairs but its value does
+// not beyond that.  av: a with the loop-invariant operands in vector registers;
+// far_span = max(table_length - k_far, 0).
+template <bool BETA0, bool FAR>
+__device__ __forceinline__ double pair_term_mixed(const RowMixed& q, double x, double y, double z, const AzizDev& a,
+                                                  const AzizDev& av, int lenm1, unsigned far_span) {
+    double yr;
+    const int k = bin_mixed(q, x, y, z, &yr);
+    if (FAR && __builtin_amdgcn_ballot_w64((unsigned)(k - a.k_far) >= far_span) == 0) return aziz_vfar(k, yr, av);
+    // k_eps <= k < tableLength, else V = 0 (include/potential.h:252-257): the
+    // clamped bin keeps the arithmetic finite, and only a wave holding such a
+    // pair pays for the select
+    const int kk = min(max(k, a.k_eps), lenm1);
+    double v = aziz_vmix<BETA0>(kk, av);
+    if (__builtin_amdgcn_ballot_w64(k != kk)) {
+        asm volatile("");
+        v = (k != kk) ? 0.0 : v;
+    }
+    return v;
+}
+
+template <int MODE, int PS, bool FACTORS, bool BETA0, bool NOWRAP, bool FAR>
 __device__ __forceinline__ double row_loop(const double* px, const double* py, const double* pz, int n, int i,
                                            int d, int d_end, int T, int G, bool mask, const BoxDev& box,
                                            const AzizDev& a, double inv_dr, const WormDev& worm, int slice) {
@@ -111,6 +137,8 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
         q.Sh = in_vgpr(-0.5 * q.S);
     }
     const int lenm1 = in_vgpr(a.table_length - 1);
+    // far bins: k - k_far < table_length - k_far, unsigned (empty when k_far >= table_length)
+    const unsigned far_span = in_vgpr((unsigned)max(a.table_length - a.k_far, 0));
     AzizDev av = a;
     if (MODE == 0) {
         av.rep_c1 = in_vgpr(a.rep_c1);
@@ -141,16 +169,7 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     auto eval = [&](const Pos& p) -> double {
         double v;
         if (MODE == 0) {
-            const int k = bin_mixed(q, p.x, p.y, p.z);
-            // k_eps <= k < tableLength, else V = 0 (include/potential.h:252-257): the
-            // clamped bin keeps the arithmetic finite, and only a wave holding such a
-            // pair pays for the select
-            const int kk = min(max(k, a.k_eps), lenm1);
-            v = aziz_vmix<BETA0>(kk, av);
-            if (__builtin_amdgcn_ballot_w64(k != kk)) {
-                asm volatile("");
-                v = (k != kk) ? 0.0 : v;
-            }
+            v = pair_term_mixed<BETA0, FAR>(q, p.x, p.y, p.z, a, av, lenm1, far_span);
         } else {
             const double r2 = sep_r2<MODE>(xi, yi, zi, p.x, p.y, p.z, box);
             v = pair_value_from_r2<MODE>(r2, a, inv_dr);
@@ -196,18 +215,79 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
 // FACTORS (a worm on the slice), BETA0 (beta = 0, mixed mode) and NOWRAP (no
 // row + offset of the tile reaches n, so the partner cursor needs no wrap even
 // without the duplicated staging).
-template <int MODE, int PS>
+template <int MODE, int PS, bool FAR = false>
 __device__ __forceinline__ double row_tile(const double* px, const double* py, const double* pz, int n, int i, int d,
                                            int d_end, int T, int G, bool mask, const BoxDev& box, const AzizDev& a,
                                            double inv_dr, bool factors, bool nowrap, const WormDev& worm,
                                            int slice) {
 #define PIMC_ROW(F_, B_, W_) \
-    row_loop<MODE, PS, F_, B_, W_>(px, py, pz, n, i, d, d_end, T, G, mask, box, a, inv_dr, worm, slice)
+    row_loop<MODE, PS, F_, B_, W_, FAR>(px, py, pz, n, i, d, d_end, T, G, mask, box, a, inv_dr, worm, slice)
     if (factors) return PIMC_ROW(true, false, false);
     const bool beta0 = MODE == 0 && a.rep_c2 == 0.0;
     if (!pair_lds_dup<PS>() && nowrap) return beta0 ? PIMC_ROW(false, true, true) : PIMC_ROW(false, false, true);
     return beta0 ? PIMC_ROW(false, true, false) : PIMC_ROW(false, false, false);
 #undef PIMC_ROW
+}
+
+// Spatial order of a slice: the Morton index of each bead's cell on a 32^3 grid
+// over the box (5 bits per axis, x the most significant of each triple).
+__device__ __forceinline__ unsigned morton_cell(double x, double y, double z, double invL) {
+    auto cell = [&](double v) { return (unsigned)min(max((int)floor(fma(v, invL, 0.5) * 32.0), 0), 31); };
+    const unsigned cx = cell(x), cy = cell(y), cz = cell(z);
+    unsigned m = 0;
+    for (int b = 4; b >= 0; --b) m = (m << 3) | (((cx >> b) & 1u) << 2) | (((cy >> b) & 1u) << 1) | ((cz >> b) & 1u);
+    return m;
+}
+
+// Spatial order of each slice (one workgroup per slice): keys (cell << 11 |
+// bead), unique, sorted ascending by a bitonic network over the next power of
+// two (padding keys 0xffffffff); perm[t] is the bead of the t-th key.  The order
+// depends only on the positions.  n <= kLdsBeads (bead indices take 11 bits).
+constexpr int kOrderThreads = 1024;
+__global__ __launch_bounds__(kOrderThreads) void k_slice_order(const SliceDesc* __restrict__ slices,
+                                                               const double* __restrict__ pos, double invL,
+                                                               int* __restrict__ perm, int stride) {
+    static_assert(kLdsBeads <= 2048, "bead index keys take 11 bits");
+    __shared__ unsigned key[kLdsBeads];
+    const SliceDesc sd = slices[blockIdx.x];
+    const int n = sd.n;
+    if (n > kLdsBeads) return;
+    const double* px = pos + sd.pos;
+    const double* py = px + sd.plane;
+    const double* pz = py + sd.plane;
+    int P = 1;
+    while (P < n) P <<= 1;
+    for (int t = threadIdx.x; t < P; t += kOrderThreads)
+        key[t] = t < n ? (morton_cell(px[t], py[t], pz[t], invL) << 11) | (unsigned)t : 0xffffffffu;
+    __syncthreads();
+    // lane t compares keys a and a + j of the 128 keys [128 w, 128 w + 128) of its
+    // wave w whenever j <= 64, so a pass with j <= 64 followed by another needs
+    // only the wave's own LDS order (51 of the 66 passes at 2048 beads)
+    static_assert(kLdsBeads / 2 <= kOrderThreads, "one compare-exchange per lane and pass");
+    const int t = threadIdx.x;
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (t < (P >> 1)) {
+                const int a = ((t & ~(j - 1)) << 1) | (t & (j - 1));   // pair (a, a + j)
+                const unsigned ka = key[a], kb = key[a + j];
+                if ((ka > kb) == ((a & k) == 0)) {
+                    key[a] = kb;
+                    key[a + j] = ka;
+                }
+            }
+            const int j_next = j > 1 ? j >> 1 : k;   // the next pass's distance
+            if (j <= 64 && j_next <= 64) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            } else {
+                __syncthreads();
+            }
+        }
+    }
+    __syncthreads();
+    int* out = perm + (int64_t)blockIdx.x * stride;
+    for (int t = threadIdx.x; t < n; t += kOrderThreads) out[t] = (int)(key[t] & 0x7ffu);
 }
 
 // Persistent: the workgroups of one XCD take consecutive tile ranges (so a
@@ -275,6 +355,196 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
                                               factors, nowrap, wk.worm, tl.m)
                          : row_tile<MODE, 0>(px, py, pz, n, i, d, d_end, T, G, mask, wk.box, a, inv_dr, factors,
                                              nowrap, wk.worm, tl.m);
+        }
+        const double s = block_sum<kPairThreads>(acc, red);
+        if (threadIdx.x == 0) partials[k] = s;
+        tl = tl_next;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Window form (mixed mode, slices in spatial order): tiles of kWinRows rows x at
+// most kWinOffsets offsets on slices of 256 beads and more.  A tile's partners
+// j = (i + d) mod n, i in [i0, i0 + 32), d in [d0, d0 + dn), are the cyclic run
+// of 32 + dn - 1 beads from (i0 + d0) mod n; only that window is staged, 26 KB
+// of LDS against 48 KB for a 2048-bead slice, so a CU holds six workgroups
+// instead of three.  Each wave takes 8 rows x 8 offset
+// groups, so its 64 pairs join two small clusters of beads and are mostly all
+// far or all near (the FAR path); the partner cursor walks the window without a
+// wrap.  Slices of fewer than 256 beads are staged whole, row_tile's layout.
+// ---------------------------------------------------------------------------
+constexpr int kWinRows = 32;      // rows per tile: 4 waves x 8
+constexpr int kWinGroups = 8;     // offset groups per wave (lane bits 3..5)
+// offsets per tile, at most (N = 2048 x 250, interleaved runs: 256 / 512 / 1024
+// offsets 711 / 687 / 676 us, bead-order staging 801 us)
+constexpr int kWinOffsets = 1024;
+// window slots per plane: 32 + 1024 - 1 staged, the masked last iteration of a
+// lane reads at most 7 slots further; rounded up to a multiple of 64
+constexpr int kWinBeads = 1088;
+static_assert(kWinRows + kWinOffsets - 1 + kWinGroups - 1 <= kWinBeads, "window too small");
+static_assert(pair_lds_plane<256>() <= kWinBeads, "small slices are staged whole in the window's planes");
+
+// One lane of a window tile: row bead (xi, yi, zi) of global index `row`,
+// partners at window slots slot0, slot0 + 8, ... (offsets d, d + 8, ... < d_end,
+// T iterations as in row_loop, the same masking of d = n/2), w0: the slice index
+// of window slot 0 (FACTORS: partner index (w0 + slot) mod n, bead order).
+template <bool FACTORS, bool BETA0>
+__device__ __forceinline__ double row_loop_win(const double* lw, double xi, double yi, double zi, int row, int n,
+                                               int slot0, int d, int d_end, int T, bool mask, int w0,
+                                               const BoxDev& box, const AzizDev& a, double inv_dr,
+                                               const WormDev& worm, int slice) {
+    RowMixed q;
+    q.invL = box.invL;
+    q.cx = fma(-xi, box.invL, 0.5);
+    q.cy = fma(-yi, box.invL, 0.5);
+    q.cz = fma(-zi, box.invL, 0.5);
+    q.S = box.L * inv_dr;
+    q.Sh = in_vgpr(-0.5 * q.S);
+    const int lenm1 = in_vgpr(a.table_length - 1);
+    const unsigned far_span = in_vgpr((unsigned)max(a.table_length - a.k_far, 0));
+    AzizDev av = a;
+    av.rep_c1 = in_vgpr(a.rep_c1);
+    av.eC8k = in_vgpr(a.eC8k);
+    const int skip_d = (mask && row >= (n >> 1)) ? (n >> 1) : -1;
+    const char* lb = reinterpret_cast<const char*>(lw);
+    struct Pos {
+        double x, y, z;
+        unsigned c;
+    };
+    auto load = [&](unsigned c) -> Pos {   // c: byte offset of the slot in the x plane
+        Pos p;
+        p.c = c;
+        p.x = *reinterpret_cast<const double*>(lb + c);
+        p.y = *reinterpret_cast<const double*>(lb + c + 8 * kWinBeads);
+        p.z = *reinterpret_cast<const double*>(lb + c + 16 * kWinBeads);
+        return p;
+    };
+    auto eval = [&](const Pos& p) -> double {
+        double v = pair_term_mixed<BETA0, true>(q, p.x, p.y, p.z, a, av, lenm1, far_span);
+        if (FACTORS) {
+            int j = w0 + (int)(p.c >> 3);
+            if (j >= n) j -= n;
+            const int lo = row < j ? row : j, hi = row < j ? j : row;
+            v *= pair_factor(bead_state(worm, slice, lo), bead_state(worm, slice, hi));
+        }
+        return v;
+    };
+    constexpr unsigned kStep = 8u * kWinGroups;
+    unsigned jc = (unsigned)slot0 * 8u;
+    const bool ok = (d + (T - 1) * kWinGroups < d_end) && (d + (T - 1) * kWinGroups) != skip_d;
+    if (T == 1) {
+        const double v = eval(load(ok ? jc : 0u));
+        return ok ? v : 0.0;
+    }
+    double acc = 0.0;
+    Pos p = load(jc);
+    for (int t = 1; t < T - 1; ++t) {
+        jc += kStep;
+        const Pos pn = load(jc);
+        acc += eval(p);
+        p = pn;
+    }
+    jc += kStep;
+    const Pos pl = load(ok ? jc : 0u);
+    acc += eval(p);
+    const double v = eval(pl);
+    acc += ok ? v : 0.0;
+    return acc;
+}
+
+__global__ __launch_bounds__(kPairThreads) void k_pair_window(const RowTile* __restrict__ tiles,
+                                                              const int* __restrict__ ranges,
+                                                              const WalkerDesc* __restrict__ walkers,
+                                                              const double* __restrict__ pos, AzizDev az,
+                                                              double* __restrict__ partials,
+                                                              const int* __restrict__ perm, int perm_stride) {
+    constexpr int PSL = pair_lds_plane<256>();   // plane stride of a small slice staged whole
+    __shared__ double lds[3 * kWinBeads + kPairThreads / 64];
+    double* red = lds + 3 * kWinBeads;
+    const int r = xcd_contiguous_item(blockIdx.x, gridDim.x);
+    const int k_begin = ranges[r], k_end = ranges[r + 1];
+    if (k_begin >= k_end) return;
+    int staged = -1;   // the small slice staged whole, if any
+    RowTile tl = tiles[k_begin];
+    for (int k = k_begin; k < k_end; ++k) {
+        const RowTile tl_next = tiles[k + 1 < k_end ? k + 1 : k];
+        const WalkerDesc& wk = walkers[tl.walker];
+        AzizDev a = az;
+        a.table_length = wk.table_length;
+        const double inv_dr = 1.0 / a.dr;
+        const int n = tl.n;
+        const double* px = pos + tl.pos;
+        const double* py = px + tl.plane;
+        const double* pz = py + tl.plane;
+        // spatial order, or bead order where the worm's pair factors need the
+        // bead index
+        const bool factors = worm_touches_slice(wk.worm, tl.m);
+        const int* pm = factors ? nullptr : perm + (int64_t)tl.slice * perm_stride;
+        const bool mask_n = (n & 1) == 0;
+        const int d_end = tl.d0 + tl.dn;
+        double acc = 0.0;
+        if (n >= kPairThreads) {
+            // the window: slots t < 32 + dn - 1 hold beads (w0 + t) mod n (i0 + d0
+            // < 3n/2, 32 + dn - 1 < n: one wrap at most)
+            int w0 = tl.i0 + tl.d0;
+            if (w0 >= n) w0 -= n;
+            const int len = (tl.i_end - tl.i0) + tl.dn - 1;
+            __syncthreads();   // the previous window is no longer read
+            for (int t = threadIdx.x; t < len; t += kPairThreads) {
+                int b = w0 + t;
+                if (b >= n) b -= n;
+                const int src = pm ? min(pm[b], n - 1) : b;
+                lds[t] = px[src];
+                lds[kWinBeads + t] = py[src];
+                lds[2 * kWinBeads + t] = pz[src];
+            }
+            staged = -1;
+            __syncthreads();
+            const int row = tl.i0 + 8 * (threadIdx.x >> 6) + (threadIdx.x & 7);
+            const int c = (threadIdx.x >> 3) & 7;
+            if (row < tl.i_end) {
+                const int rs = pm ? min(pm[row], n - 1) : row;
+                const double xi = px[rs], yi = py[rs], zi = pz[rs];
+                const int T = (tl.dn + kWinGroups - 1) / kWinGroups;
+                const bool mask = mask_n && d_end > (n >> 1) && tl.i_end > (n >> 1);
+                const int slot0 = (row - tl.i0) + c;
+                const int d = tl.d0 + c;
+#define PIMC_WIN(F_, B_) \
+    row_loop_win<F_, B_>(lds, xi, yi, zi, row, n, slot0, d, d_end, T, mask, w0, wk.box, a, inv_dr, wk.worm, tl.m)
+                if (factors)
+                    acc = PIMC_WIN(true, false);
+                else
+                    acc = a.rep_c2 == 0.0 ? PIMC_WIN(false, true) : PIMC_WIN(false, false);
+#undef PIMC_WIN
+            }
+        } else {
+            if (staged != tl.slice) {
+                __syncthreads();
+                for (int t = threadIdx.x; t < n; t += kPairThreads) {
+                    const int src = pm ? min(pm[t], n - 1) : t;
+                    const double x = px[src], y = py[src], z = pz[src];
+                    lds[t] = x;
+                    lds[PSL + t] = y;
+                    lds[2 * PSL + t] = z;
+                    if (t <= (n >> 1)) {
+                        lds[n + t] = x;
+                        lds[PSL + n + t] = y;
+                        lds[2 * PSL + n + t] = z;
+                    }
+                }
+                staged = tl.slice;
+                __syncthreads();
+            }
+            const int G = kPairThreads / n;
+            const int grp = threadIdx.x / n;
+            const int i = tl.i0 + (threadIdx.x - grp * n);
+            if (grp < G && i < tl.i_end) {
+                const int d = tl.d0 + grp;
+                const int T = (tl.dn + G - 1) / G;
+                const bool mask = mask_n && d_end > (n >> 1) && tl.i_end > (n >> 1);
+                acc = row_tile<0, 256, true>(lds, nullptr, nullptr, n, i, d, d_end, T, G, mask, wk.box, a, inv_dr,
+                                             factors, true, wk.worm, tl.m);
+            }
         }
         const double s = block_sum<kPairThreads>(acc, red);
         if (threadIdx.x == 0) partials[k] = s;
@@ -1179,9 +1449,11 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
                               const SliceDesc* slices, int n_slices, const WalkerDesc* walkers, int n_walkers,
                               const double* pos, const AzizDev& az, int lds_beads, double* partials, double* v_slice,
                               double* action, const KernelTimer* timer, bool finalize, unsigned long long* done,
-                              unsigned long long seq, const double* g2_part, int g2_blocks, double* g2_slice) {
+                              unsigned long long seq, const double* g2_part, int g2_blocks, double* g2_slice,
+                              const int* perm, int perm_stride) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (n_tiles <= 0 || G <= 0) return hipSuccess;
+    const bool spatial = perm != nullptr;
     dim3 g(G), b(kPairThreads);
     // LDS plane stride: the smallest of 256 / 512 / 1024 / 2048 beads that holds
     // the largest staged slice (a compile-time stride lets the three plane
@@ -1191,6 +1463,11 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
     auto go = [&](auto kern) { hipLaunchKernelGGL(kern, g, b, 0, st, tiles, ranges, walkers, pos, az, partials); };
     auto by_ps = [&](auto mode_tag) {
         constexpr int MODE = decltype(mode_tag)::value;
+        if (MODE == 0 && spatial) {
+            hipLaunchKernelGGL(k_pair_window, g, b, 0, st, tiles, ranges, walkers, pos, az, partials, perm,
+                               perm_stride);
+            return;
+        }
         if (PS == 0)
             go(k_pair_action<MODE, 0>);
         else if (PS == 256)
@@ -1216,6 +1493,14 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
     return hipGetLastError();
 }
 
+hipError_t launch_slice_order(hipStream_t st, const SliceDesc* slices, int n_slices, const double* pos, double invL,
+                              int* perm, int stride) {
+    (void)hipGetLastError();  // drop any stale error of an earlier call
+    if (n_slices <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slice_order, dim3(n_slices), dim3(kOrderThreads), 0, st, slices, pos, invL, perm, stride);
+    return hipGetLastError();
+}
+
 int pair_grid_for(int lds_beads, int num_cus) {
     // Sixteen workgroups per CU, more than LDS holds at once for large slices:
     // short per-workgroup ranges overlap one workgroup's slice staging with
@@ -1226,15 +1511,17 @@ int pair_grid_for(int lds_beads, int num_cus) {
     return 16 * std::max(1, num_cus);
 }
 
-int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lane, std::vector<RowTile>& tiles) {
+int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lane, std::vector<RowTile>& tiles,
+                         bool window) {
     const int n = sd.n;
     if (n < 2) return 0;
     auto tile = [&](int i0, int i_end, int d0, int dn) {
         tiles.push_back(RowTile{slice_index, i0, i_end, d0, dn, n, sd.slice, sd.walker, sd.pos, sd.plane});
     };
-    const int R = n < kPairThreads ? n : kPairThreads;
+    const int R = n < kPairThreads ? n : window ? kWinRows : kPairThreads;
     const int G = kPairThreads / R;
-    const int dn = std::max(1, pairs_per_lane) * G;    // offsets per tile
+    int dn = std::max(1, pairs_per_lane) * G;    // offsets per tile
+    if (window && n >= kPairThreads) dn = std::min(dn, kWinOffsets);
     if (n < kPairThreads) {
         // one row block: offsets 1..n/2 in one run, d = n/2 of an even n masked to
         // rows i < n/2 inside the kernel (a tile of its own would mostly idle)
@@ -1242,7 +1529,7 @@ int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lan
         for (int d0 = 1; d0 <= dmax; d0 += dn) tile(0, n, d0, std::min(dn, dmax - d0 + 1));
         return (int64_t)n * (n - 1) / 2;
     }
-    // row blocks of 256 against offsets 1..n/2; the d = n/2 pairs of an even n
+    // row blocks of R against offsets 1..n/2; the d = n/2 pairs of an even n
     // count once: rows >= n/2 drop that offset, in the last iteration of the
     // block's last tile
     const int dmax = n / 2;

@@ -148,6 +148,94 @@ def test_total_action_lds_strides(orc, gpu, mode, N):
     assert abs(a - b) <= TOL[mode] * abs(b), (N, a, b)
 
 
+@pytest.mark.parametrize("N", [512, 700, 1024, 1537, 2048])
+def test_total_action_spatial_order(orc, gpu, N):
+    """Slices staged in spatial order with the far-pair path (mixed mode): the
+    oracle's action and per-slice V at 1e-6, and the bead-order staging's at
+    1e-12 (the same bins; only the far pairs' 1/k^2, the dropped repulsion below
+    2^-53 of V and the summation order differ).  Index-order staging is forced
+    with stage_in_lds = 1, spatial order with 2; the automatic shape takes
+    spatial order above 1024 beads."""
+    M = 4
+    ref, ctx, _, _ = setup_pair(orc, gpu, N, M, 0.004, rc_frac=0.5, kind=1)
+    b = ref.total_action()
+    vs_ref = np.array([ref.V_slice(m)[1] for m in range(M)])
+    got = {}
+    for lds in (1, 2, -1):
+        ctx.set_launch_shape(0, 0, lds)
+        a, vs = ctx.potentialAction(with_slices=True)
+        vs = vs[:, 1].copy()
+        assert abs(a - b) <= TOL[0] * abs(b), (lds, a, b)
+        assert np.allclose(vs, vs_ref, rtol=TOL[0], atol=0), lds
+        got[lds] = (a, vs)
+    a1, a2 = got[1][0], got[2][0]
+    print(f"N{N}: spatial - index order {abs(a2 - a1) / abs(a1):.2e}, oracle {abs(a2 - b) / abs(b):.2e}")
+    assert abs(a2 - a1) <= 1e-12 * abs(a1)
+    assert np.allclose(got[2][1], got[1][1], rtol=1e-12, atol=0)
+    assert got[-1][0] == (a2 if N > 1024 else a1)
+    # the same slices twice: the spatial order depends only on the positions
+    ctx.set_launch_shape(0, 0, 2)
+    assert ctx.potentialAction() == a2
+
+
+def test_total_action_spatial_uneven_slices(orc, gpu):
+    """The window form over slices of uneven bead counts (numBeadsAtSlice, left
+    packed): windows on slices of 256 beads and more, whole-slice staging below
+    (255, 100), no pairs at one bead; the oracle at 1e-6, bead-order staging at
+    1e-12."""
+    N, M = 700, 6
+    ref, ctx, beads, L = setup_pair(orc, gpu, N, M, 0.004, rc_frac=0.5, kind=1)
+    nb = np.array([700, 255, 100, 1, 256, 517], dtype=np.int32)
+    on = np.zeros((M, N), dtype=np.uint8)
+    for m in range(M):
+        on[m, :nb[m]] = 1
+    nxt = np.zeros((M, N, 2), dtype=np.int32)
+    prv = np.zeros((M, N, 2), dtype=np.int32)
+    for m in range(M):
+        for i in range(N):
+            nxt[m, i] = ((m + 1) % M, i)
+            prv[m, i] = ((m - 1) % M, i)
+    ref.set_topology(on, nb, nxt, prv)
+    ctx.upload_path(beads, bead_on=on, num_beads=nb)
+    b = ref.total_action()
+    vs_ref = np.array([ref.V_slice(m)[1] for m in range(M)])
+    got = []
+    for lds in (1, 2):
+        ctx.set_launch_shape(0, 0, lds)
+        a, vs = ctx.potentialAction(with_slices=True)
+        assert abs(a - b) <= TOL[0] * abs(b), (lds, a, b)
+        for m in range(M):
+            assert abs(vs[m, 1] - vs_ref[m]) <= TOL[0] * max(abs(vs_ref[m]), 1.0), (lds, m, vs[m, 1], vs_ref[m])
+        got.append(vs[:, 1].copy())
+    assert np.allclose(got[1], got[0], rtol=1e-12, atol=1e-12)
+
+
+def test_total_action_spatial_order_worm(orc, gpu):
+    """Spatial staging with a worm: slices the worm touches are staged in bead
+    order for the pair factors (src/action.cpp:925-940), the others in spatial
+    order; the oracle's action at 1e-6 and the bead-order staging's at 1e-12."""
+    N, M = 600, 6
+    ref, ctx, beads, L = setup_pair(orc, gpu, N, M, 0.004, rc_frac=0.5, kind=1)
+    on = np.ones((M, N), dtype=np.uint8)
+    nb = np.full(M, N, dtype=np.int32)
+    nxt = np.zeros((M, N, 2), dtype=np.int32)
+    prv = np.zeros((M, N, 2), dtype=np.int32)
+    for m in range(M):
+        for i in range(N):
+            nxt[m, i] = ((m + 1) % M, i)
+            prv[m, i] = ((m - 1) % M, i)
+    head, tail, s1 = (1, 20), (3, 500), (4, 7)
+    ref.set_topology(on, nb, nxt, prv, head=head, tail=tail, special1=s1)
+    ctx.set_worm(head=head, tail=tail, special1=s1)
+    b = ref.total_action()
+    ctx.set_launch_shape(0, 0, 1)
+    a1 = ctx.potentialAction()
+    ctx.set_launch_shape(0, 0, 2)
+    a2 = ctx.potentialAction()
+    assert abs(a2 - b) <= TOL[0] * abs(b), (a2, b)
+    assert abs(a2 - a1) <= 1e-12 * abs(a1), (a1, a2)
+
+
 def _bisection_case(orc, gpu, N, M, seg, seed, mode=0, rc_frac=1.0):
     ref, ctx, beads, L = setup_pair(orc, gpu, N, M, 0.004, rc_frac=rc_frac, seed=seed, mode=mode)
     rng = np.random.default_rng(seed)

@@ -87,18 +87,24 @@ constexpr int pair_lds_plane() {
 }
 
 // One mixed-mode pair term against the row bead of q.  FAR: a wave whose pairs
-// all lie in far bins (k_far <= k < table_length) takes aziz_vfar, no exp2,
-// damping test or clamp; any other wave the full term.  The far value differs
-// from the full one by at most an FP64 rounding and a ~1e-12 relative 1/k^2, so
-// which path a pair takes depends on its wave's other pairs but its value does
-// not beyond that.  av: a with the loop-invariant operands in vector registers;
-// far_span = max(table_length - k_far, 0).
+// all lie in far bins (k_far <= k < table_length) takes aziz_vfar (no exp2,
+// damping test or clamp), a wave whose pairs all lie at or beyond D (k_d <= k <
+// table_length) aziz_vmid (no damping test or clamp), any other wave the full
+// term.  The tiers differ from the full term by at most an FP64 rounding and a
+// ~1e-12 relative 1/k^2, so which one a pair takes depends on its wave's other
+// pairs but its value does not beyond that.  av: a with the loop-invariant
+// operands in vector registers; far_span = max(table_length - k_far, 0),
+// mid_span = max(table_length - k_d, 0).
 template <bool BETA0, bool FAR>
 __device__ __forceinline__ double pair_term_mixed(const RowMixed& q, double x, double y, double z, const AzizDev& a,
-                                                  const AzizDev& av, int lenm1, unsigned far_span) {
+                                                  const AzizDev& av, int lenm1, unsigned far_span,
+                                                  unsigned mid_span = 0) {
     double yr;
     const int k = bin_mixed(q, x, y, z, &yr);
-    if (FAR && __builtin_amdgcn_ballot_w64((unsigned)(k - a.k_far) >= far_span) == 0) return aziz_vfar(k, yr, av);
+    if (FAR) {
+        if (__builtin_amdgcn_ballot_w64((unsigned)(k - a.k_far) >= far_span) == 0) return aziz_vfar(k, yr, av);
+        if (__builtin_amdgcn_ballot_w64((unsigned)(k - a.k_d) >= mid_span) == 0) return aziz_vmid<BETA0>(k, yr, av);
+    }
     // k_eps <= k < tableLength, else V = 0 (include/potential.h:252-257): the
     // clamped bin keeps the arithmetic finite, and only a wave holding such a
     // pair pays for the select
@@ -137,8 +143,10 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
         q.Sh = in_vgpr(-0.5 * q.S);
     }
     const int lenm1 = in_vgpr(a.table_length - 1);
-    // far bins: k - k_far < table_length - k_far, unsigned (empty when k_far >= table_length)
+    // far bins: k - k_far < table_length - k_far, unsigned (empty when k_far >=
+    // table_length); bins beyond D likewise
     const unsigned far_span = in_vgpr((unsigned)max(a.table_length - a.k_far, 0));
+    const unsigned mid_span = in_vgpr((unsigned)max(a.table_length - a.k_d, 0));
     AzizDev av = a;
     if (MODE == 0) {
         av.rep_c1 = in_vgpr(a.rep_c1);
@@ -169,7 +177,7 @@ __device__ __forceinline__ double row_loop(const double* px, const double* py, c
     auto eval = [&](const Pos& p) -> double {
         double v;
         if (MODE == 0) {
-            v = pair_term_mixed<BETA0, FAR>(q, p.x, p.y, p.z, a, av, lenm1, far_span);
+            v = pair_term_mixed<BETA0, FAR>(q, p.x, p.y, p.z, a, av, lenm1, far_span, mid_span);
         } else {
             const double r2 = sep_r2<MODE>(xi, yi, zi, p.x, p.y, p.z, box);
             v = pair_value_from_r2<MODE>(r2, a, inv_dr);
@@ -379,8 +387,12 @@ constexpr int kWinGroups = 8;     // offset groups per wave (lane bits 3..5)
 // offsets 711 / 687 / 676 us, bead-order staging 801 us)
 constexpr int kWinOffsets = 1024;
 // window slots per plane: 32 + 1024 - 1 staged, the masked last iteration of a
-// lane reads at most 7 slots further; rounded up to a multiple of 64
-constexpr int kWinBeads = 1088;
+// lane reads at most 7 slots further; rounded up to a multiple of 64, plus 2 so
+// the plane stride (8720 bytes) is no multiple of 512: the three reads of a
+// partner are then single reads with immediate offsets from the cursor (a
+// stride of 64 x 512 bytes pairs the x and y reads into one read2st64 whose
+// offset units cannot hold the next slot's 64 bytes, which costs an add per pair)
+constexpr int kWinBeads = 1090;
 static_assert(kWinRows + kWinOffsets - 1 + kWinGroups - 1 <= kWinBeads, "window too small");
 static_assert(pair_lds_plane<256>() <= kWinBeads, "small slices are staged whole in the window's planes");
 
@@ -402,6 +414,7 @@ __device__ __forceinline__ double row_loop_win(const double* lw, double xi, doub
     q.Sh = in_vgpr(-0.5 * q.S);
     const int lenm1 = in_vgpr(a.table_length - 1);
     const unsigned far_span = in_vgpr((unsigned)max(a.table_length - a.k_far, 0));
+    const unsigned mid_span = in_vgpr((unsigned)max(a.table_length - a.k_d, 0));
     AzizDev av = a;
     av.rep_c1 = in_vgpr(a.rep_c1);
     av.eC8k = in_vgpr(a.eC8k);
@@ -420,7 +433,7 @@ __device__ __forceinline__ double row_loop_win(const double* lw, double xi, doub
         return p;
     };
     auto eval = [&](const Pos& p) -> double {
-        double v = pair_term_mixed<BETA0, true>(q, p.x, p.y, p.z, a, av, lenm1, far_span);
+        double v = pair_term_mixed<BETA0, true>(q, p.x, p.y, p.z, a, av, lenm1, far_span, mid_span);
         if (FACTORS) {
             int j = w0 + (int)(p.c >> 3);
             if (j >= n) j -= n;

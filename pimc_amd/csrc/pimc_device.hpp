@@ -137,14 +137,13 @@ __device__ __forceinline__ double aziz_disp_y(double kd, double y, const AzizDev
     const double iq = fma(r0, fma(-q, r0, 1.0), r0);
     return (iq * iq * iq) * fma(fma(a.eC10k, iq, a.eC8k), iq, a.eC6k);
 }
-__device__ __forceinline__ double aziz_vfar(int k, double y, const AzizDev& a) { return -aziz_disp_y((double)k, y, a); }
+__device__ __forceinline__ double aziz_vfar(double kd, double y, const AzizDev& a) { return -aziz_disp_y(kd, y, a); }
 
 // valueV of a bin at or beyond D, k_d <= k < table_length (F = 1): repulsion
 // minus dispersion, 1/k^2 as in aziz_vfar (2/k + 2^-22 <= 2e-6 before the Newton
 // step at x >= D >= 1.24, ~4e-12 after).
 template <bool BETA0 = false>
-__device__ __forceinline__ double aziz_vmid(int k, double y, const AzizDev& a) {
-    const double kd = (double)k;
+__device__ __forceinline__ double aziz_vmid(double kd, double y, const AzizDev& a) {
     const double rep = split_exp2(BETA0 ? fma(a.rep_c1, kd, a.rep_c0) : fma(fma(a.rep_c2, kd, a.rep_c1), kd, a.rep_c0));
     return rep - aziz_disp_y(kd, y, a);
 }

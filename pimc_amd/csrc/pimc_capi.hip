@@ -850,8 +850,7 @@ int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* a
         if (cor && (r = run_gradv2(c, az))) return r;
         const bool spatial = c->pair_spatial && c->desc.v_mode == PIMC_V_MIXED;
         if (spatial && c->perm_version != c->path_version) {
-            HIP_OR_FAIL(c, launch_slice_order(c->stream, c->d_slices, M, c->d_pos, c->box.invL, c->d_perm,
-                                              c->desc.num_alloc));
+            HIP_OR_FAIL(c, launch_slice_order(c->stream, c->d_slices, M, c->d_pos, c->box.invL, c->d_perm));
             c->perm_version = c->path_version;
         }
         const KernelTimer kt = c->kernel_timer(0);
@@ -860,7 +859,7 @@ int pimc_total_action(pimc_ctx* c, int shift, const double vfactor[2], double* a
                                           c->d_slices, M, c->d_walker, 1, c->d_pos, az, c->pair_lds, c->d_partials,
                                           c->d_vslice, c->d_map, &kt, true, c->d_done(), seq,
                                           cor ? c->d_g2part : nullptr, c->g2_blocks, c->d_g2slice,
-                                          spatial ? c->d_perm : nullptr, c->desc.num_alloc));
+                                          spatial ? c->d_perm : nullptr));
         if (v_slices) {
             HIP_OR_FAIL(c, hipMemcpyAsync(c->h_stage, c->d_vslice, sizeof(double) * M, hipMemcpyDeviceToHost,
                                           c->stream));

@@ -63,7 +63,7 @@ struct RowTile {
 // Appends the tiles of slice `slice_index` (described by sd) with about
 // pairs_per_lane pairs per lane each; returns the slice's pair count n(n-1)/2.
 // window: the tiles of the window form (k_pair_window: 32 rows x at most 1024
-// offsets on slices of 256 beads and more).
+// offsets on slices of 64 beads and more).
 int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lane, std::vector<RowTile>& tiles,
                          bool window = false);
 // Splits tiles [0, T) into G contiguous ranges of about equal pair count:
@@ -197,19 +197,20 @@ __device__ __forceinline__ int xcd_contiguous_item(int b, int G) {
 // (written to g2_slice) enter the walkers' actions with the gradient correction.
 // perm (or null): mixed mode, the window form (k_pair_window, tiles from
 // append_row_tiles(..., window = true)) over slices in the spatial order
-// launch_slice_order wrote, rows [n_slices][perm_stride].
+// launch_slice_order wrote (slice s at perm + slices[s].pos).
 hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, int n_tiles, const int* ranges, int G,
                               const SliceDesc* slices, int n_slices, const WalkerDesc* walkers, int n_walkers,
                               const double* pos, const AzizDev& az, int lds_beads, double* partials, double* v_slice,
                               double* action, const KernelTimer* timer = nullptr, bool finalize = true,
                               unsigned long long* done = nullptr, unsigned long long seq = 0,
                               const double* g2_part = nullptr, int g2_blocks = 0, double* g2_slice = nullptr,
-                              const int* perm = nullptr, int perm_stride = 0);
-// Spatial order of every slice (slices of at most kLdsBeads beads): perm row s
-// (stride ints) lists the slice's beads by the Morton index of their cell on a
-// 32^3 grid over the box, ties by bead index.
+                              const int* perm = nullptr);
+// Spatial order of every slice (slices of at most kLdsBeads beads, one box):
+// perm[slices[s].pos + t], t < slices[s].n, lists slice s's beads by the index of
+// their cell along a Hilbert curve through a 32^3 grid over the box, ties by
+// bead index (perm spans the x plane of the position array).
 hipError_t launch_slice_order(hipStream_t st, const SliceDesc* slices, int n_slices, const double* pos, double invL,
-                              int* perm, int stride);
+                              int* perm);
 // Workgroups of the persistent pair kernel for a slice staging of lds_beads beads.
 int pair_grid_for(int lds_beads, int num_cus);
 hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,

@@ -282,12 +282,14 @@ __device__ __forceinline__ unsigned hilbert_cell(double x, double y, double z, d
 
 // Spatial order of each slice (one workgroup per slice): keys (Hilbert cell << 11
 // | bead), unique, sorted ascending by a bitonic network over the next power of
-// two (padding keys 0xffffffff); perm[t] is the bead of the t-th key.  The order
-// depends only on the positions.  n <= kLdsBeads (bead indices take 11 bits).
+// two (padding keys 0xffffffff); perm[pos + t] is the bead of the t-th key, pos
+// the slice's offset in the x plane (so one int array the size of the position
+// planes holds every slice's order).  The order depends only on the positions.
+// n <= kLdsBeads (bead indices take 11 bits).
 constexpr int kOrderThreads = 1024;
 __global__ __launch_bounds__(kOrderThreads) void k_slice_order(const SliceDesc* __restrict__ slices,
                                                                const double* __restrict__ pos, double invL,
-                                                               int* __restrict__ perm, int stride) {
+                                                               int* __restrict__ perm) {
     static_assert(kLdsBeads <= 2048, "bead index keys take 11 bits");
     __shared__ unsigned key[kLdsBeads];
     const SliceDesc sd = slices[blockIdx.x];
@@ -327,7 +329,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_slice_order(const SliceDesc* 
         }
     }
     __syncthreads();
-    int* out = perm + (int64_t)blockIdx.x * stride;
+    int* out = perm + sd.pos;
     for (int t = threadIdx.x; t < n; t += kOrderThreads) out[t] = (int)(key[t] & 0x7ffu);
 }
 
@@ -405,16 +407,17 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_action(const RowTile* __r
 
 // ---------------------------------------------------------------------------
 // Window form (mixed mode, slices in spatial order): tiles of kWinRows rows x at
-// most kWinOffsets offsets on slices of 256 beads and more.  A tile's partners
+// most kWinOffsets offsets on slices of kWinMinBeads beads and more.  A tile's partners
 // j = (i + d) mod n, i in [i0, i0 + 32), d in [d0, d0 + dn), are the cyclic run
 // of 32 + dn - 1 beads from (i0 + d0) mod n; only that window is staged, 26 KB
 // of LDS against 48 KB for a 2048-bead slice, so a CU holds six workgroups
 // instead of three.  Each wave takes 8 rows x 8 offset
 // groups, so its 64 pairs join two small clusters of beads and are mostly all
 // far or all near (the FAR path); the partner cursor walks the window without a
-// wrap.  Slices of fewer than 256 beads are staged whole, row_tile's layout.
+// wrap.  Slices of fewer than 64 beads are staged whole, row_tile's layout.
 // ---------------------------------------------------------------------------
 constexpr int kWinRows = 32;      // rows per tile: 4 waves x 8
+constexpr int kWinMinBeads = 64;  // window tiles from this slice size (32 + n/2 - 1 < n)
 constexpr int kWinGroups = 8;     // offset groups per wave (lane bits 3..5)
 // offsets per tile, at most (N = 2048 x 250, interleaved runs: 256 / 512 / 1024
 // offsets 711 / 687 / 676 us, bead-order staging 801 us)
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_window(const RowTile* __r
                                                               const WalkerDesc* __restrict__ walkers,
                                                               const double* __restrict__ pos, AzizDev az,
                                                               double* __restrict__ partials,
-                                                              const int* __restrict__ perm, int perm_stride) {
+                                                              const int* __restrict__ perm) {
     constexpr int PSL = pair_lds_plane<256>();   // plane stride of a small slice staged whole
     __shared__ double lds[3 * kWinBeads + kPairThreads / 64];
     double* red = lds + 3 * kWinBeads;
@@ -535,11 +538,11 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_window(const RowTile* __r
         // spatial order, or bead order where the worm's pair factors need the
         // bead index
         const bool factors = worm_touches_slice(wk.worm, tl.m);
-        const int* pm = factors ? nullptr : perm + (int64_t)tl.slice * perm_stride;
+        const int* pm = factors ? nullptr : perm + tl.pos;
         const bool mask_n = (n & 1) == 0;
         const int d_end = tl.d0 + tl.dn;
         double acc = 0.0;
-        if (n >= kPairThreads) {
+        if (n >= kWinMinBeads) {
             // the window: slots t < 32 + dn - 1 hold beads (w0 + t) mod n (i0 + d0
             // < 3n/2, 32 + dn - 1 < n: one wrap at most)
             int w0 = tl.i0 + tl.d0;
@@ -549,7 +552,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_window(const RowTile* __r
             for (int t = threadIdx.x; t < len; t += kPairThreads) {
                 int b = w0 + t;
                 if (b >= n) b -= n;
-                const int src = pm ? min(pm[b], n - 1) : b;
+                const int src = pm ? min(max(pm[b], 0), n - 1) : b;
                 lds[t] = px[src];
                 lds[kWinBeads + t] = py[src];
                 lds[2 * kWinBeads + t] = pz[src];
@@ -559,7 +562,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_window(const RowTile* __r
             const int row = tl.i0 + 8 * (threadIdx.x >> 6) + (threadIdx.x & 7);
             const int c = (threadIdx.x >> 3) & 7;
             if (row < tl.i_end) {
-                const int rs = pm ? min(pm[row], n - 1) : row;
+                const int rs = pm ? min(max(pm[row], 0), n - 1) : row;
                 const double xi = px[rs], yi = py[rs], zi = pz[rs];
                 const int T = (tl.dn + kWinGroups - 1) / kWinGroups;
                 const bool mask = mask_n && d_end > (n >> 1) && tl.i_end > (n >> 1);
@@ -577,7 +580,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_window(const RowTile* __r
             if (staged != tl.slice) {
                 __syncthreads();
                 for (int t = threadIdx.x; t < n; t += kPairThreads) {
-                    const int src = pm ? min(pm[t], n - 1) : t;
+                    const int src = pm ? min(max(pm[t], 0), n - 1) : t;
                     const double x = px[src], y = py[src], z = pz[src];
                     lds[t] = x;
                     lds[PSL + t] = y;
@@ -1506,7 +1509,7 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
                               const double* pos, const AzizDev& az, int lds_beads, double* partials, double* v_slice,
                               double* action, const KernelTimer* timer, bool finalize, unsigned long long* done,
                               unsigned long long seq, const double* g2_part, int g2_blocks, double* g2_slice,
-                              const int* perm, int perm_stride) {
+                              const int* perm) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (n_tiles <= 0 || G <= 0) return hipSuccess;
     const bool spatial = perm != nullptr;
@@ -1520,8 +1523,7 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
     auto by_ps = [&](auto mode_tag) {
         constexpr int MODE = decltype(mode_tag)::value;
         if (MODE == 0 && spatial) {
-            hipLaunchKernelGGL(k_pair_window, g, b, 0, st, tiles, ranges, walkers, pos, az, partials, perm,
-                               perm_stride);
+            hipLaunchKernelGGL(k_pair_window, g, b, 0, st, tiles, ranges, walkers, pos, az, partials, perm);
             return;
         }
         if (PS == 0)
@@ -1550,10 +1552,10 @@ hipError_t launch_pair_action(int mode, hipStream_t st, const RowTile* tiles, in
 }
 
 hipError_t launch_slice_order(hipStream_t st, const SliceDesc* slices, int n_slices, const double* pos, double invL,
-                              int* perm, int stride) {
+                              int* perm) {
     (void)hipGetLastError();  // drop any stale error of an earlier call
     if (n_slices <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_slice_order, dim3(n_slices), dim3(kOrderThreads), 0, st, slices, pos, invL, perm, stride);
+    hipLaunchKernelGGL(k_slice_order, dim3(n_slices), dim3(kOrderThreads), 0, st, slices, pos, invL, perm);
     return hipGetLastError();
 }
 
@@ -1574,11 +1576,12 @@ int64_t append_row_tiles(int slice_index, const SliceDesc& sd, int pairs_per_lan
     auto tile = [&](int i0, int i_end, int d0, int dn) {
         tiles.push_back(RowTile{slice_index, i0, i_end, d0, dn, n, sd.slice, sd.walker, sd.pos, sd.plane});
     };
-    const int R = n < kPairThreads ? n : window ? kWinRows : kPairThreads;
+    const bool win = window && n >= kWinMinBeads;
+    const int R = win ? kWinRows : n < kPairThreads ? n : kPairThreads;
     const int G = kPairThreads / R;
     int dn = std::max(1, pairs_per_lane) * G;    // offsets per tile
-    if (window && n >= kPairThreads) dn = std::min(dn, kWinOffsets);
-    if (n < kPairThreads) {
+    if (win) dn = std::min(dn, kWinOffsets);
+    if (!win && n < kPairThreads) {
         // one row block: offsets 1..n/2 in one run, d = n/2 of an even n masked to
         // rows i < n/2 inside the kernel (a tile of its own would mostly idle)
         const int dmax = n / 2;

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "pimc_kernels.hpp"
 
@@ -393,8 +394,9 @@ hipError_t launch_rho(bool lattice, hipStream_t st, const RhoArgs& A) {
         // products' space first holds the row scan (256 QPT ints).
         const int rows_cap = qpt >= 4 ? kRhoRowsMax : 0;
         const size_t lds = tables + sizeof(float2) * rows_cap * kRhoBeadChunk + sizeof(int2) * rows_cap;
-        static bool attr_set = false;
-        if (!attr_set) {
+        // set once per process (two threads racing here both set the same value)
+        static std::atomic<bool> attr_set{false};
+        if (!attr_set.load(std::memory_order_acquire)) {
             // up to 63 + 32 + 1 KB at |n_d| <= 40, above the default 64 KB of dynamic LDS
             // (the kernel's static LDS comes on top, within the 160 KB of a CU)
             const void* fns[4] = {(const void*)k_rho_lattice<1>, (const void*)k_rho_lattice<2>,
@@ -403,7 +405,7 @@ hipError_t launch_rho(bool lattice, hipStream_t st, const RhoArgs& A) {
                 const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024);
                 if (e != hipSuccess) return e;
             }
-            attr_set = true;
+            attr_set.store(true, std::memory_order_release);
         }
         if (qpt == 8) hipLaunchKernelGGL(k_rho_lattice<8>, g, b, lds, st, A, rows_cap);
         else if (qpt == 4) hipLaunchKernelGGL(k_rho_lattice<4>, g, b, lds, st, A, rows_cap);
@@ -424,13 +426,13 @@ hipError_t launch_isf(hipStream_t st, const double* rho, int nq, int M, double i
     } else {
         dim3 g((nq + kIsfQ - 1) / kIsfQ), b(256);
         const size_t lds = sizeof(double2) * kIsfQ * (2 * (size_t)M + 64);
-        static bool attr_set = false;
-        if (!attr_set) {
+        static std::atomic<bool> attr_set{false};
+        if (!attr_set.load(std::memory_order_acquire)) {
             // rows of M <= 512 slices need up to 68 KB of the 160 KB LDS
             hipError_t e = hipFuncSetAttribute((const void*)k_isf_from_rho,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
             if (e != hipSuccess) return e;
-            attr_set = true;
+            attr_set.store(true, std::memory_order_release);
         }
         hipLaunchKernelGGL(k_isf_from_rho, g, b, lds, st, rho, nq, M, inv_n, fqt);
     }

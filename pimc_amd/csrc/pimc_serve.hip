@@ -300,6 +300,22 @@ struct Slot {
     double* d_rho = nullptr;          // S(q): rho_q(m) of one run [M][run][2]
     double* h_sq = nullptr;           // S(q): mapped results [kMaxSsfQ]
     double* m_sq = nullptr;           // the device's address of h_sq
+    // the batch's requests and chains, picked under the server lock and read by
+    // build_batch after it (requests are immutable until retired, and the chain
+    // deque keeps its elements in place)
+    std::vector<std::pair<const Request*, const Chain*>> items;
+};
+
+// Chain extents readable without the server lock: an append-only table,
+// replaced by a copy of twice the capacity when full (the old tables stay alive
+// until the server goes, so a reader holding one is never left dangling).
+struct ChainExtents {
+    int num_slices, num_alloc;
+};
+struct ExtentsTable {
+    std::unique_ptr<ChainExtents[]> v;
+    int cap = 0;
+    std::atomic<int> n{0};
 };
 
 // One packed batch, handed from pack_batch (under the lock) to fire_batch (without).
@@ -329,6 +345,8 @@ struct pimc_server {
     // (remove_chain and download hold one across a wait that drops the lock)
     std::deque<Chain> chains;
     std::deque<int> waiting_chains;
+    std::atomic<ExtentsTable*> extents{nullptr};
+    std::vector<std::unique_ptr<ExtentsTable>> extents_all;   // every table ever published
     std::vector<ServeChainDev> chain_table;
     ServeChainDev* d_chains = nullptr;
     int chain_cap = 0;
@@ -361,6 +379,7 @@ struct pimc_server {
 
     void run();
     int pack_batch(int slot, BatchLaunch* L);
+    void build_batch(BatchLaunch* L);
     hipError_t fire_batch(const BatchLaunch& L);
     hipError_t fire_ssf(const BatchLaunch& L);
     void complete(Slot& s);
@@ -497,6 +516,8 @@ void pimc_server::admit_waiting() {
 // Packs every ready request that fits (at most one in flight per chain) into
 // slot S's pinned descriptor block and marks the slot busy.  Runs under the
 // server lock; returns 1 when a batch was packed, 0 when nothing is ready.
+// Picks a slot's batch from the ready requests (under the server lock): an S(q)
+// request alone, or Delta V requests up to the batch limits.
 int pimc_server::pack_batch(int si, BatchLaunch* L) {
     Slot& S = slots[si];
     if (ready.empty()) return 0;
@@ -505,28 +526,21 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
         kernel_error_msg = "chain table upload failed";
         return -1;
     }
-    std::vector<ServeReqDev>& rq = pack_rq;
-    std::vector<VnnBead>& bd = pack_bd;
-    std::vector<ServeGroupDev>& gr = pack_gr;
-    rq.clear();
-    bd.clear();
-    gr.clear();
     S.reqs.clear();
+    S.items.clear();
     if (reqs[ready.front()].kind == 1) {
         // an S(q) request takes the slot alone
         const uint64_t id = ready.front();
         ready.pop_front();
         Request& R = reqs[id];
         Chain& C = chains[R.chain];
-        const int nq = (int)R.qn.size() / 3;
-        std::memcpy(S.h_desc, R.qn.data(), sizeof(int) * R.qn.size());
         L->kind = 1;
         L->slot = si;
         L->chain = C.dev;
-        L->nq = nq;
+        L->nq = (int)R.qn.size() / 3;
         L->inv_n = 1.0 / C.d.num_alloc;
         L->invL = 1.0 / C.d.side;
-        L->bytes = sizeof(int) * R.qn.size();
+        S.items.emplace_back(&R, &C);
         S.reqs.push_back(id);
         S.kind = 1;
         C.busy = true;
@@ -540,16 +554,57 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
     }
     S.kind = 0;
     L->kind = 0;
-    while (!ready.empty() && (int)rq.size() < kMaxReqPerBatch) {
+    int n_bd = 0, n_gr = 0;
+    while (!ready.empty() && (int)S.items.size() < kMaxReqPerBatch) {
         const uint64_t id = ready.front();
         Request& R = reqs[id];
         if (R.kind == 1) break;       // S(q) goes in the next free slot
         Chain& C = chains[R.chain];
         const int chunks = (C.d.num_alloc + kVnnChunk - 1) / kVnnChunk;
         const int nb = (int)R.beads.size();
-        if (!rq.empty() && ((int)bd.size() + nb > kMaxBeadsPerBatch || (int)gr.size() + nb * chunks > kMaxGroupsPerBatch))
-            break;
+        if (!S.items.empty() && (n_bd + nb > kMaxBeadsPerBatch || n_gr + nb * chunks > kMaxGroupsPerBatch)) break;
         ready.pop_front();
+        S.items.emplace_back(&R, &C);
+        n_bd += nb;
+        n_gr += nb * chunks;
+        S.reqs.push_back(id);
+        C.busy = true;
+        pairs_done += 2LL * nb * (C.d.num_alloc - 1);
+    }
+    if (S.items.empty()) return 0;
+    L->n_reqs = (int)S.items.size();
+    S.n_reqs = L->n_reqs;
+    L->slot = si;
+    L->chains = d_chains;
+    S.busy = true;
+    S.seq += 1;   // the sequence number this batch publishes when it completes
+    S.t_fire = Clock::now();
+    ++launched_batches;
+    launched_reqs += (int64_t)S.items.size();
+    return 1;
+}
+
+// A picked batch's descriptors into the slot's staging block (worker thread,
+// without the server lock).
+void pimc_server::build_batch(BatchLaunch* L) {
+    Slot& S = slots[L->slot];
+    if (L->kind == 1) {
+        const Request& R = *S.items[0].first;
+        std::memcpy(S.h_desc, R.qn.data(), sizeof(int) * R.qn.size());
+        L->bytes = sizeof(int) * R.qn.size();
+        return;
+    }
+    std::vector<ServeReqDev>& rq = pack_rq;
+    std::vector<VnnBead>& bd = pack_bd;
+    std::vector<ServeGroupDev>& gr = pack_gr;
+    rq.clear();
+    bd.clear();
+    gr.clear();
+    for (const auto& it : S.items) {
+        const Request& R = *it.first;
+        const Chain& C = *it.second;
+        const int chunks = (C.d.num_alloc + kVnnChunk - 1) / kVnnChunk;
+        const int nb = (int)R.beads.size();
         ServeReqDev d{};
         d.chain = C.dev_index;
         d.n_beads = nb;
@@ -567,11 +622,7 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
             for (int c = 0; c < chunks; ++c) gr.push_back(ServeGroupDev{(int)rq.size(), d.first_bead + k, c});
         }
         rq.push_back(d);
-        S.reqs.push_back(id);
-        C.busy = true;
-        pairs_done += 2LL * nb * (C.d.num_alloc - 1);
     }
-    if (rq.empty()) return 0;
     const size_t b_rq = sizeof(ServeReqDev) * rq.size();
     const size_t b_bd = sizeof(VnnBead) * bd.size();
     const size_t b_gr = sizeof(ServeGroupDev) * gr.size();
@@ -579,9 +630,6 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
     L->o_gr = (L->o_bd + b_bd + 255) & ~size_t(255);
     L->bytes = L->o_gr + b_gr;
     L->n_groups = (int)gr.size();
-    L->n_reqs = (int)rq.size();
-    S.n_reqs = L->n_reqs;
-    L->slot = si;
     std::memcpy(S.h_desc, rq.data(), b_rq);
     std::memcpy(S.h_desc + L->o_bd, bd.data(), b_bd);
     std::memcpy(S.h_desc + L->o_gr, gr.data(), b_gr);
@@ -590,13 +638,6 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
         std::string e;
         L->az.table = cached_table(desc.aziz_year, 0, (int)(max_table_sep / ((1.0E-6) * params.rm)) + 1, &e);
     }
-    L->chains = d_chains;
-    S.busy = true;
-    S.seq += 1;   // the sequence number this batch publishes when it completes
-    S.t_fire = Clock::now();
-    ++launched_batches;
-    launched_reqs += (int64_t)rq.size();
-    return 1;
 }
 
 // Copies a packed batch to the device and launches it on its slot's stream.
@@ -697,6 +738,7 @@ void pimc_server::complete(Slot& S) {
         if (!C.queue.empty() && C.state == 1) ready.push_back(C.queue.front());
     }
     S.reqs.clear();
+    S.items.clear();
     S.busy = false;
 }
 
@@ -784,6 +826,7 @@ void pimc_server::run() {
             }
         lk.unlock();
         for (int t = 0; t < nl; ++t) {
+            build_batch(&L[t]);
             const hipError_t e = fire_batch(L[t]);
             if (e != hipSuccess) {
                 lk.lock();
@@ -882,6 +925,22 @@ int pimc_server_add_chain(pimc_server* s, const pimc_desc* d, const double* bead
     if (c.bytes > s->arena.size) return sfail(s, PIMC_E_NOMEM, "chain larger than the HBM budget");
     const int id = (int)s->chains.size();
     s->chains.push_back(c);
+    // publish the chain's extents to the lock-free submit path
+    ExtentsTable* t = s->extents.load(std::memory_order_relaxed);
+    if (!t || t->n.load(std::memory_order_relaxed) == t->cap) {
+        auto nt = std::make_unique<ExtentsTable>();
+        nt->cap = t ? 2 * t->cap : 256;
+        nt->v.reset(new ChainExtents[nt->cap]);
+        const int n = t ? t->n.load(std::memory_order_relaxed) : 0;
+        for (int k = 0; k < n; ++k) nt->v[k] = t->v[k];
+        nt->n.store(n, std::memory_order_relaxed);
+        t = nt.get();
+        s->extents_all.push_back(std::move(nt));
+        s->extents.store(t, std::memory_order_release);
+    }
+    const int n_ext = t->n.load(std::memory_order_relaxed);
+    t->v[n_ext] = ChainExtents{d->num_slices, d->num_alloc};
+    t->n.store(n_ext + 1, std::memory_order_release);
     const bool queue_empty = s->waiting_chains.empty();
     int r = queue_empty ? place_chain(s, id, beads) : 1;
     if (r < 0) return sfail(s, PIMC_E_HIP, "chain upload failed");
@@ -949,6 +1008,7 @@ int pimc_server_download(pimc_server* s, int chain, double* beads) {
 // have been removed); the worker sees it at once when the chain is idle.
 static int server_enqueue(pimc_server* s, Request&& R, uint64_t* req_id) {
     std::lock_guard<std::mutex> g(s->mu);
+    if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
     const int chain = R.chain;
     if (chain < 0 || chain >= (int)s->chains.size() || s->chains[chain].state == 2)
         return sfail(s, PIMC_E_ARG, "unknown chain");
@@ -970,16 +1030,18 @@ static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, c
     if (!s || !slice_ptcl || !new_pos || n < 1 || !req_id || shift < 1 || commit_mode < 0 || commit_mode > 2)
         return PIMC_E_ARG;
     if (n > kMaxBeadsPerBatch / 4) return sfail(s, PIMC_E_ARG, "too many beads in one request");
-    // the chain's extents under the lock; the request is built outside it, so
-    // the server lock is held only to queue it
+    // the chain's extents from the lock-free table (the state and a kernel error
+    // are checked when the request is queued); the request is built outside the
+    // lock, so the server lock is held only to queue it
     int num_slices = 0, num_alloc = 0;
     {
-        std::lock_guard<std::mutex> g(s->mu);
-        if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
-        if (chain < 0 || chain >= (int)s->chains.size() || s->chains[chain].state == 2)
+        const ExtentsTable* t = s->extents.load(std::memory_order_acquire);
+        if (chain < 0 || !t || chain >= t->n.load(std::memory_order_acquire)) {
+            std::lock_guard<std::mutex> g(s->mu);
             return sfail(s, PIMC_E_ARG, "unknown chain");
-        num_slices = s->chains[chain].d.num_slices;
-        num_alloc = s->chains[chain].d.num_alloc;
+        }
+        num_slices = t->v[chain].num_slices;
+        num_alloc = t->v[chain].num_alloc;
     }
     std::vector<char> seen(num_slices, 0);
     Request R;

@@ -367,3 +367,32 @@ def test_server_ssf_requests(orc):
         srv.wait(rid)
     assert srv.wait_ssf(rid)[0].shape == (3,)
     srv.close()
+
+
+def test_server_request_errors(orc):
+    """Requests the server refuses with an error and nothing queued: an unknown
+    chain (below zero, past the last one), a removed chain, a bead locator outside
+    the chain, two beads on one slice; a chain added after the first requests is
+    reachable at once (its extents published to the submit path)."""
+    import pimc_amd
+    w, b = walker(orc, 32, 16, 1.5, SEED0 + 7, kind=0)
+    srv = pimc_amd.Server(1 << 27, streams=2, max_side=40.0)
+    c0 = srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b)
+    sp = np.array([[1, 3], [2, 3]], dtype=np.int32)
+    new = b[sp[:, 0], sp[:, 1]] + 0.05
+    for chain in (-1, c0 + 1):
+        with pytest.raises(pimc_amd.PimcError, match="unknown chain"):
+            srv.submit(chain, sp, new)
+    with pytest.raises(pimc_amd.PimcError, match="out of range"):
+        srv.submit(c0, np.array([[w["M"], 0]], dtype=np.int32), new[:1])
+    with pytest.raises(pimc_amd.PimcError, match="distinct slices"):
+        srv.submit(c0, np.array([[1, 3], [1, 4]], dtype=np.int32), new)
+    r0 = srv.wait(srv.submit(c0, sp, new))
+    c1 = srv.add_chain(w["side"], w["M"], w["N"], w["tau"], LAMBDA, w["rc"], b)
+    r1 = srv.wait(srv.submit(c1, sp, new))
+    assert np.array_equal(r1["raw"][:4], r0["raw"][:4])
+    srv.remove_chain(c1)
+    with pytest.raises(pimc_amd.PimcError, match="unknown chain"):
+        srv.submit(c1, sp, new)
+    assert srv.stats()["requests"] == 2
+    srv.close()

@@ -443,6 +443,12 @@ int pimc_set_timing(pimc_ctx* ctx, int enable);
  * off.  A caller that needs one kernel's duration leaves the others' launches
  * free of event records. */
 int pimc_set_timing_kinds(pimc_ctx* ctx, unsigned kinds);
+/* Besides the names above, "delta_packets" counts the Delta V requests sent as
+ * dispatch packets straight into the device's AQL queue (total_ms 0): requests
+ * of up to 32 beads on slices of up to 1024, without the gradient correction
+ * or per-bead outputs, while timing of "delta_action" is off.  The environment
+ * variable PIMC_DIRECT_DISPATCH=0, read when a context is created, sends them
+ * through hipLaunchKernel instead. */
 int pimc_kernel_time(pimc_ctx* ctx, const char* name, double* total_ms, long* count);
 int pimc_device_synchronize(void);
 /* A plain device-to-device streaming copy of `bytes` run `iters` times (no

@@ -16,7 +16,10 @@ using namespace pimc;
 namespace {
 
 int sfail(pimc_ctx* c, int code, const std::string& m) {
-    if (c) c->err = m;
+    if (c) {
+        c->err = m;
+        ++c->fail_count;
+    }
     return code;
 }
 

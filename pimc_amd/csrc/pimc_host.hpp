@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/pimc_gpu.h"
+#include "pimc_direct.hpp"
 #include "pimc_kernels.hpp"
 
 namespace pimc {
@@ -75,7 +76,22 @@ struct pimc_ctx {
     // the direct Delta V form's per-bead results (mapped, coherent pinned block)
     pimc::VnnDirectOut* h_vdir = nullptr;
     pimc::VnnDirectOut* d_vdir = nullptr;
-    int wait_direct(int n, unsigned long long s);   // every slot of n carries s
+    int wait_direct(int n, unsigned long long s, bool packet);   // every slot of n carries s
+    // short Delta V requests dispatched through the device's AQL queue (the
+    // kernel-argument ring and k_vnn_direct per mode; direct_state -1: not set
+    // up yet, 0: launched through HIP, 1: packets)
+    pimc::DirectRing dring;
+    pimc::DirectKernel dk_vdir[3];
+    int direct_state = -1;
+    long direct_count = 0;   // requests dispatched as packets
+    bool direct_ready(int mode);
+    // Every entry point returns only after its work on the stream has finished
+    // or its last kernel has published its result (nothing of that kernel is
+    // left but its exit), so the stream is quiet between calls -- unless a call
+    // failed part-way.  Failures are counted; the first packet after one waits
+    // for the stream.
+    unsigned long long fail_count = 0, fail_seen = 0;
+    bool stream_quiet();
     int lds_beads = 0;            // slice positions staged in LDS (0: read from HBM)
     int pair_lds = 0;             // the pair kernel's staging (lds_beads, or 0 when tuned off)
     // launch shape of the pair kernel (pimc_set_launch_shape / pimc_autotune; 0 / -1: automatic)

@@ -12,6 +12,7 @@ namespace pimc {
 
 constexpr int kPairThreads = 256;   // workgroup of the all-pairs kernel
 constexpr int kVnnThreads = 256;    // workgroup of the Vnn (Delta V) kernel
+constexpr int kVnnDirectThreads = 2 * kVnnThreads;   // k_vnn_direct: old and new position
 constexpr int kLdsBeads = 2048;     // slice positions staged in LDS up to this size
 
 // One imaginary-time slice of one walker.
@@ -220,6 +221,14 @@ hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInlin
 // most kVnnChunk beads per slice; out [n_beads] is host-mapped.
 hipError_t launch_vnn_direct(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,
                              VnnDirectOut* out, const KernelTimer* timer = nullptr);
+// k_vnn_direct<mode>'s host stub, and its argument block as the AMDGPU ABI lays
+// it out (each by-value argument at its alignment), for a dispatch packet
+const void* vnn_direct_stub(int mode);
+constexpr uint32_t karg_align(uint32_t o, uint32_t a) { return (o + a - 1) / a * a; }
+constexpr uint32_t kVdirOffReq = karg_align(sizeof(VnnArgs), alignof(VnnInline));
+constexpr uint32_t kVdirOffAz = karg_align(kVdirOffReq + sizeof(VnnInline), alignof(AzizDev));
+constexpr uint32_t kVdirOffOut = karg_align(kVdirOffAz + sizeof(AzizDev), alignof(VnnDirectOut*));
+constexpr uint32_t kVdirBytes = kVdirOffOut + sizeof(VnnDirectOut*);
 // gradVSquared(slice) partials [M][blocks_per_slice] for every slice.
 hipError_t launch_gradv2(int mode, hipStream_t st, const double* pos, int64_t plane, int nalloc, int num_slices,
                          const int* nbs, const BoxDev& box, const AzizDev& az, int lds_beads, double* part,

@@ -44,7 +44,10 @@ std::mutex g_scratch_mu;
 void* g_scratch_ptr[kScratchDevices] = {};
 
 int nfail(pimc_ctx* c, int code, const std::string& m) {
-    if (c) c->err = m;
+    if (c) {
+        c->err = m;
+        ++c->fail_count;
+    }
     return code;
 }
 

@@ -1236,7 +1236,6 @@ __global__ __launch_bounds__(256) void k_vnn_finalize(VnnArgs A, VnnInline req) 
 // beads' weighted terms in bead order once every slot carries the sequence
 // number.  A lane's partners, and so its sum, are those of the two-position
 // kernels' lanes, and the waves are added in the same order.
-constexpr int kVnnDirectThreads = 2 * kVnnThreads;
 template <int MODE>
 __global__ __launch_bounds__(kVnnDirectThreads) void k_vnn_direct(VnnArgs A, VnnInline req, AzizDev az,
                                                                   VnnDirectOut* out) {
@@ -1637,6 +1636,12 @@ hipError_t launch_vnn_direct(int mode, hipStream_t st, const VnnArgs& A, const V
         hipLaunchKernelGGL(k_vnn_direct<2>, g, b, 0, st, A, req, az, out);
     if (timer && timer->stop) (void)hipEventRecord(timer->stop, st);
     return hipGetLastError();
+}
+
+const void* vnn_direct_stub(int mode) {
+    if (mode == 0) return reinterpret_cast<const void*>(&k_vnn_direct<0>);
+    if (mode == 1) return reinterpret_cast<const void*>(&k_vnn_direct<1>);
+    return reinterpret_cast<const void*>(&k_vnn_direct<2>);
 }
 
 hipError_t launch_vnn(int mode, hipStream_t st, const VnnArgs& A, const VnnInline& req, const AzizDev& az,

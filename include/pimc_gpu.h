@@ -300,6 +300,12 @@ int  pimc_server_submit_ssf(pimc_server* s, int chain, const double* q, int nq, 
 int  pimc_server_wait_ssf(pimc_server* s, uint64_t req_id, double* sq, double* latency_us);
 int  pimc_server_stats(pimc_server* s, int64_t* batches, int64_t* requests, int64_t* pair_evals,
                        size_t* free_bytes);
+/* Delta V batches by how they reached the GPU (no counterpart in the reference):
+ * as AQL packets on the launching slot's own queue, their descriptors written
+ * into device memory through the BAR (batches of up to 64 KB of descriptors),
+ * or as a copy and two hipLaunchKernel calls on the slot's stream (larger
+ * batches, and every batch when PIMC_DIRECT_DISPATCH=0 was set at creation). */
+int  pimc_server_dispatch_stats(pimc_server* s, int64_t* packet_batches, int64_t* hip_batches);
 typedef struct {
     int n_chains;       /* chains 0..n_chains-1 receive the load */
     double rate;        /* Poisson arrival rate [requests/s] over all chains; <= 0: closed loop */

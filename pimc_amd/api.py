@@ -125,6 +125,7 @@ SIGNATURES = {
     "pimc_server_submit_ssf": (C.c_int, [_vp, C.c_int, _dp, C.c_int, _u64p]),
     "pimc_server_wait_ssf": (C.c_int, [_vp, C.c_uint64, _dp, _dp]),
     "pimc_server_stats": (C.c_int, [_vp, _i64p, _i64p, _i64p, C.POINTER(C.c_size_t)]),
+    "pimc_server_dispatch_stats": (C.c_int, [_vp, _i64p, _i64p]),
     "pimc_server_run_synthetic": (C.c_int, [_vp, C.POINTER(_LoadDesc), _dp]),
     "pimc_set_action_factors": (C.c_int, [C.c_void_p, _dp, _dp, C.c_int]),
     "pimc_chain_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -630,6 +631,12 @@ class Server:
         f = C.c_size_t()
         self._check(lib().pimc_server_stats(self.h, C.byref(b), C.byref(r), C.byref(p), C.byref(f)), "stats")
         return dict(batches=b.value, requests=r.value, pair_evals=p.value, free_bytes=f.value)
+
+    def dispatch_stats(self):
+        """Delta V batches sent as AQL packets and through hipLaunchKernel."""
+        pk, hp = C.c_int64(), C.c_int64()
+        self._check(lib().pimc_server_dispatch_stats(self.h, C.byref(pk), C.byref(hp)), "dispatch_stats")
+        return dict(packet_batches=pk.value, hip_batches=hp.value)
 
     def run_synthetic(self, n_chains, rate, duration_s, seg=(4, 64), p_com=0.1, com_sigma=0.1, lam=6.0596,
                       seed=139853, threads=1, p_ssf=0.0, ssf_nmax=4):

@@ -240,46 +240,83 @@ hsa_status_t find_symbol(hsa_executable_t e, void* d) {
 
 }  // namespace
 
-bool direct_ring_create(int device, DirectRing* r) {
+namespace {
+
+// A barrier packet on q completes after every packet before it; waits for it.
+void drain(hsa_queue_t* q, hsa_signal_t sig) {
+    hsa_signal_store_relaxed(sig, 1);
+    const uint64_t idx = hsa_queue_add_write_index_relaxed(q, 1);
+    while (idx - hsa_queue_load_read_index_scacquire(q) >= q->size) _mm_pause();
+    hsa_barrier_and_packet_t* p = static_cast<hsa_barrier_and_packet_t*>(q->base_address) + (idx & (q->size - 1));
+    std::memset(reinterpret_cast<char*>(p) + 4, 0, sizeof(*p) - 4);
+    p->completion_signal = sig;
+    const uint16_t header = (HSA_PACKET_TYPE_BARRIER_AND << HSA_PACKET_HEADER_TYPE) |
+                            (1 << HSA_PACKET_HEADER_BARRIER) |
+                            (HSA_FENCE_SCOPE_SYSTEM << HSA_PACKET_HEADER_SCACQUIRE_FENCE_SCOPE) |
+                            (HSA_FENCE_SCOPE_SYSTEM << HSA_PACKET_HEADER_SCRELEASE_FENCE_SCOPE);
+    __atomic_store_n(reinterpret_cast<uint32_t*>(p), (uint32_t)header, __ATOMIC_RELEASE);
+    hsa_signal_store_screlease(q->doorbell_signal, (hsa_signal_value_t)idx);
+    (void)hsa_signal_wait_scacquire(sig, HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX, HSA_WAIT_STATE_BLOCKED);
+}
+
+void* alloc_visible(DeviceQueue* D, size_t bytes) {
+    void* p = nullptr;
+    if (hsa_amd_memory_pool_allocate(D->vram, bytes, 0, &p) != HSA_STATUS_SUCCESS) return nullptr;
+    if (hsa_amd_agents_allow_access(1, &D->cpu, nullptr, p) != HSA_STATUS_SUCCESS) {
+        (void)hsa_amd_memory_pool_free(p);
+        return nullptr;
+    }
+    return p;
+}
+
+}  // namespace
+
+bool direct_ring_create(int device, DirectRing* r, bool own_queue) {
     DeviceQueue* D = device_queue(device);
     if (!D) return false;
-    void* p = nullptr;
-    if (hsa_amd_memory_pool_allocate(D->vram, (size_t)kDirectSlots * kDirectSlotBytes, 0, &p) != HSA_STATUS_SUCCESS)
-        return false;
     // the host writes the argument blocks through the BAR
-    if (hsa_amd_agents_allow_access(1, &D->cpu, nullptr, p) != HSA_STATUS_SUCCESS) {
+    void* p = alloc_visible(D, (size_t)kDirectSlots * kDirectSlotBytes);
+    if (!p) return false;
+    hsa_queue_t* q = nullptr;
+    if (own_queue && hsa_queue_create(D->gpu, kQueueSize, HSA_QUEUE_TYPE_SINGLE, nullptr, nullptr, UINT32_MAX,
+                                      UINT32_MAX, &q) != HSA_STATUS_SUCCESS) {
         (void)hsa_amd_memory_pool_free(p);
         return false;
     }
     r->base = static_cast<char*>(p);
     r->device = device;
     r->next = 0;
+    r->own_queue = q;
     return true;
 }
 
 void direct_ring_release(DirectRing* r) {
     if (!r->base) return;
     DeviceQueue& D = g_dev[r->device];
-    {
-        // a barrier packet completes after every packet before it
+    if (r->own_queue) {
+        hsa_queue_t* q = static_cast<hsa_queue_t*>(r->own_queue);
+        hsa_signal_t sig;
+        if (hsa_signal_create(0, 0, nullptr, &sig) == HSA_STATUS_SUCCESS) {
+            drain(q, sig);
+            (void)hsa_signal_destroy(sig);
+        }
+        (void)hsa_queue_destroy(q);
+        r->own_queue = nullptr;
+    } else {
         std::lock_guard<std::mutex> lk(D.m);
-        hsa_signal_store_relaxed(D.drain, 1);
-        const uint64_t idx = hsa_queue_add_write_index_relaxed(D.q, 1);
-        while (idx - hsa_queue_load_read_index_scacquire(D.q) >= D.q->size) _mm_pause();
-        hsa_barrier_and_packet_t* p =
-            static_cast<hsa_barrier_and_packet_t*>(D.q->base_address) + (idx & (D.q->size - 1));
-        std::memset(reinterpret_cast<char*>(p) + 4, 0, sizeof(*p) - 4);
-        p->completion_signal = D.drain;
-        const uint16_t header = (HSA_PACKET_TYPE_BARRIER_AND << HSA_PACKET_HEADER_TYPE) |
-                                (1 << HSA_PACKET_HEADER_BARRIER) |
-                                (HSA_FENCE_SCOPE_SYSTEM << HSA_PACKET_HEADER_SCACQUIRE_FENCE_SCOPE) |
-                                (HSA_FENCE_SCOPE_SYSTEM << HSA_PACKET_HEADER_SCRELEASE_FENCE_SCOPE);
-        __atomic_store_n(reinterpret_cast<uint32_t*>(p), (uint32_t)header, __ATOMIC_RELEASE);
-        hsa_signal_store_screlease(D.q->doorbell_signal, (hsa_signal_value_t)idx);
-        (void)hsa_signal_wait_scacquire(D.drain, HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX, HSA_WAIT_STATE_BLOCKED);
+        drain(D.q, D.drain);
     }
     (void)hsa_amd_memory_pool_free(r->base);
     r->base = nullptr;
+}
+
+void* direct_alloc_visible(int device, size_t bytes) {
+    DeviceQueue* D = device_queue(device);
+    return D ? alloc_visible(D, bytes) : nullptr;
+}
+
+void direct_free_visible(void* p) {
+    if (p) (void)hsa_amd_memory_pool_free(p);
 }
 
 bool direct_kernel(int device, const void* host_stub, hipStream_t stream, uint32_t bytes, DirectKernel* k) {
@@ -323,19 +360,25 @@ bool direct_kernel(int device, const void* host_stub, hipStream_t stream, uint32
     return true;
 }
 
-void direct_dispatch(DirectRing* r, const DirectKernel& k, const ArgPiece* pieces, int n_pieces, uint32_t grid_x,
-                     uint32_t block_x) {
-    DeviceQueue& D = g_dev[r->device];
+char* direct_stage(DirectRing* r, const ArgPiece* pieces, int n_pieces) {
     char* slot = r->base + (size_t)(r->next++ % kDirectSlots) * kDirectSlotBytes;
     for (int i = 0; i < n_pieces; ++i) std::memcpy(slot + pieces[i].offset, pieces[i].src, pieces[i].bytes);
+    return slot;
+}
+
+void direct_flush(int device) {
+    DeviceQueue& D = g_dev[device];
     _mm_sfence();
     *D.hdp = 1u;
-    (void)*D.hdp;   // non-posted: the argument writes and the flush have reached the device
-    std::lock_guard<std::mutex> lk(D.m);
-    const uint64_t idx = hsa_queue_add_write_index_relaxed(D.q, 1);
-    while (idx - hsa_queue_load_read_index_scacquire(D.q) >= D.q->size) _mm_pause();
-    hsa_kernel_dispatch_packet_t* p = static_cast<hsa_kernel_dispatch_packet_t*>(D.q->base_address) +
-                                      (idx & (D.q->size - 1));
+    (void)*D.hdp;   // non-posted: the BAR writes and the flush have reached the device
+}
+
+namespace {
+void write_packet(hsa_queue_t* q, const DirectKernel& k, const char* kernargs, uint32_t grid_x, uint32_t block_x) {
+    const uint64_t idx = hsa_queue_add_write_index_relaxed(q, 1);
+    while (idx - hsa_queue_load_read_index_scacquire(q) >= q->size) _mm_pause();
+    hsa_kernel_dispatch_packet_t* p = static_cast<hsa_kernel_dispatch_packet_t*>(q->base_address) +
+                                      (idx & (q->size - 1));
     p->workgroup_size_x = (uint16_t)block_x;
     p->workgroup_size_y = 1;
     p->workgroup_size_z = 1;
@@ -346,19 +389,38 @@ void direct_dispatch(DirectRing* r, const DirectKernel& k, const ArgPiece* piece
     p->private_segment_size = 0;
     p->group_segment_size = k.group_bytes;
     p->kernel_object = k.object;
-    p->kernarg_address = slot;
+    p->kernarg_address = const_cast<char*>(kernargs);
     p->reserved2 = 0;
     p->completion_signal.handle = 0;
-    // agent-scope fences: the kernel publishes its results with system-scope
-    // stores itself, and an agent-scope acquire invalidates the caches that
-    // could hold positions another kernel changed
+    // agent-scope fences: the kernels publish their results with system-scope
+    // stores themselves, and an agent-scope acquire invalidates the caches that
+    // could hold data another kernel changed; the barrier bit keeps a queue's
+    // packets in order
     const uint16_t header = (HSA_PACKET_TYPE_KERNEL_DISPATCH << HSA_PACKET_HEADER_TYPE) |
                             (1 << HSA_PACKET_HEADER_BARRIER) |
                             (HSA_FENCE_SCOPE_AGENT << HSA_PACKET_HEADER_SCACQUIRE_FENCE_SCOPE) |
                             (HSA_FENCE_SCOPE_AGENT << HSA_PACKET_HEADER_SCRELEASE_FENCE_SCOPE);
     const uint16_t setup = 1 << HSA_KERNEL_DISPATCH_PACKET_SETUP_DIMENSIONS;
     __atomic_store_n(reinterpret_cast<uint32_t*>(p), (uint32_t)header | ((uint32_t)setup << 16), __ATOMIC_RELEASE);
-    hsa_signal_store_screlease(D.q->doorbell_signal, (hsa_signal_value_t)idx);
+    hsa_signal_store_screlease(q->doorbell_signal, (hsa_signal_value_t)idx);
+}
+}  // namespace
+
+void direct_submit(DirectRing* r, const DirectKernel& k, const char* kernargs, uint32_t grid_x, uint32_t block_x) {
+    if (r->own_queue) {
+        write_packet(static_cast<hsa_queue_t*>(r->own_queue), k, kernargs, grid_x, block_x);
+        return;
+    }
+    DeviceQueue& D = g_dev[r->device];
+    std::lock_guard<std::mutex> lk(D.m);
+    write_packet(D.q, k, kernargs, grid_x, block_x);
+}
+
+void direct_dispatch(DirectRing* r, const DirectKernel& k, const ArgPiece* pieces, int n_pieces, uint32_t grid_x,
+                     uint32_t block_x) {
+    const char* slot = direct_stage(r, pieces, n_pieces);
+    direct_flush(r->device);
+    direct_submit(r, k, slot, grid_x, block_x);
 }
 
 }  // namespace pimc

@@ -35,6 +35,7 @@ struct DirectRing {
     char* base = nullptr;
     int device = -1;
     unsigned next = 0;
+    void* own_queue = nullptr;   // a queue of the ring's own (in order), or null: the device's shared queue
 };
 
 // A piece of the argument block: bytes [offset, offset + bytes) come from src.
@@ -45,9 +46,17 @@ struct ArgPiece {
 
 // Sets up the device's queue on first use and allocates the ring; false when
 // direct dispatch is unavailable (the caller launches through HIP instead).
-bool direct_ring_create(int device, DirectRing* r);
-// Waits for every packet queued so far on the device's queue, then frees the ring.
+// own_queue: the ring gets an in-order queue of its own, for one thread whose
+// packets depend on each other; otherwise it shares the device's queue, whose
+// packets the caller orders by waiting for each result.
+bool direct_ring_create(int device, DirectRing* r, bool own_queue = false);
+// Waits for every packet queued so far on the ring's queue, then frees the ring
+// (and its own queue).
 void direct_ring_release(DirectRing* r);
+// Device memory the host writes through the BAR (made visible by
+// direct_flush), for data a packet's kernel reads; null when unavailable.
+void* direct_alloc_visible(int device, size_t bytes);
+void direct_free_visible(void* p);
 // Resolves a __global__ function (its host stub) in the code object this
 // library registered; false unless the code object's argument block is exactly
 // 'bytes' long and the kernel needs no scratch.
@@ -56,5 +65,11 @@ bool direct_kernel(int device, const void* host_stub, hipStream_t stream, uint32
 // the device, and queues one dispatch of grid_x workgroups of block_x lanes.
 void direct_dispatch(DirectRing* r, const DirectKernel& k, const ArgPiece* pieces, int n_pieces, uint32_t grid_x,
                      uint32_t block_x);
+// The same in three steps, for several packets behind one flush: copy the
+// pieces into the next slot (returns its address), make every BAR write so far
+// visible to the device, queue the packet.
+char* direct_stage(DirectRing* r, const ArgPiece* pieces, int n_pieces);
+void direct_flush(int device);
+void direct_submit(DirectRing* r, const DirectKernel& k, const char* kernargs, uint32_t grid_x, uint32_t block_x);
 
 }  // namespace pimc

@@ -25,6 +25,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -202,6 +203,10 @@ __global__ void k_serve_cells(const double* __restrict__ pos, int64_t plane, int
 using namespace pimc;
 using Clock = std::chrono::steady_clock;
 constexpr size_t kZeroCopyBytes = 16384;   // batches read in place from mapped host memory up to this size
+// Delta V batches whose descriptors fit this many bytes are written through the
+// BAR into the slot's device block and sent as AQL packets on the slot's own
+// queue (larger ones: one DMA copy and two HIP launches on the slot's stream)
+constexpr size_t kPacketDescBytes = 65536;
 
 namespace {
 
@@ -304,6 +309,9 @@ struct Slot {
     // build_batch after it (requests are immutable until retired, and the chain
     // deque keeps its elements in place)
     std::vector<std::pair<const Request*, const Chain*>> items;
+    DirectRing ring;                  // kernel arguments and the slot's own AQL queue
+    char* v_desc = nullptr;           // descriptor block in device memory, written through the BAR
+    bool packet = false;              // the batch in flight went as packets
 };
 
 // Chain extents readable without the server lock: an append-only table,
@@ -330,6 +338,27 @@ struct BatchLaunch {
     ServeChainDev chain{};
     int nq = 0;
     double inv_n = 0.0, invL = 0.0;
+    bool packet = false;              // descriptors in the slot's device block, kernels as packets
+};
+
+// k_serve_vnn's and k_serve_finalize's argument blocks as the AMDGPU ABI lays
+// them out (each argument at its natural alignment)
+struct ServeVnnArgs {
+    const ServeGroupDev* groups;
+    const ServeReqDev* reqs;
+    const VnnBead* beads;
+    const ServeChainDev* chains;
+    AzizDev az;
+    int chunk;
+    double* partials;
+};
+struct ServeFinArgs {
+    const ServeReqDev* reqs;
+    const VnnBead* beads;
+    const ServeChainDev* chains;
+    const double* partials;
+    ServeOut* out;
+    unsigned long long seq;
 };
 
 }  // namespace
@@ -376,6 +405,13 @@ struct pimc_server {
     std::vector<ServeGroupDev> pack_gr;
 
     int64_t launched_ssf = 0;
+
+    // Delta V batches as AQL packets (PIMC_DIRECT_DISPATCH=0: HIP launches only)
+    int device = 0;
+    bool direct = false;
+    DirectKernel dk_vnn[3], dk_fin;
+    std::atomic<int64_t> packet_batches{0}, hip_batches{0};   // Delta V batches by how they went out
+    bool setup_direct();
 
     void run();
     int pack_batch(int slot, BatchLaunch* L);
@@ -472,21 +508,41 @@ int place_chain(pimc_server* s, int id, const double* beads) {
     return 0;
 }
 
+// A batch has finished when its records (Delta V) or its completion word (S(q))
+// carry its sequence number; its kernels have then read everything they read.
+bool slot_finished(const Slot& S) {
+    if (S.kind == 1) return __atomic_load_n(S.h_done, __ATOMIC_ACQUIRE) == S.seq;
+    for (int k = S.n_reqs - 1; k >= 0; --k)
+        if (__atomic_load_n(&S.h_out[k].seq, __ATOMIC_ACQUIRE) != S.seq) return false;
+    return true;
+}
+
+// Every batch in flight, launched through HIP or sent as packets, is done with
+// the chain table (a packet batch is waited for through its records, for at
+// most 10 s).
+void quiesce_slots(pimc_server* s) {
+    for (auto& sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.busy && sl.packet) {
+            const auto t0 = Clock::now();
+            while (!slot_finished(sl) && Clock::now() - t0 < std::chrono::seconds(10)) __builtin_ia32_pause();
+        }
+    }
+}
+
 int sync_chain_table(pimc_server* s, hipStream_t st) {
     if (!s->chains_dirty) return 0;
     const int n = (int)s->chain_table.size();
     if (n > s->chain_cap) {
         if (s->d_chains) {
-            for (auto& sl : s->slots)
-                if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+            quiesce_slots(s);
             (void)hipFree(s->d_chains);
         }
         s->chain_cap = std::max(64, 2 * n);
         if (hipMalloc(&s->d_chains, sizeof(ServeChainDev) * s->chain_cap) != hipSuccess) return -1;
     }
-    // every slot stream must see the new table before its next batch
-    for (auto& sl : s->slots)
-        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    // every slot must see the new table before its next batch
+    quiesce_slots(s);
     if (hipMemcpy(s->d_chains, s->chain_table.data(), sizeof(ServeChainDev) * n, hipMemcpyHostToDevice) != hipSuccess)
         return -1;
     s->chains_dirty = false;
@@ -545,6 +601,7 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
         S.kind = 1;
         C.busy = true;
         S.busy = true;
+        S.packet = false;   // not fired yet
         S.seq += 1;
         S.t_fire = Clock::now();
         ++launched_batches;
@@ -577,6 +634,7 @@ int pimc_server::pack_batch(int si, BatchLaunch* L) {
     L->slot = si;
     L->chains = d_chains;
     S.busy = true;
+    S.packet = false;   // not fired yet
     S.seq += 1;   // the sequence number this batch publishes when it completes
     S.t_fire = Clock::now();
     ++launched_batches;
@@ -630,14 +688,53 @@ void pimc_server::build_batch(BatchLaunch* L) {
     L->o_gr = (L->o_bd + b_bd + 255) & ~size_t(255);
     L->bytes = L->o_gr + b_gr;
     L->n_groups = (int)gr.size();
-    std::memcpy(S.h_desc, rq.data(), b_rq);
-    std::memcpy(S.h_desc + L->o_bd, bd.data(), b_bd);
-    std::memcpy(S.h_desc + L->o_gr, gr.data(), b_gr);
+    L->packet = direct && S.v_desc && L->bytes <= kPacketDescBytes;
+    char* dst = L->packet ? S.v_desc : S.h_desc;
+    std::memcpy(dst, rq.data(), b_rq);
+    std::memcpy(dst + L->o_bd, bd.data(), b_bd);
+    std::memcpy(dst + L->o_gr, gr.data(), b_gr);
     L->az = az;
     if (desc.v_mode == PIMC_V_TABLE) {
         std::string e;
         L->az.table = cached_table(desc.aziz_year, 0, (int)(max_table_sep / ((1.0E-6) * params.rm)) + 1, &e);
     }
+}
+
+// Kernel objects of k_serve_vnn / k_serve_finalize and, for each slot in use, an
+// argument ring with a queue of its own (the slot's two packets run in order)
+// and a descriptor block in device memory; false (nothing kept) when any part
+// is unavailable, and every batch then launches through HIP.
+bool pimc_server::setup_direct() {
+    if (hipGetDevice(&device) != hipSuccess) return false;
+    static_assert(sizeof(ServeVnnArgs) == 272 && offsetof(ServeVnnArgs, az) == 32 &&
+                      offsetof(ServeVnnArgs, chunk) == 256 && offsetof(ServeVnnArgs, partials) == 264 &&
+                      sizeof(ServeFinArgs) == 48,
+                  "serve argument blocks");
+    const void* vnn[3] = {reinterpret_cast<const void*>(&k_serve_vnn<0>),
+                          reinterpret_cast<const void*>(&k_serve_vnn<1>),
+                          reinterpret_cast<const void*>(&k_serve_vnn<2>)};
+    bool ok = true;
+    for (int m = 0; m < 3 && ok; ++m)
+        ok = direct_kernel(device, vnn[m], slots[0].stream, (uint32_t)sizeof(ServeVnnArgs), &dk_vnn[m]);
+    ok = ok && direct_kernel(device, reinterpret_cast<const void*>(&k_serve_finalize), slots[0].stream,
+                             (uint32_t)sizeof(ServeFinArgs), &dk_fin);
+    for (int k = 0; k < n_slots && ok; ++k) {
+        ok = direct_ring_create(device, &slots[k].ring, true);
+        if (ok) {
+            slots[k].v_desc = static_cast<char*>(direct_alloc_visible(device, kPacketDescBytes));
+            ok = slots[k].v_desc != nullptr;
+        }
+    }
+    if (!ok) {
+        for (auto& S : slots) {
+            direct_ring_release(&S.ring);
+            direct_free_visible(S.v_desc);
+            S.v_desc = nullptr;
+        }
+        return false;
+    }
+    direct = true;
+    return true;
 }
 
 // Copies a packed batch to the device and launches it on its slot's stream.
@@ -646,7 +743,26 @@ void pimc_server::build_batch(BatchLaunch* L) {
 hipError_t pimc_server::fire_batch(const BatchLaunch& L) {
     Slot& S = slots[L.slot];
     (void)hipGetLastError();
+    S.packet = L.kind == 0 && L.packet;
     if (L.kind == 1) return fire_ssf(L);
+    if (L.packet) {
+        // both kernels' arguments into the slot's ring, one flush for them and
+        // the descriptors, two packets in order on the slot's queue
+        const ServeReqDev* d_rq = (const ServeReqDev*)S.v_desc;
+        const VnnBead* d_bd = (const VnnBead*)(S.v_desc + L.o_bd);
+        const ServeGroupDev* d_gr = (const ServeGroupDev*)(S.v_desc + L.o_gr);
+        const ServeVnnArgs va{d_gr, d_rq, d_bd, L.chains, L.az, kVnnChunk, S.d_partials};
+        const ServeFinArgs fa{d_rq, d_bd, L.chains, S.d_partials, S.d_out, S.seq};
+        const ArgPiece pv{&va, 0, (uint32_t)sizeof(va)}, pf{&fa, 0, (uint32_t)sizeof(fa)};
+        const char* kv = direct_stage(&S.ring, &pv, 1);
+        const char* kf = direct_stage(&S.ring, &pf, 1);
+        direct_flush(device);
+        direct_submit(&S.ring, dk_vnn[desc.v_mode], kv, (uint32_t)L.n_groups, (uint32_t)kVnnThreads);
+        direct_submit(&S.ring, dk_fin, kf, (uint32_t)L.n_reqs, 64u);
+        packet_batches.fetch_add(1, std::memory_order_relaxed);
+        return hipSuccess;
+    }
+    hip_batches.fetch_add(1, std::memory_order_relaxed);
     // A small batch's descriptors are read in place from mapped host memory: a
     // copy would put a DMA start in front of the kernels.  Larger batches go
     // over in one copy, since every workgroup re-reads its descriptors.
@@ -746,14 +862,7 @@ void pimc_server::complete(Slot& S) {
 // lock released; takes the lock only to retire results and to pack requests.
 void pimc_server::run() {
     std::unique_lock<std::mutex> lk(mu, std::defer_lock);
-    // a batch has finished when its records (Delta V) or its completion word
-    // (S(q)) carry its sequence number
-    auto finished = [](const Slot& S) {
-        if (S.kind == 1) return __atomic_load_n(S.h_done, __ATOMIC_ACQUIRE) == S.seq;
-        for (int k = S.n_reqs - 1; k >= 0; --k)
-            if (__atomic_load_n(&S.h_out[k].seq, __ATOMIC_ACQUIRE) != S.seq) return false;
-        return true;
-    };
+    auto finished = [](const Slot& S) { return slot_finished(S); };
     while (!stop.load()) {
         int done_k[kSlots];
         hipError_t done_e[kSlots];
@@ -769,6 +878,12 @@ void pimc_server::run() {
             if (finished(S)) {
                 done_k[nd] = k;
                 done_e[nd++] = hipSuccess;
+            } else if (S.packet) {
+                // packets carry no stream: a batch 10 s late has failed
+                if (Clock::now() - S.t_fire > std::chrono::seconds(10) && !finished(S)) {
+                    done_k[nd] = k;
+                    done_e[nd++] = hipErrorLaunchTimeOut;
+                }
             } else if (Clock::now() - S.t_fire > std::chrono::milliseconds(200)) {
                 // a batch this late may have failed: ask the stream
                 const hipError_t q = hipStreamQuery(S.stream);
@@ -882,6 +997,9 @@ int pimc_server_create(const pimc_server_desc* d, pimc_server** out) {
             return PIMC_E_NOMEM;
         if (hipHostGetDevicePointer((void**)&S.m_sq, S.h_sq, 0) != hipSuccess) return PIMC_E_HIP;
     }
+    // PIMC_DIRECT_DISPATCH=0: every batch launches through HIP
+    const char* dd = std::getenv("PIMC_DIRECT_DISPATCH");
+    if (!(dd && std::strcmp(dd, "0") == 0)) (void)s->setup_direct();
     pimc_server* raw = s.get();
     raw->worker = std::thread([raw] { raw->run(); });
     *out = s.release();
@@ -894,6 +1012,8 @@ void pimc_server_destroy(pimc_server* s) {
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     for (auto& S : s->slots) {
+        direct_ring_release(&S.ring);   // waits for the slot's packets
+        direct_free_visible(S.v_desc);
         if (S.stream) (void)hipStreamSynchronize(S.stream);
         if (S.ev) (void)hipEventDestroy(S.ev);
         if (S.h_desc) (void)hipHostFree(S.h_desc);
@@ -1191,6 +1311,13 @@ int pimc_server_stats(pimc_server* s, int64_t* batches, int64_t* requests, int64
     if (requests) *requests = s->launched_reqs;
     if (pairs) *pairs = s->pairs_done;
     if (free_bytes) *free_bytes = s->arena.free_bytes();
+    return PIMC_OK;
+}
+
+int pimc_server_dispatch_stats(pimc_server* s, int64_t* packet_batches, int64_t* hip_batches) {
+    if (!s) return PIMC_E_ARG;
+    if (packet_batches) *packet_batches = s->packet_batches.load(std::memory_order_relaxed);
+    if (hip_batches) *hip_batches = s->hip_batches.load(std::memory_order_relaxed);
     return PIMC_OK;
 }
 

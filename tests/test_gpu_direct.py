@@ -138,3 +138,81 @@ def test_packets_with_timing_and_many_contexts(orc, gpu):
         assert np.array_equal(a.download_path(), b.download_path())
         a.close()
         b.close()
+
+
+def _server(gpu, budget, packets, **kw):
+    old = os.environ.get("PIMC_DIRECT_DISPATCH")
+    try:
+        if packets:
+            os.environ.pop("PIMC_DIRECT_DISPATCH", None)
+        else:
+            os.environ["PIMC_DIRECT_DISPATCH"] = "0"
+        return gpu.Server(budget, **kw)
+    finally:
+        if old is None:
+            os.environ.pop("PIMC_DIRECT_DISPATCH", None)
+        else:
+            os.environ["PIMC_DIRECT_DISPATCH"] = old
+
+
+def test_server_packets_match_hip_launches(orc, gpu):
+    """The serving form with its Delta V batches as AQL packets (descriptors in
+    device memory) against a server launching every batch through HIP: the same
+    requests -- one at a time, then 60 at once in mixed batches, commits and
+    Metropolis decisions among them, batches of more than 64 KB of descriptors
+    going through HIP in the packet server too -- give bit-identical results and
+    leave bit-identical chains."""
+    specs = [(32, 40), (64, 32), (128, 24), (200, 16), (256, 64), (96, 48)] + [(64, 64)] * 18
+    chains = []
+    for k, (N, M) in enumerate(specs):
+        L = orc.box_side(N, DENSITY)
+        tau = 1.0 / (M * 1.5)
+        b = orc.gen_worldlines(SEED0 + 80 + k, L, M, N, LAMBDA, tau, 0)
+        chains.append((L, M, N, tau, b))
+    srv = {p: _server(gpu, 1 << 28, p, streams=4, max_side=60.0) for p in (True, False)}
+    ids = {p: [s.add_chain(L, M, N, tau, LAMBDA, L, b) for (L, M, N, tau, b) in chains] for p, s in srv.items()}
+    rng = np.random.default_rng(3)
+
+    def request(k, nb):
+        L, M, N, tau, b = chains[k]
+        start, p = int(rng.integers(M)), int(rng.integers(N))
+        sp = np.array([[(start + j) % M, p] for j in range(1, nb + 1)], dtype=np.int32)
+        new = b[sp[:, 0], sp[:, 1]] + rng.normal(scale=0.3, size=(nb, 3))
+        new -= L * np.floor(new / L + 0.5)
+        mode = int(rng.integers(3))
+        return k, sp, new, mode, float(rng.uniform())
+
+    # one at a time: every batch holds one request
+    for _ in range(30):
+        k, sp, new, mode, u = request(int(rng.integers(len(chains))), int(rng.integers(3, 16)))
+        out = {p: s.wait(s.submit(ids[p][k], sp, new, commit_mode=mode, u=u)) for p, s in srv.items()}
+        assert np.array_equal(out[True]["raw"][:4], out[False]["raw"][:4]), (k, mode)
+    # sixty at once, segments up to the longest slice count (large batches among them)
+    reqs = [request(int(rng.integers(len(chains))), int(rng.integers(4, 16))) for _ in range(60)]
+    for p, s in srv.items():
+        rid = [s.submit(ids[p][k], sp, new, commit_mode=mode, u=u) for (k, sp, new, mode, u) in reqs]
+        reqs_out = [s.wait(r)["raw"][:4].copy() for r in rid]
+        srv[p] = (s, reqs_out)
+    for a, b in zip(srv[True][1], srv[False][1]):
+        assert np.array_equal(a, b)
+    # long segments on every chain at once (a batch holds one request per chain):
+    # 24 requests of up to 63 beads pass 64 KB of descriptors
+    big = []
+    for rep in range(4):
+        for k in range(len(chains)):
+            M = chains[k][1]
+            big.append(request(k, min(M - 1, 63)))
+    res = {}
+    for p in (True, False):
+        s = srv[p][0]
+        rid = [s.submit(ids[p][k], sp, new, commit_mode=0) for (k, sp, new, mode, u) in big]
+        res[p] = [s.wait(r)["raw"][:3].copy() for r in rid]
+    for a, b in zip(res[True], res[False]):
+        assert np.array_equal(a, b)
+    for k in range(len(chains)):
+        assert np.array_equal(srv[True][0].download(ids[True][k]), srv[False][0].download(ids[False][k]))
+    dp, dh = srv[True][0].dispatch_stats(), srv[False][0].dispatch_stats()
+    assert dp["packet_batches"] >= 30 and dh["packet_batches"] == 0 and dh["hip_batches"] >= 30
+    print("dispatch", dp, dh)
+    for p in (True, False):
+        srv[p][0].close()

@@ -1,14 +1,17 @@
 // pimc_direct.hpp -- kernel dispatch straight into a user-mode AQL queue, for the
-// launches whose latency a Markov chain waits on (short Delta V requests).
+// launches whose latency a Markov chain waits on (short Delta V requests of a
+// context, the serving form's Delta V batches).
 //
 // hipLaunchKernel spends ~3 us of host time per launch here and its tail reaches
 // 7-11 us; a dispatch packet written into an HSA queue of our own, with the
 // kernel arguments in device memory written through the PCIe BAR, costs ~1.9 us
 // of host time (most of it the read-back that orders the argument writes before
-// the doorbell) with a flat tail.  Every call is synchronous: the caller waits
-// for the kernel's published results before it returns, and the context's HIP
-// stream is idle when the packet is written, so the two queues never hold work
-// of one context at the same time.
+// the doorbell) with a flat tail.  A context's call is synchronous: it waits for
+// the kernel's published results before it returns, and the context's HIP
+// stream is quiet when the packet is written, so the two queues never run work
+// of one context at the same time.  The server sends a batch's two kernels as
+// packets on a queue of the slot's own, in order, and retires the batch from
+// its published records.
 #pragma once
 #include <hip/hip_runtime.h>
 

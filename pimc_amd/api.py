@@ -207,6 +207,7 @@ class Context:
         self._out = np.zeros(4)         # reused result buffer of the per-move calls
         self._out_ptr = self._out.ctypes.data
         self._fn_dv = lib().pimc_delta_action
+        self._fn_ta = lib().pimc_total_action
 
     def close(self):
         if getattr(self, "h", None):
@@ -271,6 +272,12 @@ class Context:
 
     def potentialAction(self, shift=1, vfactor=None, with_slices=False):
         """vfactor None: the context's VFactor (set_action_factors; {1, 1} by default)."""
+        if vfactor is None and not with_slices:
+            # the per-step call: one ctypes call into the reused result buffer
+            rc = self._fn_ta(self.h, shift, None, self._out_ptr, None)
+            if rc != 0:
+                self._check(rc, "total_action")
+            return float(self._out[0])
         vf = None if vfactor is None else _f64(vfactor)
         vs = np.zeros((self.M, 2)) if with_slices else None
         self._check(lib().pimc_total_action(self.h, shift, None if vf is None else _ptr(vf), self._out_ptr,

@@ -2,10 +2,14 @@
 //
 // One HSA queue per device, shared by the library's contexts and created on
 // first use; each context owns a ring of kernel-argument blocks in device
-// memory.  A dispatch copies the arguments through the PCIe BAR, flushes the
-// host data path (HDP) and reads the flush register back -- a non-posted read
-// that returns only after the writes before it have reached the device -- then
-// writes the packet body, its header last, and rings the doorbell.
+// memory (a serving slot's ring comes with a queue of its own, so the slot's
+// packets run in order without holding up the others).  A dispatch copies the
+// arguments through the PCIe BAR, flushes the host data path (HDP) and reads
+// the flush register back -- a non-posted read that returns only after the
+// writes before it have reached the device -- then writes the packet body, its
+// header last, and rings the doorbell.  The packet's agent-scope acquire
+// invalidates the scalar cache and the non-coherent cache lines, so a slot
+// rewritten through the BAR is read afresh.
 #include "pimc_direct.hpp"
 
 #include <dlfcn.h>

@@ -50,17 +50,18 @@ def _requests(rng, beads, L, M, N, tau, n_req, segs=(8, 16, 32)):
 
 @pytest.mark.parametrize("mode", [0, 2])
 def test_packets_match_hip_launches(orc, gpu, mode):
-    """120 Bisection requests of 7/15/31 beads on N=256 x 167 (every third one
+    """400 Bisection requests of 7/15/31 beads on N=256 x 167 (every third one
     committed), a total action after each commit: packets and HIP launches agree
-    bit for bit, the packet context sent every request as a packet, and the
-    paths and cell tables end identical."""
+    bit for bit (each of the 64 argument slots is rewritten through the BAR six
+    times, with other requests' contents), the packet context sent every request
+    as a packet, and the paths and cell tables end identical."""
     N, M, tau = 256, 167, 0.004
     L = orc.box_side(N, DENSITY)
     beads = orc.gen_worldlines(SEED0 + 21, L, M, N, LAMBDA, tau, 0)
     rc = L / 2 if mode == 0 else L
     a = _context(gpu, L, M, N, tau, rc, mode, beads, packets=True)
     b = _context(gpu, L, M, N, tau, rc, mode, beads, packets=False)
-    reqs = _requests(np.random.default_rng(5), beads, L, M, N, tau, 120)
+    reqs = _requests(np.random.default_rng(5), beads, L, M, N, tau, 400)
     for sp, new, commit in reqs:
         oa = a.delta_action(sp, new, commit=commit)
         ob = b.delta_action(sp, new, commit=commit)

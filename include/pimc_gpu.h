@@ -457,6 +457,13 @@ int pimc_set_timing_kinds(pimc_ctx* ctx, unsigned kinds);
  * through hipLaunchKernel instead. */
 int pimc_kernel_time(pimc_ctx* ctx, const char* name, double* total_ms, long* count);
 int pimc_device_synchronize(void);
+/* Page-locked host memory for large results (no counterpart in the reference,
+ * whose estimators write into their own arrays): F(q,tau) over 100 401 q and 250
+ * lags is 200 MB, which the device copies into page-locked memory at the full
+ * link rate but into pageable memory through staging buffers at a fraction of
+ * it.  Any entry point's output pointer may point into such a block. */
+int pimc_host_alloc(size_t bytes, void** ptr);
+int pimc_host_free(void* ptr);
 /* A plain device-to-device streaming copy of `bytes` run `iters` times (no
  * reference counterpart): a known-size kernel against which the HBM byte
  * counters of the hot kernels are scaled.  *avg_ms receives the mean duration. */

@@ -100,6 +100,8 @@ SIGNATURES = {
     "pimc_set_timing_kinds": (C.c_int, [C.c_void_p, C.c_uint]),
     "pimc_kernel_time": (C.c_int, [C.c_void_p, C.c_char_p, _dp, C.POINTER(C.c_long)]),
     "pimc_device_synchronize": (C.c_int, []),
+    "pimc_host_alloc": (C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    "pimc_host_free": (C.c_int, [C.c_void_p]),
     "pimc_copy_probe": (C.c_int, [C.c_size_t, C.c_int, _dp]),
     "pimc_gen_worldlines": (C.c_int, [C.c_uint32, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                                       _dp]),
@@ -405,9 +407,16 @@ class Context:
         self._check(lib().pimc_ssf_noalloc(self.h, _p(q, _dp), q.shape[0], _p(o, _dp)), "ssf_noalloc")
         return o
 
-    def isf(self, q):
+    def isf(self, q, out=None):
+        """F(q, tau) [nq][M]; out: a C-contiguous float64 array of that shape to
+        write into (pinned_empty gives one in page-locked memory)."""
         q = _f64(q, (-1, 3))
-        o = np.empty((q.shape[0], self.M))   # every row is written
+        if out is None:
+            o = np.empty((q.shape[0], self.M))   # every row is written
+        else:
+            o = out
+            if o.dtype != np.float64 or o.shape != (q.shape[0], self.M) or not o.flags.c_contiguous:
+                raise PimcError(f"isf: out must be a C-contiguous float64 array of shape ({q.shape[0]}, {self.M})")
         self._check(lib().pimc_isf(self.h, _p(q, _dp), q.shape[0], _p(o, _dp)), "isf")
         return o
 
@@ -855,6 +864,34 @@ def copy_probe(nbytes, iters=20):
     if rc_ != 0:
         raise PimcError(f"copy probe failed ({rc_})")
     return float(ms[0])
+
+
+class _PinnedBlock:
+    """Page-locked host memory from pimc_host_alloc, freed with the last array
+    viewing it."""
+
+    def __init__(self, nbytes):
+        p = C.c_void_p()
+        rc = lib().pimc_host_alloc(nbytes, C.byref(p))
+        if rc != 0:
+            raise PimcError(f"pimc_host_alloc: {lib().pimc_last_error(None).decode()} ({rc})")
+        self.ptr, self.nbytes = p.value, nbytes
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().pimc_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float64):
+    """An uninitialised array in page-locked host memory (for large results such
+    as F(q, tau)); the memory is released with the last array viewing it."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) if np.ndim(shape) else int(shape)
+    blk = _PinnedBlock(max(1, n) * dt.itemsize)
+    buf = (C.c_char * blk.nbytes).from_address(blk.ptr)
+    buf._pimc_block = blk          # the ctypes buffer keeps the block alive
+    return np.frombuffer(buf, dtype=dt, count=n).reshape(shape)
 
 
 def device_synchronize():

@@ -1638,4 +1638,22 @@ int pimc_device_synchronize(void) {
     return PIMC_OK;
 }
 
+int pimc_host_alloc(size_t bytes, void** ptr) {
+    if (!ptr || bytes == 0) return fail(nullptr, PIMC_E_ARG, "pimc_host_alloc: null pointer or zero bytes");
+    *ptr = nullptr;
+    hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return fail(nullptr, PIMC_E_NOMEM, std::string("pimc_host_alloc: ") + hipGetErrorString(e));
+    }
+    return PIMC_OK;
+}
+
+int pimc_host_free(void* ptr) {
+    if (!ptr) return PIMC_OK;
+    hipError_t e = hipHostFree(ptr);
+    if (e != hipSuccess) return fail(nullptr, PIMC_E_HIP, std::string("pimc_host_free: ") + hipGetErrorString(e));
+    return PIMC_OK;
+}
+
 }  // extern "C"

@@ -418,6 +418,11 @@ def test_isf_small(orc, gpu):
     fg, fr = ctx.isf(q), ref.isf(q)
     assert np.all(np.abs(fg - fr) <= 1e-5 * np.maximum(np.abs(fr), 16 * 20 * 1e-3))
     assert np.allclose(fg[:, 1:], fg[:, :0:-1], rtol=1e-12, atol=1e-9)
+    # into a page-locked result array: the same bits; a wrong shape is refused
+    out = gpu.pinned_empty((q.shape[0], 20))
+    assert ctx.isf(q, out=out) is out and np.array_equal(out, fg)
+    with pytest.raises(gpu.PimcError):
+        ctx.isf(q, out=gpu.pinned_empty((q.shape[0], 19)))
 
 
 @pytest.mark.parametrize("mode", [0, 1, 2])

@@ -268,15 +268,21 @@ static const char* kKernelNames[] = {"pair_action", "delta_action", "rho", "ssf"
 
 constexpr int kTimingSlots = 2048;
 
+// The event pool is created when timing is first switched on (not inside the
+// first timed call, where its ~0.5 ms would count against that call).
+bool pimc_ctx::ensure_event_pool() {
+    if (!ev_pool.empty()) return true;
+    ev_pool.assign(2 * kTimingSlots, nullptr);
+    for (auto& e : ev_pool)
+        if (hipEventCreate(&e) != hipSuccess) return false;
+    return true;
+}
+
 int pimc_ctx::timed_slot(int k) {
     if (!timing || !((timing_kinds >> k) & 1u)) return -1;
-    if (ev_pool.empty()) {
-        ev_pool.assign(2 * kTimingSlots, nullptr);
-        for (auto& e : ev_pool)
-            if (hipEventCreate(&e) != hipSuccess) {
-                timing = false;
-                return -1;
-            }
+    if (!ensure_event_pool()) {
+        timing = false;
+        return -1;
     }
     if (ev_next == kTimingSlots) resolve_timing();
     const int slot = ev_next++;
@@ -1604,6 +1610,7 @@ int pimc_set_timing(pimc_ctx* c, int enable) {
     c->resolve_timing();
     c->timing = enable != 0;
     c->timing_kinds = ~0u;
+    if (c->timing && !c->ensure_event_pool()) return fail(c, PIMC_E_HIP, "hipEventCreate failed");
     return PIMC_OK;
 }
 
@@ -1612,6 +1619,7 @@ int pimc_set_timing_kinds(pimc_ctx* c, unsigned kinds) {
     c->resolve_timing();
     c->timing = kinds != 0;
     c->timing_kinds = kinds;
+    if (c->timing && !c->ensure_event_pool()) return fail(c, PIMC_E_HIP, "hipEventCreate failed");
     return PIMC_OK;
 }
 

@@ -169,6 +169,7 @@ struct pimc_ctx {
     double kernel_ms[kKinds] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long kernel_count[kKinds] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     int timed_slot(int kind);          // -1 when timing is off
+    bool ensure_event_pool();
     void timed_begin(int kind);
     void timed_end();
     pimc::KernelTimer kernel_timer(int kind);   // events for one kernel's own duration

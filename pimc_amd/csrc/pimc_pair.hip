@@ -1182,19 +1182,43 @@ __device__ __forceinline__ void vnn_finalize_body(const VnnArgs& A, const VnnInl
         // gradVnnSquared = sum of the chunks' three-body terms + |F1|^2
         double cor[2] = {0.0, 0.0};
         const double gvf = (slice & 1) ? A.gvf1 : A.gvf0;
-        if (A.gnn_part && (A.shift == 1 || A.cor_any_shift) && gvf > kCorrEps)
-            for (int w = 0; w < 2; ++w) {
-                const double* g = A.gnn_part + ((int64_t)k * 2 + w) * A.gnn_chunks * 4;
-                double t = 0.0, f1x = 0.0, f1y = 0.0, f1z = 0.0;
-                for (int cc = 0; cc < A.gnn_chunks; ++cc) {
-                    t += g[4 * cc + 0];
-                    f1x += g[4 * cc + 1];
-                    f1y += g[4 * cc + 2];
-                    f1z += g[4 * cc + 3];
+        if (A.gnn_part && (A.shift == 1 || A.cor_any_shift) && gvf > kCorrEps) {
+            // both positions' chunk terms, in chunk order per sum; four chunks'
+            // loads are issued together before their additions
+            const double* g0 = A.gnn_part + ((int64_t)k * 2 + 0) * A.gnn_chunks * 4;
+            const double* g1 = g0 + (int64_t)A.gnn_chunks * 4;
+            double t[2] = {0.0, 0.0}, fx[2] = {0.0, 0.0}, fy[2] = {0.0, 0.0}, fz[2] = {0.0, 0.0};
+            int cc = 0;
+            for (; cc + 4 <= A.gnn_chunks; cc += 4) {
+                double v0[16], v1[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    v0[u] = g0[4 * cc + u];
+                    v1[u] = g1[4 * cc + u];
                 }
-                t += norm2_exact(f1x, f1y, f1z);
-                cor[w] = gvf * tau_s * tau_s * tau_s * A.lambda * t;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    t[0] += v0[4 * u + 0];
+                    fx[0] += v0[4 * u + 1];
+                    fy[0] += v0[4 * u + 2];
+                    fz[0] += v0[4 * u + 3];
+                    t[1] += v1[4 * u + 0];
+                    fx[1] += v1[4 * u + 1];
+                    fy[1] += v1[4 * u + 2];
+                    fz[1] += v1[4 * u + 3];
+                }
             }
+            for (; cc < A.gnn_chunks; ++cc)
+                for (int w = 0; w < 2; ++w) {
+                    const double* g = w ? g1 : g0;
+                    t[w] += g[4 * cc + 0];
+                    fx[w] += g[4 * cc + 1];
+                    fy[w] += g[4 * cc + 2];
+                    fz[w] += g[4 * cc + 3];
+                }
+            for (int w = 0; w < 2; ++w)
+                cor[w] = gvf * tau_s * tau_s * tau_s * A.lambda * (t[w] + norm2_exact(fx[w], fy[w], fz[w]));
+        }
         if (A.per_bead_cor) {
             A.per_bead_cor[2 * k + 0] = cor[0];
             A.per_bead_cor[2 * k + 1] = cor[1];

@@ -64,3 +64,19 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in line["error"]
+
+
+def test_argument_checks_without_a_device():
+    """Entry points that refuse bad arguments before touching the device do so
+    here too: page-locked allocation of zero bytes or into a null pointer, and
+    the server's dispatch counters of a null server."""
+    import ctypes
+    from pimc_amd import api
+    E_ARG = -1   # PIMC_E_ARG (include/pimc_gpu.h)
+    lib = api.lib()
+    p = ctypes.c_void_p()
+    assert lib.pimc_host_alloc(0, ctypes.byref(p)) == E_ARG and not p.value
+    assert lib.pimc_host_alloc(64, None) == E_ARG
+    assert lib.pimc_host_free(None) == 0
+    a = ctypes.c_int64(-1)
+    assert lib.pimc_server_dispatch_stats(None, ctypes.byref(a), None) == E_ARG and a.value == -1

@@ -384,6 +384,18 @@ def test_worm_factors(orc, gpu):
     assert ctx.potentialAction() == pytest.approx(ref.total_action(), rel=1e-12)
     sp = np.array([[1, 2], [3, 0], [4, 4]], dtype=np.int32)
     assert ctx.action_list(sp) == pytest.approx(ref.action_list(sp), rel=1e-12)
+    # Delta V of the three beads (distinct slices, the short-request kernel) with
+    # the worm's pair factors, then committed
+    new = beads[sp[:, 0], sp[:, 1]] + np.random.default_rng(4).normal(scale=0.2, size=(3, 3))
+    new -= L * np.floor(new / L + 0.5)
+    a_old = ref.action_list(sp)
+    for (m, i), q in zip(sp, new):
+        ref.update_bead(int(m), int(i), q)
+    a_new = ref.action_list(sp)
+    out = ctx.delta_action(sp, new, commit=True)
+    scale = abs(a_old) + abs(a_new)
+    assert abs(out[0] - a_old) <= 1e-12 * scale and abs(out[1] - a_new) <= 1e-12 * scale
+    assert ctx.potentialAction() == pytest.approx(ref.total_action(), rel=1e-12)
 
 
 def test_energy_and_sep_hist(orc, gpu):

@@ -25,6 +25,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
 F_PAIR = 50          # algorithmic FP32 flops per bead-pair evaluation
+# Delta V reads each partner once for the old and the new position (two pair
+# evaluations): FP64 x, y, z, its on-byte and its cell number
+B_PARTNER = 3 * 8 + 1 + 4
 TIMING_STRIDE = 8    # plain leg: the pair kernel of every 8th timed step carries HIP events
 F_QB = 9             # per (q, bead) phasor
 F_QMT = 4            # per (q, m, tau) lag product
@@ -186,6 +189,9 @@ def main():
         "delta_v": {str(s): {"pairs": pairs_dv[s], "p50_us": pct(lat[s], 50), "p99_us": pct(lat[s], 99)}
                     for s in segs},
         "delta_v_pair_kernel_avg_us": dv_ms / max(dv_n, 1) * 1e3,
+        # latency-bound: the partner bytes the kernel must read against its duration
+        "delta_v_hbm_gb_per_s_algorithmic": (sum(pairs_dv.values()) / 2 * B_PARTNER)
+                                            / (dv_ms / max(dv_n, 1) * 1e-3 * len(segs)) / 1e9,
         "isf": {"n_q": int(q.shape[0]), "lags": M, "ms": isf_s * 1e3, "ms_pageable_result": isf_pageable_s * 1e3,
                 "rho_kernel_ms": rho_ms / max(rho_n, 1), "lag_kernel_ms": isf_ms / max(isf_n, 1),
                 "q_bead_phasors_per_s": q.shape[0] * M * N / isf_s,
@@ -549,6 +555,7 @@ def served_leg(pimc_amd, plain_dv_rate, seconds):
         st = srv.run_synthetic(1024, rate=rate, duration_s=seconds, threads=threads, p_ssf=p_ssf, ssf_nmax=4)
         d = {k: float(v) for k, v in st.items()}
         d["vs_plain_delta_v_cxx"] = d["pair_evals_per_s"] / plain_dv_rate
+        d["hbm_gb_per_s_algorithmic"] = d["pair_evals_per_s"] / 2 * B_PARTNER / 1e9
         out["rates"][f"{int(rate)}"] = d
     # closed loop (every chain always has a request in flight): the server's
     # throughput limit, with more client threads since submitting is the limit there
@@ -557,6 +564,7 @@ def served_leg(pimc_amd, plain_dv_rate, seconds):
     out["closed_loop"] = {k: float(v) for k, v in st.items()}
     out["closed_loop"]["client_threads"] = closed_threads
     out["closed_loop"]["vs_plain_delta_v_cxx"] = out["closed_loop"]["pair_evals_per_s"] / plain_dv_rate
+    out["closed_loop"]["hbm_gb_per_s_algorithmic"] = out["closed_loop"]["pair_evals_per_s"] / 2 * B_PARTNER / 1e9
     top = out["rates"]["200000"]
     out["pair_evals_per_s"] = top["pair_evals_per_s"]
     out["vs_plain_delta_v"] = top["pair_evals_per_s"] / plain_dv_rate

@@ -1125,9 +1125,9 @@ int pimc_server_download(pimc_server* s, int chain, double* beads) {
 }  // extern "C"
 
 // Queues a built request behind its chain's earlier ones (the chain must not
-// have been removed); the worker sees it at once when the chain is idle.
-static int server_enqueue(pimc_server* s, Request&& R, uint64_t* req_id) {
-    std::lock_guard<std::mutex> g(s->mu);
+// have been removed); the worker sees it at once when the chain is idle.  The
+// server lock is held by the caller.
+static int enqueue_locked(pimc_server* s, Request&& R, uint64_t* req_id) {
     if (s->kernel_error) return sfail(s, s->kernel_error, s->kernel_error_msg);
     const int chain = R.chain;
     if (chain < 0 || chain >= (int)s->chains.size() || s->chains[chain].state == 2)
@@ -1140,14 +1140,36 @@ static int server_enqueue(pimc_server* s, Request&& R, uint64_t* req_id) {
     c.queue.push_back(id);
     if (c.queue.size() == 1 && c.state == 1 && !c.busy) s->ready.push_back(id);
     *req_id = id;
+    return PIMC_OK;
+}
+
+// The submit counter moves under the lock: the worker reads it under the lock
+// before it blocks, so a bump cannot fall between its check and its wait.
+static int server_enqueue(pimc_server* s, Request&& R, uint64_t* req_id) {
+    std::lock_guard<std::mutex> g(s->mu);
+    const int r = enqueue_locked(s, std::move(R), req_id);
+    if (r) return r;
     s->work_events.fetch_add(1, std::memory_order_release);
     s->cv_work.notify_one();
     return PIMC_OK;
 }
 
-static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos, int n,
-                         int shift, const double vfactor[2], int commit_mode, double u, int tag, uint64_t* req_id) {
-    if (!s || !slice_ptcl || !new_pos || n < 1 || !req_id || shift < 1 || commit_mode < 0 || commit_mode > 2)
+// Several built requests under one acquisition of the server lock (a client
+// holding many chains resubmits after each completion list); ids[k] receives
+// request k's id.  Stops at the first refused request and returns its code.
+static int server_enqueue_many(pimc_server* s, std::vector<Request>& rs, uint64_t* ids) {
+    int r = PIMC_OK;
+    std::lock_guard<std::mutex> g(s->mu);
+    for (size_t k = 0; k < rs.size() && !r; ++k) r = enqueue_locked(s, std::move(rs[k]), &ids[k]);
+    s->work_events.fetch_add(1, std::memory_order_release);
+    s->cv_work.notify_one();
+    return r;
+}
+
+// Checks and builds a Delta V request (without the server lock).
+static int server_build(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos, int n,
+                        int shift, const double vfactor[2], int commit_mode, double u, int tag, Request* out) {
+    if (!s || !slice_ptcl || !new_pos || n < 1 || !out || shift < 1 || commit_mode < 0 || commit_mode > 2)
         return PIMC_E_ARG;
     if (n > kMaxBeadsPerBatch / 4) return sfail(s, PIMC_E_ARG, "too many beads in one request");
     // the chain's extents from the lock-free table (the state and a kernel error
@@ -1194,12 +1216,22 @@ static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, c
         }
     }
     R.tag = tag;
+    *out = std::move(R);
+    return PIMC_OK;
+}
+
+static int server_submit(pimc_server* s, int chain, const int32_t* slice_ptcl, const double* new_pos, int n,
+                         int shift, const double vfactor[2], int commit_mode, double u, int tag, uint64_t* req_id) {
+    if (!req_id) return PIMC_E_ARG;
+    Request R;
+    const int r = server_build(s, chain, slice_ptcl, new_pos, n, shift, vfactor, commit_mode, u, tag, &R);
+    if (r) return r;
     return server_enqueue(s, std::move(R), req_id);
 }
 
 // S(q) of a chain's current worldlines for the lattice vectors qn [nq][3]
 // (q = 2 pi n / L), queued behind the chain's earlier requests.
-static int server_submit_ssf(pimc_server* s, int chain, std::vector<int>&& qn, int tag, uint64_t* req_id) {
+static int server_build_ssf(pimc_server* s, int chain, std::vector<int>&& qn, int tag, Request* out) {
     const int nq = (int)qn.size() / 3;
     if (nq < 1 || nq > kMaxSsfQ) return sfail(s, PIMC_E_ARG, "S(q) requests take 1..65536 q-vectors");
     for (int v : qn)
@@ -1209,6 +1241,14 @@ static int server_submit_ssf(pimc_server* s, int chain, std::vector<int>&& qn, i
     R.chain = chain;
     R.qn = std::move(qn);
     R.tag = tag;
+    *out = std::move(R);
+    return PIMC_OK;
+}
+
+static int server_submit_ssf(pimc_server* s, int chain, std::vector<int>&& qn, int tag, uint64_t* req_id) {
+    Request R;
+    const int r = server_build_ssf(s, chain, std::move(qn), tag, &R);
+    if (r) return r;
     return server_enqueue(s, std::move(R), req_id);
 }
 
@@ -1405,28 +1445,46 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
             }
         }
     };
-    auto submit = [&](int c, Clock::time_point t_arr) -> int {
-        GenChain& g = gc[c];
-        uint64_t id = 0;
-        int r;
-        g.inflight_ssf = !ssf_qn.empty() && U(rng) < L->p_ssf;
-        if (g.inflight_ssf) {
-            r = server_submit_ssf(s, c, std::vector<int>(ssf_qn), t, &id);
-        } else {
-            propose(c);
-            const double vf[2] = {1.0, 1.0};
-            r = server_submit(s, c, g.sp.data(), g.prop.data(), (int)g.sp.size() / 2, 1, vf, 1, U(rng), t, &id);
+    // a pass's new requests are built without the lock and queued under one
+    // acquisition of it
+    std::vector<std::pair<int, Clock::time_point>> batch;
+    std::vector<Request> built;
+    std::vector<uint64_t> ids;
+    auto submit_batch = [&]() -> int {
+        built.resize(batch.size());
+        for (size_t k = 0; k < batch.size(); ++k) {
+            const int c = batch[k].first;
+            GenChain& g = gc[c];
+            int r;
+            g.inflight_ssf = !ssf_qn.empty() && U(rng) < L->p_ssf;
+            if (g.inflight_ssf) {
+                r = server_build_ssf(s, c, std::vector<int>(ssf_qn), t, &built[k]);
+            } else {
+                propose(c);
+                const double vf[2] = {1.0, 1.0};
+                r = server_build(s, c, g.sp.data(), g.prop.data(), (int)g.sp.size() / 2, 1, vf, 1, U(rng), t,
+                                 &built[k]);
+            }
+            if (r) return r;
         }
+        ids.assign(batch.size(), 0);
+        const int r = server_enqueue_many(s, built, ids.data());
         if (r) return r;
-        g.inflight = id;
-        g.t_arrival = t_arr;
-        by_id[id] = c;
-        ++inflight;
+        for (size_t k = 0; k < batch.size(); ++k) {
+            GenChain& g = gc[batch[k].first];
+            g.inflight = ids[k];
+            g.t_arrival = batch[k].second;
+            by_id[ids[k]] = batch[k].first;
+            ++inflight;
+        }
+        batch.clear();
+        built.clear();
         return 0;
     };
-    if (closed)
-        for (int c : mine)
-            if ((st.err = submit(c, Clock::now()))) return;
+    if (closed) {
+        for (int c : mine) batch.emplace_back(c, Clock::now());
+        if ((st.err = submit_batch())) return;
+    }
     std::vector<uint64_t> finished;
     std::vector<double> outs;
     uint64_t seen_done = 0;
@@ -1487,8 +1545,7 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
             g.inflight = 0;
             --inflight;
             if (!closed && !g.arrivals.empty()) runnable.push_back(c);
-            if (closed && open)
-                if ((st.err = submit(c, Clock::now()))) return;
+            if (closed && open) batch.emplace_back(c, Clock::now());
         }
         while (!runnable.empty()) {
             const int c = runnable.front();
@@ -1497,9 +1554,11 @@ void gen_thread(pimc_server* s, const pimc_load_desc* L, int t, int T, std::vect
             if (!g.inflight && !g.arrivals.empty()) {
                 const auto ta = g.arrivals.front();
                 g.arrivals.pop_front();
-                if ((st.err = submit(c, ta))) return;
+                batch.emplace_back(c, ta);
+                g.inflight = ~0ull;   // claimed by this pass's batch
             }
         }
+        if (!batch.empty() && (st.err = submit_batch())) return;
         if (finished.empty()) std::this_thread::yield();
         if (!open && inflight == 0) break;
         if (!open && now - t_end > std::chrono::seconds(30)) {

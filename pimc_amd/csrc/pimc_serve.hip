@@ -902,7 +902,9 @@ void pimc_server::run() {
                 continue;
             }
             // idle: spin on the submit counter for up to 2 ms (a sleeping
-            // worker costs every request a thread wake-up), then block
+            // worker costs every request a thread wake-up), then block; the
+            // counter is read every 8 pauses (every 64 took ~0.4 us longer to
+            // see a submit)
             const auto t0 = Clock::now();
             bool woke = false;
             while (!stop.load(std::memory_order_relaxed) && Clock::now() - t0 < std::chrono::milliseconds(2)) {
@@ -910,7 +912,7 @@ void pimc_server::run() {
                     woke = true;
                     break;
                 }
-                for (int p = 0; p < 64; ++p) __builtin_ia32_pause();
+                for (int p = 0; p < 8; ++p) __builtin_ia32_pause();
             }
             if (!woke) {
                 lk.lock();

@@ -389,6 +389,12 @@ bool pimc_ctx::stream_quiet() {
 
 pimc_ctx::~pimc_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (copy_stream) {
+        (void)hipStreamSynchronize(copy_stream);
+        (void)hipStreamDestroy(copy_stream);
+    }
+    for (auto& e : chunk_ev)
+        if (e) (void)hipEventDestroy(e);
     pimc::direct_ring_release(&dring);
     resolve_timing();
     for (auto& e : ev_pool)
@@ -1629,6 +1635,11 @@ int pimc_kernel_time(pimc_ctx* c, const char* name, double* total_ms, long* coun
     if (std::strcmp(name, "delta_packets") == 0) {
         if (total_ms) *total_ms = 0.0;
         if (count) *count = c->direct_count;
+        return PIMC_OK;
+    }
+    if (std::strcmp(name, "isf_chunks") == 0) {
+        if (total_ms) *total_ms = 0.0;
+        if (count) *count = c->isf_chunks;
         return PIMC_OK;
     }
     for (int k = 0; k < pimc_ctx::kKinds; ++k)

@@ -52,8 +52,8 @@ int true_num_particles(pimc_ctx* c) {
 }
 
 // rho_q(m) for a group of q-vectors into c->d_rho ([M][nq][2])
-int compute_rho(pimc_ctx* c, bool lattice, const std::vector<int>& qn, const std::vector<double>& qv, int nq) {
-    const int M = c->desc.num_slices;
+// Room for nq q-vectors in both device q lists.
+int ensure_q(pimc_ctx* c, int nq) {
     int r;
     size_t capq = (size_t)c->cap_q;
     if ((size_t)nq * 3 > capq) {
@@ -62,21 +62,37 @@ int compute_rho(pimc_ctx* c, bool lattice, const std::vector<int>& qn, const std
         if ((r = grow(c, &c->d_qn, &cn, (size_t)nq * 3))) return r;
         c->cap_q = (int)std::min(cq, cn);
     }
-    if ((r = grow(c, &c->d_rho, &c->cap_rho, (size_t)M * nq * 2))) return r;
+    return PIMC_OK;
+}
+
+// The density-mode arguments of the context's worldlines for nq q-vectors,
+// modes into c->d_rho; for lattice q, qn (host, nq x 3) sets the phasor ranges.
+RhoArgs rho_args(pimc_ctx* c, int nq, const int* qn) {
     RhoArgs A{};
     A.pos = c->d_pos;
     A.plane = c->plane;
     A.nalloc = c->desc.num_alloc;
-    A.num_slices = M;
+    A.num_slices = c->desc.num_slices;
     A.nbs = c->d_nbs;
     A.invL = 1.0 / c->desc.side;
     A.nq = nq;
     A.rho = c->d_rho;
-    if (lattice) {
+    if (qn) {
         int nm[3] = {0, 0, 0};
         for (int k = 0; k < nq; ++k)
             for (int d = 0; d < 3; ++d) nm[d] = std::max(nm[d], std::abs(qn[3 * k + d]));
         for (int d = 0; d < 3; ++d) A.nmax[d] = nm[d];
+    }
+    return A;
+}
+
+int compute_rho(pimc_ctx* c, bool lattice, const std::vector<int>& qn, const std::vector<double>& qv, int nq) {
+    const int M = c->desc.num_slices;
+    int r;
+    if ((r = ensure_q(c, nq))) return r;
+    if ((r = grow(c, &c->d_rho, &c->cap_rho, (size_t)M * nq * 2))) return r;
+    RhoArgs A = rho_args(c, nq, lattice ? qn.data() : nullptr);
+    if (lattice) {
         SHIP(c, hipMemcpyAsync(c->d_qn, qn.data(), sizeof(int) * 3 * nq, hipMemcpyHostToDevice, c->stream));
         A.qn = c->d_qn;
     } else {
@@ -86,6 +102,70 @@ int compute_rho(pimc_ctx* c, bool lattice, const std::vector<int>& qn, const std
     c->timed_begin(2);
     SHIP(c, launch_rho(lattice, c->stream, A));
     c->timed_end();
+    return PIMC_OK;
+}
+
+// F(q, tau) of a long lattice q list written straight into page-locked memory:
+// the list runs in chunks, each chunk's density modes and lags on the context's
+// stream, and each chunk's rows are copied out on a second stream while the
+// next chunk computes (100 401 q x 250 lags make a 200 MB result whose copy
+// takes ~6 ms after ~18 ms of kernels).  A q's modes and lags come from the same
+// expressions whatever q share its launch, so the values are those of the
+// one-launch path bit for bit.
+// q per chunk: at 100 401 q x 250 lags, chunks of 34 000 / 25 600 / 18 432 q
+// took 22.0 / 22.2 / 22.6 ms against 23.9 ms in one launch (interleaved), and
+// 14 400 (4 q per lane in k_rho_lattice) 28.5 ms; chunks on two alternating
+// streams with a mode buffer each were no faster (22.2 / 22.5 / 22.8 ms)
+constexpr int kIsfChunkQ = 34000;                  // 8 q per lane in k_rho_lattice from M = 121 on
+constexpr size_t kIsfChunkMinBytes = 64u << 20;    // smaller results: one launch per kernel
+constexpr int kIsfChunksMax = 64;
+
+bool isf_chunks_pay(pimc_ctx* c, int n, int M, const double* fqt) {
+    if (n <= kIsfChunkQ || (size_t)n * M * sizeof(double) < kIsfChunkMinBytes) return false;
+    // a timed call keeps one launch per kernel, so its durations stay per call
+    if (c->timing && (c->timing_kinds & ((1u << 2) | (1u << 4)))) return false;
+    // a pageable destination is copied through staging with the host waiting
+    hipPointerAttribute_t at{};
+    const hipError_t e = hipPointerGetAttributes(&at, fqt);
+    (void)hipGetLastError();   // a pageable pointer may report an error here
+    return e == hipSuccess && at.type == hipMemoryTypeHost;
+}
+
+int isf_lattice_chunked(pimc_ctx* c, const std::vector<int>& qn, int n, int M, double inv_n, double* fqt) {
+    const int per = std::max(kIsfChunkQ, (n + kIsfChunksMax - 1) / kIsfChunksMax);
+    const int chunks = (n + per - 1) / per;
+    int r;
+    if ((r = ensure_q(c, n))) return r;
+    if ((r = grow(c, &c->d_rho, &c->cap_rho, (size_t)M * per * 2))) return r;
+    if ((r = grow(c, &c->d_fqt, &c->cap_fqt, (size_t)n * M))) return r;
+    if (!c->copy_stream) SHIP(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+    while ((int)c->chunk_ev.size() < chunks) {
+        hipEvent_t ev = nullptr;
+        SHIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        c->chunk_ev.push_back(ev);
+    }
+    SHIP(c, hipMemcpyAsync(c->d_qn, qn.data(), sizeof(int) * 3 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < chunks && e == hipSuccess; ++k) {
+        const int off = k * per, nc = std::min(per, n - off);
+        RhoArgs A = rho_args(c, nc, qn.data() + 3 * (size_t)off);
+        A.qn = c->d_qn + 3 * (size_t)off;
+        double* rows = c->d_fqt + (size_t)off * M;
+        // the chunk's modes overwrite d_rho after the previous chunk's lags (one stream)
+        e = launch_rho(true, c->stream, A);
+        if (e == hipSuccess) e = launch_isf(c->stream, c->d_rho, nc, M, inv_n, rows, 0);
+        if (e == hipSuccess) e = hipEventRecord(c->chunk_ev[k], c->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c->copy_stream, c->chunk_ev[k], 0);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(fqt + (size_t)off * M, rows, sizeof(double) * (size_t)nc * M, hipMemcpyDeviceToHost,
+                               c->copy_stream);
+        if (e == hipSuccess) ++c->isf_chunks;
+    }
+    // both streams end quiet, after an error too (no copy may land later)
+    const hipError_t e1 = hipStreamSynchronize(c->copy_stream);
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = e1 != hipSuccess ? e1 : e2;
+    if (e != hipSuccess) return sfail(c, PIMC_E_HIP, std::string("F(q,tau) in chunks: ") + hipGetErrorString(e));
     return PIMC_OK;
 }
 
@@ -192,6 +272,10 @@ int pimc_isf(pimc_ctx* c, const double* q, int nq, double* fqt) {
         if (idx.empty()) continue;
         const int n = (int)idx.size();
         int r;
+        if (pass == 0 && n == nq && isf_chunks_pay(c, n, M, fqt)) {
+            if ((r = isf_lattice_chunked(c, qn, n, M, inv_n, fqt))) return r;
+            continue;
+        }
         if ((r = compute_rho(c, pass == 0, qn, qg, n))) return r;
         if ((r = grow(c, &c->d_fqt, &c->cap_fqt, (size_t)n * M))) return r;
         c->timed_begin(4);

@@ -142,6 +142,11 @@ struct pimc_ctx {
     size_t cap_rho = 0;
     double* d_fqt = nullptr;
     size_t cap_fqt = 0;
+    // F(q,tau) of long lattice q lists into page-locked memory: a second stream
+    // copies each chunk's rows out while the next chunk computes
+    hipStream_t copy_stream = nullptr;
+    std::vector<hipEvent_t> chunk_ev;
+    long isf_chunks = 0;               // chunks computed that way (pimc_kernel_time "isf_chunks")
     // staging
     double* h_stage = nullptr;    // >= 3*M*N doubles, pinned
     double* h_out = nullptr;      // 64 doubles, pinned

@@ -44,11 +44,25 @@ def _case(orc, N, M, kind, qmax):
     print(f"N={N} M={M}: {q.shape[0]} q, |n_d| <= {np.abs(nvec).max()}", flush=True)
     s_gpu = ctx.ssf(q)
     f_gpu = ctx.isf(q)
+    extra = {}
+    if N == 2048:
+        # into page-locked memory: the chunked form (modes and lags per chunk of
+        # q, each chunk's rows copied out while the next computes), twice in a
+        # row; then with the rho kernel timed, which keeps one launch per kernel
+        f_pin = [pimc_amd.pinned_empty((q.shape[0], M)) for _ in range(3)]
+        ctx.isf(q, out=f_pin[0])
+        ctx.isf(q, out=f_pin[1])
+        extra["chunks"] = ctx.kernel_time("isf_chunks")[1]
+        ctx.set_timing(True, kinds=["rho"])
+        ctx.isf(q, out=f_pin[2])
+        ctx.set_timing(False)
+        extra["chunks_timed"] = ctx.kernel_time("isf_chunks")[1] - extra["chunks"]
+        extra["f_pin"] = [np.array(f) for f in f_pin]
     ctx.close()
     rho = lattice_rho(beads, nvec, L)
     f_ref = isf_from_rho(rho, N)
     return dict(L=L, beads=beads, q=q, n=nvec, s_gpu=s_gpu, f_gpu=f_gpu, s_ref=f_ref[:, 0].copy(), f_ref=f_ref,
-                N=N, M=M)
+                N=N, M=M, **extra)
 
 
 @pytest.fixture(scope="module")
@@ -111,3 +125,13 @@ def test_ssf_n2048_m250_oracle_pair_sum(orc, large):
     print(f"S(0,0,28): gpu {c['s_gpu'][k]:.10g} oracle {s_orc:.10g} fp64 phasor {c['s_ref'][k]:.10g}, err {err:.2e}")
     assert abs(c["s_ref"][k] - s_orc) <= 1e-10 * s_orc
     assert err <= TOL
+
+
+def test_isf_n2048_m250_chunked_into_pinned(large):
+    """F(q,tau) of the 100 401 q into page-locked memory takes the chunked form
+    (3 chunks of at most 34 000 q, the last one ragged): bit-identical to the one-launch
+    result, on two calls in a row, and not taken while the rho kernel is timed."""
+    assert large["chunks"] == 2 * 3
+    assert large["chunks_timed"] == 0
+    for f in large["f_pin"]:
+        assert np.array_equal(f, large["f_gpu"])

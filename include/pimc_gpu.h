@@ -141,9 +141,12 @@ int pimc_path_version(const pimc_ctx* ctx, uint64_t* version);
  * weights its transverse part with |gradV|, as the reference does.  window is
  * virialWindow (>= 1, the reference's default 5); next_links as in pimc_energy
  * (NULL: periodic closed worldlines); every active bead must link to an active
- * bead (PIMC_E_ARG otherwise).  FP64 in every mode (PIMC_V_MIXED evaluates dV/dr
- * and d2V/dr2 in closed form like PIMC_V_FP64; PIMC_V_TABLE gathers lookupdVdr
- * and lookupd2Vdr2). */
+ * bead (PIMC_E_ARG otherwise).  dV/dr and d2V/dr2 at the reference's bin:
+ * PIMC_V_MIXED with FP32 exponentials of split exponents (1e-6 relative),
+ * PIMC_V_FP64 in FP64 closed form, PIMC_V_TABLE gathered from lookupdVdr and
+ * lookupd2Vdr2; sums in FP64.  The links of the last call stay on the device:
+ * a call with the same links on the same beads per slice neither checks nor
+ * uploads them again. */
 int pimc_virial_terms(pimc_ctx* ctx, const int32_t* next_links, int window, double* out6);
 /* VirialEnergyEstimator::accumulate (src/estimator.cpp:1086-1251): the 19 values
  * one call adds, in the reference's column order

@@ -944,10 +944,16 @@ static int run_virial(pimc_ctx* c, const int32_t* next_links, int window, std::v
         if (!az.dtable || !az.d2table) return fail(c, PIMC_E_NOMEM, err);
     }
     // links: next as given, prev as its inverse; every active bead must link to
-    // an active bead (closed worldlines, the estimator's diagonal sector)
+    // an active bead (closed worldlines, the estimator's diagonal sector).  The
+    // last call's links stay on the device: the same links on the same beads per
+    // slice are neither checked nor uploaded again.
+    const size_t n_links = 2 * (size_t)M * N;
+    const bool same_links = next_links && c->links_ok && c->links_nbs == c->h_nbs &&
+                            std::memcmp(c->links_next.data(), next_links, sizeof(int32_t) * n_links) == 0;
     std::vector<int32_t> prev;
-    if (next_links) {
-        prev.assign(2 * (size_t)M * N, -1);
+    if (next_links && !same_links) {
+        c->links_ok = false;
+        prev.assign(n_links, -1);
         for (int m = 0; m < M; ++m)
             for (int i = 0; i < c->h_nbs[m]; ++i) {
                 const size_t t = (size_t)m * N + i;
@@ -961,13 +967,23 @@ static int run_virial(pimc_ctx* c, const int32_t* next_links, int window, std::v
             }
     }
     int* d_links = nullptr;
-    if (next_links) {
-        if ((r = ctx_dalloc(c, &d_links, 4 * (size_t)M * N))) return r;
-        HIP_OR_FAIL(c, hipMemcpyAsync(d_links, next_links, sizeof(int) * 2 * (size_t)M * N, hipMemcpyHostToDevice,
+    if (next_links && !same_links) {
+        if (c->cap_links < 2 * n_links) {
+            free_dev(c, c->d_links);
+            c->d_links = nullptr;
+            c->cap_links = 0;
+            if ((r = ctx_dalloc(c, &c->d_links, 2 * n_links))) return r;
+            c->cap_links = 2 * n_links;
+        }
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->d_links, next_links, sizeof(int) * n_links, hipMemcpyHostToDevice,
                                       c->stream));
-        HIP_OR_FAIL(c, hipMemcpyAsync(d_links + 2 * (size_t)M * N, prev.data(), sizeof(int) * 2 * (size_t)M * N,
+        HIP_OR_FAIL(c, hipMemcpyAsync(c->d_links + n_links, prev.data(), sizeof(int) * n_links,
                                       hipMemcpyHostToDevice, c->stream));
+        c->links_next.assign(next_links, next_links + n_links);
+        c->links_nbs = c->h_nbs;
+        c->links_ok = true;
     }
+    if (next_links) d_links = c->d_links;
     const int B = (N + 255) / 256;
     const size_t need = (size_t)M * B * kVirialTerms;
     if (c->cap_virial < need) {
@@ -981,7 +997,7 @@ static int run_virial(pimc_ctx* c, const int32_t* next_links, int window, std::v
     A.num_slices = M;
     A.nbs = c->d_nbs;
     A.next = d_links;
-    A.prev = d_links ? d_links + 2 * (size_t)M * N : nullptr;
+    A.prev = d_links ? d_links + n_links : nullptr;
     A.window = window;
     A.t_even = c->gvf[0] > kCorrEps;
     A.t_odd = c->gvf[1] > kCorrEps;
@@ -1008,11 +1024,6 @@ static int run_virial(pimc_ctx* c, const int32_t* next_links, int window, std::v
     HIP_OR_FAIL(c, hipMemcpyAsync(part.data(), c->d_virial, sizeof(double) * need, hipMemcpyDeviceToHost,
                                   c->stream));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-    if (d_links) {
-        (void)hipFree(d_links);
-        c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)d_links),
-                            c->dev_allocs.end());
-    }
     S.assign((size_t)M * kVirialTerms, 0.0);
     for (int m = 0; m < M; ++m)
         for (int b = 0; b < blocks; ++b)

@@ -130,6 +130,14 @@ struct pimc_ctx {
     double* d_vf = nullptr;             // virial chunk partials [M][chunks][N][9]
     size_t cap_vf = 0;
     size_t cap_virial = 0;
+    // the virial estimator's links (next, then prev) on the device, and the next
+    // links and beads per slice they were checked against: a call with the same
+    // links on the same beads uploads nothing
+    int* d_links = nullptr;
+    size_t cap_links = 0;
+    std::vector<int32_t> links_next;
+    std::vector<int> links_nbs;
+    bool links_ok = false;
     double* d_per_bead_cor = nullptr;   // per-bead corrections [cap_req][2]
     double* h_per_bead_cor = nullptr;
     double* d_vnn_partials = nullptr;

@@ -98,3 +98,31 @@ def test_virial_rejects_open_links(orc, gpu):
         ctx.virial_terms(2, next_links=bad)
     with pytest.raises(gpu.PimcError):
         ctx.virial_terms(0)
+
+
+def test_virial_links_change_between_calls(orc, gpu):
+    """The previous call's links stay on the device: a call with other links,
+    or with the first links again after a refused call, uses the links it is
+    given (bit-identical to a context that saw only those links)."""
+    M, N = 24, 32
+    _, ctx, swap = make(orc, gpu, N, M, 0.015, 2, GSF, swap=True)
+    per = np.zeros((M, N, 2), dtype=np.int32)
+    for m in range(M):
+        per[m, :, 0] = (m + 1) % M
+        per[m, :, 1] = np.arange(N)
+    a = ctx.virial_terms(3, next_links=swap)
+    a_again = ctx.virial_terms(3, next_links=swap)
+    b = ctx.virial_terms(3, next_links=per)
+    bad = swap.copy()
+    bad[2, 3] = (3, 4)
+    with pytest.raises(gpu.PimcError):
+        ctx.virial_terms(3, next_links=bad)
+    a_after = ctx.virial_terms(3, next_links=swap)
+    _, only_swap, _ = make(orc, gpu, N, M, 0.015, 2, GSF)
+    _, only_per, _ = make(orc, gpu, N, M, 0.015, 2, GSF)
+    a_ref = only_swap.virial_terms(3, next_links=swap)
+    b_ref = only_per.virial_terms(3, next_links=per)
+    assert not np.array_equal(a_ref, b_ref)   # the exchange changes the link-window terms
+    for x in (a, a_again, a_after):
+        assert np.array_equal(x, a_ref)
+    assert np.array_equal(b, b_ref)

@@ -118,7 +118,9 @@ int pimc_potential_slices(pimc_ctx* ctx, double* v_slices, int32_t* sep_hist);
 /* EnergyEstimator::accumulate (src/estimator.cpp:940-1029) with the context's
  * action factors: the nine columns in the reference's order (:922)
  *   out9 = {K, V, V_ext, V_int, E, E_mu, K/N, V/N, E/N},  E_mu = E - mu N.
- * next_links [M][N_alloc][2] may be NULL (periodic closed worldlines). */
+ * next_links [M][N_alloc][2] may be NULL (periodic closed worldlines); the
+ * links of the last call stay on the device, and the same links again are
+ * neither checked nor uploaded. */
 int pimc_energy(pimc_ctx* ctx, const int32_t* next_links, double mu, double* out9);
 /* LocalAction::gradVSquared(slice) (src/action.cpp:1188-1223) for every slice:
  * g2 [M] = sum over active beads i of |sum_{j != i} gradV(r_i - r_j)|^2, the

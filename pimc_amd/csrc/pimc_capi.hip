@@ -1221,26 +1221,36 @@ int pimc_energy(pimc_ctx* c, const int32_t* next_links, double mu, double* out9)
     if (r) return r;
     int* d_next = nullptr;
     if (next_links) {
-        for (size_t t = 0; t < (size_t)M * N; ++t) {
-            const int a = next_links[2 * t], b = next_links[2 * t + 1];
-            const bool null_link = (a == -1 && b == -1);
-            if (!null_link && (a < 0 || a >= M || b < 0 || b >= N))
-                return fail(c, PIMC_E_ARG, "next link out of range");
+        // the last call's links stay on the device: the same links again are
+        // neither checked nor uploaded
+        const size_t nl = 2 * (size_t)M * N;
+        const bool same = c->enext_ok && std::memcmp(c->enext.data(), next_links, sizeof(int32_t) * nl) == 0;
+        if (!same) {
+            c->enext_ok = false;
+            for (size_t t = 0; t < (size_t)M * N; ++t) {
+                const int a = next_links[2 * t], b = next_links[2 * t + 1];
+                const bool null_link = (a == -1 && b == -1);
+                if (!null_link && (a < 0 || a >= M || b < 0 || b >= N))
+                    return fail(c, PIMC_E_ARG, "next link out of range");
+            }
+            if (c->cap_enext < nl) {
+                free_dev(c, c->d_enext);
+                c->d_enext = nullptr;
+                c->cap_enext = 0;
+                if ((r = ctx_dalloc(c, &c->d_enext, nl))) return r;
+                c->cap_enext = nl;
+            }
+            HIP_OR_FAIL(c, hipMemcpyAsync(c->d_enext, next_links, sizeof(int) * nl, hipMemcpyHostToDevice, c->stream));
+            c->enext.assign(next_links, next_links + nl);
+            c->enext_ok = true;
         }
-        if ((r = ctx_dalloc(c, &d_next, 2 * (size_t)M * N))) return r;
-        HIP_OR_FAIL(c, hipMemcpyAsync(d_next, next_links, sizeof(int) * 2 * (size_t)M * N, hipMemcpyHostToDevice,
-                                      c->stream));
+        d_next = c->d_enext;
     }
     HIP_OR_FAIL(c, launch_kinetic(c->stream, c->d_pos, c->plane, N, M, c->d_nbs, d_next, c->box, c->d_kin_partials,
                                   1024, c->d_out + 8));
     HIP_OR_FAIL(c, hipMemcpyAsync(c->h_out, c->d_out + 8, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     const double link_sum = c->h_out[0];
-    if (d_next) {
-        (void)hipFree(d_next);
-        c->dev_allocs.erase(std::remove(c->dev_allocs.begin(), c->dev_allocs.end(), (void*)d_next),
-                            c->dev_allocs.end());
-    }
     // EnergyEstimator::accumulate (src/estimator.cpp:940-1029) with the context's action factors
     long on = 0;
     for (auto v : c->h_on) on += v;

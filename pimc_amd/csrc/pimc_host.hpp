@@ -138,6 +138,12 @@ struct pimc_ctx {
     std::vector<int32_t> links_next;
     std::vector<int> links_nbs;
     bool links_ok = false;
+    // the energy estimator's next links on the device and the links they were
+    // checked against (the same links again upload nothing)
+    int* d_enext = nullptr;
+    size_t cap_enext = 0;
+    std::vector<int32_t> enext;
+    bool enext_ok = false;
     double* d_per_bead_cor = nullptr;   // per-bead corrections [cap_req][2]
     double* h_per_bead_cor = nullptr;
     double* d_vnn_partials = nullptr;

@@ -65,3 +65,24 @@ def test_worm_state_scores(orc, gpu):
     # the virial estimator needs closed worldlines
     with pytest.raises(gpu.PimcError):
         ctx.virial_energy(3, next_links=p["next"])
+
+
+def test_energy_links_change_between_calls(orc, gpu):
+    """The energy estimator keeps the last call's links on the device: the same
+    links again give the same bits, other links (two worldlines exchanged
+    across the last link) the values of a context that saw only those, and the
+    first links after them the first values again."""
+    ctx, ref, p = loaded(orc, gpu, 12, 16, 16, 0, 1)
+    nxt = np.array(p["next"], dtype=np.int32, copy=True)
+    M = nxt.shape[0]
+    swapped = nxt.copy()
+    swapped[M - 1, 0], swapped[M - 1, 1] = nxt[M - 1, 1].copy(), nxt[M - 1, 0].copy()
+    e1 = ctx.energy(next_links=nxt)
+    e1b = ctx.energy(next_links=nxt)
+    e2 = ctx.energy(next_links=swapped)
+    e1c = ctx.energy(next_links=nxt)
+    other, _, _ = loaded(orc, gpu, 12, 16, 16, 0, 1)
+    e2_ref = other.energy(next_links=swapped)
+    assert e1 == e1b == e1c
+    assert e2 == e2_ref
+    assert e1 != e2   # the exchange changes the kinetic link sum

@@ -216,3 +216,25 @@ def test_path_version(orc, gpu):
         w = ctx.path_version()
         assert w > v, k
         v = w
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("N,M,rc_frac", [(24, 32, 1.0), (300, 16, 0.25)])
+def test_delta_one_launch_matches_two(orc, gpu, mode, N, M, rc_frac):
+    """With the gradient correction an untimed Delta V runs k_vnn's and
+    k_gradvnn's workgroups in one launch, a timed one launches them apart (each
+    kernel keeps its own duration): the same bits either way."""
+    _, ctx, beads, L = pair(orc, gpu, N, M, 0.01, rc_frac, mode, GSF)
+    rng = np.random.default_rng(9)
+    for trial in range(3):
+        seg = int(rng.integers(3, 9))
+        start, p = int(rng.integers(M)), int(rng.integers(N))
+        sp = np.array([[(start + k) % M, p] for k in range(seg)], dtype=np.int32)
+        new = beads[sp[:, 0], sp[:, 1]] + rng.normal(scale=0.3, size=(seg, 3))
+        new -= L * np.floor(new / L + 0.5)
+        one = np.asarray(ctx.delta_action(sp, new))
+        ctx.set_timing(True, kinds=["delta_action", "gradvnn"])
+        two = np.asarray(ctx.delta_action(sp, new))
+        ctx.set_timing(False)
+        assert np.array_equal(one, two), (trial, one, two)
+        assert ctx.kernel_time("gradvnn")[1] == trial + 1

@@ -460,12 +460,13 @@ int pimc_set_timing_kinds(pimc_ctx* ctx, unsigned kinds);
  * or per-bead outputs, while timing of "delta_action" is off.  The environment
  * variable PIMC_DIRECT_DISPATCH=0, read when a context is created, sends them
  * through hipLaunchKernel instead.  "isf_chunks" counts the q chunks of
- * pimc_isf calls that took the chunked form (total_ms 0): lattice q only,
- * results of 64 MB and more into page-locked memory (pimc_host_alloc), while
- * timing of "rho" and "isf" is off; each chunk of 34 000 q (more when the list
- * exceeds 64 chunks) computes its density modes and lags while the previous
- * chunk's rows are copied out on a second stream, with the same values as one
- * launch per kernel. */
+ * pimc_isf calls that took the chunked form (total_ms 0): lattice q only, at
+ * least 2048 of them, results of 4 MB and more into page-locked memory
+ * (pimc_host_alloc), while timing of "rho" and "isf" is off; lists of up to
+ * 68 000 q run as two halves, longer ones in chunks of 34 000 q (more when the
+ * list exceeds 64 chunks); each chunk computes its density modes and lags
+ * while the previous chunk's rows are copied out on a second stream, with the
+ * same values as one launch per kernel. */
 int pimc_kernel_time(pimc_ctx* ctx, const char* name, double* total_ms, long* count);
 int pimc_device_synchronize(void);
 /* Page-locked host memory for large results (no counterpart in the reference,

@@ -117,11 +117,18 @@ int compute_rho(pimc_ctx* c, bool lattice, const std::vector<int>& qn, const std
 // 14 400 (4 q per lane in k_rho_lattice) 28.5 ms; chunks on two alternating
 // streams with a mode buffer each were no faster (22.2 / 22.5 / 22.8 ms)
 constexpr int kIsfChunkQ = 34000;                  // 8 q per lane in k_rho_lattice from M = 121 on
-constexpr size_t kIsfChunkMinBytes = 64u << 20;    // smaller results: one launch per kernel
+constexpr int kIsfChunkMinQ = 1024;                // shorter lists: one launch per kernel
+constexpr size_t kIsfChunkMinBytes = 4u << 20;     // smaller results: one launch per kernel
 constexpr int kIsfChunksMax = 64;
 
+// q per chunk: lists of up to two chunks of kIsfChunkQ run as two halves
+// (5377 q x 167 lags: 7.2 MB, whose copy is half the one-launch wall)
+int isf_chunk_q(int n) {
+    return n > 2 * kIsfChunkQ ? std::max(kIsfChunkQ, (n + kIsfChunksMax - 1) / kIsfChunksMax) : (n + 1) / 2;
+}
+
 bool isf_chunks_pay(pimc_ctx* c, int n, int M, const double* fqt) {
-    if (n <= kIsfChunkQ || (size_t)n * M * sizeof(double) < kIsfChunkMinBytes) return false;
+    if (n < 2 * kIsfChunkMinQ || (size_t)n * M * sizeof(double) < kIsfChunkMinBytes) return false;
     // a timed call keeps one launch per kernel, so its durations stay per call
     if (c->timing && (c->timing_kinds & ((1u << 2) | (1u << 4)))) return false;
     // a pageable destination is copied through staging with the host waiting
@@ -132,7 +139,7 @@ bool isf_chunks_pay(pimc_ctx* c, int n, int M, const double* fqt) {
 }
 
 int isf_lattice_chunked(pimc_ctx* c, const std::vector<int>& qn, int n, int M, double inv_n, double* fqt) {
-    const int per = std::max(kIsfChunkQ, (n + kIsfChunksMax - 1) / kIsfChunksMax);
+    const int per = isf_chunk_q(n);
     const int chunks = (n + per - 1) / per;
     int r;
     if ((r = ensure_q(c, n))) return r;

@@ -45,6 +45,13 @@ def _case(orc, N, M, kind, qmax):
     s_gpu = ctx.ssf(q)
     f_gpu = ctx.isf(q)
     extra = {}
+    if N == 256:
+        # 7.2 MB into page-locked memory: two halves, each copied out while the
+        # other computes
+        f_pin = pimc_amd.pinned_empty((q.shape[0], M))
+        ctx.isf(q, out=f_pin)
+        extra["chunks"] = ctx.kernel_time("isf_chunks")[1]
+        extra["f_pin"] = [np.array(f_pin)]
     if N == 2048:
         # into page-locked memory: the chunked form (modes and lags per chunk of
         # q, each chunk's rows copied out while the next computes), twice in a
@@ -103,6 +110,13 @@ def test_isf_n256_m167_all_q_all_lags(walker):
     assert walker["q"].shape[0] == 5377
     _check_identities(walker)
     _check_values(walker)
+
+
+def test_isf_n256_m167_halves_into_pinned(walker):
+    """F(q,tau) of the 5377 q into page-locked memory runs as two halves of the
+    q list: bit-identical to the one-launch result."""
+    assert walker["chunks"] == 2
+    assert np.array_equal(walker["f_pin"][0], walker["f_gpu"])
 
 
 def test_ssf_isf_n2048_m250_all_q_all_lags(large):
